@@ -1,0 +1,265 @@
+"""Python view of the MI355X IBIM-PB FMM library (ctypes over include/ibim.h).
+
+The product is the C ABI in ``_build/libibim_amd.so`` (HIP, gfx950) and the
+C++ host code beside it; this module only binds that ABI for tests and the
+benchmark.  There is no CPU fallback: if the library is missing, or no GPU is
+visible, calls fail with an error.
+
+The names follow the reference's interfaces (GaZ3ll3/ibim-levelset):
+``Plan.kernel_fmm`` is ``bbfmm::kernel::run`` (bbfmm.h:244-270),
+``Plan.matvec`` is the ``dmapping`` lambda (electric.cpp:294-388),
+``Plan.rhs`` the ``load`` loop (electric.cpp:395-413),
+``Plan.polarize`` is ``polarizeMap`` (electric.cpp:442-485) and
+``Plan.gmres`` is ``GMRES`` (gmres.cpp:50-164) on the plan's matvec.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libibim_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "ibim.h")
+
+KERNELS = ["K1x", "K1y", "K1z", "K2", "K3xx", "K3yy", "K3zz", "K3xy", "K3yz", "K3zx",
+           "K3yx", "K3zy", "K3xz", "K4x", "K4y", "K4z"]
+KID = {k: i for i, k in enumerate(KERNELS)}
+
+PREC_FP64, PREC_MIXED = 0, 1
+KERNEL_PB, KERNEL_POLY = 0, 1
+IBIM_OK, IBIM_NOT_CONVERGED = 0, 1
+
+
+class IbimError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [("p", C.c_int), ("max_points", C.c_int), ("max_level", C.c_int),
+                ("kappa", C.c_double), ("eps_in", C.c_double), ("eps_out", C.c_double),
+                ("vacant_radius", C.c_double), ("precision", C.c_int), ("kernel_mode", C.c_int)]
+
+
+class TreeStats(C.Structure):
+    _fields_ = [("n_points", C.c_int64), ("n_nodes", C.c_int64), ("n_leaves", C.c_int64),
+                ("n_empty_leaves", C.c_int64), ("depth", C.c_int),
+                ("n_u", C.c_int64), ("n_v", C.c_int64), ("n_w", C.c_int64), ("n_x", C.c_int64),
+                ("p2p_pairs", C.c_int64), ("m2l_pairs", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("matvecs", C.c_int), ("rel_residual", C.c_double),
+                ("converged", C.c_int), ("outer_steps", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def build(jobs=8):
+    """Compile the HIP library (hipcc cross-compiles gfx950 without a GPU)."""
+    subprocess.run(["make", "-s", f"-j{jobs}", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise IbimError(f"{LIB_PATH} is missing: build it with `make -C {HERE}`")
+    L = C.CDLL(LIB_PATH)
+    d = C.POINTER(C.c_double)
+    i = C.POINTER(C.c_int)
+    vp = C.c_void_p
+    pp = C.POINTER(C.c_void_p)
+    i64 = C.c_int64
+    L.ibim_plan_create.argtypes = [C.POINTER(Params), i64, d, d, d, pp]
+    L.ibim_plan_create_device.argtypes = [C.POINTER(Params), i64, vp, vp, vp, pp]
+    L.ibim_plan_destroy.argtypes = [vp]
+    L.ibim_plan_destroy.restype = None
+    L.ibim_plan_sync.argtypes = [vp]
+    L.ibim_plan_set_precision.argtypes = [vp, C.c_int]
+    L.ibim_plan_footprint.argtypes = [vp]
+    L.ibim_plan_footprint.restype = i64
+    L.ibim_matvec.argtypes = [vp, d, d]
+    L.ibim_matvec_device.argtypes = [vp, vp, vp]
+    L.ibim_kernel_fmm.argtypes = [vp, C.c_int, d, d]
+    L.ibim_rhs.argtypes = [vp, i64, d, d, d]
+    L.ibim_rhs_device.argtypes = [vp, i64, vp, vp, vp]
+    L.ibim_polarize.argtypes = [vp, i64, d, d, d]
+    L.ibim_polarize_device.argtypes = [vp, i64, vp, vp, vp]
+    L.ibim_gmres.argtypes = [vp, d, d, C.c_int, C.c_int, C.c_double, C.POINTER(SolveInfo)]
+    L.ibim_gmres_device.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_double, C.POINTER(SolveInfo)]
+    L.ibim_tree_stats_get.argtypes = [vp, C.POINTER(TreeStats)]
+    L.ibim_tree_export.argtypes = [vp, i, i, i, i, i, i, d, d, i, i, i, i]
+    L.ibim_tree_lists.argtypes = [vp, C.c_int, i, i, i64]
+    L.ibim_tree_points.argtypes = [vp, C.c_int, i, i64]
+    L.ibim_last_error.restype = C.c_char_p
+    L.ibim_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def declared_symbols():
+    """Function names declared in include/ibim.h."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(ibim_[a-z_0-9]+)\s*\(", txt)))
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise IbimError(f"{what} failed ({rc}): {lib().ibim_last_error().decode()}")
+    return rc
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.reshape(shape) if shape is not None else a
+
+
+def params(p=4, max_points=64, max_level=10, kappa=0.0, eps_in=1.0, eps_out=80.0, vacant_radius=0.0,
+           precision=PREC_FP64, kernel_mode=KERNEL_PB):
+    return Params(p, max_points, max_level, kappa, eps_in, eps_out, vacant_radius, precision, kernel_mode)
+
+
+class Plan:
+    """A device-resident IBIM plan: octree, lists and FMM buffers for one
+    surface point set (xyz, normals: (n,3); weights: (n,))."""
+
+    def __init__(self, xyz, normals=None, weights=None, device_ptrs=None, **kw):
+        self.prm = params(**kw)
+        self._h = C.c_void_p()
+        if device_ptrs is not None:
+            n, dxyz, dnrm, dw = device_ptrs
+            _check(lib().ibim_plan_create_device(C.byref(self.prm), n, dxyz, dnrm, dw, C.byref(self._h)),
+                   "ibim_plan_create_device")
+            self.n = int(n)
+        else:
+            xyz = _f64(xyz, (-1, 3))
+            n = len(xyz)
+            nrm = _f64(np.zeros((n, 3)) if normals is None else normals, (-1, 3))
+            w = _f64(np.ones(n) if weights is None else weights)
+            _check(lib().ibim_plan_create(C.byref(self.prm), n, _d(xyz), _d(nrm), _d(w), C.byref(self._h)),
+                   "ibim_plan_create")
+            self.n = n
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().ibim_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------- hot path --
+    def matvec(self, x):
+        x = _f64(x)
+        assert x.shape == (2 * self.n,)
+        y = np.zeros(2 * self.n)
+        _check(lib().ibim_matvec(self._h, _d(x), _d(y)), "ibim_matvec")
+        return y
+
+    def matvec_device(self, d_x, d_y):
+        _check(lib().ibim_matvec_device(self._h, d_x, d_y), "ibim_matvec_device")
+
+    def kernel_fmm(self, kernel, charge):
+        kid = KID[kernel] if isinstance(kernel, str) else int(kernel)
+        q = _f64(charge)
+        out = np.zeros(self.n)
+        _check(lib().ibim_kernel_fmm(self._h, kid, _d(q), _d(out)), "ibim_kernel_fmm")
+        return out
+
+    def rhs(self, atom_xyz, atom_q):
+        axyz = _f64(atom_xyz, (-1, 3))
+        aq = _f64(atom_q)
+        b = np.zeros(2 * self.n)
+        _check(lib().ibim_rhs(self._h, len(axyz), _d(axyz), _d(aq), _d(b)), "ibim_rhs")
+        return b
+
+    def polarize(self, atom_xyz, x):
+        axyz = _f64(atom_xyz, (-1, 3))
+        out = np.zeros(len(axyz))
+        _check(lib().ibim_polarize(self._h, len(axyz), _d(axyz), _d(_f64(x)), _d(out)), "ibim_polarize")
+        return out
+
+    def gmres(self, b, x0=None, restart=20, maxit=100, tol=1e-8):
+        b = _f64(b)
+        x = np.zeros(2 * self.n) if x0 is None else _f64(x0).copy()
+        info = SolveInfo()
+        rc = _check(lib().ibim_gmres(self._h, _d(b), _d(x), restart, maxit, tol, C.byref(info)), "ibim_gmres")
+        return x, rc, info.as_dict()
+
+    def sync(self):
+        _check(lib().ibim_plan_sync(self._h), "ibim_plan_sync")
+
+    def set_precision(self, precision):
+        _check(lib().ibim_plan_set_precision(self._h, precision), "ibim_plan_set_precision")
+
+    def footprint(self):
+        return int(lib().ibim_plan_footprint(self._h))
+
+    # -------------------------------------------------------- inspection --
+    def stats(self):
+        s = TreeStats()
+        _check(lib().ibim_tree_stats_get(self._h, C.byref(s)), "ibim_tree_stats_get")
+        return s.as_dict()
+
+    def tree(self):
+        nn = self.stats()["n_nodes"]
+        ints = {k: np.zeros(nn, np.int32) for k in
+                ["level", "parent", "slot", "leaf", "empty", "nsrc", "nu", "nv", "nw", "nx"]}
+        centre = np.zeros((nn, 3))
+        radius = np.zeros((nn, 3))
+        _check(lib().ibim_tree_export(self._h, _i(ints["level"]), _i(ints["parent"]), _i(ints["slot"]),
+                                      _i(ints["leaf"]), _i(ints["empty"]), _i(ints["nsrc"]), _d(centre),
+                                      _d(radius), _i(ints["nu"]), _i(ints["nv"]), _i(ints["nw"]),
+                                      _i(ints["nx"])), "ibim_tree_export")
+        ints["centre"] = centre
+        ints["radius"] = radius
+        return ints
+
+    def lists(self, kind):
+        k = {"U": 0, "V": 1, "W": 2, "X": 3}[kind]
+        nn = self.stats()["n_nodes"]
+        off = np.zeros(nn + 1, np.int32)
+        _check(lib().ibim_tree_lists(self._h, k, _i(off), None, 0), "ibim_tree_lists")
+        items = np.zeros(max(int(off[-1]), 1), np.int32)
+        _check(lib().ibim_tree_lists(self._h, k, _i(off), _i(items), len(items)), "ibim_tree_lists")
+        return off, items[:off[-1]]
+
+    def points(self, node):
+        cnt = _check(lib().ibim_tree_points(self._h, node, None, 0), "ibim_tree_points")
+        out = np.zeros(max(cnt, 1), np.int32)
+        _check(lib().ibim_tree_points(self._h, node, _i(out), len(out)), "ibim_tree_points")
+        return out[:cnt]

@@ -1,0 +1,576 @@
+// ibim_fmm.hip -- the Chebyshev black-box FMM passes, all 16 PB kernels fused.
+//
+// Reference: bbfmm::kernel::upPass / downPass (bbfmm.h:500-623) run once per
+// kernel closure (electric.cpp:352-367).  Every operator of that FMM is linear
+// and shared by the 16 kernels, so one pass carries 4 multipole fields
+// (the weighted charges qx, qy, qz, qpsi of electric.cpp:321-329) up the tree
+// and 4 local fields (F0 = sum_a K1a[qa] - K2[qpsi], Fa = sum_b K3ab[qb] -
+// K4a[qpsi]) down it.  That is the combination dmapping takes of the 16 FMM
+// results (electric.cpp:369-384), reassociated.
+//
+//   P2M   leaf multipole M = S^T q, S the tensor Chebyshev interpolation
+//         matrix at the standardized source coordinates (bbfmm.h:392-433,519)
+//   M2M   M_parent += R[c]^T M_child, applied as three p x p contractions
+//         instead of the dense r x r R[c] (bbfmm.h:328-367,530-534)
+//   M2L   L_B += K(grid_B, grid_A) M_A over the V and X lists (bbfmm.h:547-561)
+//   L2L   L_child += R[c] L_parent (bbfmm.h:566-567)
+//   P2P   near field over the U and W lists, W boxes with all their subtree
+//         sources (bbfmm.h:577-595)
+//   L2P   pot += S L_leaf at the standardized target coordinate (bbfmm.h:600)
+#include <cmath>
+
+#include "ibim_internal.h"
+
+namespace ibim {
+
+namespace {
+
+inline int nblk(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
+
+// ----------------------------------------------------------------- P2M ---
+template <int P>
+__global__ __launch_bounds__(256) void k_p2m(const int *__restrict__ leaves, const double *__restrict__ sx,
+                                             const double *__restrict__ sy, const double *__restrict__ sz,
+                                             const double *__restrict__ Q, int n, const int *__restrict__ sbeg,
+                                             const int *__restrict__ send, const double *__restrict__ ctr,
+                                             const int *__restrict__ level, const double *__restrict__ lrad,
+                                             ChebTab tab, double *__restrict__ M) {
+    constexpr int R = P * P * P;
+    constexpr int PER = (R + 255) / 256;
+    constexpr int CH = 64;
+    __shared__ double sS[3][P][CH];
+    __shared__ double sq[4][CH];
+    const int b = leaves[blockIdx.x];
+    const int tid = threadIdx.x;
+    const double cx = ctr[3 * b], cy = ctr[3 * b + 1], cz = ctr[3 * b + 2];
+    const double *rd = lrad + 3 * level[b];
+    const double rx = rd[0], ry = rd[1], rz = rd[2];
+    double acc[PER][4];
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+#pragma unroll
+        for (int f = 0; f < 4; ++f) acc[k][f] = 0.0;
+    const int beg = sbeg[b], end = send[b];
+    for (int base = beg; base < end; base += CH) {
+        const int cnt = min(CH, end - base);
+        if (tid < cnt) {
+            const int s = base + tid;
+            double v[P];
+            sp_values<P>((sx[s] - cx) / rx, tab, v);
+#pragma unroll
+            for (int m = 0; m < P; ++m) sS[0][m][tid] = v[m];
+            sp_values<P>((sy[s] - cy) / ry, tab, v);
+#pragma unroll
+            for (int m = 0; m < P; ++m) sS[1][m][tid] = v[m];
+            sp_values<P>((sz[s] - cz) / rz, tab, v);
+#pragma unroll
+            for (int m = 0; m < P; ++m) sS[2][m][tid] = v[m];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) sq[f][tid] = Q[(size_t)f * n + s];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int m = tid + 256 * k;
+            if (m < R) {
+                const int i = m % P, j = (m / P) % P, l = m / (P * P);
+                for (int s = 0; s < cnt; ++s) {
+                    const double w = sS[0][i][s] * sS[1][j][s] * sS[2][l][s];
+                    acc[k][0] += w * sq[0][s];
+                    acc[k][1] += w * sq[1][s];
+                    acc[k][2] += w * sq[2][s];
+                    acc[k][3] += w * sq[3][s];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double *Mb = M + (size_t)b * 4 * R;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int m = tid + 256 * k;
+        if (m < R)
+#pragma unroll
+            for (int f = 0; f < 4; ++f) Mb[f * R + m] = acc[k][f];
+    }
+}
+
+// load the three p x p child transfer factors of child slot c into LDS
+template <int P>
+__device__ __forceinline__ void load_transfer(const ChebTab &tab, int c, double (*sT)[P][P]) {
+    const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
+    for (int e = threadIdx.x; e < 3 * P * P; e += blockDim.x) {
+        const int a = e / (P * P), i = (e / P) % P, l = e % P;
+        const int b = a == 0 ? bx : (a == 1 ? by : bz);
+        // kernel-argument arrays indexed at run time are read through a copy
+        double v = 0.0;
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int ii = 0; ii < P; ++ii)
+#pragma unroll
+                for (int ll = 0; ll < P; ++ll)
+                    if (bb == b && ii == i && ll == l) v = tab.tr[bb][ii][ll];
+        sT[a][i][l] = v;
+    }
+}
+
+// ----------------------------------------------------------------- M2M ---
+// out(f,l,m,n) += sum_i Tz(i,l) sum_j Ty(j,m) sum_k Tx(k,n) Mc(f,i,j,k)
+// with T(i,l) = S_p(child node i, parent node l), index i*P*P + j*P + k
+template <int P>
+__global__ __launch_bounds__(256) void k_m2m(int lo, const int *__restrict__ flags, const int *__restrict__ fc,
+                                             ChebTab tab, double *__restrict__ M) {
+    constexpr int R = P * P * P;
+    constexpr int R4 = 4 * R;
+    constexpr int PER = (R4 + 255) / 256;
+    __shared__ double sIn[R4];
+    __shared__ double sA[R4];
+    __shared__ double sT[3][P][P];
+    const int id = lo + blockIdx.x;
+    if (flags[id] & kLeaf) return;
+    const int tid = threadIdx.x;
+    double acc[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) acc[k] = 0.0;
+    const int c0 = fc[id];
+    for (int c = 0; c < 8; ++c) {
+        const int ch = c0 + c;
+        if (flags[ch] & kEmpty) continue;  // bbfmm.h:531
+        load_transfer<P>(tab, c, sT);
+        for (int e = tid; e < R4; e += 256) sIn[e] = M[(size_t)ch * R4 + e];
+        __syncthreads();
+        for (int e = tid; e < R4; e += 256) {  // A(f,i,j,n)
+            const int f = e / R, rem = e % R, i = rem / (P * P), j = (rem / P) % P, nn = rem % P;
+            const double *in = sIn + f * R + i * P * P + j * P;
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < P; ++k) s += sT[0][k][nn] * in[k];
+            sA[e] = s;
+        }
+        __syncthreads();
+        for (int e = tid; e < R4; e += 256) {  // B(f,i,m,n) -> sIn
+            const int f = e / R, rem = e % R, i = rem / (P * P), m = (rem / P) % P, nn = rem % P;
+            const double *in = sA + f * R + i * P * P + nn;
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < P; ++j) s += sT[1][j][m] * in[j * P];
+            sIn[e] = s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {  // out(f,l,m,n)
+            const int e = tid + 256 * k;
+            if (e < R4) {
+                const int f = e / R, rem = e % R, l = rem / (P * P), m = (rem / P) % P, nn = rem % P;
+                const double *in = sIn + f * R + m * P + nn;
+                double s = 0.0;
+#pragma unroll
+                for (int i = 0; i < P; ++i) s += sT[2][i][l] * in[i * P * P];
+                acc[k] += s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int e = tid + 256 * k;
+        if (e < R4) M[(size_t)id * R4 + e] = acc[k];
+    }
+}
+
+// ----------------------------------------------------------------- L2L ---
+// Lc(f,i,j,k) += sum_l Tz(i,l) sum_m Ty(j,m) sum_n Tx(k,n) Lp(f,l,m,n)
+template <int P>
+__global__ __launch_bounds__(256) void k_l2l(int lo, const unsigned char *__restrict__ need,
+                                             const int *__restrict__ parent, const int *__restrict__ slot,
+                                             ChebTab tab, double *__restrict__ L) {
+    constexpr int R = P * P * P;
+    constexpr int R4 = 4 * R;
+    __shared__ double sIn[R4];
+    __shared__ double sA[R4];
+    __shared__ double sT[3][P][P];
+    const int id = lo + blockIdx.x;
+    if (!need[id]) return;
+    const int tid = threadIdx.x;
+    const int pid = parent[id];
+    load_transfer<P>(tab, slot[id], sT);
+    for (int e = tid; e < R4; e += 256) sIn[e] = L[(size_t)pid * R4 + e];
+    __syncthreads();
+    for (int e = tid; e < R4; e += 256) {  // A(f,l,m,k) = sum_n Tx(k,n) Lp(f,l,m,n)
+        const int f = e / R, rem = e % R, l = rem / (P * P), m = (rem / P) % P, k = rem % P;
+        const double *in = sIn + f * R + l * P * P + m * P;
+        double s = 0.0;
+#pragma unroll
+        for (int nn = 0; nn < P; ++nn) s += sT[0][k][nn] * in[nn];
+        sA[e] = s;
+    }
+    __syncthreads();
+    for (int e = tid; e < R4; e += 256) {  // B(f,l,j,k) = sum_m Ty(j,m) A(f,l,m,k)
+        const int f = e / R, rem = e % R, l = rem / (P * P), j = (rem / P) % P, k = rem % P;
+        const double *in = sA + f * R + l * P * P + k;
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < P; ++m) s += sT[1][j][m] * in[m * P];
+        sIn[e] = s;
+    }
+    __syncthreads();
+    for (int e = tid; e < R4; e += 256) {  // C(f,i,j,k) = sum_l Tz(i,l) B(f,l,j,k)
+        const int f = e / R, rem = e % R, i = rem / (P * P), j = (rem / P) % P, k = rem % P;
+        const double *in = sIn + f * R + j * P + k;
+        double s = 0.0;
+#pragma unroll
+        for (int l = 0; l < P; ++l) s += sT[2][i][l] * in[l * P * P];
+        L[(size_t)id * R4 + e] += s;
+    }
+}
+
+// ------------------------------------------------------ M2L, direct form ---
+// L_B = sum over non-empty A in V(B) and X(B) of K(grid_B, grid_A) M_A, the
+// kernel matrix evaluated at the tensor Chebyshev grids of both boxes
+// (kernelEvalChebyshev, bbfmm.h:456-479).
+template <int P, int BS>
+__global__ __launch_bounds__(BS) void k_m2l_direct(const int *__restrict__ work, const int *__restrict__ voff,
+                                                   const int *__restrict__ vit, const int *__restrict__ xoff,
+                                                   const int *__restrict__ xit, const int *__restrict__ flags,
+                                                   const double *__restrict__ ctr, const int *__restrict__ level,
+                                                   const double *__restrict__ lrad, ChebTab tab, PBConst kc,
+                                                   const double *__restrict__ M, double *__restrict__ L) {
+    constexpr int R = P * P * P;
+    constexpr int PER = (R + BS - 1) / BS;
+    __shared__ double spx[R], spy[R], spz[R];
+    __shared__ double sm[4][R];
+    __shared__ double sxb[P];
+    const int id = work[blockIdx.x];
+    const int tid = threadIdx.x;
+    if (tid < P) {
+        double v = 0.0;
+#pragma unroll
+        for (int i = 0; i < P; ++i)
+            if (i == tid) v = tab.xbar[i];
+        sxb[tid] = v;
+    }
+    __syncthreads();
+    const double *rb = lrad + 3 * level[id];
+    double tx[PER], ty[PER], tz[PER], acc[PER][4];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int m = tid + BS * k;
+        const int i = m % P, j = (m / P) % P, l = (m / (P * P)) % P;
+        tx[k] = ctr[3 * id] + rb[0] * sxb[i];
+        ty[k] = ctr[3 * id + 1] + rb[1] * sxb[j];
+        tz[k] = ctr[3 * id + 2] + rb[2] * sxb[l];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) acc[k][f] = 0.0;
+    }
+    for (int lst = 0; lst < 2; ++lst) {
+        const int *off = lst == 0 ? voff : xoff;
+        const int *it = lst == 0 ? vit : xit;
+        for (int q = off[id]; q < off[id + 1]; ++q) {
+            const int a = it[q];
+            if (flags[a] & kEmpty) continue;
+            const double *ra = lrad + 3 * level[a];
+            const double ax = ctr[3 * a], ay = ctr[3 * a + 1], az = ctr[3 * a + 2];
+            for (int e = tid; e < R; e += BS) {
+                const int i = e % P, j = (e / P) % P, l = e / (P * P);
+                spx[e] = ax + ra[0] * sxb[i];
+                spy[e] = ay + ra[1] * sxb[j];
+                spz[e] = az + ra[2] * sxb[l];
+            }
+            const double *Ma = M + (size_t)a * 4 * R;
+            for (int e = tid; e < 4 * R; e += BS) sm[e / R][e % R] = Ma[e];
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                if (tid + BS * k < R) {
+                    double f0 = acc[k][0], fx = acc[k][1], fy = acc[k][2], fz = acc[k][3];
+                    for (int s = 0; s < R; ++s)
+                        pair_accum<double>(spx[s] - tx[k], spy[s] - ty[k], spz[s] - tz[k], sm[0][s], sm[1][s],
+                                           sm[2][s], sm[3][s], kc, f0, fx, fy, fz);
+                    acc[k][0] = f0;
+                    acc[k][1] = fx;
+                    acc[k][2] = fy;
+                    acc[k][3] = fz;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    double *Lb = L + (size_t)id * 4 * R;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int m = tid + BS * k;
+        if (m < R)
+#pragma unroll
+            for (int f = 0; f < 4; ++f) Lb[f * R + m] = acc[k][f];
+    }
+}
+
+// ----------------------------------------------------------- P2P + L2P ---
+template <int P>
+__global__ __launch_bounds__(64) void k_p2p_l2p(
+    const int *__restrict__ tleaves, const int *__restrict__ tbeg, const int *__restrict__ tend,
+    const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ sx, const double *__restrict__ sy, const double *__restrict__ sz,
+    const double *__restrict__ Q, int ns, const int *__restrict__ uoff, const int *__restrict__ uit,
+    const int *__restrict__ woff, const int *__restrict__ wit, const int *__restrict__ flags,
+    const int *__restrict__ sbeg, const int *__restrict__ send, const double *__restrict__ ctr,
+    const int *__restrict__ level, const double *__restrict__ lrad, ChebTab tab, PBConst kc,
+    const double *__restrict__ L, double *__restrict__ F, int nt) {
+    constexpr int R = P * P * P;
+    constexpr int T = 64;
+    __shared__ double s_x[T], s_y[T], s_z[T], s_q[4][T];
+    __shared__ double sL[4 * R];
+    const int b = tleaves[blockIdx.x];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < 4 * R; e += T) sL[e] = L[(size_t)b * 4 * R + e];
+    const double cx = ctr[3 * b], cy = ctr[3 * b + 1], cz = ctr[3 * b + 2];
+    const double *rd = lrad + 3 * level[b];
+    const int t_beg = tbeg[b], t_end = tend[b];
+    for (int t0 = t_beg; t0 < t_end; t0 += T) {
+        const int t = t0 + tid;
+        const bool valid = t < t_end;
+        const double px = valid ? tx[t] : 0.0, py = valid ? ty[t] : 0.0, pz = valid ? tz[t] : 0.0;
+        double f0 = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int lst = 0; lst < 2; ++lst) {
+            const int *off = lst == 0 ? uoff : woff;
+            const int *it = lst == 0 ? uit : wit;
+            for (int q = off[b]; q < off[b + 1]; ++q) {
+                const int a = it[q];
+                if (flags[a] & kEmpty) continue;
+                const int s_end = send[a];
+                for (int s0 = sbeg[a]; s0 < s_end; s0 += T) {
+                    const int cnt = min(T, s_end - s0);
+                    __syncthreads();
+                    if (tid < cnt) {
+                        const int s = s0 + tid;
+                        s_x[tid] = sx[s];
+                        s_y[tid] = sy[s];
+                        s_z[tid] = sz[s];
+                        s_q[0][tid] = Q[s];
+                        s_q[1][tid] = Q[(size_t)ns + s];
+                        s_q[2][tid] = Q[2 * (size_t)ns + s];
+                        s_q[3][tid] = Q[3 * (size_t)ns + s];
+                    }
+                    __syncthreads();
+                    if (valid)
+                        for (int s = 0; s < cnt; ++s)
+                            pair_accum<double>(s_x[s] - px, s_y[s] - py, s_z[s] - pz, s_q[0][s], s_q[1][s],
+                                               s_q[2][s], s_q[3][s], kc, f0, fx, fy, fz);
+                }
+            }
+        }
+        __syncthreads();  // sL loaded
+        if (valid) {
+            double Sx[P], Sy[P], Sz[P];
+            sp_values<P>((px - cx) / rd[0], tab, Sx);
+            sp_values<P>((py - cy) / rd[1], tab, Sy);
+            sp_values<P>((pz - cz) / rd[2], tab, Sz);
+            double g0 = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+            for (int l = 0; l < P; ++l)
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const double wyz = Sy[j] * Sz[l];
+#pragma unroll
+                    for (int i = 0; i < P; ++i) {
+                        const int m = l * P * P + j * P + i;
+                        const double w = Sx[i] * wyz;
+                        g0 += w * sL[m];
+                        gx += w * sL[R + m];
+                        gy += w * sL[2 * R + m];
+                        gz += w * sL[3 * R + m];
+                    }
+                }
+            F[t] = f0 + g0;
+            F[(size_t)nt + t] = fx + gx;
+            F[2 * (size_t)nt + t] = fy + gy;
+            F[3 * (size_t)nt + t] = fz + gz;
+        }
+    }
+}
+
+// ------------------------------------------------------------- helpers ---
+__global__ void k_charges(int n, const int *__restrict__ perm, const double *__restrict__ snx,
+                          const double *__restrict__ sny, const double *__restrict__ snz,
+                          const double *__restrict__ sw, const double *__restrict__ x, double *__restrict__ Q) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = perm[i];
+    const double phi = x[o], psi = x[(size_t)n + o], w = sw[i];
+    Q[i] = phi * snx[i] * w;  // electric.cpp:325-328
+    Q[(size_t)n + i] = phi * sny[i] * w;
+    Q[2 * (size_t)n + i] = phi * snz[i] * w;
+    Q[3 * (size_t)n + i] = psi * w;
+}
+
+__global__ void k_combine(int n, const int *__restrict__ perm, const double *__restrict__ snx,
+                          const double *__restrict__ sny, const double *__restrict__ snz, const double *__restrict__ F,
+                          const double *__restrict__ x, double *__restrict__ y, double c1, double c2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = perm[i];
+    const double f0 = F[i], fx = F[(size_t)n + i], fy = F[2 * (size_t)n + i], fz = F[3 * (size_t)n + i];
+    y[o] = x[o] + f0 / c1;  // electric.cpp:382-383
+    y[(size_t)n + o] = x[(size_t)n + o] + (snx[i] * fx + sny[i] * fy + snz[i] * fz) / c2;
+}
+
+// one input field from a charge vector, the others zero
+__global__ void k_single_charge(int n, const int *__restrict__ perm, const double *__restrict__ q, int field,
+                                double *__restrict__ Q) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = q[perm[i]];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) Q[(size_t)f * n + i] = f == field ? v : 0.0;
+}
+
+__global__ void k_scatter_field(int n, const int *__restrict__ perm, const double *__restrict__ F, int field,
+                                double sign, double *__restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[perm[i]] = sign * F[(size_t)field * n + i];
+}
+
+template <int P>
+int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt) {
+    constexpr int R = P * P * P;
+    constexpr int BS = R <= 64 ? 64 : (R <= 128 ? 128 : 256);
+    hipStream_t st = pl->stream;
+    // upward pass
+    if (pl->n_leaves_ne > 0)
+        k_p2m<P><<<pl->n_leaves_ne, 256, 0, st>>>(pl->leaves_ne, pl->sx, pl->sy, pl->sz, pl->Q, pl->n, pl->sbeg,
+                                                   pl->send, pl->ctr, pl->level, pl->d_lvl_rad, pl->tab, pl->M);
+    IBIM_LAUNCH_CHECK();
+    for (int l = pl->nlevels - 2; l >= 0; --l) {
+        const int cnt = pl->lvl_off[l + 1] - pl->lvl_off[l];
+        k_m2m<P><<<cnt, 256, 0, st>>>(pl->lvl_off[l], pl->flags, pl->first_child, pl->tab, pl->M);
+    }
+    IBIM_LAUNCH_CHECK();
+    // downward pass
+    IBIM_HIP_CHECK(hipMemsetAsync(pl->L, 0, (size_t)pl->nnodes * 4 * R * sizeof(double), st));
+    if (ts.n_m2l_work > 0)
+        k_m2l_direct<P, BS><<<ts.n_m2l_work, BS, 0, st>>>(ts.m2l_work, pl->loff[1], pl->litem[1], pl->loff[3],
+                                                          pl->litem[3], pl->flags, pl->ctr, pl->level,
+                                                          pl->d_lvl_rad, pl->tab, pl->kc, pl->M, pl->L);
+    IBIM_LAUNCH_CHECK();
+    for (int l = 1; l < pl->nlevels; ++l) {
+        const int cnt = pl->lvl_off[l + 1] - pl->lvl_off[l];
+        k_l2l<P><<<cnt, 256, 0, st>>>(pl->lvl_off[l], ts.need_local, pl->parent, pl->slot, pl->tab, pl->L);
+    }
+    IBIM_LAUNCH_CHECK();
+    if (ts.nleaves > 0)
+        k_p2p_l2p<P><<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->sx, pl->sy,
+                                                pl->sz, pl->Q, pl->n, pl->loff[0], pl->litem[0], pl->loff[2],
+                                                pl->litem[2], pl->flags, pl->sbeg, pl->send, pl->ctr, pl->level,
+                                                pl->d_lvl_rad, pl->tab, pl->kc, pl->L, F, nt);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
+}  // namespace
+
+int build_chebyshev(ibim_plan *pl) {
+    const int p = pl->p;
+    ChebTab &t = pl->tab;
+    t = ChebTab{};
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < p; ++i) t.xbar[i] = -cos((i + 0.5) * pi / p);  // bbfmm.h:276
+    for (int m = 0; m < p; ++m) {                                       // bbfmm.h:280-303
+        t.Tbar[m][0] = 1.0;
+        if (p > 1) t.Tbar[m][1] = t.xbar[m];
+        for (int k = 2; k < p; ++k) t.Tbar[m][k] = 2.0 * t.xbar[m] * t.Tbar[m][k - 1] - t.Tbar[m][k - 2];
+    }
+    t.invp = 1.0 / p;
+    for (int b = 0; b < 2; ++b)  // bbfmm.h:305-326
+        for (int i = 0; i < p; ++i) {
+            const double u = 0.5 * t.xbar[i] + (b ? 0.5 : -0.5);
+            double T[kMaxP];
+            T[0] = 1.0;
+            if (p > 1) T[1] = u;
+            for (int k = 2; k < p; ++k) T[k] = 2.0 * u * T[k - 1] - T[k - 2];
+            for (int l = 0; l < p; ++l) {
+                double acc = 0.0;
+                for (int k = 0; k < p; ++k) acc += T[k] * t.Tbar[l][k];
+                t.tr[b][i][l] = (2.0 * acc - 1.0) * t.invp;
+            }
+        }
+    const ibim_params &q = pl->prm;
+    PBConst &k = pl->kc;
+    k.kappa = q.kappa;
+    k.eps = q.eps_out / q.eps_in;
+    k.inv_eps = q.eps_in / q.eps_out;
+    k.vr = q.vacant_radius;
+    k.inv4pi = 1.0 / (4.0 * pi);
+    k.mode = q.kernel_mode;
+    const double vr = q.vacant_radius, kap = q.kappa;  // electric.cpp:94-97
+    k.k2reg = (kap == 0.0) ? 0.0 : (exp(-kap * vr) - 1.0 + kap * vr) / kap / 2.0 / pi / vr / vr;
+    return IBIM_OK;
+}
+
+int fmm_forward(ibim_plan *pl, const TargetSet &ts, double *F, int nt) {
+    switch (pl->p) {
+    case 2: return fmm_forward_p<2>(pl, ts, F, nt);
+    case 3: return fmm_forward_p<3>(pl, ts, F, nt);
+    case 4: return fmm_forward_p<4>(pl, ts, F, nt);
+    case 5: return fmm_forward_p<5>(pl, ts, F, nt);
+    case 6: return fmm_forward_p<6>(pl, ts, F, nt);
+    case 7: return fmm_forward_p<7>(pl, ts, F, nt);
+    case 8: return fmm_forward_p<8>(pl, ts, F, nt);
+    }
+    set_error("unsupported Chebyshev order");
+    return IBIM_ERR_ARG;
+}
+
+int form_charges(ibim_plan *pl, const double *d_x) {
+    k_charges<<<nblk(pl->n), 256, 0, pl->stream>>>(pl->n, pl->perm, pl->snx, pl->sny, pl->snz, pl->sw, d_x, pl->Q);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
+int combine(ibim_plan *pl, const double *d_x, double *d_y) {
+    const double dE = pl->prm.eps_out, dI = pl->prm.eps_in;
+    k_combine<<<nblk(pl->n), 256, 0, pl->stream>>>(pl->n, pl->perm, pl->snx, pl->sny, pl->snz, pl->F, d_x, d_y,
+                                                   0.5 * (1 + dE / dI), 0.5 * (1 + dI / dE));
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
+TargetSet source_targets(ibim_plan *pl) {
+    TargetSet ts;
+    ts.n = pl->n;
+    ts.x = pl->sx;
+    ts.y = pl->sy;
+    ts.z = pl->sz;
+    ts.tbeg = pl->sbeg;
+    ts.tend = pl->send;
+    ts.leaves = pl->leaves_ne;
+    ts.nleaves = pl->n_leaves_ne;
+    ts.m2l_work = pl->m2l_work;
+    ts.n_m2l_work = pl->n_m2l_work;
+    ts.need_local = pl->need_local;
+    return ts;
+}
+
+// single kernel FMM: which charge field feeds it, which output field holds
+// it and with which sign (see pair_accum for the fused combination)
+int kernel_fmm_device(ibim_plan *pl, int kid, const double *d_q, double *d_out) {
+    static const int fin[16] = {0, 1, 2, 3, 0, 1, 2, 0, 1, 2, 0, 1, 2, 3, 3, 3};
+    static const int fout[16] = {0, 0, 0, 0, 1, 2, 3, 2, 3, 1, 2, 3, 1, 1, 2, 3};
+    static const double sgn[16] = {1, 1, 1, -1, 1, 1, 1, 1, 1, 1, 1, 1, 1, -1, -1, -1};
+    if (kid < 0 || kid > 15) {
+        set_error("kernel_id out of range");
+        return IBIM_ERR_ARG;
+    }
+    hipStream_t st = pl->stream;
+    k_single_charge<<<nblk(pl->n), 256, 0, st>>>(pl->n, pl->perm, d_q, fin[kid], pl->Q);
+    IBIM_LAUNCH_CHECK();
+    TargetSet ts = source_targets(pl);
+    int rc = fmm_forward(pl, ts, pl->F, pl->n);
+    if (rc) return rc;
+    k_scatter_field<<<nblk(pl->n), 256, 0, st>>>(pl->n, pl->perm, pl->F, fout[kid], sgn[kid], d_out);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
+}  // namespace ibim

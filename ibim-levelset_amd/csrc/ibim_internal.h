@@ -1,0 +1,234 @@
+// ibim_internal.h -- plan layout and device helpers shared by the HIP sources.
+//
+// Data layout in HBM (one plan = one surface point set):
+//   points      sorted by octree key (stable in the original index), SoA
+//               x, y, z, nx, ny, nz, w; perm[sorted] = original index
+//   nodes       breadth first; the 8 children of a split node consecutive;
+//               centre (3 doubles), level, parent, slot, flags, source range
+//               [sbeg, send) in sorted order (a subtree is one range)
+//   lists       CSR per kind (U, V, W, X) over all nodes
+//   expansions  M (multipole) and L (local): 4 fields x r doubles per node,
+//               fields = (qx, qy, qz, qpsi) upward, (F0, Fx, Fy, Fz) downward
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ibim.h"
+
+namespace ibim {
+
+constexpr double kEps = 1e-12;  // adjacency / matching tolerance, utils.h:40
+constexpr int kMaxP = 8;
+constexpr int kLeaf = 1;
+constexpr int kEmpty = 2;
+
+void set_error(const std::string &msg);
+
+#define IBIM_HIP_CHECK(expr)                                                              \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            ibim::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));           \
+            return IBIM_ERR_HIP;                                                          \
+        }                                                                                 \
+    } while (0)
+
+#define IBIM_LAUNCH_CHECK()                                                               \
+    do {                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                \
+        if (e_ != hipSuccess) {                                                           \
+            ibim::set_error(std::string("kernel launch: ") + hipGetErrorString(e_));      \
+            return IBIM_ERR_HIP;                                                          \
+        }                                                                                 \
+    } while (0)
+
+// Chebyshev tables, passed by value to kernels (bbfmm.h:273-367)
+struct ChebTab {
+    double xbar[kMaxP];            // standard nodes -cos((i+1/2) pi / p)
+    double Tbar[kMaxP][kMaxP];     // Tbar[m][k] = T_k(xbar_m)
+    double tr[2][kMaxP][kMaxP];    // tr[b][i][l] = S_p(child-b node i, xbar_l)
+    double invp;                   // 1/p
+};
+
+// kernel constants of electric.cpp:52-58
+struct PBConst {
+    double kappa;
+    double eps;       // eps_out / eps_in
+    double inv_eps;   // eps_in / eps_out
+    double vr;        // vacant radius tau*h
+    double k2reg;     // K2 below the vacant radius (disk average), 0 if kappa == 0
+    double inv4pi;
+    int mode;         // IBIM_KERNEL_*
+};
+
+// --------------------------------------------------------------------------
+// The 16 kernels fused: one source s with weighted charges (qx, qy, qz, qp)
+// acting on one target t, d = s - t.  Accumulates
+//   F0 += sum_a K1a qa - K2 qp
+//   Fa += sum_b K3ab qb - K4a qp
+// which is exactly what dmapping forms from the 16 separate FMM results
+// (electric.cpp:369-384); the kernels are electric.cpp:67-291:
+//   K1a = d_a (eps e (t+1) - 1) / (4 pi r^3)          t = kappa r, e = exp(-t)
+//   K2  = (1 - e) / (4 pi r)
+//   K3ab = delta_ab (1 - e (t+1)) / r^3 + (e (t^2 + 3t + 3) - 3) d_a d_b / r^5
+//   K4a = -d_a (e (t+1) / eps - 1) / (4 pi r^3)
+// and below the vacant radius K1 = K3 = K4 = 0, K2 = k2reg.
+template <typename T>
+__device__ __forceinline__ void pair_accum(T dx, T dy, T dz, T qx, T qy, T qz, T qp, const PBConst &k,
+                                           T &f0, T &fx, T &fy, T &fz) {
+    if (k.mode == IBIM_KERNEL_POLY) {
+        // K1a = d_a, K2 = 0.5 + 0.25 r^2, K3ab = delta_ab + 0.5 d_a d_b, K4a = -0.75 d_a
+        T r2 = dx * dx + dy * dy + dz * dz;
+        T dq = dx * qx + dy * qy + dz * qz;
+        f0 += dq - (T(0.5) + T(0.25) * r2) * qp;
+        T s = T(0.5) * dq + T(0.75) * qp;
+        fx += qx + dx * s;
+        fy += qy + dy * s;
+        fz += qz + dz * s;
+        return;
+    }
+    T r2 = dx * dx + dy * dy + dz * dz;
+    T r = sqrt(r2);
+    if (r < T(k.vr)) {
+        f0 -= T(k.k2reg) * qp;
+        return;
+    }
+    T ir = T(1) / r;
+    T ir2 = ir * ir;
+    T ir3 = ir2 * ir;
+    T t = T(k.kappa) * r;
+    T e = exp(-t);
+    T et1 = e * (t + T(1));
+    T c4 = T(k.inv4pi) * ir3;
+    T g1 = (T(k.eps) * et1 - T(1)) * c4;
+    T k2 = (T(1) - e) * ir * T(k.inv4pi);
+    T a3 = (T(1) - et1) * ir3;
+    T b3 = (e * ((t + T(3)) * t + T(3)) - T(3)) * ir3 * ir2;
+    T g4 = (T(k.inv_eps) * et1 - T(1)) * c4;
+    T dq = dx * qx + dy * qy + dz * qz;
+    f0 += g1 * dq - k2 * qp;
+    T s = b3 * dq + g4 * qp;
+    fx += a3 * qx + dx * s;
+    fy += a3 * qy + dy * s;
+    fz += a3 * qz + dz * s;
+}
+
+// S_p(u, xbar_m) for m < P at one standardized coordinate u (bbfmm.h:408-419)
+template <int P>
+__device__ __forceinline__ void sp_values(double u, const ChebTab &tab, double *out) {
+    double T[P];
+    T[0] = 1.0;
+    if (P > 1) T[1] = u;
+#pragma unroll
+    for (int k = 2; k < P; ++k) T[k] = 2.0 * u * T[k - 1] - T[k - 2];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) acc += T[k] * tab.Tbar[m][k];
+        out[m] = (2.0 * acc - 1.0) * tab.invp;
+    }
+}
+
+// a set of targets sorted into the leaves of a plan's tree
+struct TargetSet {
+    int n = 0;
+    const double *x = nullptr, *y = nullptr, *z = nullptr;  // sorted target coords
+    const int *tbeg = nullptr, *tend = nullptr;             // per node ranges
+    const int *leaves = nullptr;                            // leaves with targets
+    int nleaves = 0;
+    const int *m2l_work = nullptr;                          // nodes needing a local expansion
+    int n_m2l_work = 0;
+    const unsigned char *need_local = nullptr;              // per node
+};
+
+}  // namespace ibim
+
+struct ibim_plan {
+    ibim_params prm{};
+    int n = 0;
+    int p = 0, r = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool owns_stream = false;
+
+    // sorted geometry
+    double *sx = nullptr, *sy = nullptr, *sz = nullptr;
+    double *snx = nullptr, *sny = nullptr, *snz = nullptr, *sw = nullptr;
+    int *perm = nullptr;
+
+    // tree
+    int nnodes = 0, nlevels = 0, depth = 0;
+    double root_c[3] = {0, 0, 0}, root_r[3] = {0, 0, 0};
+    std::vector<int> lvl_off;        // host, nlevels+1
+    std::vector<double> lvl_rad;     // host, 3*nlevels
+    double *d_lvl_rad = nullptr;
+    double *ctr = nullptr;           // 3*nnodes
+    uint64_t *nkey = nullptr;        // prefix key per node
+    int *level = nullptr, *parent = nullptr, *first_child = nullptr, *slot = nullptr, *flags = nullptr;
+    int *sbeg = nullptr, *send = nullptr;
+    int *loff[4] = {nullptr, nullptr, nullptr, nullptr};   // U V W X offsets
+    int *litem[4] = {nullptr, nullptr, nullptr, nullptr};
+    int64_t ltot[4] = {0, 0, 0, 0};
+
+    // work lists for the source set as targets
+    int *leaves_ne = nullptr;   // non-empty leaves
+    int n_leaves_ne = 0;
+    int *m2l_work = nullptr;    // non-empty non-root nodes
+    int n_m2l_work = 0;
+    unsigned char *need_local = nullptr;
+
+    // FMM buffers
+    double *M = nullptr, *L = nullptr;   // nnodes * 4 * r
+    double *Q = nullptr;                 // 4 * n sorted charges
+    double *F = nullptr;                 // 4 * n sorted fields
+
+    // GMRES workspace (allocated on first use)
+    int gm_restart = 0;
+    double *gm_V = nullptr;              // (restart+1) * 2n
+    double *gm_w = nullptr, *gm_r = nullptr, *gm_x = nullptr, *gm_b = nullptr;
+    double *gm_part = nullptr;           // partial sums
+    double *gm_small = nullptr;          // H, s, cs, sn, scalars on device
+
+    ibim::ChebTab tab{};
+    ibim::PBConst kc{};
+
+    std::vector<void *> allocs;          // everything to free
+    int64_t footprint = 0;
+    ibim_tree_stats stats{};
+};
+
+namespace ibim {
+// allocation helper: records for release, counts footprint
+template <typename T>
+int dalloc(ibim_plan *pl, T **ptr, size_t count) {
+    *ptr = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)ptr, count * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes: " + hipGetErrorString(e));
+        return IBIM_ERR_NOMEM;
+    }
+    pl->allocs.push_back((void *)*ptr);
+    pl->footprint += (int64_t)(count * sizeof(T));
+    return IBIM_OK;
+}
+
+// stages implemented in the .hip files
+int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const double *d_w);
+int build_chebyshev(ibim_plan *pl);
+int fmm_forward(ibim_plan *pl, const TargetSet &ts, double *F, int nt);
+int form_charges(ibim_plan *pl, const double *d_x);
+int combine(ibim_plan *pl, const double *d_x, double *d_y);
+int make_target_set(ibim_plan *pl, int m, const double *d_txyz, TargetSet &ts, std::vector<void *> &tmp,
+                    int **perm_out);
+TargetSet source_targets(ibim_plan *pl);
+int rhs_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_aq, double *d_b);
+int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int maxit, double tol,
+                 ibim_solve_info *info);
+int compute_stats(ibim_plan *pl);
+}  // namespace ibim

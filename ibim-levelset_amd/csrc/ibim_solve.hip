@@ -1,0 +1,507 @@
+// ibim_solve.hip -- the steps either side of the matvec, on the device:
+// right-hand side (electric.cpp:395-413), target sets for the reaction
+// potential at atom centres (polarizeMap, electric.cpp:442-485) and restarted
+// GMRES with modified Gram-Schmidt and Givens rotations (gmres.cpp:10-164).
+//
+// GMRES keeps the Krylov basis, the Hessenberg matrix, the rotations and the
+// residual estimate in HBM; the host only reads the residual estimate once
+// per Arnoldi step to decide when to stop, as the reference does.
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+
+#include "ibim_internal.h"
+
+namespace ibim {
+
+int kernel_fmm_device(ibim_plan *pl, int kid, const double *d_q, double *d_out);
+int matvec_device(ibim_plan *pl, const double *d_x, double *d_y);
+
+namespace {
+
+inline int nblk(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
+
+// ------------------------------------------------------------------ RHS ---
+__global__ __launch_bounds__(256) void k_rhs(int n, const int *__restrict__ perm, const double *__restrict__ sx,
+                                             const double *__restrict__ sy, const double *__restrict__ sz,
+                                             const double *__restrict__ snx, const double *__restrict__ sny,
+                                             const double *__restrict__ snz, int m, const double *__restrict__ axyz,
+                                             const double *__restrict__ aq, double k1, double k2,
+                                             double *__restrict__ b) {
+    __shared__ double s_a[4][256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const double px = valid ? sx[i] : 0.0, py = valid ? sy[i] : 0.0, pz = valid ? sz[i] : 0.0;
+    const double nx = valid ? snx[i] : 0.0, ny = valid ? sny[i] : 0.0, nz = valid ? snz[i] : 0.0;
+    double b1 = 0.0, b2 = 0.0;
+    for (int a0 = 0; a0 < m; a0 += 256) {
+        const int cnt = min(256, m - a0);
+        __syncthreads();
+        if ((int)threadIdx.x < cnt) {
+            const int a = a0 + threadIdx.x;
+            s_a[0][threadIdx.x] = axyz[3 * (size_t)a];
+            s_a[1][threadIdx.x] = axyz[3 * (size_t)a + 1];
+            s_a[2][threadIdx.x] = axyz[3 * (size_t)a + 2];
+            s_a[3][threadIdx.x] = aq[a];
+        }
+        __syncthreads();
+        if (valid)
+            for (int a = 0; a < cnt; ++a) {
+                const double dx = px - s_a[0][a], dy = py - s_a[1][a], dz = pz - s_a[2][a];
+                const double d = dx * dx + dy * dy + dz * dz;
+                const double ir = 1.0 / sqrt(d);
+                const double q = s_a[3][a];
+                b1 += q * ir * k1;
+                b2 -= q * ir * ir * ir * (nx * dx + ny * dy + nz * dz) * k2;
+            }
+    }
+    if (valid) {
+        const int o = perm[i];
+        b[o] = b1;
+        b[(size_t)n + o] = b2;
+    }
+}
+
+// ------------------------------------------------------- target sets ---
+__global__ void k_target_keys(const double *__restrict__ xyz, int n, double cx, double cy, double cz, double rx,
+                              double ry, double rz, int maxlev, uint64_t *__restrict__ keys, int *__restrict__ idx) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double px = xyz[3 * (size_t)i], py = xyz[3 * (size_t)i + 1], pz = xyz[3 * (size_t)i + 2];
+    uint64_t key = 0;
+    for (int l = 0; l < maxlev; ++l) {  // same comparisons as the sources (bbfmm.cpp:121-131)
+        int xb = px < cx ? 0 : 1;
+        int yb = py < cy ? 0 : 1;
+        int zb = pz < cz ? 0 : 1;
+        key = (key << 3) | (uint64_t)(4 * zb + 2 * yb + xb);
+        cx = cx + ((double)xb - 0.5) * rx;
+        cy = cy + ((double)yb - 0.5) * ry;
+        cz = cz + ((double)zb - 0.5) * rz;
+        rx = rx * 0.5;
+        ry = ry * 0.5;
+        rz = rz * 0.5;
+    }
+    keys[i] = key;
+    idx[i] = i;
+}
+
+__global__ void k_gather3(const int *__restrict__ perm, int n, const double *__restrict__ xyz, double *x, double *y,
+                          double *z) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    size_t o = (size_t)perm[i];
+    x[i] = xyz[3 * o];
+    y[i] = xyz[3 * o + 1];
+    z[i] = xyz[3 * o + 2];
+}
+
+// target range of every node from its key prefix
+__global__ void k_target_ranges(int nnodes, const uint64_t *__restrict__ nkey, const int *__restrict__ level,
+                                const int *__restrict__ flags, int maxlev, const uint64_t *__restrict__ tkeys, int m,
+                                int *__restrict__ tbeg, int *__restrict__ tend, unsigned char *__restrict__ is_tleaf,
+                                unsigned char *__restrict__ need, unsigned char *__restrict__ m2l) {
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nnodes) return;
+    const int sh = 3 * (maxlev - level[id]);
+    const uint64_t lo = nkey[id] << sh, hi = (nkey[id] + 1) << sh;
+    int a = 0, b = m;
+    while (a < b) {
+        int mid = (a + b) >> 1;
+        if (tkeys[mid] < lo) a = mid + 1;
+        else b = mid;
+    }
+    int c = a, d = m;
+    while (c < d) {
+        int mid = (c + d) >> 1;
+        if (tkeys[mid] < hi) c = mid + 1;
+        else d = mid;
+    }
+    tbeg[id] = a;
+    tend[id] = c;
+    const bool has = c > a;
+    is_tleaf[id] = (has && (flags[id] & kLeaf)) ? 1 : 0;
+    need[id] = has ? 1 : 0;
+    m2l[id] = (has && id != 0) ? 1 : 0;
+}
+
+__global__ void k_scatter_neg_f0(int m, const int *__restrict__ tperm, const double *__restrict__ F,
+                                 double *__restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    out[tperm[i]] = -F[i];  // K2[qpsi] - sum K1a[qa] = -F0 (electric.cpp:482)
+}
+
+// -------------------------------------------------------------- GMRES ---
+constexpr int kRB = 256;   // threads per reduction block
+constexpr int kRG = 512;   // reduction blocks (fixed: deterministic sums)
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kRB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// sum of the kRG partials, same order in every block
+__device__ __forceinline__ double sum_partials(const double *__restrict__ part, double *sh) {
+    double v = 0.0;
+    for (int g = threadIdx.x; g < kRG; g += kRB) v += part[g];
+    return block_sum(v, sh);
+}
+
+__global__ __launch_bounds__(kRB) void k_dot_part(const double *__restrict__ a, const double *__restrict__ b, int n,
+                                                  double *__restrict__ part) {
+    __shared__ double sh[kRB];
+    double v = 0.0;
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) v += a[i] * b[i];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// h = <w, v_k> from partials; H(k,i) = h; w -= h v_k; next partial of
+// <w, v_{k+1}> (or <w, w> when last)
+__global__ __launch_bounds__(kRB) void k_mgs(double *__restrict__ w, const double *__restrict__ vk,
+                                             const double *__restrict__ vnext, int n,
+                                             const double *__restrict__ part_in, double *__restrict__ part_out,
+                                             double *__restrict__ hslot) {
+    __shared__ double sh[kRB];
+    const double h = sum_partials(part_in, sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *hslot = h;
+    double v = 0.0;
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) {
+        const double wi = w[i] - h * vk[i];
+        w[i] = wi;
+        v += wi * (vnext ? vnext[i] : wi);
+    }
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) part_out[blockIdx.x] = v;
+}
+
+// H(i+1,i) = |w|; v_{i+1} = w * (1/|w|)
+__global__ __launch_bounds__(kRB) void k_normalize(const double *__restrict__ w, double *__restrict__ vout, int n,
+                                                   const double *__restrict__ part_in, double *__restrict__ hslot) {
+    __shared__ double sh[kRB];
+    const double nr = sqrt(sum_partials(part_in, sh));
+    if (blockIdx.x == 0 && threadIdx.x == 0) *hslot = nr;
+    const double inv = 1.0 / nr;
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) vout[i] = w[i] * inv;
+}
+
+// v0 = r * (1/beta); s = 0; s(0) = beta
+__global__ __launch_bounds__(kRB) void k_start(const double *__restrict__ r, double *__restrict__ v0, int n,
+                                               const double *__restrict__ part_in, double *__restrict__ s, int m,
+                                               double *__restrict__ beta_out) {
+    __shared__ double sh[kRB];
+    const double beta = sqrt(sum_partials(part_in, sh));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int q = 0; q <= m; ++q) s[q] = 0.0;
+        s[0] = beta;
+        *beta_out = beta;
+    }
+    const double inv = 1.0 / beta;
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) v0[i] = r[i] * inv;
+}
+
+__device__ void gen_rot(double dx, double dy, double &cs, double &sn) {  // gmres.cpp:23-36
+    if (dy == 0.0) {
+        cs = 1.0;
+        sn = 0.0;
+    } else if (fabs(dy) > fabs(dx)) {
+        double temp = dx / dy;
+        sn = 1.0 / sqrt(1.0 + temp * temp);
+        cs = temp * sn;
+    } else {
+        double temp = dy / dx;
+        cs = 1.0 / sqrt(1.0 + temp * temp);
+        sn = temp * cs;
+    }
+}
+
+__device__ void apply_rot(double &dx, double &dy, double cs, double sn) {  // gmres.cpp:39-44
+    double temp = cs * dx + sn * dy;
+    dy = -sn * dx + cs * dy;
+    dx = temp;
+}
+
+// gmres.cpp:121-130 on column i of H (column-major, leading dimension ld)
+__global__ void k_givens(double *__restrict__ H, int ld, double *__restrict__ s, double *__restrict__ cs,
+                         double *__restrict__ sn, int i, double normb, double *__restrict__ resid) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double *h = H + (size_t)i * ld;
+    for (int k = 0; k < i; ++k) apply_rot(h[k], h[k + 1], cs[k], sn[k]);
+    gen_rot(h[i], h[i + 1], cs[i], sn[i]);
+    apply_rot(h[i], h[i + 1], cs[i], sn[i]);
+    apply_rot(s[i], s[i + 1], cs[i], sn[i]);
+    *resid = fabs(s[i + 1]) / normb;
+}
+
+// Update (gmres.cpp:10-21): back substitution, then x += sum_j y_j v_j
+__global__ __launch_bounds__(kRB) void k_update(double *__restrict__ x, const double *__restrict__ V, size_t ldv, int n,
+                                                const double *__restrict__ H, int ld, const double *__restrict__ s,
+                                                int k) {
+    __shared__ double y[128];
+    if (threadIdx.x == 0) {
+        for (int i = 0; i <= k; ++i) y[i] = s[i];
+        for (int i = k; i >= 0; i--) {
+            y[i] /= H[(size_t)i * ld + i];
+            for (int j = i - 1; j >= 0; j--) y[j] -= H[(size_t)i * ld + j] * y[i];
+        }
+    }
+    __syncthreads();
+    for (int e = blockIdx.x * kRB + threadIdx.x; e < n; e += gridDim.x * kRB) {
+        double acc = x[e];
+        for (int j = 0; j <= k; ++j) acc += y[j] * V[(size_t)j * ldv + e];
+        x[e] = acc;
+    }
+}
+
+// r = b - p and partial |r|^2
+__global__ __launch_bounds__(kRB) void k_residual(const double *__restrict__ b, const double *__restrict__ p,
+                                                  double *__restrict__ r, int n, double *__restrict__ part) {
+    __shared__ double sh[kRB];
+    double v = 0.0;
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) {
+        const double ri = b[i] - p[i];
+        r[i] = ri;
+        v += ri * ri;
+    }
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ void k_sqrt_sum(const double *__restrict__ part, double *__restrict__ out) {
+    __shared__ double sh[kRB];
+    double v = sum_partials(part, sh);
+    if (threadIdx.x == 0) *out = sqrt(v);
+}
+
+}  // namespace
+
+int rhs_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_aq, double *d_b) {
+    const double pi = 3.14159265358979323846;
+    const double dE = pl->prm.eps_out, dI = pl->prm.eps_in;
+    const double k1 = 1.0 / (dI * 4.0 * pi * (0.5 * (1 + dE / dI)));
+    const double k2 = 1.0 / (dI * 4.0 * pi * (0.5 * (1 + dI / dE)));
+    k_rhs<<<nblk(pl->n), 256, 0, pl->stream>>>(pl->n, pl->perm, pl->sx, pl->sy, pl->sz, pl->snx, pl->sny, pl->snz, m,
+                                               d_axyz, d_aq, k1, k2, d_b);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
+int make_target_set(ibim_plan *pl, int m, const double *d_txyz, TargetSet &ts, std::vector<void *> &tmp,
+                    int **perm_out) {
+    hipStream_t st = pl->stream;
+    const int nn = pl->nnodes, maxlev = pl->prm.max_level;
+    auto talloc = [&](auto **p, size_t cnt) -> int {
+        hipError_t e = hipMalloc((void **)p, (cnt ? cnt : 1) * sizeof(**p));
+        if (e != hipSuccess) {
+            set_error(std::string("hipMalloc (targets): ") + hipGetErrorString(e));
+            return IBIM_ERR_NOMEM;
+        }
+        tmp.push_back((void *)*p);
+        return IBIM_OK;
+    };
+    uint64_t *k_in, *k_out;
+    int *i_in, *perm, *tbeg, *tend, *leaves, *work, *d_num;
+    double *x, *y, *z;
+    unsigned char *fl_leaf, *need, *fl_m2l;
+    int rc;
+    if ((rc = talloc(&k_in, m)) || (rc = talloc(&k_out, m)) || (rc = talloc(&i_in, m)) || (rc = talloc(&perm, m)) ||
+        (rc = talloc(&x, m)) || (rc = talloc(&y, m)) || (rc = talloc(&z, m)) || (rc = talloc(&tbeg, nn)) ||
+        (rc = talloc(&tend, nn)) || (rc = talloc(&leaves, nn)) || (rc = talloc(&work, nn)) ||
+        (rc = talloc(&fl_leaf, nn)) || (rc = talloc(&need, nn)) || (rc = talloc(&fl_m2l, nn)) ||
+        (rc = talloc(&d_num, 2)))
+        return rc;
+    k_target_keys<<<nblk(m), 256, 0, st>>>(d_txyz, m, pl->root_c[0], pl->root_c[1], pl->root_c[2], pl->root_r[0],
+                                           pl->root_r[1], pl->root_r[2], maxlev, k_in, i_in);
+    IBIM_LAUNCH_CHECK();
+    size_t bytes = 0;
+    IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k_in, k_out, i_in, perm, m, 0,
+                                                      std::max(1, 3 * maxlev), st));
+    void *sort_tmp;
+    if ((rc = talloc((char **)&sort_tmp, bytes))) return rc;
+    IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp, bytes, k_in, k_out, i_in, perm, m, 0,
+                                                      std::max(1, 3 * maxlev), st));
+    k_gather3<<<nblk(m), 256, 0, st>>>(perm, m, d_txyz, x, y, z);
+    k_target_ranges<<<nblk(nn), 256, 0, st>>>(nn, pl->nkey, pl->level, pl->flags, maxlev, k_out, m, tbeg, tend,
+                                              fl_leaf, need, fl_m2l);
+    IBIM_LAUNCH_CHECK();
+    hipcub::CountingInputIterator<int> it(0);
+    size_t b1 = 0, b2 = 0;
+    IBIM_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, b1, it, fl_leaf, leaves, d_num, nn, st));
+    IBIM_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, b2, it, fl_m2l, work, d_num + 1, nn, st));
+    void *sel_tmp;
+    if ((rc = talloc((char **)&sel_tmp, std::max(b1, b2)))) return rc;
+    IBIM_HIP_CHECK(hipcub::DeviceSelect::Flagged(sel_tmp, b1, it, fl_leaf, leaves, d_num, nn, st));
+    IBIM_HIP_CHECK(hipcub::DeviceSelect::Flagged(sel_tmp, b2, it, fl_m2l, work, d_num + 1, nn, st));
+    int h_num[2];
+    IBIM_HIP_CHECK(hipMemcpyAsync(h_num, d_num, sizeof(h_num), hipMemcpyDeviceToHost, st));
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    ts.n = m;
+    ts.x = x;
+    ts.y = y;
+    ts.z = z;
+    ts.tbeg = tbeg;
+    ts.tend = tend;
+    ts.leaves = leaves;
+    ts.nleaves = h_num[0];
+    ts.m2l_work = work;
+    ts.n_m2l_work = h_num[1];
+    ts.need_local = need;
+    *perm_out = perm;
+    return IBIM_OK;
+}
+
+int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_x, double *d_out) {
+    std::vector<void *> tmp;
+    TargetSet ts;
+    int *tperm = nullptr;
+    double *F = nullptr;
+    int rc = make_target_set(pl, m, d_axyz, ts, tmp, &tperm);
+    if (rc == IBIM_OK) {
+        hipError_t e = hipMalloc((void **)&F, 4 * (size_t)m * sizeof(double));
+        if (e != hipSuccess) {
+            set_error("hipMalloc (polarize fields)");
+            rc = IBIM_ERR_NOMEM;
+        } else {
+            tmp.push_back(F);
+        }
+    }
+    if (rc == IBIM_OK) rc = form_charges(pl, d_x);
+    if (rc == IBIM_OK) rc = fmm_forward(pl, ts, F, m);
+    if (rc == IBIM_OK) {
+        k_scatter_neg_f0<<<nblk(m), 256, 0, pl->stream>>>(m, tperm, F, d_out);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
+        if (e != hipSuccess) {
+            set_error(std::string("polarize: ") + hipGetErrorString(e));
+            rc = IBIM_ERR_HIP;
+        }
+    } else {
+        (void)hipStreamSynchronize(pl->stream);
+    }
+    for (void *p : tmp) (void)hipFree(p);
+    return rc;
+}
+
+static int gmres_alloc(ibim_plan *pl, int restart) {
+    if (pl->gm_V && pl->gm_restart >= restart) return IBIM_OK;
+    const size_t n2 = 2 * (size_t)pl->n;
+    int rc;
+    // earlier buffers stay on the plan's allocation list and are freed with it
+    if ((rc = dalloc(pl, &pl->gm_V, (size_t)(restart + 1) * n2)) || (rc = dalloc(pl, &pl->gm_w, n2)) ||
+        (rc = dalloc(pl, &pl->gm_r, n2)) || (rc = dalloc(pl, &pl->gm_x, n2)) ||
+        (rc = dalloc(pl, &pl->gm_part, 4 * (size_t)kRG)) ||
+        (rc = dalloc(pl, &pl->gm_small, (size_t)(restart + 1) * (restart + 1) + 4 * (size_t)(restart + 1) + 8)))
+        return rc;
+    pl->gm_restart = restart;
+    return IBIM_OK;
+}
+
+// restarted GMRES, gmres.cpp:50-164, all vectors in HBM
+int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int maxit, double tol,
+                 ibim_solve_info *info) {
+    if (restart < 1 || restart > 120 || maxit < 0) {
+        set_error("gmres: restart must be in 1..120 and maxit >= 0");
+        return IBIM_ERR_ARG;
+    }
+    int rc = gmres_alloc(pl, restart);
+    if (rc) return rc;
+    hipStream_t st = pl->stream;
+    const int m = restart;
+    const int n2 = 2 * pl->n;
+    const size_t ldv = (size_t)n2;
+    const int ld = m + 1;
+    double *V = pl->gm_V, *w = pl->gm_w, *r = pl->gm_r, *p = pl->gm_x;
+    double *partA = pl->gm_part, *partB = pl->gm_part + kRG;
+    double *H = pl->gm_small;
+    double *s = H + (size_t)ld * ld, *cs = s + ld, *sn = cs + ld, *scal = sn + ld;  // scal: beta, resid, normb
+    IBIM_HIP_CHECK(hipMemsetAsync(H, 0, (size_t)ld * ld * sizeof(double), st));
+
+    int nmv = 0, inner = 0;
+    // normb
+    k_dot_part<<<kRG, kRB, 0, st>>>(d_b, d_b, n2, partA);
+    k_sqrt_sum<<<1, kRB, 0, st>>>(partA, scal + 2);
+    IBIM_LAUNCH_CHECK();
+    double normb = 0.0;
+    IBIM_HIP_CHECK(hipMemcpyAsync(&normb, scal + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+    // r = b - A x
+    if ((rc = matvec_device(pl, d_x, p))) return rc;
+    nmv++;
+    k_residual<<<kRG, kRB, 0, st>>>(d_b, p, r, n2, partA);
+    k_sqrt_sum<<<1, kRB, 0, st>>>(partA, scal);
+    IBIM_LAUNCH_CHECK();
+    double beta = 0.0;
+    IBIM_HIP_CHECK(hipMemcpyAsync(&beta, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    if (normb == 0.0) normb = 1;
+    double res = beta / normb;
+    int ret = IBIM_NOT_CONVERGED;
+    if (res <= tol) {
+        ret = IBIM_OK;
+    } else {
+        int i = 0, j = 1;
+        bool done = false;
+        while (j <= maxit && !done) {
+            // v0 = r / beta (partials of |r|^2 are in partA)
+            k_start<<<kRG, kRB, 0, st>>>(r, V, n2, partA, s, m, scal);
+            IBIM_LAUNCH_CHECK();
+            for (i = 0; i < m && j <= maxit; i++, j++) {
+                double *vi = V + (size_t)i * ldv;
+                if ((rc = matvec_device(pl, vi, w))) return rc;
+                nmv++;
+                inner++;
+                double *hcol = H + (size_t)i * ld;
+                k_dot_part<<<kRG, kRB, 0, st>>>(w, V, n2, partA);  // <w, v_0>
+                double *pin = partA, *pout = partB;
+                for (int k = 0; k <= i; ++k) {
+                    const double *vnext = (k < i) ? V + (size_t)(k + 1) * ldv : nullptr;
+                    k_mgs<<<kRG, kRB, 0, st>>>(w, V + (size_t)k * ldv, vnext, n2, pin, pout, hcol + k);
+                    std::swap(pin, pout);
+                }
+                k_normalize<<<kRG, kRB, 0, st>>>(w, V + (size_t)(i + 1) * ldv, n2, pin, hcol + i + 1);
+                k_givens<<<1, 64, 0, st>>>(H, ld, s, cs, sn, i, normb, scal + 1);
+                IBIM_LAUNCH_CHECK();
+                IBIM_HIP_CHECK(hipMemcpyAsync(&res, scal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+                IBIM_HIP_CHECK(hipStreamSynchronize(st));
+                if (res < tol) {
+                    k_update<<<kRG, kRB, 0, st>>>(d_x, V, ldv, n2, H, ld, s, i);
+                    IBIM_LAUNCH_CHECK();
+                    ret = IBIM_OK;
+                    done = true;
+                    break;
+                }
+            }
+            if (done) break;
+            k_update<<<kRG, kRB, 0, st>>>(d_x, V, ldv, n2, H, ld, s, i - 1);
+            IBIM_LAUNCH_CHECK();
+            if ((rc = matvec_device(pl, d_x, p))) return rc;
+            nmv++;
+            k_residual<<<kRG, kRB, 0, st>>>(d_b, p, r, n2, partA);
+            k_sqrt_sum<<<1, kRB, 0, st>>>(partA, scal);
+            IBIM_LAUNCH_CHECK();
+            IBIM_HIP_CHECK(hipMemcpyAsync(&beta, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+            IBIM_HIP_CHECK(hipStreamSynchronize(st));
+            if ((res = beta / normb) < tol) {
+                ret = IBIM_OK;
+                break;
+            }
+        }
+    }
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    if (info) {
+        info->iterations = inner;
+        info->matvecs = nmv;
+        info->rel_residual = res;
+        info->converged = ret == IBIM_OK ? 1 : 0;
+        info->outer_steps = 0;
+    }
+    return ret;
+}
+
+}  // namespace ibim

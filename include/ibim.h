@@ -1,0 +1,153 @@
+/*
+ * ibim.h -- C ABI of the MI355X-native IBIM Poisson-Boltzmann FMM matvec.
+ *
+ * One plan = one surface point set (the level-set tube nodes with normals
+ * and weights).  Creating a plan builds the reference octree and its U/V/W/X
+ * interaction lists on the GPU once; every matvec, FMM, RHS, energy and GMRES
+ * call then reuses that device-resident state.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Functions without a `_device` suffix
+ *     take host arrays and return when the results are in them.  `_device`
+ *     variants take device pointers (HBM) and are ordered on the plan's
+ *     stream; call ibim_plan_sync() before reading their results.
+ *   - Points are AoS: xyz[3*i+0..2]; normals the same; one weight per point.
+ *   - Return 0 (IBIM_OK) on success, a negative IBIM_ERR_* otherwise;
+ *     ibim_last_error() describes the last failure of the calling thread.
+ *   - Plans are independent: different plans may be used from different
+ *     threads.  One plan must not be used by two threads at once.
+ *
+ * Reference interfaces each entry point replaces (GaZ3ll3/ibim-levelset):
+ *   ibim_plan_create      bbfmm::kernel::initialize, bbfmm.h:220-242, with the
+ *                         tree build tree::populate, bbfmm.cpp:8-41, done once
+ *                         per geometry instead of once per kernel per matvec
+ *   ibim_kernel_fmm       bbfmm::kernel::run, bbfmm.h:244-270, for one of the
+ *                         16 kernel closures of electric.cpp:67-291
+ *   ibim_matvec           the dmapping lambda, electric.cpp:294-388, called by
+ *                         GMRES through FullMap, electric.cpp:390-392
+ *   ibim_rhs              the `load` loop, electric.cpp:395-413
+ *   ibim_polarize         polarizeMap, electric.cpp:442-485
+ *   ibim_gmres            GMRES(A, x, b, m, max_iter, tol), gmres.h:20-21,
+ *                         with A = the matvec of this plan
+ *   ibim_tree_*           tree::output, bbfmm.cpp:283-302 (per-node dump)
+ */
+#ifndef IBIM_H
+#define IBIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IBIM_OK 0
+#define IBIM_ERR_ARG (-1)     /* invalid argument */
+#define IBIM_ERR_HIP (-2)     /* HIP runtime failure (no device, launch error) */
+#define IBIM_ERR_NOMEM (-3)   /* device allocation failed */
+#define IBIM_ERR_STATE (-4)   /* call not valid for this plan */
+#define IBIM_NOT_CONVERGED 1  /* ibim_gmres: iteration limit reached */
+
+#define IBIM_PREC_FP64 0      /* everything in FP64 (the reference's arithmetic) */
+#define IBIM_PREC_MIXED 1     /* FP32 far/near field inside FP64 refinement */
+
+#define IBIM_KERNEL_PB 0      /* the 16 PB kernels of electric.cpp:67-291 */
+#define IBIM_KERNEL_POLY 1    /* polynomial surrogates (FMM-exact; for testing) */
+
+/* kernel ids of ibim_kernel_fmm, in the order of electric.cpp:299 */
+enum {
+    IBIM_K1X = 0, IBIM_K1Y, IBIM_K1Z, IBIM_K2,
+    IBIM_K3XX, IBIM_K3YY, IBIM_K3ZZ, IBIM_K3XY, IBIM_K3YZ, IBIM_K3ZX,
+    IBIM_K3YX, IBIM_K3ZY, IBIM_K3XZ,
+    IBIM_K4X, IBIM_K4Y, IBIM_K4Z
+};
+
+typedef struct ibim_plan ibim_plan;
+
+typedef struct ibim_params {
+    int p;                /* Chebyshev order, fmm_np (2..8) */
+    int max_points;       /* leaf capacity, fmm_maxpoint (>= 1) */
+    int max_level;        /* deepest tree level, fmm_maxlevel (0..20) */
+    double kappa;         /* solvent_kappa, 1/Angstrom (>= 0) */
+    double eps_in;        /* solvent_dI (> 0) */
+    double eps_out;       /* solvent_dE (> 0) */
+    double vacant_radius; /* tau * h, Angstrom: kernels regularised below it */
+    int precision;        /* IBIM_PREC_* */
+    int kernel_mode;      /* IBIM_KERNEL_* */
+} ibim_params;
+
+typedef struct ibim_tree_stats {
+    int64_t n_points;
+    int64_t n_nodes;
+    int64_t n_leaves;
+    int64_t n_empty_leaves;
+    int depth;            /* deepest leaf level */
+    int64_t n_u, n_v, n_w, n_x;      /* total list lengths (all nodes) */
+    int64_t p2p_pairs;    /* sum over target leaves of nT * sum_{U,W} nS */
+    int64_t m2l_pairs;    /* non-empty V + X interactions into nodes with targets */
+} ibim_tree_stats;
+
+typedef struct ibim_solve_info {
+    int iterations;       /* Arnoldi steps (inner iterations) */
+    int matvecs;          /* operator applications incl. residual checks */
+    double rel_residual;  /* final |b - Ax| / |b| estimate */
+    int converged;        /* 1 if rel_residual < tol */
+    int outer_steps;      /* FP64 refinement steps (mixed precision only) */
+} ibim_solve_info;
+
+/* ---- plan --------------------------------------------------------------- */
+int  ibim_plan_create(const ibim_params *prm, int64_t n, const double *xyz, const double *normals,
+                      const double *weights, ibim_plan **out);
+int  ibim_plan_create_device(const ibim_params *prm, int64_t n, const double *d_xyz,
+                             const double *d_normals, const double *d_weights, ibim_plan **out);
+void ibim_plan_destroy(ibim_plan *plan);
+int  ibim_plan_sync(ibim_plan *plan);
+int  ibim_plan_set_precision(ibim_plan *plan, int precision);
+/* device memory held by the plan, bytes */
+int64_t ibim_plan_footprint(const ibim_plan *plan);
+
+/* ---- the hot path -------------------------------------------------------- */
+/* y = A x, x = [phi (n); psi (n)], exactly the reference dmapping */
+int ibim_matvec(ibim_plan *plan, const double *x, double *y);
+int ibim_matvec_device(ibim_plan *plan, const double *d_x, double *d_y);
+
+/* one black-box FMM: potential[t] = sum_s K(src_s, tgt_t) charge[s] for one
+ * of the 16 kernels (kernel_id = IBIM_K*), far field by Chebyshev FMM */
+int ibim_kernel_fmm(ibim_plan *plan, int kernel_id, const double *charge, double *potential);
+
+/* ---- either side of the solve -------------------------------------------- */
+/* right-hand side (length 2n) from m atoms (AoS centres, charges) */
+int ibim_rhs(ibim_plan *plan, int64_t m, const double *atom_xyz, const double *atom_q, double *b);
+int ibim_rhs_device(ibim_plan *plan, int64_t m, const double *d_atom_xyz, const double *d_atom_q,
+                    double *d_b);
+
+/* reaction potential at m target points (atom centres) from a solution x */
+int ibim_polarize(ibim_plan *plan, int64_t m, const double *atom_xyz, const double *x, double *phi_reac);
+int ibim_polarize_device(ibim_plan *plan, int64_t m, const double *d_atom_xyz, const double *d_x,
+                         double *d_phi_reac);
+
+/* restarted GMRES(restart) on A x = b with this plan's matvec; x holds the
+ * initial guess on entry.  Returns IBIM_OK, IBIM_NOT_CONVERGED or an error. */
+int ibim_gmres(ibim_plan *plan, const double *b, double *x, int restart, int maxit, double tol,
+               ibim_solve_info *info);
+int ibim_gmres_device(ibim_plan *plan, const double *d_b, double *d_x, int restart, int maxit,
+                      double tol, ibim_solve_info *info);
+
+/* ---- inspection ----------------------------------------------------------- */
+int ibim_tree_stats_get(ibim_plan *plan, ibim_tree_stats *out);
+/* per-node dump (arrays of length n_nodes; centre/radius 3 per node) in the
+ * plan's node order (breadth first, children of a node consecutive) */
+int ibim_tree_export(ibim_plan *plan, int *level, int *parent, int *slot, int *leaf, int *empty,
+                     int *nsrc, double *centre, double *radius, int *nu, int *nv, int *nw, int *nx);
+/* interaction list (kind 0=U 1=V 2=W 3=X) of every node: offsets (n_nodes+1)
+ * and items (offsets[n_nodes]) */
+int ibim_tree_lists(ibim_plan *plan, int kind, int *offsets, int *items, int64_t cap);
+/* source points of node id, original indices ascending */
+int ibim_tree_points(ibim_plan *plan, int id, int *out, int64_t cap);
+
+const char *ibim_last_error(void);
+const char *ibim_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,161 @@
+/*
+ * ibim_oracle.h -- CPU restatement of the reference IBIM Poisson-Boltzmann
+ * FMM matvec path (GaZ3ll3/ibim-levelset), used ONLY as test infrastructure:
+ * the parity checker for the HIP library and the timed CPU baseline.
+ * Nothing in the product links or calls this code.
+ *
+ * What it restates (reference file:line):
+ *   octree + interaction lists ...... bbfmm.cpp:8-280, bbfmm.h:24-198
+ *   Chebyshev black-box FMM ......... bbfmm.h:200-623
+ *   16 PB kernel closures ........... electric.cpp:67-291
+ *   matvec dmapping ................. electric.cpp:294-388
+ *   RHS load ........................ electric.cpp:395-413
+ *   polarizeMap + energy ............ electric.cpp:432-505
+ *   restarted GMRES ................. gmres.cpp:10-164
+ *
+ * Pinning (how each function is checked without running the reference,
+ * which needs an external CBLAS library this image lacks):
+ *   Chebyshev nodes/polys/S_p/R[8] -- closed forms and polynomial
+ *     reproduction (tests/test_oracle_cheb.py);
+ *   tree -- partition/containment/tie-breaking invariants and list
+ *     symmetries (tests/test_oracle_tree.py);
+ *   FMM engine -- with the polynomial surrogate kernels (mode 1) the
+ *     Chebyshev FMM is exact, so it must equal the direct O(N^2) sum to
+ *     rounding; this fails for any dropped/double-counted interaction,
+ *     wrong transfer index or sign (tests/test_oracle_fmm.py);
+ *   kernels -- finite differences of the Laplace/Yukawa Green's functions,
+ *     kappa=0 and eps_in=eps_out special cases (tests/test_oracle_kernels.py);
+ *   matvec -- FMM vs dense O(N^2) operator at p=6..8, A = I when
+ *     eps_in = eps_out and kappa = 0;
+ *   GMRES -- small dense systems against LAPACK (numpy.linalg.solve);
+ *   whole solve -- the Born sphere energy.
+ */
+#ifndef IBIM_ORACLE_H
+#define IBIM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* kernel ids, in the order of electric.cpp:299 */
+enum {
+    ORC_K1X = 0, ORC_K1Y, ORC_K1Z, ORC_K2,
+    ORC_K3XX, ORC_K3YY, ORC_K3ZZ, ORC_K3XY, ORC_K3YZ, ORC_K3ZX,
+    ORC_K3YX, ORC_K3ZY, ORC_K3XZ,
+    ORC_K4X, ORC_K4Y, ORC_K4Z,
+    ORC_NKERNEL
+};
+
+typedef struct {
+    double kappa;          /* solvent_kappa */
+    double eps_in;         /* solvent_dI */
+    double eps_out;        /* solvent_dE */
+    double vacant_radius;  /* tau * h  (electric.cpp:58) */
+    int mode;              /* 0: PB kernels; 1: polynomial surrogate (test) */
+} orc_kparams;
+
+typedef struct {
+    int p;                 /* fmm_np */
+    int max_point;         /* fmm_maxpoint */
+    int max_level;         /* fmm_maxlevel */
+    orc_kparams k;
+} orc_fmm_params;
+
+typedef struct {
+    int parent;
+    int child[8];
+    int level;
+    int slot;              /* child slot in the parent (nodeIndex), root 0 */
+    double c[3], r[3];     /* centre and per-axis radius */
+    int nsrc, ntgt;
+    int *src, *tgt;        /* original indices, ascending */
+    int leaf, empty;
+    int nu, nv, nw, nx;
+    int *u, *v, *w, *x;    /* interaction lists (node ids) */
+    int cu, cv, cw, cx;    /* capacities */
+} orc_node;
+
+typedef struct {
+    int nnodes, cap;
+    orc_node *nodes;
+    int nsrc, ntgt;
+    double *src;           /* 3*nsrc AoS copy */
+    double *tgt;           /* 3*ntgt AoS copy */
+    int max_point, max_level;
+    int depth;             /* deepest leaf level */
+} orc_tree;
+
+/* tree (bbfmm.cpp:8-280) */
+int  orc_tree_build(orc_tree *t, const double *src, int nsrc, const double *tgt, int ntgt,
+                    int max_point, int max_level);
+void orc_tree_free(orc_tree *t);
+
+/* flat export for tests: arrays sized nnodes (centre/radius 3*nnodes) */
+int  orc_tree_nnodes(const orc_tree *t);
+void orc_tree_export(const orc_tree *t, int *level, int *parent, int *slot, int *leaf, int *empty,
+                     int *nsrc, int *ntgt, double *centre, double *radius,
+                     int *nu, int *nv, int *nw, int *nx);
+/* list of kind 0=U 1=V 2=W 3=X of node id into out (capacity cap); returns its length */
+int  orc_tree_list(const orc_tree *t, int id, int kind, int *out, int cap);
+/* sources/targets of node id (original indices); returns count */
+int  orc_tree_points(const orc_tree *t, int id, int which, int *out, int cap);
+
+/* Chebyshev building blocks (bbfmm.h:273-433) */
+void orc_cheb_nodes(int p, double *x);
+void orc_cheb_poly(int p, int n, const double *x, double *T /* n x p col-major */);
+void orc_sp(int p, int n, const double *x, double *S /* n x p col-major */);
+void orc_transfer_R(int p, double *R /* 8 blocks of r x r col-major */);
+
+/* one kernel closure: a = source, b = target (electric.cpp:67-291) */
+double orc_kernel_eval(int kid, const orc_kparams *kp, const double *a, const double *b);
+
+/* one black-box FMM (bbfmm::kernel::initialize + run, bbfmm.h:220-270) on a
+ * built tree; charge in source order, potential in target order. */
+int orc_fmm_run(const orc_tree *t, int p, int kid, const orc_kparams *kp,
+                const double *charge, double *potential);
+
+/* direct O(nsrc*ntgt) sum of the same kernel, for brute-force checks */
+void orc_direct(int kid, const orc_kparams *kp, const double *src, int nsrc,
+                const double *tgt, int ntgt, const double *charge, double *potential);
+
+/* the matvec y = A x, x = [phi; psi] (electric.cpp:294-388).
+ * xyz, nrm: 3n AoS; w: n.  share_tree=0 builds one tree per kernel like the
+ * reference; 1 builds one tree and reuses it (same result, faster). */
+int orc_dmapping(int n, const double *xyz, const double *nrm, const double *w,
+                 const orc_fmm_params *fp, const double *x, double *y, int share_tree);
+
+/* dense O(N^2) version of the same operator (every kernel summed directly) */
+void orc_dmapping_direct(int n, const double *xyz, const double *nrm, const double *w,
+                         const orc_kparams *kp, const double *x, double *y);
+
+/* RHS `load` (electric.cpp:395-413) */
+void orc_rhs(int n, const double *xyz, const double *nrm, int m, const double *atom_xyz,
+             const double *atom_q, double eps_in, double eps_out, double *b);
+
+/* polarizeMap (electric.cpp:442-485): reaction potential at the m atoms */
+int orc_polarize(int n, const double *xyz, const double *nrm, const double *w, int m,
+                 const double *atom_xyz, const orc_fmm_params *fp, const double *x,
+                 double *phi_reac);
+
+/* restarted GMRES(m) with MGS and Givens (gmres.cpp:50-164).
+ * Returns 0 on convergence, 1 otherwise; *iters = inner iterations used,
+ * *resid = final relative residual, *nmatvec = operator applications. */
+typedef void (*orc_matvec_fn)(void *ctx, const double *x, double *y);
+int orc_gmres(orc_matvec_fn A, void *ctx, int n, const double *b, double *x, int m, int maxit,
+              double tol, int *iters, double *resid, int *nmatvec);
+
+/* GMRES on a dense col-major n x n matrix (for tests) */
+int orc_gmres_dense(int n, const double *Amat, const double *b, double *x, int m, int maxit,
+                    double tol, int *iters, double *resid);
+
+/* GMRES with the FMM matvec as operator; share_tree as in orc_dmapping */
+int orc_solve(int n, const double *xyz, const double *nrm, const double *w,
+              const orc_fmm_params *fp, const double *b, double *x, int m, int maxit,
+              double tol, int share_tree, int *iters, double *resid, int *nmatvec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
