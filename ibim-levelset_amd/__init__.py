@@ -31,6 +31,7 @@ KID = {k: i for i, k in enumerate(KERNELS)}
 PREC_FP64, PREC_MIXED = 0, 1
 KERNEL_PB, KERNEL_POLY = 0, 1
 IBIM_OK, IBIM_NOT_CONVERGED = 0, 1
+STAGES = ["P2M", "M2M", "M2L", "L2L", "P2P", "MATVEC", "ORTH", "RHS", "POLARIZE"]
 
 
 class IbimError(RuntimeError):
@@ -59,6 +60,25 @@ class SolveInfo(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GeomParams(C.Structure):
+    _fields_ = [("h", C.c_double), ("lo", C.c_double * 3), ("dims", C.c_int * 3), ("pad", C.c_double),
+                ("probe", C.c_double), ("band", C.c_int), ("thickness", C.c_int), ("thres", C.c_double),
+                ("reinit_time", C.c_double), ("scale", C.c_double)]
+
+
+class SurfaceInfo(C.Structure):
+    _fields_ = [("n_nodes", C.c_int64), ("dims", C.c_int * 3), ("lo", C.c_double * 3), ("h", C.c_double),
+                ("inclusions", C.c_int64), ("evolve_steps", C.c_int), ("reinit_steps", C.c_int * 2),
+                ("quality", C.c_double * 2), ("area", C.c_double)]
+
+    def as_dict(self):
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = list(v) if hasattr(v, "__len__") else v
+        return out
 
 
 def build(jobs=8):
@@ -103,6 +123,16 @@ def lib():
     L.ibim_tree_export.argtypes = [vp, i, i, i, i, i, i, d, d, i, i, i, i]
     L.ibim_tree_lists.argtypes = [vp, C.c_int, i, i, i64]
     L.ibim_tree_points.argtypes = [vp, C.c_int, i, i64]
+    L.ibim_plan_profile.argtypes = [vp, C.c_int]
+    L.ibim_plan_stage_times.argtypes = [vp, d, C.POINTER(C.c_int64), C.c_int]
+    L.ibim_surface_from_atoms.argtypes = [C.POINTER(GeomParams), i64, d, d, pp]
+    L.ibim_surface_from_grid.argtypes = [C.POINTER(GeomParams), d, pp]
+    L.ibim_surface_get_info.argtypes = [vp, C.POINTER(SurfaceInfo)]
+    L.ibim_surface_copy.argtypes = [vp, d, d, d]
+    L.ibim_surface_device.argtypes = [vp, pp, pp, pp]
+    L.ibim_surface_destroy.argtypes = [vp]
+    L.ibim_surface_destroy.restype = None
+    L.ibim_synthetic_globule.argtypes = [i64, C.c_uint64, C.c_double, d, d, d]
     L.ibim_last_error.restype = C.c_char_p
     L.ibim_version.restype = C.c_char_p
     _lib = L
@@ -223,6 +253,31 @@ class Plan:
     def sync(self):
         _check(lib().ibim_plan_sync(self._h), "ibim_plan_sync")
 
+    # ---------------------------------------------- device-pointer variants --
+    def rhs_device(self, m, d_axyz, d_aq, d_b):
+        _check(lib().ibim_rhs_device(self._h, m, d_axyz, d_aq, d_b), "ibim_rhs_device")
+
+    def gmres_device(self, d_b, d_x, restart=20, maxit=100, tol=1e-8):
+        info = SolveInfo()
+        rc = _check(lib().ibim_gmres_device(self._h, d_b, d_x, restart, maxit, tol, C.byref(info)),
+                    "ibim_gmres_device")
+        return rc, info.as_dict()
+
+    def polarize_device(self, m, d_axyz, d_x, d_phi):
+        _check(lib().ibim_polarize_device(self._h, m, d_axyz, d_x, d_phi), "ibim_polarize_device")
+
+    # ------------------------------------------------------------ timing --
+    def profile(self, enable=True):
+        _check(lib().ibim_plan_profile(self._h, int(enable)), "ibim_plan_profile")
+
+    def stage_times(self, reset=True):
+        """{stage: (milliseconds, launches)} from HIP events on the plan's stream."""
+        ms = np.zeros(16)
+        cnt = np.zeros(16, np.int64)
+        _check(lib().ibim_plan_stage_times(self._h, _d(ms), cnt.ctypes.data_as(C.POINTER(C.c_int64)), int(reset)),
+               "ibim_plan_stage_times")
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(STAGES)}
+
     def set_precision(self, precision):
         _check(lib().ibim_plan_set_precision(self._h, precision), "ibim_plan_set_precision")
 
@@ -263,3 +318,77 @@ class Plan:
         out = np.zeros(max(cnt, 1), np.int32)
         _check(lib().ibim_tree_points(self._h, node, _i(out), len(out)), "ibim_tree_points")
         return out[:cnt]
+
+
+# ------------------------------------------------------------ geometry --
+def geom_params(h, lo=(0.0, 0.0, 0.0), dims=(0, 0, 0), pad=None, probe=1.4, band=8, thickness=2, thres=1e-4,
+                reinit_time=None, scale=1.0):
+    """Grid and level-set options.  reinit_time defaults to 26 h, the final time
+    the reference's input.cfg (reinit_step = 60) gives at its usual grid of
+    256 cells over 600 scaled units (main.cpp:38-40)."""
+    if pad is None:
+        pad = (band + 4) * h
+    if reinit_time is None:
+        reinit_time = 26.0 * h
+    return GeomParams(h, (C.c_double * 3)(*lo), (C.c_int * 3)(*dims), pad, probe, band, thickness, thres,
+                      reinit_time, scale)
+
+
+class Surface:
+    """Tube quadrature nodes of a level set, resident on the GPU."""
+
+    def __init__(self, handle):
+        self._h = handle
+        info = SurfaceInfo()
+        _check(lib().ibim_surface_get_info(self._h, C.byref(info)), "ibim_surface_get_info")
+        self.info = info.as_dict()
+        self.n = int(info.n_nodes)
+
+    @classmethod
+    def from_atoms(cls, atom_xyz, radii, **kw):
+        gp = geom_params(**kw)
+        axyz = _f64(atom_xyz, (-1, 3))
+        h = C.c_void_p()
+        _check(lib().ibim_surface_from_atoms(C.byref(gp), len(axyz), _d(axyz), _d(_f64(radii)), C.byref(h)),
+               "ibim_surface_from_atoms")
+        return cls(h)
+
+    @classmethod
+    def from_grid(cls, phi, h, lo, thickness=2, scale=1.0):
+        phi = _f64(phi)
+        gp = geom_params(h, lo=lo, dims=phi.shape, thickness=thickness, scale=scale)
+        hd = C.c_void_p()
+        _check(lib().ibim_surface_from_grid(C.byref(gp), _d(phi.ravel()), C.byref(hd)), "ibim_surface_from_grid")
+        return cls(hd)
+
+    def arrays(self):
+        xyz = np.zeros((self.n, 3))
+        nrm = np.zeros((self.n, 3))
+        w = np.zeros(self.n)
+        _check(lib().ibim_surface_copy(self._h, _d(xyz), _d(nrm), _d(w)), "ibim_surface_copy")
+        return xyz, nrm, w
+
+    def device_ptrs(self):
+        a, b, c = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _check(lib().ibim_surface_device(self._h, C.byref(a), C.byref(b), C.byref(c)), "ibim_surface_device")
+        return self.n, a, b, c
+
+    def close(self):
+        if self._h:
+            lib().ibim_surface_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synthetic_globule(m, seed, confine_radius):
+    """Seeded synthetic molecule: (xyz (m,3), radii (m,), charges (m,))."""
+    xyz = np.zeros((m, 3))
+    r = np.zeros(m)
+    q = np.zeros(m)
+    _check(lib().ibim_synthetic_globule(m, seed, confine_radius, _d(xyz), _d(r), _d(q)), "ibim_synthetic_globule")
+    return xyz, r, q

@@ -13,6 +13,7 @@ int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_
 
 // one dmapping (electric.cpp:294-388): charges, fused FMM, combine
 int matvec_device(ibim_plan *pl, const double *d_x, double *d_y) {
+    StageScope sc(pl, kStMatvec);
     int rc = form_charges(pl, d_x);
     if (rc) return rc;
     TargetSet ts = source_targets(pl);
@@ -105,6 +106,11 @@ const char *ibim_version(void) { return "ibim-levelset-amd 0.1 (gfx950)"; }
 void ibim_plan_destroy(ibim_plan *pl) {
     if (!pl) return;
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+    for (auto &r : pl->timer.pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : pl->timer.pool) (void)hipEventDestroy(e);
     for (void *p : pl->allocs) (void)hipFree(p);
     if (pl->owns_stream && pl->stream) (void)hipStreamDestroy(pl->stream);
     delete pl;
@@ -212,6 +218,43 @@ int ibim_plan_set_precision(ibim_plan *pl, int precision) {
 }
 
 int64_t ibim_plan_footprint(const ibim_plan *pl) { return pl ? pl->footprint : 0; }
+
+int ibim_plan_profile(ibim_plan *pl, int enable) {
+    if (!pl) {
+        set_error("plan is NULL");
+        return IBIM_ERR_ARG;
+    }
+    pl->timer.on = enable != 0;
+    return IBIM_OK;
+}
+
+int ibim_plan_stage_times(ibim_plan *pl, double *ms, int64_t *launches, int reset) {
+    if (!pl) {
+        set_error("plan is NULL");
+        return IBIM_ERR_ARG;
+    }
+    StageTimer &t = pl->timer;
+    for (auto &r : t.pending) {
+        IBIM_HIP_CHECK(hipEventSynchronize(r.b));
+        float el = 0.f;
+        IBIM_HIP_CHECK(hipEventElapsedTime(&el, r.a, r.b));
+        t.ms[r.stage] += el;
+        t.count[r.stage] += 1;
+        t.pool.push_back(r.a);
+        t.pool.push_back(r.b);
+    }
+    t.pending.clear();
+    for (int i = 0; i < IBIM_STAGE_COUNT; ++i) {
+        if (ms) ms[i] = t.ms[i];
+        if (launches) launches[i] = t.count[i];
+    }
+    if (reset)
+        for (int i = 0; i < 16; ++i) {
+            t.ms[i] = 0;
+            t.count[i] = 0;
+        }
+    return IBIM_OK;
+}
 
 int ibim_matvec_device(ibim_plan *pl, const double *d_x, double *d_y) {
     if (!pl || !d_x || !d_y) {

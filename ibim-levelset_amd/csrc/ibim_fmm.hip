@@ -438,33 +438,48 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt) {
     constexpr int BS = R <= 64 ? 64 : (R <= 128 ? 128 : 256);
     hipStream_t st = pl->stream;
     // upward pass
+    {
+    StageScope sc(pl, kStP2M);
     if (pl->n_leaves_ne > 0)
         k_p2m<P><<<pl->n_leaves_ne, 256, 0, st>>>(pl->leaves_ne, pl->sx, pl->sy, pl->sz, pl->Q, pl->n, pl->sbeg,
                                                    pl->send, pl->ctr, pl->level, pl->d_lvl_rad, pl->tab, pl->M);
     IBIM_LAUNCH_CHECK();
+    }
+    {
+    StageScope sc(pl, kStM2M);
     for (int l = pl->nlevels - 2; l >= 0; --l) {
         const int cnt = pl->lvl_off[l + 1] - pl->lvl_off[l];
         k_m2m<P><<<cnt, 256, 0, st>>>(pl->lvl_off[l], pl->flags, pl->first_child, pl->tab, pl->M);
     }
     IBIM_LAUNCH_CHECK();
+    }
     // downward pass
     IBIM_HIP_CHECK(hipMemsetAsync(pl->L, 0, (size_t)pl->nnodes * 4 * R * sizeof(double), st));
+    {
+    StageScope sc(pl, kStM2L);
     if (ts.n_m2l_work > 0)
         k_m2l_direct<P, BS><<<ts.n_m2l_work, BS, 0, st>>>(ts.m2l_work, pl->loff[1], pl->litem[1], pl->loff[3],
                                                           pl->litem[3], pl->flags, pl->ctr, pl->level,
                                                           pl->d_lvl_rad, pl->tab, pl->kc, pl->M, pl->L);
     IBIM_LAUNCH_CHECK();
+    }
+    {
+    StageScope sc(pl, kStL2L);
     for (int l = 1; l < pl->nlevels; ++l) {
         const int cnt = pl->lvl_off[l + 1] - pl->lvl_off[l];
         k_l2l<P><<<cnt, 256, 0, st>>>(pl->lvl_off[l], ts.need_local, pl->parent, pl->slot, pl->tab, pl->L);
     }
     IBIM_LAUNCH_CHECK();
+    }
+    {
+    StageScope sc(pl, kStP2P);
     if (ts.nleaves > 0)
         k_p2p_l2p<P><<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->sx, pl->sy,
                                                 pl->sz, pl->Q, pl->n, pl->loff[0], pl->litem[0], pl->loff[2],
                                                 pl->litem[2], pl->flags, pl->sbeg, pl->send, pl->ctr, pl->level,
                                                 pl->d_lvl_rad, pl->tab, pl->kc, pl->L, F, nt);
     IBIM_LAUNCH_CHECK();
+    }
     return IBIM_OK;
 }
 

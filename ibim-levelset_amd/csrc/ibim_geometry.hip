@@ -1,0 +1,615 @@
+// ibim_geometry.hip -- the geometry pre-step on the GPU: the solvent-excluded
+// surface as a level set on a uniform grid, and the IBIM tube quadrature
+// nodes around it.
+//
+// Reference (levelset.cpp, main.cpp:47-62):
+//   expand        max over atoms of (r + probe) - |x - c| in a window of
+//                 (r + probe)/h + band cells (levelset.cpp:166-184)
+//   evolve        TVD-RK3 of u_t = -v |grad u| with WENO5 one-sided
+//                 derivatives and a Godunov norm, moving the surface in by
+//                 the probe radius (levelset.cpp:186-262, 365-463)
+//   reinitialize  TVD-RK3 of u_t = -sign(phi0)(|grad u| - 1) until every tube
+//                 point has ||grad u| - 1| <= thres (levelset.cpp:266-362, 473-507)
+//   Surface       exterior flood fill from grid point 0, cavities pushed out
+//                 of the tube, then for each tube point the projection
+//                 P - grad(phi) phi with the averaged one-sided gradient (not
+//                 renormalised) and the cosine weight (levelset.cpp:512-661)
+// Each grid-wide stage is one kernel; the grid is HBM resident throughout.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+#include "ibim_internal.h"
+
+struct ibim_surface {
+    int64_t n = 0;
+    double *xyz = nullptr, *nrm = nullptr, *w = nullptr;
+    ibim_surface_info info{};
+};
+
+namespace ibim {
+namespace {
+
+struct GridDims {
+    int nx, ny, nz;
+    double sx, sy, sz, h;
+};
+
+inline int nblk(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
+
+__device__ __forceinline__ double gget(const double *__restrict__ g, const GridDims &d, int i, int j, int k) {
+    i = min(max(i, 0), d.nx - 1);  // clamped read (levelset.cpp:103-108)
+    j = min(max(j, 0), d.ny - 1);
+    k = min(max(k, 0), d.nz - 1);
+    return g[((size_t)i * d.ny + j) * d.nz + k];
+}
+
+// WENO5 forward (up) and backward (un) derivatives along dir (levelset.cpp:405-463)
+__device__ __forceinline__ void weno(const double *__restrict__ g, const GridDims &d, int i, int j, int k, int dir,
+                                     double &up, double &un) {
+    double w[7];
+#pragma unroll
+    for (int l = 0; l < 7; ++l) {
+        const int o = l - 3;
+        w[l] = gget(g, d, i + (dir == 0 ? o : 0), j + (dir == 1 ? o : 0), k + (dir == 2 ? o : 0));
+    }
+    const double EPS_LS = 1e-6;
+    double a = w[1] - w[0], b = w[2] - w[1], c = w[3] - w[2], dd = w[4] - w[3], e = w[5] - w[4], f = w[6] - w[5];
+    const double df = (-b + 7.0 * (c + dd) - e) / 12.0;
+    a = b - a;
+    b = c - b;
+    c = dd - c;
+    dd = e - dd;
+    e = f - e;
+    double is0 = 13.0 * (e - dd) * (e - dd) + 3.0 * (e - 3.0 * dd) * (e - 3.0 * dd);
+    double is1 = 13.0 * (dd - c) * (dd - c) + 3.0 * (dd + c) * (dd + c);
+    double is2 = 13.0 * (c - b) * (c - b) + 3.0 * (3.0 * c - b) * (3.0 * c - b);
+    double a0 = 1.0 / ((EPS_LS + is0) * (EPS_LS + is0));
+    double a1 = 6.0 / ((EPS_LS + is1) * (EPS_LS + is1));
+    double a2 = 3.0 / ((EPS_LS + is2) * (EPS_LS + is2));
+    double w1 = 1.0 / (a0 + a1 + a2);
+    double w0 = a0 * w1, w2 = a2 * w1;
+    up = (df + w0 * (e - 2 * dd + c) / 3.0 + (w2 - 0.5) * (dd - 2.0 * c + b) / 6.0) / d.h;
+    is0 = 13.0 * (a - b) * (a - b) + 3.0 * (a - 3.0 * b) * (a - 3.0 * b);
+    is1 = 13.0 * (b - c) * (b - c) + 3.0 * (b + c) * (b + c);
+    is2 = 13.0 * (c - dd) * (c - dd) + 3.0 * (3.0 * c - dd) * (3.0 * c - dd);
+    a0 = 1.0 / ((EPS_LS + is0) * (EPS_LS + is0));
+    a1 = 6.0 / ((EPS_LS + is1) * (EPS_LS + is1));
+    a2 = 3.0 / ((EPS_LS + is2) * (EPS_LS + is2));
+    w1 = 1.0 / (a0 + a1 + a2);
+    w0 = a0 * w1;
+    w2 = a2 * w1;
+    un = (df - w0 * (a - 2 * b + c) / 3.0 - (w2 - 0.5) * (b - 2 * c + dd) / 6.0) / d.h;
+}
+
+// Godunov norm getNorm(A, B) = sqrt(sum max(A+, B-)^2) (levelset.cpp:369-376)
+__device__ __forceinline__ double godunov(const double *A, const double *B) {
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double t = fmax(A[q] > 0. ? A[q] : 0., B[q] < 0. ? -B[q] : 0.);
+        v += t * t;
+    }
+    return sqrt(v);
+}
+
+__device__ __forceinline__ void index3(int64_t I, const GridDims &d, int &i, int &j, int &k) {
+    k = (int)(I % d.nz);
+    const int64_t t = I / d.nz;
+    j = (int)(t % d.ny);
+    i = (int)(t / d.ny);
+}
+
+__global__ void k_fill(double *__restrict__ g, int64_t n, double v) {
+    int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I < n) g[I] = v;
+}
+
+// order-preserving map of doubles onto unsigned integers for atomicMax
+__device__ __forceinline__ unsigned long long ord(double v) {
+    unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double unord(unsigned long long u) {
+    u = (u & 0x8000000000000000ull) ? (u & 0x7fffffffffffffffull) : ~u;
+    return __longlong_as_double((long long)u);
+}
+
+__global__ void k_to_ord(double *__restrict__ g, int64_t n) {
+    int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I < n) {
+        unsigned long long u = ord(g[I]);
+        g[I] = __longlong_as_double((long long)u);
+    }
+}
+__global__ void k_from_ord(double *__restrict__ g, int64_t n) {
+    int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I < n) g[I] = unord((unsigned long long)__double_as_longlong(g[I]));
+}
+
+// expand: one block row per atom, window cells over blockIdx.x (levelset.cpp:166-184)
+__global__ void k_expand(unsigned long long *__restrict__ g, GridDims d, const double *__restrict__ axyz,
+                         const double *__restrict__ rad, int m, double probe, int band, int wmax) {
+    const int a = blockIdx.y + gridDim.y * blockIdx.z;
+    if (a >= m) return;
+    const double cx = axyz[3 * (size_t)a], cy = axyz[3 * (size_t)a + 1], cz = axyz[3 * (size_t)a + 2];
+    const double rp = rad[a] + probe;
+    const int r = (int)(rp / d.h) + band;
+    const int icx = (int)((cx - d.sx) / d.h), icy = (int)((cy - d.sy) / d.h), icz = (int)((cz - d.sz) / d.h);
+    const int w = 2 * r + 1;
+    const int64_t cells = (int64_t)w * w * w;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cells; t += (int64_t)gridDim.x * blockDim.x) {
+        const int oc = (int)(t % w), ob = (int)((t / w) % w), oa = (int)(t / ((int64_t)w * w));
+        const int ia = icx - r + oa, ib = icy - r + ob, ic = icz - r + oc;
+        // the reference writes without a bounds check; cells outside the grid are skipped here
+        if (ia < 0 || ib < 0 || ic < 0 || ia >= d.nx || ib >= d.ny || ic >= d.nz) continue;
+        const double px = d.sx + ia * d.h - cx, py = d.sy + ib * d.h - cy, pz = d.sz + ic * d.h - cz;
+        const double dist = rp - sqrt(px * px + py * py + pz * pz);
+        atomicMax(&g[((size_t)ia * d.ny + ib) * d.nz + ic], ord(dist));
+    }
+    (void)wmax;
+}
+
+// one TVD-RK3 stage of evolve (mode 0) or reinitialize (mode 1)
+//   stage 1: out = base - T(base)
+//   stage 2: out = (3 base + src - T(src)) / 4
+//   stage 3: out = (base + 2 (src - T(src))) / 3      (out may alias base)
+// T(u) = dtv |grad u|_G (evolve) or dtv sign(phi0) (|grad u|_G - 1) (reinit)
+__global__ __launch_bounds__(256) void k_ls_stage(const double *base, const double *__restrict__ src,
+                                                  const double *__restrict__ phi0, double *out, GridDims d, int mode,
+                                                  int stage, double dtv, double eps) {
+    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= n) return;
+    int i, j, k;
+    index3(I, d, i, j, k);
+    double up[3], un[3];
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) weno(src, d, i, j, k, dir, up[dir], un[dir]);
+    double T;
+    if (mode == 0) {
+        T = dtv * godunov(un, up);
+    } else {
+        const double p0 = phi0[I];
+        const double sgn = p0 / sqrt(p0 * p0 + eps);
+        const double nd = sgn > 0. ? godunov(un, up) : godunov(up, un);
+        T = dtv * sgn * (nd - 1.0);
+    }
+    const double b = base[I];
+    double v;
+    if (stage == 1) v = b - T;
+    else if (stage == 2) v = (3 * b + src[I] - T) / 4.0;
+    else v = (b + 2 * (src[I] - T)) / 3.0;
+    out[I] = v;
+}
+
+__global__ void k_scale(double *__restrict__ g, int64_t n, double s) {
+    int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I < n) g[I] *= s;
+}
+
+// countGradient (levelset.cpp:473-507): tube points, bad points, sum of errors
+__global__ __launch_bounds__(256) void k_count(const double *__restrict__ g, GridDims d, double tube, double thres,
+                                               unsigned long long *__restrict__ counts, double *__restrict__ err) {
+    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= n || !(fabs(g[I]) < tube)) return;
+    int i, j, k;
+    index3(I, d, i, j, k);
+    double up[3], un[3];
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) weno(g, d, i, j, k, dir, up[dir], un[dir]);
+    const double nr = godunov(un, up);
+    atomicAdd(&counts[0], 1ull);
+    if (fabs(nr - 1.0) > thres) atomicAdd(&counts[1], 1ull);
+    atomicAdd(err, fabs(nr - 1.0));
+}
+
+// exterior flood fill (levelset.cpp:519-599): a point is reached when a
+// reached neighbour has phi > -tube
+__global__ void k_flood(const double *__restrict__ g, GridDims d, unsigned char *vis, double neg_tube,
+                        int *__restrict__ changed) {
+    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= n || vis[I]) return;
+    int i, j, k;
+    index3(I, d, i, j, k);
+    const int64_t sI = (int64_t)d.ny * d.nz, sJ = d.nz;
+    bool reach = false;
+    if (i > 0) reach |= vis[I - sI] && g[I - sI] > neg_tube;
+    if (i < d.nx - 1) reach |= vis[I + sI] && g[I + sI] > neg_tube;
+    if (j > 0) reach |= vis[I - sJ] && g[I - sJ] > neg_tube;
+    if (j < d.ny - 1) reach |= vis[I + sJ] && g[I + sJ] > neg_tube;
+    if (k > 0) reach |= vis[I - 1] && g[I - 1] > neg_tube;
+    if (k < d.nz - 1) reach |= vis[I + 1] && g[I + 1] > neg_tube;
+    if (reach) {
+        vis[I] = 1;
+        *changed = 1;
+    }
+}
+
+__global__ void k_inclusion(double *__restrict__ g, int64_t n, const unsigned char *__restrict__ vis, double tube,
+                            unsigned long long *__restrict__ cnt) {
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= n) return;
+    if (!vis[I] && fabs(g[I]) < tube) {
+        g[I] = -(tube + 1e-15);
+        atomicAdd(cnt, 1ull);
+    }
+}
+
+__global__ void k_tube_flag(const double *__restrict__ g, int64_t n, double tube, int *__restrict__ flag) {
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I < n) flag[I] = fabs(g[I]) < tube ? 1 : 0;
+}
+
+// tube nodes, normals and weights in grid order (levelset.cpp:615-651),
+// converted as electric.cpp:17-41 (nodes / s, weights * s * (h/s)^3)
+__global__ __launch_bounds__(256) void k_tube_emit(const double *__restrict__ g, GridDims d,
+                                                   const int *__restrict__ flag, const int *__restrict__ pos,
+                                                   double tube, double inv_s, double wscale, double *__restrict__ xyz,
+                                                   double *__restrict__ nrm, double *__restrict__ w) {
+    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= n || !flag[I]) return;
+    int i, j, k;
+    index3(I, d, i, j, k);
+    double up[3], un[3], gr[3];
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+        weno(g, d, i, j, k, dir, up[dir], un[dir]);
+        gr[dir] = (un[dir] + up[dir]) * 0.5;
+    }
+    const double dist = gget(g, d, i, j, k);
+    const double P[3] = {d.sx + i * d.h, d.sy + j * d.h, d.sz + k * d.h};
+    const int o = pos[I];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        xyz[3 * (size_t)o + q] = (P[q] - gr[q] * dist) * inv_s;
+        nrm[3 * (size_t)o + q] = gr[q];
+    }
+    w[o] = 0.5 * (1.0 + cos(M_PI * dist / tube)) / tube * wscale;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------
+struct LsRun {
+    GridDims d{};
+    int64_t n = 0;
+    double *g = nullptr, *phi0 = nullptr, *u1 = nullptr, *u2 = nullptr;
+    hipStream_t st = nullptr;
+};
+
+static int ls_rk3(LsRun &L, int mode, double final_t, double vel, double cfl, double tube, double thres,
+                  int *steps_out, double *quality) {
+    double dt = cfl * L.d.h / vel;
+    const int num_steps = (int)(final_t / dt) + 1;
+    dt = final_t / (double)num_steps;
+    const double dtv = dt * vel;
+    const double eps = L.d.h * L.d.h;
+    const int nb = nblk(L.n);
+    unsigned long long *cnt = nullptr;
+    double *err = nullptr;
+    IBIM_HIP_CHECK(hipMalloc(&cnt, 2 * sizeof(unsigned long long)));
+    IBIM_HIP_CHECK(hipMalloc(&err, sizeof(double)));
+    auto count = [&](unsigned long long *h) -> int {
+        IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), L.st));
+        IBIM_HIP_CHECK(hipMemsetAsync(err, 0, sizeof(double), L.st));
+        k_count<<<nb, 256, 0, L.st>>>(L.g, L.d, tube, thres, cnt, err);
+        IBIM_LAUNCH_CHECK();
+        double e = 0;
+        IBIM_HIP_CHECK(hipMemcpyAsync(h, cnt, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, L.st));
+        IBIM_HIP_CHECK(hipMemcpyAsync(&e, err, sizeof(double), hipMemcpyDeviceToHost, L.st));
+        IBIM_HIP_CHECK(hipStreamSynchronize(L.st));
+        if (quality) {
+            quality[0] = h[0] ? e / (double)h[0] : 0.0;
+            quality[1] = h[0] ? (double)h[1] / (double)h[0] : 0.0;
+        }
+        return IBIM_OK;
+    };
+    int step = 0, rc = IBIM_OK;
+    unsigned long long hc[2] = {0, 1};
+    if (mode == 1 && (rc = count(hc))) return rc;
+    while (step < num_steps && (mode == 0 || hc[1] != 0)) {
+        ++step;
+        k_ls_stage<<<nb, 256, 0, L.st>>>(L.g, L.g, L.phi0, L.u1, L.d, mode, 1, dtv, eps);
+        k_ls_stage<<<nb, 256, 0, L.st>>>(L.g, L.u1, L.phi0, L.u2, L.d, mode, 2, dtv, eps);
+        k_ls_stage<<<nb, 256, 0, L.st>>>(L.g, L.u2, L.phi0, L.g, L.d, mode, 3, dtv, eps);
+        IBIM_LAUNCH_CHECK();
+        if (mode == 1 && (rc = count(hc))) return rc;
+    }
+    if (mode == 0) IBIM_HIP_CHECK(hipStreamSynchronize(L.st));
+    (void)hipFree(cnt);
+    (void)hipFree(err);
+    *steps_out = step;
+    return IBIM_OK;
+}
+
+// Surface: flood fill, inclusions, tube extraction into a new ibim_surface
+static int ls_surface(LsRun &L, const ibim_geom_params &gp, ibim_surface *S) {
+    const double tube = gp.thickness * L.d.h;
+    const int nb = nblk(L.n);
+    unsigned char *vis = nullptr;
+    int *changed = nullptr;
+    IBIM_HIP_CHECK(hipMalloc(&vis, (size_t)L.n));
+    IBIM_HIP_CHECK(hipMalloc(&changed, sizeof(int)));
+    IBIM_HIP_CHECK(hipMemsetAsync(vis, 0, (size_t)L.n, L.st));
+    const unsigned char one = 1;
+    IBIM_HIP_CHECK(hipMemcpyAsync(vis, &one, 1, hipMemcpyHostToDevice, L.st));
+    for (int it = 0;; ++it) {
+        IBIM_HIP_CHECK(hipMemsetAsync(changed, 0, sizeof(int), L.st));
+        for (int r = 0; r < 8; ++r) k_flood<<<nb, 256, 0, L.st>>>(L.g, L.d, vis, -tube, changed);
+        IBIM_LAUNCH_CHECK();
+        int hch = 0;
+        IBIM_HIP_CHECK(hipMemcpyAsync(&hch, changed, sizeof(int), hipMemcpyDeviceToHost, L.st));
+        IBIM_HIP_CHECK(hipStreamSynchronize(L.st));
+        if (!hch) break;
+        if (it > 1000000) {
+            set_error("flood fill did not terminate");
+            return IBIM_ERR_STATE;
+        }
+    }
+    unsigned long long *cnt = nullptr;
+    IBIM_HIP_CHECK(hipMalloc(&cnt, sizeof(unsigned long long)));
+    IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), L.st));
+    k_inclusion<<<nb, 256, 0, L.st>>>(L.g, L.n, vis, tube, cnt);
+    IBIM_LAUNCH_CHECK();
+    unsigned long long hinc = 0;
+    IBIM_HIP_CHECK(hipMemcpyAsync(&hinc, cnt, sizeof(hinc), hipMemcpyDeviceToHost, L.st));
+    (void)hipFree(vis);
+    (void)hipFree(changed);
+    (void)hipFree(cnt);
+
+    int *flag = nullptr, *pos = nullptr;
+    IBIM_HIP_CHECK(hipMalloc(&flag, ((size_t)L.n + 1) * sizeof(int)));
+    IBIM_HIP_CHECK(hipMalloc(&pos, ((size_t)L.n + 1) * sizeof(int)));
+    IBIM_HIP_CHECK(hipMemsetAsync(flag, 0, ((size_t)L.n + 1) * sizeof(int), L.st));
+    k_tube_flag<<<nb, 256, 0, L.st>>>(L.g, L.n, tube, flag);
+    IBIM_LAUNCH_CHECK();
+    size_t bytes = 0;
+    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flag, pos, (int)(L.n + 1), L.st));
+    void *tmp = nullptr;
+    IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
+    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, flag, pos, (int)(L.n + 1), L.st));
+    int total = 0;
+    IBIM_HIP_CHECK(hipMemcpyAsync(&total, pos + L.n, sizeof(int), hipMemcpyDeviceToHost, L.st));
+    IBIM_HIP_CHECK(hipStreamSynchronize(L.st));
+    (void)hipFree(tmp);
+    if (total <= 0) {
+        (void)hipFree(flag);
+        (void)hipFree(pos);
+        set_error("the level set has no tube points");
+        return IBIM_ERR_STATE;
+    }
+    S->n = total;
+    IBIM_HIP_CHECK(hipMalloc(&S->xyz, 3 * (size_t)total * sizeof(double)));
+    IBIM_HIP_CHECK(hipMalloc(&S->nrm, 3 * (size_t)total * sizeof(double)));
+    IBIM_HIP_CHECK(hipMalloc(&S->w, (size_t)total * sizeof(double)));
+    const double s = gp.scale > 0 ? gp.scale : 1.0;
+    const double hs = L.d.h / s;
+    k_tube_emit<<<nb, 256, 0, L.st>>>(L.g, L.d, flag, pos, tube, 1.0 / s, s * hs * hs * hs, S->xyz, S->nrm, S->w);
+    IBIM_LAUNCH_CHECK();
+    IBIM_HIP_CHECK(hipStreamSynchronize(L.st));
+    (void)hipFree(flag);
+    (void)hipFree(pos);
+    std::vector<double> hw(total);
+    IBIM_HIP_CHECK(hipMemcpy(hw.data(), S->w, (size_t)total * sizeof(double), hipMemcpyDeviceToHost));
+    double area = 0.0;
+    for (double v : hw) area += v;
+    ibim_surface_info &inf = S->info;
+    inf.n_nodes = total;
+    inf.dims[0] = L.d.nx;
+    inf.dims[1] = L.d.ny;
+    inf.dims[2] = L.d.nz;
+    inf.lo[0] = L.d.sx;
+    inf.lo[1] = L.d.sy;
+    inf.lo[2] = L.d.sz;
+    inf.h = L.d.h;
+    inf.inclusions = (int64_t)hinc;
+    inf.area = area;
+    return IBIM_OK;
+}
+
+static int check_geom(const ibim_geom_params *gp) {
+    if (!gp || !(gp->h > 0) || gp->thickness < 1 || gp->band < 0 || !(gp->probe >= 0) || !(gp->reinit_time >= 0) ||
+        !(gp->scale >= 0)) {
+        set_error("bad geometry parameters (h > 0, thickness >= 1, band >= 0, probe >= 0 required)");
+        return IBIM_ERR_ARG;
+    }
+    return IBIM_OK;
+}
+
+static int alloc_grid(LsRun &L, bool full) {
+    L.n = (int64_t)L.d.nx * L.d.ny * L.d.nz;
+    if (L.d.nx < 7 || L.d.ny < 7 || L.d.nz < 7 || L.n > ((int64_t)1 << 31) - 2) {
+        set_error("grid dimensions must be >= 7 and hold < 2^31 points");
+        return IBIM_ERR_ARG;
+    }
+    IBIM_HIP_CHECK(hipMalloc(&L.g, (size_t)L.n * sizeof(double)));
+    if (full) {
+        IBIM_HIP_CHECK(hipMalloc(&L.phi0, (size_t)L.n * sizeof(double)));
+        IBIM_HIP_CHECK(hipMalloc(&L.u1, (size_t)L.n * sizeof(double)));
+        IBIM_HIP_CHECK(hipMalloc(&L.u2, (size_t)L.n * sizeof(double)));
+    }
+    return IBIM_OK;
+}
+
+static void free_grid(LsRun &L) {
+    for (double *p : {L.g, L.phi0, L.u1, L.u2})
+        if (p) (void)hipFree(p);
+    if (L.st) (void)hipStreamDestroy(L.st);
+}
+
+}  // namespace ibim
+
+using namespace ibim;
+
+extern "C" {
+
+void ibim_surface_destroy(ibim_surface *s) {
+    if (!s) return;
+    for (double *p : {s->xyz, s->nrm, s->w})
+        if (p) (void)hipFree(p);
+    delete s;
+}
+
+int ibim_surface_from_atoms(const ibim_geom_params *gp, int64_t m, const double *axyz, const double *radii,
+                            ibim_surface **out) {
+    if (!out || m < 1 || !axyz || !radii) {
+        set_error("bad argument to ibim_surface_from_atoms");
+        return IBIM_ERR_ARG;
+    }
+    *out = nullptr;
+    int rc = check_geom(gp);
+    if (rc) return rc;
+    LsRun L;
+    L.d.h = gp->h;
+    if (gp->dims[0] > 0 && gp->dims[1] > 0 && gp->dims[2] > 0) {
+        L.d.nx = gp->dims[0];
+        L.d.ny = gp->dims[1];
+        L.d.nz = gp->dims[2];
+        L.d.sx = gp->lo[0];
+        L.d.sy = gp->lo[1];
+        L.d.sz = gp->lo[2];
+    } else {
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t a = 0; a < m; ++a)
+            for (int q = 0; q < 3; ++q) {
+                lo[q] = std::min(lo[q], axyz[3 * a + q] - radii[a] - gp->probe);
+                hi[q] = std::max(hi[q], axyz[3 * a + q] + radii[a] + gp->probe);
+            }
+        int *dd[3] = {&L.d.nx, &L.d.ny, &L.d.nz};
+        double *ss[3] = {&L.d.sx, &L.d.sy, &L.d.sz};
+        for (int q = 0; q < 3; ++q) {
+            *ss[q] = lo[q] - gp->pad;
+            *dd[q] = (int)std::ceil((hi[q] + gp->pad - *ss[q]) / gp->h) + 1;
+        }
+    }
+    IBIM_HIP_CHECK(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+    if ((rc = alloc_grid(L, true))) {
+        free_grid(L);
+        return rc;
+    }
+    ibim_surface *S = new ibim_surface();
+    const int nb = nblk(L.n);
+    double *d_at = nullptr;
+    auto fail = [&](int code) {
+        if (d_at) (void)hipFree(d_at);
+        free_grid(L);
+        ibim_surface_destroy(S);
+        return code;
+    };
+    // expand (levelset.cpp:166-184) onto g = -2 * max(dims) * h
+    if (hipMalloc(&d_at, 4 * (size_t)m * sizeof(double)) != hipSuccess) {
+        set_error("hipMalloc (atoms)");
+        return fail(IBIM_ERR_NOMEM);
+    }
+    std::vector<double> at(4 * (size_t)m);
+    std::copy(axyz, axyz + 3 * m, at.begin());
+    std::copy(radii, radii + m, at.begin() + 3 * m);
+    if (hipMemcpy(d_at, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("hipMemcpy (atoms)");
+        return fail(IBIM_ERR_HIP);
+    }
+    const double init = -2.0 * std::max(L.d.nx, std::max(L.d.ny, L.d.nz)) * L.d.h;
+    k_fill<<<nb, 256, 0, L.st>>>(L.g, L.n, init);
+    k_to_ord<<<nb, 256, 0, L.st>>>(L.g, L.n);
+    {
+        const int gy = (int)std::min<int64_t>(m, 65535), gz = (int)((m + gy - 1) / gy);
+        dim3 grid(64, gy, gz);
+        k_expand<<<grid, 256, 0, L.st>>>((unsigned long long *)L.g, L.d, d_at, d_at + 3 * m, (int)m, gp->probe,
+                                         gp->band, 0);
+    }
+    k_from_ord<<<nb, 256, 0, L.st>>>(L.g, L.n);
+    if (hipGetLastError() != hipSuccess) {
+        set_error("expand launch failed");
+        return fail(IBIM_ERR_HIP);
+    }
+    const double tube = gp->thickness * L.d.h;
+    // evolve inward by the probe radius (main.cpp:51: final time 1.4, speed s)
+    if ((rc = ls_rk3(L, 0, gp->probe, 1.0, 0.2, tube, gp->thres, &S->info.evolve_steps, nullptr))) return fail(rc);
+    // flip the sign, then reinitialise twice with phi0 the current level set (main.cpp:53-60)
+    k_scale<<<nb, 256, 0, L.st>>>(L.g, L.n, -1.0);
+    for (int pass = 0; pass < 2; ++pass) {
+        if (hipMemcpyAsync(L.phi0, L.g, (size_t)L.n * sizeof(double), hipMemcpyDeviceToDevice, L.st) != hipSuccess) {
+            set_error("hipMemcpy (phi0)");
+            return fail(IBIM_ERR_HIP);
+        }
+        if ((rc = ls_rk3(L, 1, gp->reinit_time, 1.0, 0.5, tube, gp->thres, &S->info.reinit_steps[pass],
+                         S->info.quality)))
+            return fail(rc);
+    }
+    if ((rc = ls_surface(L, *gp, S))) return fail(rc);
+    (void)hipFree(d_at);
+    d_at = nullptr;
+    free_grid(L);
+    *out = S;
+    return IBIM_OK;
+}
+
+int ibim_surface_from_grid(const ibim_geom_params *gp, const double *phi, ibim_surface **out) {
+    if (!out || !phi) {
+        set_error("bad argument to ibim_surface_from_grid");
+        return IBIM_ERR_ARG;
+    }
+    *out = nullptr;
+    int rc = check_geom(gp);
+    if (rc) return rc;
+    LsRun L;
+    L.d = GridDims{gp->dims[0], gp->dims[1], gp->dims[2], gp->lo[0], gp->lo[1], gp->lo[2], gp->h};
+    IBIM_HIP_CHECK(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+    if ((rc = alloc_grid(L, false))) {
+        free_grid(L);
+        return rc;
+    }
+    if (hipMemcpy(L.g, phi, (size_t)L.n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        free_grid(L);
+        set_error("hipMemcpy (phi)");
+        return IBIM_ERR_HIP;
+    }
+    ibim_surface *S = new ibim_surface();
+    if ((rc = ls_surface(L, *gp, S))) {
+        free_grid(L);
+        ibim_surface_destroy(S);
+        return rc;
+    }
+    free_grid(L);
+    *out = S;
+    return IBIM_OK;
+}
+
+int ibim_surface_get_info(const ibim_surface *s, ibim_surface_info *info) {
+    if (!s || !info) {
+        set_error("null argument");
+        return IBIM_ERR_ARG;
+    }
+    *info = s->info;
+    return IBIM_OK;
+}
+
+int ibim_surface_copy(const ibim_surface *s, double *xyz, double *normals, double *weights) {
+    if (!s) {
+        set_error("surface is NULL");
+        return IBIM_ERR_ARG;
+    }
+    if (xyz) IBIM_HIP_CHECK(hipMemcpy(xyz, s->xyz, 3 * (size_t)s->n * sizeof(double), hipMemcpyDeviceToHost));
+    if (normals) IBIM_HIP_CHECK(hipMemcpy(normals, s->nrm, 3 * (size_t)s->n * sizeof(double), hipMemcpyDeviceToHost));
+    if (weights) IBIM_HIP_CHECK(hipMemcpy(weights, s->w, (size_t)s->n * sizeof(double), hipMemcpyDeviceToHost));
+    return IBIM_OK;
+}
+
+int ibim_surface_device(const ibim_surface *s, const double **d_xyz, const double **d_normals,
+                        const double **d_weights) {
+    if (!s) {
+        set_error("surface is NULL");
+        return IBIM_ERR_ARG;
+    }
+    if (d_xyz) *d_xyz = s->xyz;
+    if (d_normals) *d_normals = s->nrm;
+    if (d_weights) *d_weights = s->w;
+    return IBIM_OK;
+}
+
+}  // extern "C"

@@ -134,6 +134,31 @@ __device__ __forceinline__ void sp_values(double u, const ChebTab &tab, double *
     }
 }
 
+// stage timers: HIP events recorded on the plan's stream around each stage
+enum Stage { kStP2M = 0, kStM2M, kStM2L, kStL2L, kStP2P, kStMatvec, kStOrth, kStRhs, kStPolarize, kStCount };
+
+struct StageTimer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    struct Rec {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    double ms[16] = {0};
+    int64_t count[16] = {0};
+    hipEvent_t take() {
+        if (pool.empty()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            return e;
+        }
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+};
+
 // a set of targets sorted into the leaves of a plan's tree
 struct TargetSet {
     int n = 0;
@@ -194,6 +219,8 @@ struct ibim_plan {
     double *gm_part = nullptr;           // partial sums
     double *gm_small = nullptr;          // H, s, cs, sn, scalars on device
 
+    ibim::StageTimer timer;
+
     ibim::ChebTab tab{};
     ibim::PBConst kc{};
 
@@ -231,4 +258,28 @@ int rhs_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_aq, d
 int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int maxit, double tol,
                  ibim_solve_info *info);
 int compute_stats(ibim_plan *pl);
+
+// scoped stage timing on the plan's stream (no-op when profiling is off)
+struct StageScope {
+    ibim_plan *pl;
+    int stage;
+    hipEvent_t a = nullptr;
+    StageScope(ibim_plan *p, int s) : pl(p), stage(s) {
+        if (pl->timer.on) {
+            a = pl->timer.take();
+            if (a) (void)hipEventRecord(a, pl->stream);
+        }
+    }
+    ~StageScope() {
+        if (a) {
+            hipEvent_t b = pl->timer.take();
+            if (b) {
+                (void)hipEventRecord(b, pl->stream);
+                pl->timer.pending.push_back({stage, a, b});
+            } else {
+                pl->timer.pool.push_back(a);
+            }
+        }
+    }
+};
 }  // namespace ibim

@@ -287,6 +287,7 @@ int rhs_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_aq, d
     const double dE = pl->prm.eps_out, dI = pl->prm.eps_in;
     const double k1 = 1.0 / (dI * 4.0 * pi * (0.5 * (1 + dE / dI)));
     const double k2 = 1.0 / (dI * 4.0 * pi * (0.5 * (1 + dI / dE)));
+    StageScope sc(pl, kStRhs);
     k_rhs<<<nblk(pl->n), 256, 0, pl->stream>>>(pl->n, pl->perm, pl->sx, pl->sy, pl->sz, pl->snx, pl->sny, pl->snz, m,
                                                d_axyz, d_aq, k1, k2, d_b);
     IBIM_LAUNCH_CHECK();
@@ -358,6 +359,7 @@ int make_target_set(ibim_plan *pl, int m, const double *d_txyz, TargetSet &ts, s
 }
 
 int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_x, double *d_out) {
+    StageScope sc(pl, kStPolarize);
     std::vector<void *> tmp;
     TargetSet ts;
     int *tperm = nullptr;
@@ -457,15 +459,18 @@ int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int
                 nmv++;
                 inner++;
                 double *hcol = H + (size_t)i * ld;
-                k_dot_part<<<kRG, kRB, 0, st>>>(w, V, n2, partA);  // <w, v_0>
-                double *pin = partA, *pout = partB;
-                for (int k = 0; k <= i; ++k) {
-                    const double *vnext = (k < i) ? V + (size_t)(k + 1) * ldv : nullptr;
-                    k_mgs<<<kRG, kRB, 0, st>>>(w, V + (size_t)k * ldv, vnext, n2, pin, pout, hcol + k);
-                    std::swap(pin, pout);
+                {
+                    StageScope sc(pl, kStOrth);
+                    k_dot_part<<<kRG, kRB, 0, st>>>(w, V, n2, partA);  // <w, v_0>
+                    double *pin = partA, *pout = partB;
+                    for (int k = 0; k <= i; ++k) {
+                        const double *vnext = (k < i) ? V + (size_t)(k + 1) * ldv : nullptr;
+                        k_mgs<<<kRG, kRB, 0, st>>>(w, V + (size_t)k * ldv, vnext, n2, pin, pout, hcol + k);
+                        std::swap(pin, pout);
+                    }
+                    k_normalize<<<kRG, kRB, 0, st>>>(w, V + (size_t)(i + 1) * ldv, n2, pin, hcol + i + 1);
+                    k_givens<<<1, 64, 0, st>>>(H, ld, s, cs, sn, i, normb, scal + 1);
                 }
-                k_normalize<<<kRG, kRB, 0, st>>>(w, V + (size_t)(i + 1) * ldv, n2, pin, hcol + i + 1);
-                k_givens<<<1, 64, 0, st>>>(H, ld, s, cs, sn, i, normb, scal + 1);
                 IBIM_LAUNCH_CHECK();
                 IBIM_HIP_CHECK(hipMemcpyAsync(&res, scal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
                 IBIM_HIP_CHECK(hipStreamSynchronize(st));

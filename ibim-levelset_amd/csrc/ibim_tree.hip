@@ -10,9 +10,10 @@
 // and centres follow the same recurrence c + (bit - 1/2) r (the product is a
 // power-of-two scaling, so fused and unfused evaluation agree).  Instead of
 // recursive splitting, every point gets its full path key (3 bits per level,
-// child slot 4z+2y+x), the keys are radix-sorted (stable, so each box keeps
-// its points in ascending original index like the reference's push_back),
-// and the boxes of each level are cut out of the sorted key ranges.  A box is
+// child slot 4z+2y+x), the keys are radix-sorted, and the boxes of each level
+// are cut out of the sorted key ranges.  A second stable sort on the keys cut
+// at each point's leaf level then restores the reference's order inside a
+// leaf (ascending original index, from its push_back in index order).  A box is
 // split when it holds more than max_points sources and is above max_level;
 // all eight children are created, empty ones become empty leaves.
 #include <hipcub/hipcub.hpp>
@@ -172,6 +173,23 @@ __global__ void k_concat_level(int nl, int off, int level, int child_off, int pa
     nctr[3 * id] = lctr[3 * j];
     nctr[3 * id + 1] = lctr[3 * j + 1];
     nctr[3 * id + 2] = lctr[3 * j + 2];
+}
+
+// key of each point truncated at its leaf's level, written at the point's
+// original index: a stable sort on it keeps the leaves in the same order and
+// puts the points of a leaf in ascending original index (bbfmm.cpp:106-116)
+__global__ void k_leaf_keys(int nnodes, const int *__restrict__ flags, const uint64_t *__restrict__ nkey,
+                            const int *__restrict__ level, const int *__restrict__ sb, const int *__restrict__ se,
+                            const int *__restrict__ perm, int maxlev, uint64_t *__restrict__ tkey) {
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nnodes || !(flags[id] & kLeaf)) return;
+    const uint64_t k = nkey[id] << (3 * (maxlev - level[id]));
+    for (int i = sb[id]; i < se[id]; ++i) tkey[perm[i]] = k;
+}
+
+__global__ void k_iota(int n, int *__restrict__ a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
 }
 
 // ------------------------------------------------------------ lists ----
@@ -538,6 +556,34 @@ int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const do
                                   hipMemcpyHostToDevice, st));
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
     cleanup();
+
+    // ---- points of a leaf in ascending original index: second stable sort on
+    // the leaf-truncated keys, starting from the original order
+    if (maxlev > 0) {
+        uint64_t *tk = nullptr;
+        int *iota = nullptr, *perm2 = nullptr;
+        IBIM_HIP_CHECK(hipMalloc(&tk, (size_t)n * sizeof(uint64_t)));
+        IBIM_HIP_CHECK(hipMalloc(&iota, (size_t)n * sizeof(int)));
+        IBIM_HIP_CHECK(hipMalloc(&perm2, (size_t)n * sizeof(int)));
+        k_leaf_keys<<<nblk(nn), kBlk, 0, st>>>(nn, pl->flags, pl->nkey, pl->level, pl->sbeg, pl->send, pl->perm,
+                                               maxlev, tk);
+        k_iota<<<nblk(n), kBlk, 0, st>>>(n, iota);
+        IBIM_LAUNCH_CHECK();
+        size_t bytes = 0;
+        IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, tk, k_out, iota, perm2, n, 0, 3 * maxlev, st));
+        void *tmp2 = nullptr;
+        IBIM_HIP_CHECK(hipMalloc(&tmp2, bytes ? bytes : 1));
+        IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp2, bytes, tk, k_out, iota, perm2, n, 0, 3 * maxlev, st));
+        IBIM_HIP_CHECK(hipMemcpyAsync(pl->perm, perm2, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, st));
+        k_gather_geometry<<<nblk(n), kBlk, 0, st>>>(pl->perm, n, d_xyz, d_nrm, d_w, pl->sx, pl->sy, pl->sz, pl->snx,
+                                                    pl->sny, pl->snz, pl->sw);
+        IBIM_LAUNCH_CHECK();
+        IBIM_HIP_CHECK(hipStreamSynchronize(st));
+        (void)hipFree(tmp2);
+        (void)hipFree(tk);
+        (void)hipFree(iota);
+        (void)hipFree(perm2);
+    }
     (void)hipFree(k_out);
 
     // ---- interaction lists (count, scan, fill)

@@ -105,6 +105,23 @@ int  ibim_plan_set_precision(ibim_plan *plan, int precision);
 /* device memory held by the plan, bytes */
 int64_t ibim_plan_footprint(const ibim_plan *plan);
 
+/* stage timing: HIP events recorded on the plan's stream around each stage
+ * while enabled (the reference's RUN timing macro, utils.h:45-56, but on the
+ * device); ibim_plan_stage_times waits for them and returns milliseconds and
+ * counts per stage (IBIM_STAGE_COUNT entries), optionally resetting them. */
+#define IBIM_STAGE_P2M 0
+#define IBIM_STAGE_M2M 1
+#define IBIM_STAGE_M2L 2
+#define IBIM_STAGE_L2L 3
+#define IBIM_STAGE_P2P 4      /* near field + L2P */
+#define IBIM_STAGE_MATVEC 5   /* one whole matvec */
+#define IBIM_STAGE_ORTH 6     /* GMRES orthogonalisation */
+#define IBIM_STAGE_RHS 7
+#define IBIM_STAGE_POLARIZE 8
+#define IBIM_STAGE_COUNT 9
+int ibim_plan_profile(ibim_plan *plan, int enable);
+int ibim_plan_stage_times(ibim_plan *plan, double *ms, int64_t *launches, int reset);
+
 /* ---- the hot path -------------------------------------------------------- */
 /* y = A x, x = [phi (n); psi (n)], exactly the reference dmapping */
 int ibim_matvec(ibim_plan *plan, const double *x, double *y);
@@ -143,6 +160,57 @@ int ibim_tree_export(ibim_plan *plan, int *level, int *parent, int *slot, int *l
 int ibim_tree_lists(ibim_plan *plan, int kind, int *offsets, int *items, int64_t cap);
 /* source points of node id, original indices ascending */
 int ibim_tree_points(ibim_plan *plan, int id, int *out, int64_t cap);
+
+/* ---- geometry pre-step: level set and tube quadrature nodes ---------------
+ * The solvent-excluded surface as a level set on a uniform grid and the IBIM
+ * quadrature nodes in the tube |phi| < thickness*h around it.  Grid index
+ * I = i*ny*nz + j*nz + k, coordinates lo + (i,j,k)*h (levelset.cpp:103-112). */
+typedef struct ibim_geom_params {
+    double h;              /* grid spacing, in the atoms' length unit */
+    double lo[3];          /* grid origin (sx, sy, sz) */
+    int dims[3];           /* nx, ny, nz; all 0: derived from the atoms and `pad` */
+    double pad;            /* margin beyond the probe-expanded atoms when dims are 0 */
+    double probe;          /* probe radius (main.cpp:35: 1.4 Angstrom) */
+    int band;              /* expand window margin in cells (main.cpp:45: 8) */
+    int thickness;         /* tube half width in cells, lvlset_thickness */
+    double thres;          /* reinitialisation gradient tolerance, lvlset_thres */
+    double reinit_time;    /* final time of each reinitialisation, reinit_step */
+    double scale;          /* s: nodes/s, weights*s*(h/s)^3 (electric.cpp:17-41); 1 for Angstrom grids */
+} ibim_geom_params;
+
+typedef struct ibim_surface ibim_surface;
+
+typedef struct ibim_surface_info {
+    int64_t n_nodes;       /* tube points */
+    int dims[3];
+    double lo[3];
+    double h;
+    int64_t inclusions;    /* cavity grid points pushed out of the tube (levelset.cpp:601-612) */
+    int evolve_steps, reinit_steps[2];
+    double quality[2];     /* mean |grad phi| - 1 and bad fraction in the tube (levelset.cpp:500-504) */
+    double area;           /* sum of weights */
+} ibim_surface_info;
+
+/* expand + evolve (probe) + flip + reinitialise twice + Surface, as the
+ * reference pipeline main.cpp:47-62 (levelset.cpp:166-362, 512-661) */
+int ibim_surface_from_atoms(const ibim_geom_params *gp, int64_t m, const double *atom_xyz, const double *radii,
+                            ibim_surface **out);
+/* Surface (levelset.cpp:512-661) of a level set given on the grid (host array) */
+int ibim_surface_from_grid(const ibim_geom_params *gp, const double *phi, ibim_surface **out);
+int ibim_surface_get_info(const ibim_surface *s, ibim_surface_info *info);
+/* host copies (AoS xyz and normals, weights) */
+int ibim_surface_copy(const ibim_surface *s, double *xyz, double *normals, double *weights);
+/* device arrays owned by the surface */
+int ibim_surface_device(const ibim_surface *s, const double **d_xyz, const double **d_normals,
+                        const double **d_weights);
+void ibim_surface_destroy(ibim_surface *s);
+
+/* ---- molecules ---------------------------------------------------------------
+ * Seeded synthetic globule: self-avoiding random walk of 3.8 A steps confined
+ * to a sphere, radii U[1.2,2.0], charges U[-0.5,0.5] shifted to net -2 e.
+ * Arrays are sized m (xyz 3m). */
+int ibim_synthetic_globule(int64_t m, uint64_t seed, double confine_radius, double *xyz, double *radii,
+                           double *charges);
 
 const char *ibim_last_error(void);
 const char *ibim_version(void);
