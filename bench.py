@@ -164,9 +164,10 @@ def main():
     rc_p, info_p, energy_p, stages, stats, _ = solve(profile=True)
     r = cfg["p"] ** 3
     nmv = info_p["matvecs"]
-    m2l_flops = 32.0 * r * r * stats["m2l_pairs"]      # per matvec (one launch)
     p2p_flops = 60.0 * stats["p2p_pairs"]
-    per_launch = {"M2L": (stages["M2L"][0] / max(stages["M2L"][1], 1), m2l_flops),
+    # V-list GEMM: 32 r^2 flops per V interaction; the X list runs in its own launch
+    v_flops = 32.0 * r * r * stats["v_pairs"]
+    per_launch = {"M2L": (stages["M2L"][0] / max(stages["M2L"][1], 1), v_flops),
                   "P2P": (stages["P2P"][0] / max(stages["P2P"][1], 1), p2p_flops)}
     dom = max(per_launch, key=lambda k: per_launch[k][0])
     ms_launch, flops = per_launch[dom]

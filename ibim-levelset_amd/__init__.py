@@ -31,7 +31,7 @@ KID = {k: i for i, k in enumerate(KERNELS)}
 PREC_FP64, PREC_MIXED = 0, 1
 KERNEL_PB, KERNEL_POLY = 0, 1
 IBIM_OK, IBIM_NOT_CONVERGED = 0, 1
-STAGES = ["P2M", "M2M", "M2L", "L2L", "P2P", "MATVEC", "ORTH", "RHS", "POLARIZE"]
+STAGES = ["P2M", "M2M", "M2L", "L2L", "P2P", "MATVEC", "ORTH", "RHS", "POLARIZE", "M2L_X"]
 
 
 class IbimError(RuntimeError):
@@ -48,7 +48,8 @@ class TreeStats(C.Structure):
     _fields_ = [("n_points", C.c_int64), ("n_nodes", C.c_int64), ("n_leaves", C.c_int64),
                 ("n_empty_leaves", C.c_int64), ("depth", C.c_int),
                 ("n_u", C.c_int64), ("n_v", C.c_int64), ("n_w", C.c_int64), ("n_x", C.c_int64),
-                ("p2p_pairs", C.c_int64), ("m2l_pairs", C.c_int64)]
+                ("p2p_pairs", C.c_int64), ("m2l_pairs", C.c_int64), ("v_pairs", C.c_int64),
+                ("x_pairs", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -174,7 +175,7 @@ class Plan:
     """A device-resident IBIM plan: octree, lists and FMM buffers for one
     surface point set (xyz, normals: (n,3); weights: (n,))."""
 
-    def __init__(self, xyz, normals=None, weights=None, device_ptrs=None, **kw):
+    def __init__(self, xyz=None, normals=None, weights=None, device_ptrs=None, **kw):
         self.prm = params(**kw)
         self._h = C.c_void_p()
         if device_ptrs is not None:

@@ -151,7 +151,21 @@ int ibim_plan_create_device(const ibim_params *prm, int64_t n, const double *d_x
         ibim_plan_destroy(pl);
         return IBIM_ERR_HIP;
     }
-    if ((rc = build_chebyshev(pl)) || (rc = build_tree(pl, d_xyz, d_nrm, d_w))) {
+    if ((rc = build_chebyshev(pl)) || (rc = build_tree(pl, d_xyz, d_nrm, d_w)) || (rc = build_m2l_operators(pl)) ||
+        (rc = build_vpairs(pl, pl->need_local, pl->vp, nullptr))) {
+        ibim_plan_destroy(pl);
+        return rc;
+    }
+    {
+        TargetSet ts = source_targets(pl);
+        if ((rc = build_p2p_ranges(pl, ts, nullptr))) {
+            ibim_plan_destroy(pl);
+            return rc;
+        }
+        pl->p2p_roff = ts.roff;
+        pl->p2p_ranges = ts.ranges;
+    }
+    if (false) {
         ibim_plan_destroy(pl);
         return rc;
     }
@@ -473,7 +487,11 @@ int ibim_tree_stats_get(ibim_plan *pl, ibim_tree_stats *st) {
         if (i != 0 && nt > 0)
             for (int k : {1, 3})
                 for (int q = off[k][i]; q < off[k][i + 1]; ++q)
-                    if (!(fl[it[k][q]] & kEmpty)) s.m2l_pairs++;
+                    if (!(fl[it[k][q]] & kEmpty)) {
+                        s.m2l_pairs++;
+                        if (k == 1) s.v_pairs++;
+                        else s.x_pairs++;
+                    }
     }
     *st = s;
     return IBIM_OK;

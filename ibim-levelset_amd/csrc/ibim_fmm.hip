@@ -230,7 +230,7 @@ __global__ __launch_bounds__(256) void k_l2l(int lo, const unsigned char *__rest
 // kernel matrix evaluated at the tensor Chebyshev grids of both boxes
 // (kernelEvalChebyshev, bbfmm.h:456-479).
 template <int P, int BS>
-__global__ __launch_bounds__(BS) void k_m2l_direct(const int *__restrict__ work, const int *__restrict__ voff,
+__global__ __launch_bounds__(BS) void k_m2l_direct(int lists, const int *__restrict__ work, const int *__restrict__ voff,
                                                    const int *__restrict__ vit, const int *__restrict__ xoff,
                                                    const int *__restrict__ xit, const int *__restrict__ flags,
                                                    const double *__restrict__ ctr, const int *__restrict__ level,
@@ -243,6 +243,9 @@ __global__ __launch_bounds__(BS) void k_m2l_direct(const int *__restrict__ work,
     __shared__ double sxb[P];
     const int id = work[blockIdx.x];
     const int tid = threadIdx.x;
+    const bool useV = (lists & 1) && voff[id + 1] > voff[id];
+    const bool useX = (lists & 2) && xoff[id + 1] > xoff[id];
+    if (!useV && !useX) return;
     if (tid < P) {
         double v = 0.0;
 #pragma unroll
@@ -264,6 +267,7 @@ __global__ __launch_bounds__(BS) void k_m2l_direct(const int *__restrict__ work,
         for (int f = 0; f < 4; ++f) acc[k][f] = 0.0;
     }
     for (int lst = 0; lst < 2; ++lst) {
+        if ((lst == 0 && !useV) || (lst == 1 && !useX)) continue;
         const int *off = lst == 0 ? voff : xoff;
         const int *it = lst == 0 ? vit : xit;
         for (int q = off[id]; q < off[id + 1]; ++q) {
@@ -302,7 +306,7 @@ __global__ __launch_bounds__(BS) void k_m2l_direct(const int *__restrict__ work,
         const int m = tid + BS * k;
         if (m < R)
 #pragma unroll
-            for (int f = 0; f < 4; ++f) Lb[f * R + m] = acc[k][f];
+            for (int f = 0; f < 4; ++f) Lb[f * R + m] += acc[k][f];
     }
 }
 
@@ -390,6 +394,142 @@ __global__ __launch_bounds__(64) void k_p2p_l2p(
     }
 }
 
+
+// ---------------------------------------------- near-field source ranges ---
+// For each target leaf: the sources of its non-empty U and W boxes as ranges
+// of the sorted point order, sorted and merged where they touch (Morton
+// order makes neighbouring boxes' ranges contiguous).
+template <int PASS>
+__global__ void k_p2p_ranges(int nleaves, const int *__restrict__ tleaves, const int *__restrict__ uoff,
+                             const int *__restrict__ uit, const int *__restrict__ woff, const int *__restrict__ wit,
+                             const int *__restrict__ flags, const int *__restrict__ sbeg, const int *__restrict__ send,
+                             int *__restrict__ cnt, const int *__restrict__ roff, int2 *__restrict__ ranges) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nleaves) return;
+    const int b = tleaves[i];
+    constexpr int CAP = 512;
+    int2 loc[CAP];
+    int n = 0;
+    bool overflow = false;
+    for (int lst = 0; lst < 2; ++lst) {
+        const int *off = lst == 0 ? uoff : woff;
+        const int *it = lst == 0 ? uit : wit;
+        for (int q = off[b]; q < off[b + 1]; ++q) {
+            const int a = it[q];
+            if (flags[a] & kEmpty) continue;
+            if (n < CAP) loc[n++] = make_int2(sbeg[a], send[a]);
+            else overflow = true;
+        }
+    }
+    // insertion sort by begin, then merge touching ranges
+    for (int x = 1; x < n; ++x) {
+        int2 v = loc[x];
+        int y = x - 1;
+        while (y >= 0 && loc[y].x > v.x) {
+            loc[y + 1] = loc[y];
+            --y;
+        }
+        loc[y + 1] = v;
+    }
+    int m = 0;
+    for (int x = 0; x < n; ++x) {
+        if (m > 0 && loc[m - 1].y == loc[x].x) loc[m - 1].y = loc[x].y;
+        else loc[m++] = loc[x];
+    }
+    if (PASS == 0) {
+        cnt[i] = overflow ? -1 : m;
+    } else {
+        for (int x = 0; x < m; ++x) ranges[roff[i] + x] = loc[x];
+    }
+}
+
+// ----------------------------------------------------- P2P + L2P, v2 ---
+// One wavefront per target leaf.  The wave's 64 lanes are tp targets x
+// (64/tp) source slices (tp = next power of two >= targets in the chunk);
+// each lane walks its slice of the leaf's merged source ranges, the slices
+// are summed with cross-lane shuffles, and the first slice adds the far
+// field (L2P) from the leaf's local expansion staged in LDS.
+template <int P>
+__global__ __launch_bounds__(256) void k_p2p_wave(
+    int nleaves, const int *__restrict__ tleaves, const int *__restrict__ tbeg, const int *__restrict__ tend,
+    const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
+    const double *__restrict__ sx, const double *__restrict__ sy, const double *__restrict__ sz,
+    const double *__restrict__ Q, int ns, const int *__restrict__ roff, const int2 *__restrict__ ranges,
+    const double *__restrict__ ctr, const int *__restrict__ level, const double *__restrict__ lrad, ChebTab tab,
+    PBConst kc, const double *__restrict__ L, double *__restrict__ F, int nt) {
+    constexpr int R = P * P * P;
+    __shared__ double sL[4][4 * R];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = blockIdx.x * 4 + wv;
+    const bool active = li < nleaves;
+    const int b = active ? tleaves[li] : 0;
+    double *myL = sL[wv];
+    if (active)
+        for (int e = lane; e < 4 * R; e += 64) myL[e] = L[(size_t)b * 4 * R + e];
+    __syncthreads();
+    if (!active) return;  // no block barrier below
+    const double cx = ctr[3 * b], cy = ctr[3 * b + 1], cz = ctr[3 * b + 2];
+    const double *rd = lrad + 3 * level[b];
+    const int r0 = roff[li], r1 = roff[li + 1];
+    const int t_beg = tbeg[b], t_end = tend[b];
+    for (int t0 = t_beg; t0 < t_end; t0 += 64) {
+        const int ntc = min(64, t_end - t0);
+        int tp = 1;
+        while (tp < ntc) tp <<= 1;
+        const int nsl = 64 / tp;
+        const int mt = lane & (tp - 1), sl = lane / tp;
+        const bool valid = mt < ntc;
+        const int t = t0 + mt;
+        const double px = valid ? tx[t] : 0.0, py = valid ? ty[t] : 0.0, pz = valid ? tz[t] : 0.0;
+        double f0 = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+        int base = 0, j = sl;
+        for (int r = r0; r < r1; ++r) {
+            const int2 rg = ranges[r];
+            const int len = rg.y - rg.x;
+            while (j - base < len) {
+                const int s = rg.x + (j - base);
+                if (valid)
+                    pair_accum<double>(sx[s] - px, sy[s] - py, sz[s] - pz, Q[s], Q[(size_t)ns + s],
+                                       Q[2 * (size_t)ns + s], Q[3 * (size_t)ns + s], kc, f0, fx, fy, fz);
+                j += nsl;
+            }
+            base += len;
+        }
+        for (int o = tp; o < 64; o <<= 1) {
+            f0 += __shfl_xor(f0, o);
+            fx += __shfl_xor(fx, o);
+            fy += __shfl_xor(fy, o);
+            fz += __shfl_xor(fz, o);
+        }
+        if (valid && sl == 0) {
+            double Sx[P], Sy[P], Sz[P];
+            sp_values<P>((px - cx) / rd[0], tab, Sx);
+            sp_values<P>((py - cy) / rd[1], tab, Sy);
+            sp_values<P>((pz - cz) / rd[2], tab, Sz);
+            double g0 = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+            for (int l = 0; l < P; ++l)
+#pragma unroll
+                for (int jj = 0; jj < P; ++jj) {
+                    const double wyz = Sy[jj] * Sz[l];
+#pragma unroll
+                    for (int i = 0; i < P; ++i) {
+                        const int m = l * P * P + jj * P + i;
+                        const double w = Sx[i] * wyz;
+                        g0 += w * myL[m];
+                        gx += w * myL[R + m];
+                        gy += w * myL[2 * R + m];
+                        gz += w * myL[3 * R + m];
+                    }
+                }
+            F[t] = f0 + g0;
+            F[(size_t)nt + t] = fx + gx;
+            F[2 * (size_t)nt + t] = fy + gy;
+            F[3 * (size_t)nt + t] = fz + gz;
+        }
+    }
+}
+
 // ------------------------------------------------------------- helpers ---
 __global__ void k_charges(int n, const int *__restrict__ perm, const double *__restrict__ snx,
                           const double *__restrict__ sny, const double *__restrict__ snz,
@@ -456,12 +596,19 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt) {
     // downward pass
     IBIM_HIP_CHECK(hipMemsetAsync(pl->L, 0, (size_t)pl->nnodes * 4 * R * sizeof(double), st));
     {
-    StageScope sc(pl, kStM2L);
+    StageScope sc(pl, ts.vpairs ? kStM2LX : kStM2L);
+    // X list (cross-level) directly; V list as GEMMs when the pairs are grouped
+    const int lists = ts.vpairs ? 2 : 3;
     if (ts.n_m2l_work > 0)
-        k_m2l_direct<P, BS><<<ts.n_m2l_work, BS, 0, st>>>(ts.m2l_work, pl->loff[1], pl->litem[1], pl->loff[3],
+        k_m2l_direct<P, BS><<<ts.n_m2l_work, BS, 0, st>>>(lists, ts.m2l_work, pl->loff[1], pl->litem[1], pl->loff[3],
                                                           pl->litem[3], pl->flags, pl->ctr, pl->level,
                                                           pl->d_lvl_rad, pl->tab, pl->kc, pl->M, pl->L);
     IBIM_LAUNCH_CHECK();
+    }
+    if (ts.vpairs) {
+        StageScope sc(pl, kStM2L);
+        int rc = m2l_gemm(pl, *ts.vpairs);
+        if (rc) return rc;
     }
     {
     StageScope sc(pl, kStL2L);
@@ -473,7 +620,12 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt) {
     }
     {
     StageScope sc(pl, kStP2P);
-    if (ts.nleaves > 0)
+    if (ts.nleaves > 0 && ts.roff)
+        k_p2p_wave<P><<<(ts.nleaves + 3) / 4, 256, 0, st>>>(ts.nleaves, ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z,
+                                                             pl->sx, pl->sy, pl->sz, pl->Q, pl->n, ts.roff,
+                                                             ts.ranges, pl->ctr, pl->level, pl->d_lvl_rad, pl->tab,
+                                                             pl->kc, pl->L, F, nt);
+    else if (ts.nleaves > 0)
         k_p2p_l2p<P><<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->sx, pl->sy,
                                                 pl->sz, pl->Q, pl->n, pl->loff[0], pl->litem[0], pl->loff[2],
                                                 pl->litem[2], pl->flags, pl->sbeg, pl->send, pl->ctr, pl->level,
@@ -484,6 +636,56 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt) {
 }
 
 }  // namespace
+
+// merged near-field source ranges of every target leaf of a target set
+int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, std::vector<void *> *tmp_owner) {
+    ts.roff = nullptr;
+    ts.ranges = nullptr;
+    if (ts.nleaves == 0) return IBIM_OK;
+    hipStream_t st = pl->stream;
+    const int nl = ts.nleaves;
+    int *cnt = nullptr, *roff = nullptr;
+    int2 *ranges = nullptr;
+    auto alloc = [&](auto **p, size_t n) -> int {
+        if (tmp_owner) {
+            hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(**p));
+            if (e != hipSuccess) {
+                set_error("hipMalloc (p2p ranges)");
+                return IBIM_ERR_NOMEM;
+            }
+            tmp_owner->push_back((void *)*p);
+            return IBIM_OK;
+        }
+        return dalloc(pl, p, n);
+    };
+    IBIM_HIP_CHECK(hipMalloc(&cnt, ((size_t)nl + 1) * sizeof(int)));
+    IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, ((size_t)nl + 1) * sizeof(int), st));
+    k_p2p_ranges<0><<<nblk(nl), 64, 0, st>>>(nl, ts.leaves, pl->loff[0], pl->litem[0], pl->loff[2], pl->litem[2],
+                                             pl->flags, pl->sbeg, pl->send, cnt, nullptr, nullptr);
+    IBIM_LAUNCH_CHECK();
+    std::vector<int> hc(nl);
+    IBIM_HIP_CHECK(hipMemcpyAsync(hc.data(), cnt, (size_t)nl * sizeof(int), hipMemcpyDeviceToHost, st));
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<int> ho(nl + 1, 0);
+    for (int i = 0; i < nl; ++i) {
+        if (hc[i] < 0) {  // more than 512 near-field boxes: keep the per-box kernel
+            (void)hipFree(cnt);
+            return IBIM_OK;
+        }
+        ho[i + 1] = ho[i] + hc[i];
+    }
+    int rc;
+    if ((rc = alloc(&roff, (size_t)nl + 1)) || (rc = alloc(&ranges, (size_t)std::max(1, ho[nl])))) return rc;
+    IBIM_HIP_CHECK(hipMemcpyAsync(roff, ho.data(), ho.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    k_p2p_ranges<1><<<nblk(nl), 64, 0, st>>>(nl, ts.leaves, pl->loff[0], pl->litem[0], pl->loff[2], pl->litem[2],
+                                             pl->flags, pl->sbeg, pl->send, nullptr, roff, ranges);
+    IBIM_LAUNCH_CHECK();
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipFree(cnt);
+    ts.roff = roff;
+    ts.ranges = ranges;
+    return IBIM_OK;
+}
 
 int build_chebyshev(ibim_plan *pl) {
     const int p = pl->p;
@@ -564,6 +766,9 @@ TargetSet source_targets(ibim_plan *pl) {
     ts.m2l_work = pl->m2l_work;
     ts.n_m2l_work = pl->n_m2l_work;
     ts.need_local = pl->need_local;
+    ts.vpairs = pl->ops ? &pl->vp : nullptr;
+    ts.roff = pl->p2p_roff;
+    ts.ranges = pl->p2p_ranges;
     return ts;
 }
 

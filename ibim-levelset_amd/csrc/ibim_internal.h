@@ -135,7 +135,7 @@ __device__ __forceinline__ void sp_values(double u, const ChebTab &tab, double *
 }
 
 // stage timers: HIP events recorded on the plan's stream around each stage
-enum Stage { kStP2M = 0, kStM2M, kStM2L, kStL2L, kStP2P, kStMatvec, kStOrth, kStRhs, kStPolarize, kStCount };
+enum Stage { kStP2M = 0, kStM2M, kStM2L, kStL2L, kStP2P, kStMatvec, kStOrth, kStRhs, kStPolarize, kStM2LX, kStCount };
 
 struct StageTimer {
     bool on = false;
@@ -159,6 +159,17 @@ struct StageTimer {
     }
 };
 
+// V-list interactions grouped by M2L operator class (level, |offset|)
+struct VPairs {
+    int npairs = 0;
+    int *A = nullptr, *B = nullptr;     // source and target box of each pair, sorted by class
+    unsigned char *mask = nullptr;      // reflection bits of the offset
+    std::vector<int> cls_off;           // host: pairs of class c in [cls_off[c], cls_off[c+1])
+    int *cls_off_d = nullptr;
+    int nblocks = 0;                    // GEMM blocks of 64 pairs
+    int *blk_cls = nullptr, *blk_j0 = nullptr;
+};
+
 // a set of targets sorted into the leaves of a plan's tree
 struct TargetSet {
     int n = 0;
@@ -169,6 +180,9 @@ struct TargetSet {
     const int *m2l_work = nullptr;                          // nodes needing a local expansion
     int n_m2l_work = 0;
     const unsigned char *need_local = nullptr;              // per node
+    const VPairs *vpairs = nullptr;                         // V interactions into those nodes
+    const int *roff = nullptr;                              // per target leaf: merged near-field
+    const int2 *ranges = nullptr;                           //   source ranges [roff[i], roff[i+1])
 };
 
 }  // namespace ibim
@@ -221,6 +235,13 @@ struct ibim_plan {
 
     ibim::StageTimer timer;
 
+    // V-list M2L operators: n_cls classes of MRP x KRP (row major), FP64
+    int n_cls = 0, MRP = 0, KRP = 0;
+    double *ops = nullptr;
+    ibim::VPairs vp;                     // V pairs for the points as their own targets
+    const int *p2p_roff = nullptr;       // merged near-field ranges of the non-empty leaves
+    const int2 *p2p_ranges = nullptr;
+
     ibim::ChebTab tab{};
     ibim::PBConst kc{};
 
@@ -258,6 +279,10 @@ int rhs_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_aq, d
 int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int maxit, double tol,
                  ibim_solve_info *info);
 int compute_stats(ibim_plan *pl);
+int build_m2l_operators(ibim_plan *pl);
+int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vector<void *> *tmp_owner);
+int m2l_gemm(ibim_plan *pl, const VPairs &vp);
+int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, std::vector<void *> *tmp_owner);
 
 // scoped stage timing on the plan's stream (no-op when profiling is off)
 struct StageScope {

@@ -1,0 +1,408 @@
+// ibim_m2l.hip -- the V-list M2L as batched dense GEMMs on the matrix cores.
+//
+// Reference: downPass, V list (bbfmm.h:547-552): L_B += K(grid_B, grid_A) M_A
+// with the kernel matrix evaluated entry by entry for every pair of
+// same-level boxes.  Two same-level boxes at offset o = (c_A - c_B)/(2 r_l)
+// (o in [-3,3]^3, |o|_inf >= 2: 316 offsets) always see the same kernel
+// matrix, so the 16 fused kernels become one (4r x 4r) operator per level and
+// offset, applied to every pair with that offset as one GEMM
+//     Y = K_{l,o} [M_A1 M_A2 ...],   L_Bj += Y_j.
+// Reflections fold the 316 offsets onto the 56 with o >= 0: with sigma the
+// sign of each component of o, the operator of o is the operator of |o| with
+// the Chebyshev node index reversed along each reflected axis and the
+// x/y/z field components negated along it (K1, K4 odd and K2, K3 even in d),
+// so the gather of M_A and the scatter into L_B apply that permutation and
+// those signs instead of storing 8x more operators.
+//
+// Operators are built once per plan, in FP64.  The GEMM runs on FP64 MFMA
+// (v_mfma_f64_16x16x4f64): 64 x 64 output tiles, 4 waves of 32 x 32, the
+// operator rows and the gathered multipoles staged through LDS; results are
+// added into L with hardware FP64 atomics (pairs with the same target box
+// come from different offsets, i.e. different GEMMs).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+#include "ibim_internal.h"
+
+namespace ibim {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+inline int nblk(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
+
+// canonical offset index of |o| in [0,3]^3 with max >= 2 (56 of them)
+__host__ __device__ inline int canon_index(int ax, int ay, int az) {
+    int idx = 0;
+    for (int z = 0; z < 4; ++z)
+        for (int y = 0; y < 4; ++y)
+            for (int x = 0; x < 4; ++x) {
+                if (x < 2 && y < 2 && z < 2) continue;
+                if (x == ax && y == ay && z == az) return idx;
+                ++idx;
+            }
+    return -1;
+}
+constexpr int kNCanon = 56;
+
+__host__ __device__ inline void canon_offset(int idx, int &ax, int &ay, int &az) {
+    int c = 0;
+    for (int z = 0; z < 4; ++z)
+        for (int y = 0; y < 4; ++y)
+            for (int x = 0; x < 4; ++x) {
+                if (x < 2 && y < 2 && z < 2) continue;
+                if (c == idx) {
+                    ax = x;
+                    ay = y;
+                    az = z;
+                    return;
+                }
+                ++c;
+            }
+}
+
+// V pairs of every target box that needs a local expansion: count / fill
+template <int PASS>
+__global__ void k_vpairs(int nnodes, const unsigned char *__restrict__ need, const int *__restrict__ voff,
+                         const int *__restrict__ vit, const int *__restrict__ flags, const int *__restrict__ level,
+                         const double *__restrict__ ctr, const double *__restrict__ lrad, int *__restrict__ cnt,
+                         const int *__restrict__ off, int *__restrict__ pA, int *__restrict__ pB,
+                         unsigned char *__restrict__ pm, int *__restrict__ pkey) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nnodes) return;
+    int c = 0;
+    if (b != 0 && need[b]) {
+        const int lev = level[b];
+        const double *rd = lrad + 3 * lev;
+        for (int q = voff[b]; q < voff[b + 1]; ++q) {
+            const int a = vit[q];
+            if (flags[a] & kEmpty) continue;
+            if (PASS == 1) {
+                int o[3];
+                for (int d = 0; d < 3; ++d) o[d] = (int)rint((ctr[3 * a + d] - ctr[3 * b + d]) / (2.0 * rd[d]));
+                const int m = (o[0] < 0 ? 1 : 0) | (o[1] < 0 ? 2 : 0) | (o[2] < 0 ? 4 : 0);
+                const int ci = canon_index(abs(o[0]), abs(o[1]), abs(o[2]));
+                const int j = off[b] + c;
+                pA[j] = a;
+                pB[j] = b;
+                pm[j] = (unsigned char)m;
+                pkey[j] = (lev - 2) * kNCanon + ci;
+            }
+            ++c;
+        }
+    }
+    if (PASS == 0) cnt[b] = c;
+}
+
+__global__ void k_permute_pairs(int np, const int *__restrict__ order, const int *__restrict__ a_in,
+                                const int *__restrict__ b_in, const unsigned char *__restrict__ m_in,
+                                int *__restrict__ a_out, int *__restrict__ b_out, unsigned char *__restrict__ m_out) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= np) return;
+    const int s = order[j];
+    a_out[j] = a_in[s];
+    b_out[j] = b_in[s];
+    m_out[j] = m_in[s];
+}
+
+// operator entries of class (level, |o|): row g*r + t, column f*r + s
+template <int P>
+__global__ void k_build_ops(int lev0, int ncls, const double *__restrict__ lrad, ChebTab tab, PBConst kc, int MRP,
+                            int KRP, double *__restrict__ ops) {
+    constexpr int R = P * P * P;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)R * R;
+    if (e >= per * ncls) return;
+    const int c = (int)(e / per);
+    const int t = (int)((e % per) / R), s = (int)(e % R);
+    const int lev = lev0 + c / kNCanon;
+    int o[3];
+    canon_offset(c % kNCanon, o[0], o[1], o[2]);
+    const double *rd = lrad + 3 * lev;
+    const int ti[3] = {t % P, (t / P) % P, t / (P * P)};
+    const int si[3] = {s % P, (s / P) % P, s / (P * P)};
+    double d[3];
+    for (int q = 0; q < 3; ++q) {
+        double xs = 0.0, xt = 0.0;
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            if (i == si[q]) xs = tab.xbar[i];
+            if (i == ti[q]) xt = tab.xbar[i];
+        }
+        d[q] = rd[q] * (2.0 * o[q] + xs - xt);
+    }
+    // column f = input field, row g = output field; pair_accum with unit
+    // charges gives each column of the 4 x 4 block
+    double blk[4][4];
+    for (int f = 0; f < 4; ++f) {
+        double q[4] = {0, 0, 0, 0};
+        q[f] = 1.0;
+        double f0 = 0, fx = 0, fy = 0, fz = 0;
+        pair_accum<double>(d[0], d[1], d[2], q[0], q[1], q[2], q[3], kc, f0, fx, fy, fz);
+        blk[0][f] = f0;
+        blk[1][f] = fx;
+        blk[2][f] = fy;
+        blk[3][f] = fz;
+    }
+    double *op = ops + (size_t)c * MRP * KRP;
+    for (int g = 0; g < 4; ++g)
+        for (int f = 0; f < 4; ++f) op[(size_t)(g * R + t) * KRP + f * R + s] = blk[g][f];
+}
+
+template <int P>
+__device__ __forceinline__ int flip_node(int s, int m) {
+    int i = s % P, j = (s / P) % P, k = s / (P * P);
+    if (m & 1) i = P - 1 - i;
+    if (m & 2) j = P - 1 - j;
+    if (m & 4) k = P - 1 - k;
+    return (k * P + j) * P + i;
+}
+
+// one (class, 64 pairs, 64 rows) tile per block
+template <int P>
+__global__ __launch_bounds__(256) void k_m2l_gemm(const int *__restrict__ blk_cls, const int *__restrict__ blk_j0,
+                                                  const int *__restrict__ cls_off, const int *__restrict__ pA,
+                                                  const int *__restrict__ pB, const unsigned char *__restrict__ pm,
+                                                  const double *__restrict__ ops, int MRP, int KRP,
+                                                  const double *__restrict__ M, double *__restrict__ L) {
+    constexpr int R = P * P * P;
+    constexpr int R4 = 4 * R;
+    constexpr int KC = 32;
+    __shared__ double sA[64][KC + 1];
+    __shared__ double sB[KC][64 + 1];
+    __shared__ int sPA[64], sPB[64], sPM[64];
+    const int tid = threadIdx.x;
+    const int c = blk_cls[blockIdx.x];
+    const int j0 = blk_j0[blockIdx.x];
+    const int jend = cls_off[c + 1];
+    const int row0 = blockIdx.y * 64;
+    if (tid < 64) {
+        const int j = j0 + tid;
+        const bool ok = j < jend;
+        sPA[tid] = ok ? pA[j] : -1;
+        sPB[tid] = ok ? pB[j] : -1;
+        sPM[tid] = ok ? (int)pm[j] : 0;
+    }
+    __syncthreads();
+    const int w = tid >> 6, l = tid & 63;
+    const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
+    d4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    const double *Aop = ops + (size_t)c * MRP * KRP + (size_t)row0 * KRP;
+    for (int kc = 0; kc < KRP; kc += KC) {
+        for (int e = tid; e < 64 * KC; e += 256) {
+            const int rr = e / KC, kk = e % KC;
+            sA[rr][kk] = Aop[(size_t)rr * KRP + kc + kk];
+        }
+        for (int e = tid; e < 64 * KC; e += 256) {
+            const int kk = e / 64, jj = e % 64;
+            const int k = kc + kk;
+            double v = 0.0;
+            const int a = sPA[jj];
+            if (a >= 0 && k < R4) {
+                const int f = k / R, s = k % R, m = sPM[jj];
+                v = M[(size_t)a * R4 + f * R + flip_node<P>(s, m)];
+                if (f < 3 && ((m >> f) & 1)) v = -v;
+            }
+            sB[kk][jj] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 4) {
+            double af[2], bf[2];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi) af[mi] = sA[wr + mi * 16 + (l & 15)][kk + (l >> 4)];
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni) bf[ni] = sB[kk + (l >> 4)][wc + ni * 16 + (l & 15)];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[mi], bf[ni], acc[mi][ni], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) {
+            const int col = wc + ni * 16 + (l & 15);
+            const int b = sPB[col];
+            if (b < 0) continue;
+            const int m = sPM[col];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                // v_mfma_f64_16x16x4f64 result: lane l, register q holds row (l >> 4) + 4q, column l & 15
+                const int row = row0 + wr + mi * 16 + (l >> 4) + 4 * q;
+                if (row >= R4) continue;
+                const int g = row / R, t = row % R;
+                double v = acc[mi][ni][q];
+                if (g > 0 && ((m >> (g - 1)) & 1)) v = -v;
+                unsafeAtomicAdd(&L[(size_t)b * R4 + g * R + flip_node<P>(t, m)], v);
+            }
+        }
+}
+
+template <int P>
+int build_ops_p(ibim_plan *pl) {
+    constexpr int R = P * P * P;
+    const int ncls = pl->n_cls;
+    if (ncls == 0) return IBIM_OK;
+    const int64_t total = (int64_t)R * R * ncls;
+    IBIM_HIP_CHECK(hipMemsetAsync(pl->ops, 0, (size_t)ncls * pl->MRP * pl->KRP * sizeof(double), pl->stream));
+    k_build_ops<P><<<nblk(total), 256, 0, pl->stream>>>(2, ncls, pl->d_lvl_rad, pl->tab, pl->kc, pl->MRP, pl->KRP,
+                                                        pl->ops);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
+template <int P>
+int gemm_p(ibim_plan *pl, const VPairs &vp) {
+    if (vp.nblocks == 0) return IBIM_OK;
+    dim3 grid(vp.nblocks, pl->MRP / 64);
+    k_m2l_gemm<P><<<grid, 256, 0, pl->stream>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, vp.mask, pl->ops,
+                                                pl->MRP, pl->KRP, pl->M, pl->L);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
+}  // namespace
+
+// operators of every (level >= 2, |o|) class, FP64
+int build_m2l_operators(ibim_plan *pl) {
+    const int R4 = 4 * pl->r;
+    pl->MRP = (R4 + 63) / 64 * 64;
+    pl->KRP = pl->MRP;
+    pl->n_cls = pl->nlevels > 2 ? (pl->nlevels - 2) * kNCanon : 0;
+    if (pl->n_cls == 0) return IBIM_OK;
+    int rc = dalloc(pl, &pl->ops, (size_t)pl->n_cls * pl->MRP * pl->KRP);
+    if (rc) return rc;
+    switch (pl->p) {
+    case 2: return build_ops_p<2>(pl);
+    case 3: return build_ops_p<3>(pl);
+    case 4: return build_ops_p<4>(pl);
+    case 5: return build_ops_p<5>(pl);
+    case 6: return build_ops_p<6>(pl);
+    case 7: return build_ops_p<7>(pl);
+    case 8: return build_ops_p<8>(pl);
+    }
+    return IBIM_ERR_ARG;
+}
+
+// V pairs of the boxes flagged in `need`, grouped by operator class
+int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vector<void *> *tmp_owner) {
+    hipStream_t st = pl->stream;
+    const int nn = pl->nnodes;
+    auto alloc = [&](auto **p, size_t cnt) -> int {
+        if (tmp_owner) {
+            hipError_t e = hipMalloc((void **)p, (cnt ? cnt : 1) * sizeof(**p));
+            if (e != hipSuccess) {
+                set_error("hipMalloc (V pairs)");
+                return IBIM_ERR_NOMEM;
+            }
+            tmp_owner->push_back((void *)*p);
+            return IBIM_OK;
+        }
+        return dalloc(pl, p, cnt);
+    };
+    vp = VPairs{};
+    if (pl->n_cls == 0) return IBIM_OK;
+    int *cnt = nullptr, *off = nullptr;
+    IBIM_HIP_CHECK(hipMalloc(&cnt, ((size_t)nn + 1) * sizeof(int)));
+    IBIM_HIP_CHECK(hipMalloc(&off, ((size_t)nn + 1) * sizeof(int)));
+    IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, ((size_t)nn + 1) * sizeof(int), st));
+    k_vpairs<0><<<nblk(nn), 256, 0, st>>>(nn, need, pl->loff[1], pl->litem[1], pl->flags, pl->level, pl->ctr,
+                                          pl->d_lvl_rad, cnt, nullptr, nullptr, nullptr, nullptr, nullptr);
+    IBIM_LAUNCH_CHECK();
+    size_t bytes = 0;
+    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt, off, nn + 1, st));
+    void *tmp = nullptr;
+    IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
+    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, cnt, off, nn + 1, st));
+    int np = 0;
+    IBIM_HIP_CHECK(hipMemcpyAsync(&np, off + nn, sizeof(int), hipMemcpyDeviceToHost, st));
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipFree(tmp);
+    vp.npairs = np;
+    int rc;
+    int *a0 = nullptr, *b0 = nullptr, *key0 = nullptr, *key1 = nullptr, *iota = nullptr, *order = nullptr;
+    unsigned char *m0 = nullptr;
+    if (np > 0) {
+        IBIM_HIP_CHECK(hipMalloc(&a0, (size_t)np * sizeof(int)));
+        IBIM_HIP_CHECK(hipMalloc(&b0, (size_t)np * sizeof(int)));
+        IBIM_HIP_CHECK(hipMalloc(&m0, (size_t)np));
+        IBIM_HIP_CHECK(hipMalloc(&key0, (size_t)np * sizeof(int)));
+        IBIM_HIP_CHECK(hipMalloc(&key1, (size_t)np * sizeof(int)));
+        IBIM_HIP_CHECK(hipMalloc(&iota, (size_t)np * sizeof(int)));
+        IBIM_HIP_CHECK(hipMalloc(&order, (size_t)np * sizeof(int)));
+        k_vpairs<1><<<nblk(nn), 256, 0, st>>>(nn, need, pl->loff[1], pl->litem[1], pl->flags, pl->level, pl->ctr,
+                                              pl->d_lvl_rad, nullptr, off, a0, b0, m0, key0);
+        hipcub::CountingInputIterator<int> it(0);
+        // iota via a copy of the counting iterator
+        IBIM_HIP_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, iota, cnt, np,
+                                                [] __device__(int) { return true; }, st));
+        IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
+        IBIM_HIP_CHECK(hipcub::DeviceSelect::If(tmp, bytes, it, iota, cnt, np, [] __device__(int) { return true; },
+                                                st));
+        IBIM_HIP_CHECK(hipStreamSynchronize(st));
+        (void)hipFree(tmp);
+        int endbit = 1;
+        while ((1 << endbit) <= pl->n_cls) ++endbit;
+        bytes = 0;
+        IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key0, key1, iota, order, np, 0, endbit, st));
+        IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
+        IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, key0, key1, iota, order, np, 0, endbit, st));
+        IBIM_HIP_CHECK(hipStreamSynchronize(st));
+        (void)hipFree(tmp);
+        if ((rc = alloc(&vp.A, np)) || (rc = alloc(&vp.B, np)) || (rc = alloc(&vp.mask, np))) return rc;
+        k_permute_pairs<<<nblk(np), 256, 0, st>>>(np, order, a0, b0, m0, vp.A, vp.B, vp.mask);
+        IBIM_LAUNCH_CHECK();
+    }
+    // class offsets and the GEMM block table, on the host
+    std::vector<int> hkey(np);
+    if (np > 0) IBIM_HIP_CHECK(hipMemcpyAsync(hkey.data(), key1, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, st));
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    vp.cls_off.assign(pl->n_cls + 1, 0);
+    for (int j = 0; j < np; ++j) vp.cls_off[hkey[j] + 1]++;
+    for (int c = 0; c < pl->n_cls; ++c) vp.cls_off[c + 1] += vp.cls_off[c];
+    std::vector<int> bc, bj;
+    for (int c = 0; c < pl->n_cls; ++c)
+        for (int j = vp.cls_off[c]; j < vp.cls_off[c + 1]; j += 64) {
+            bc.push_back(c);
+            bj.push_back(j);
+        }
+    vp.nblocks = (int)bc.size();
+    if ((rc = alloc(&vp.cls_off_d, pl->n_cls + 1)) || (rc = alloc(&vp.blk_cls, std::max(1, vp.nblocks))) ||
+        (rc = alloc(&vp.blk_j0, std::max(1, vp.nblocks))))
+        return rc;
+    IBIM_HIP_CHECK(hipMemcpyAsync(vp.cls_off_d, vp.cls_off.data(), (pl->n_cls + 1) * sizeof(int),
+                                  hipMemcpyHostToDevice, st));
+    if (vp.nblocks > 0) {
+        IBIM_HIP_CHECK(hipMemcpyAsync(vp.blk_cls, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        IBIM_HIP_CHECK(hipMemcpyAsync(vp.blk_j0, bj.data(), bj.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    for (void *p : {(void *)cnt, (void *)off, (void *)a0, (void *)b0, (void *)m0, (void *)key0, (void *)key1,
+                    (void *)iota, (void *)order})
+        if (p) (void)hipFree(p);
+    return IBIM_OK;
+}
+
+int m2l_gemm(ibim_plan *pl, const VPairs &vp) {
+    switch (pl->p) {
+    case 2: return gemm_p<2>(pl, vp);
+    case 3: return gemm_p<3>(pl, vp);
+    case 4: return gemm_p<4>(pl, vp);
+    case 5: return gemm_p<5>(pl, vp);
+    case 6: return gemm_p<6>(pl, vp);
+    case 7: return gemm_p<7>(pl, vp);
+    case 8: return gemm_p<8>(pl, vp);
+    }
+    return IBIM_ERR_ARG;
+}
+
+}  // namespace ibim

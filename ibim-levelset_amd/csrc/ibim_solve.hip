@@ -374,6 +374,12 @@ int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_
             tmp.push_back(F);
         }
     }
+    VPairs vp;
+    if (rc == IBIM_OK && pl->ops) {
+        rc = build_vpairs(pl, ts.need_local, vp, &tmp);
+        ts.vpairs = &vp;
+    }
+    if (rc == IBIM_OK) rc = build_p2p_ranges(pl, ts, &tmp);
     if (rc == IBIM_OK) rc = form_charges(pl, d_x);
     if (rc == IBIM_OK) rc = fmm_forward(pl, ts, F, m);
     if (rc == IBIM_OK) {

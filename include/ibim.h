@@ -84,6 +84,8 @@ typedef struct ibim_tree_stats {
     int64_t n_u, n_v, n_w, n_x;      /* total list lengths (all nodes) */
     int64_t p2p_pairs;    /* sum over target leaves of nT * sum_{U,W} nS */
     int64_t m2l_pairs;    /* non-empty V + X interactions into nodes with targets */
+    int64_t v_pairs;      /* the V part of m2l_pairs */
+    int64_t x_pairs;      /* the X part of m2l_pairs */
 } ibim_tree_stats;
 
 typedef struct ibim_solve_info {
@@ -111,14 +113,15 @@ int64_t ibim_plan_footprint(const ibim_plan *plan);
  * counts per stage (IBIM_STAGE_COUNT entries), optionally resetting them. */
 #define IBIM_STAGE_P2M 0
 #define IBIM_STAGE_M2M 1
-#define IBIM_STAGE_M2L 2
+#define IBIM_STAGE_M2L 2      /* V list (GEMM) or all of M2L (direct form) */
 #define IBIM_STAGE_L2L 3
 #define IBIM_STAGE_P2P 4      /* near field + L2P */
 #define IBIM_STAGE_MATVEC 5   /* one whole matvec */
 #define IBIM_STAGE_ORTH 6     /* GMRES orthogonalisation */
 #define IBIM_STAGE_RHS 7
 #define IBIM_STAGE_POLARIZE 8
-#define IBIM_STAGE_COUNT 9
+#define IBIM_STAGE_M2L_X 9    /* X list, direct, when the V list runs as GEMM */
+#define IBIM_STAGE_COUNT 10
 int ibim_plan_profile(ibim_plan *plan, int enable);
 int ibim_plan_stage_times(ibim_plan *plan, double *ms, int64_t *launches, int reset);
 

@@ -2,8 +2,9 @@
 restated in oracle/ibim_oracle.c), called through the C ABI.
 
 Bars: tree boxes, point sets and interaction lists bit-exact; FP64 FMM and
-matvec results within 1e-12 relative L2 of the oracle (the two differ only
-in summation order); FMM-exact polynomial kernels within 1e-12 of the direct
+matvec results within 1e-10 relative L2 of the oracle (the parity tolerance
+for the FP64 path; the two differ only in summation order, typically
+1e-14..1e-12); FMM-exact polynomial kernels within 1e-10 of the direct
 O(N^2) sum."""
 import numpy as np
 import pytest
@@ -13,7 +14,7 @@ import geometry as G
 pytestmark = pytest.mark.gpu
 
 PB = dict(kappa=0.1257, eps_in=2.0, eps_out=80.0, vacant_radius=0.05)
-TOL = 1e-12
+TOL = 1e-10
 
 
 def rel(a, b):
