@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--config", default="globule500", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (solves of different molecules "
@@ -129,10 +130,10 @@ def main():
     P = lambda t: C.c_void_p(t.data_ptr())
     m = cfg["atoms"]
 
-    def solve(profile=False):
+    def solve(profile=False, precision=ibim.PREC_FP64):
         plan = ibim.Plan(device_ptrs=ptrs, p=cfg["p"], max_points=cfg["leaf"], max_level=cfg["max_level"],
                          kappa=cfg["kappa"], eps_in=cfg["eps_in"], eps_out=cfg["eps_out"],
-                         vacant_radius=cfg["tau"] * cfg["h"])
+                         vacant_radius=cfg["tau"] * cfg["h"], precision=precision)
         if profile:
             plan.profile(True)
         plan.rhs_device(m, P(d_atoms), P(d_q), P(d_b))
@@ -159,6 +160,26 @@ def main():
         lat.append(time.perf_counter() - ts)
     torch.cuda.synchronize()
     total = time.perf_counter() - t_start
+
+    # the mixed-precision form of the same solve (FP32 far and near field
+    # inside FP64 iterative refinement), timed the same way
+    mixed = None
+    if not args.no_mixed:
+        for _ in range(args.warmup):
+            solve(precision=ibim.PREC_MIXED)
+        torch.cuda.synchronize()
+        lat_m = []
+        tm0 = time.perf_counter()
+        for _ in range(args.steps):
+            ts = time.perf_counter()
+            rc_m, info_m, energy_m, _, _, _ = solve(precision=ibim.PREC_MIXED)
+            lat_m.append(time.perf_counter() - ts)
+        total_m = time.perf_counter() - tm0
+        lat_m = sorted(1e3 * x for x in lat_m)
+        mixed = {"solves_per_s": args.steps / total_m, "speedup_vs_fp64": (args.steps / total_m) / (args.steps / total),
+                 "energy": energy_m, "rel_energy_err_vs_fp64": abs(energy_m - energy) / abs(energy),
+                 "latency_ms": {"p50": float(np.percentile(lat_m, 50)), "p99": float(np.percentile(lat_m, 99))},
+                 "solve": info_m}
 
     # one more solve with stage timers on (outside the timed region)
     rc_p, info_p, energy_p, stages, stats, _ = solve(profile=True)
@@ -205,7 +226,7 @@ def main():
                   "converged": info["converged"], "energy": energy, "plan_bytes": foot},
         "matvecs_per_s": info["matvecs"] * value, "tube_points_per_s": n * info["matvecs"] * value,
         "stage_ms_per_solve": stage_ms, "tree": stats, "mesh_s": t_mesh,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "mixed": mixed, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

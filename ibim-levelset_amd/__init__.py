@@ -31,7 +31,7 @@ KID = {k: i for i, k in enumerate(KERNELS)}
 PREC_FP64, PREC_MIXED = 0, 1
 KERNEL_PB, KERNEL_POLY = 0, 1
 IBIM_OK, IBIM_NOT_CONVERGED = 0, 1
-STAGES = ["P2M", "M2M", "M2L", "L2L", "P2P", "MATVEC", "ORTH", "RHS", "POLARIZE", "M2L_X"]
+STAGES = ["P2M", "M2M", "M2L", "L2L", "P2P", "MATVEC", "ORTH", "RHS", "POLARIZE", "M2L_X", "MATVEC_MIXED"]
 
 
 class IbimError(RuntimeError):

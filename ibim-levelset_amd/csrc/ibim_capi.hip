@@ -10,14 +10,15 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 
 int kernel_fmm_device(ibim_plan *pl, int kid, const double *d_q, double *d_out);
 int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_x, double *d_out);
+int target_fmm_device(ibim_plan *pl, int kid, int m, const double *d_axyz, const double *d_in, double *d_out);
 
 // one dmapping (electric.cpp:294-388): charges, fused FMM, combine
-int matvec_device(ibim_plan *pl, const double *d_x, double *d_y) {
-    StageScope sc(pl, kStMatvec);
+int matvec_device(ibim_plan *pl, const double *d_x, double *d_y, bool mixed) {
+    StageScope sc(pl, mixed ? kStMatvecMixed : kStMatvec);
     int rc = form_charges(pl, d_x);
     if (rc) return rc;
     TargetSet ts = source_targets(pl);
-    if ((rc = fmm_forward(pl, ts, pl->F, pl->n))) return rc;
+    if ((rc = fmm_forward(pl, ts, pl->F, pl->n, mixed))) return rc;
     return combine(pl, d_x, d_y);
 }
 
@@ -156,6 +157,10 @@ int ibim_plan_create_device(const ibim_params *prm, int64_t n, const double *d_x
         ibim_plan_destroy(pl);
         return rc;
     }
+    if (pl->prm.precision == IBIM_PREC_MIXED && (rc = build_m2l_operators_f32(pl))) {
+        ibim_plan_destroy(pl);
+        return rc;
+    }
     {
         TargetSet ts = source_targets(pl);
         if ((rc = build_p2p_ranges(pl, ts, nullptr))) {
@@ -228,6 +233,7 @@ int ibim_plan_set_precision(ibim_plan *pl, int precision) {
         return IBIM_ERR_ARG;
     }
     pl->prm.precision = precision;
+    if (precision == IBIM_PREC_MIXED) return build_m2l_operators_f32(pl);
     return IBIM_OK;
 }
 
@@ -275,7 +281,7 @@ int ibim_matvec_device(ibim_plan *pl, const double *d_x, double *d_y) {
         set_error("null argument");
         return IBIM_ERR_ARG;
     }
-    return matvec_device(pl, d_x, d_y);
+    return matvec_device(pl, d_x, d_y, pl->prm.precision == IBIM_PREC_MIXED);
 }
 
 int ibim_matvec(ibim_plan *pl, const double *x, double *y) {
@@ -284,7 +290,9 @@ int ibim_matvec(ibim_plan *pl, const double *x, double *y) {
         return IBIM_ERR_ARG;
     }
     const size_t n2 = 2 * (size_t)pl->n;
-    return with_host_vectors(pl, n2, x, n2, y, [&](const double *dx, double *dy) { return matvec_device(pl, dx, dy); });
+    return with_host_vectors(pl, n2, x, n2, y, [&](const double *dx, double *dy) {
+        return matvec_device(pl, dx, dy, pl->prm.precision == IBIM_PREC_MIXED);
+    });
 }
 
 int ibim_kernel_fmm(ibim_plan *pl, int kernel_id, const double *charge, double *potential) {
@@ -295,6 +303,21 @@ int ibim_kernel_fmm(ibim_plan *pl, int kernel_id, const double *charge, double *
     const size_t n = (size_t)pl->n;
     return with_host_vectors(pl, n, charge, n, potential,
                              [&](const double *dq, double *dp) { return kernel_fmm_device(pl, kernel_id, dq, dp); });
+}
+
+int ibim_kernel_fmm_targets(ibim_plan *pl, int kernel_id, int64_t m, const double *txyz, const double *charge,
+                            double *potential) {
+    if (!pl || m < 1 || !txyz || !charge || !potential || kernel_id < 0 || kernel_id > 15) {
+        set_error("bad argument to ibim_kernel_fmm_targets");
+        return IBIM_ERR_ARG;
+    }
+    const size_t n = (size_t)pl->n;
+    std::vector<double> packed(3 * (size_t)m + n);
+    std::memcpy(packed.data(), txyz, 3 * (size_t)m * sizeof(double));
+    std::memcpy(packed.data() + 3 * m, charge, n * sizeof(double));
+    return with_host_vectors(pl, packed.size(), packed.data(), (size_t)m, potential, [&](const double *d, double *dp) {
+        return target_fmm_device(pl, kernel_id, (int)m, d, d + 3 * m, dp);
+    });
 }
 
 int ibim_rhs_device(ibim_plan *pl, int64_t m, const double *d_axyz, const double *d_aq, double *d_b) {

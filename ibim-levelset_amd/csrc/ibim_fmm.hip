@@ -449,7 +449,7 @@ __global__ void k_p2p_ranges(int nleaves, const int *__restrict__ tleaves, const
 // each lane walks its slice of the leaf's merged source ranges, the slices
 // are summed with cross-lane shuffles, and the first slice adds the far
 // field (L2P) from the leaf's local expansion staged in LDS.
-template <int P>
+template <int P, bool MIXED>
 __global__ __launch_bounds__(256) void k_p2p_wave(
     int nleaves, const int *__restrict__ tleaves, const int *__restrict__ tbeg, const int *__restrict__ tend,
     const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
@@ -482,18 +482,37 @@ __global__ __launch_bounds__(256) void k_p2p_wave(
         const int t = t0 + mt;
         const double px = valid ? tx[t] : 0.0, py = valid ? ty[t] : 0.0, pz = valid ? tz[t] : 0.0;
         double f0 = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+        KahanF k0, k1, k2, k3;
         int base = 0, j = sl;
         for (int r = r0; r < r1; ++r) {
             const int2 rg = ranges[r];
             const int len = rg.y - rg.x;
             while (j - base < len) {
                 const int s = rg.x + (j - base);
-                if (valid)
-                    pair_accum<double>(sx[s] - px, sy[s] - py, sz[s] - pz, Q[s], Q[(size_t)ns + s],
-                                       Q[2 * (size_t)ns + s], Q[3 * (size_t)ns + s], kc, f0, fx, fy, fz);
+                if (valid) {
+                    if (MIXED) {
+                        float g0, gx, gy, gz;
+                        pair_accum_f32((float)(sx[s] - px), (float)(sy[s] - py), (float)(sz[s] - pz), (float)Q[s],
+                                       (float)Q[(size_t)ns + s], (float)Q[2 * (size_t)ns + s],
+                                       (float)Q[3 * (size_t)ns + s], kc, g0, gx, gy, gz);
+                        k0.add(g0);
+                        k1.add(gx);
+                        k2.add(gy);
+                        k3.add(gz);
+                    } else {
+                        pair_accum<double>(sx[s] - px, sy[s] - py, sz[s] - pz, Q[s], Q[(size_t)ns + s],
+                                           Q[2 * (size_t)ns + s], Q[3 * (size_t)ns + s], kc, f0, fx, fy, fz);
+                    }
+                }
                 j += nsl;
             }
             base += len;
+        }
+        if (MIXED) {
+            f0 = k0.value();
+            fx = k1.value();
+            fy = k2.value();
+            fz = k3.value();
         }
         for (int o = tp; o < 64; o <<= 1) {
             f0 += __shfl_xor(f0, o);
@@ -573,7 +592,7 @@ __global__ void k_scatter_field(int n, const int *__restrict__ perm, const doubl
 }
 
 template <int P>
-int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt) {
+int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mixed) {
     constexpr int R = P * P * P;
     constexpr int BS = R <= 64 ? 64 : (R <= 128 ? 128 : 256);
     hipStream_t st = pl->stream;
@@ -607,7 +626,7 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt) {
     }
     if (ts.vpairs) {
         StageScope sc(pl, kStM2L);
-        int rc = m2l_gemm(pl, *ts.vpairs);
+        int rc = m2l_gemm(pl, *ts.vpairs, mixed);
         if (rc) return rc;
     }
     {
@@ -620,11 +639,14 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt) {
     }
     {
     StageScope sc(pl, kStP2P);
-    if (ts.nleaves > 0 && ts.roff)
-        k_p2p_wave<P><<<(ts.nleaves + 3) / 4, 256, 0, st>>>(ts.nleaves, ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z,
-                                                             pl->sx, pl->sy, pl->sz, pl->Q, pl->n, ts.roff,
-                                                             ts.ranges, pl->ctr, pl->level, pl->d_lvl_rad, pl->tab,
-                                                             pl->kc, pl->L, F, nt);
+    if (ts.nleaves > 0 && ts.roff && mixed)
+        k_p2p_wave<P, true><<<(ts.nleaves + 3) / 4, 256, 0, st>>>(
+            ts.nleaves, ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q, pl->n, ts.roff,
+            ts.ranges, pl->ctr, pl->level, pl->d_lvl_rad, pl->tab, pl->kc, pl->L, F, nt);
+    else if (ts.nleaves > 0 && ts.roff)
+        k_p2p_wave<P, false><<<(ts.nleaves + 3) / 4, 256, 0, st>>>(
+            ts.nleaves, ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q, pl->n, ts.roff,
+            ts.ranges, pl->ctr, pl->level, pl->d_lvl_rad, pl->tab, pl->kc, pl->L, F, nt);
     else if (ts.nleaves > 0)
         k_p2p_l2p<P><<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->sx, pl->sy,
                                                 pl->sz, pl->Q, pl->n, pl->loff[0], pl->litem[0], pl->loff[2],
@@ -660,7 +682,7 @@ int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, std::vector<void *> *tmp_owne
     };
     IBIM_HIP_CHECK(hipMalloc(&cnt, ((size_t)nl + 1) * sizeof(int)));
     IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, ((size_t)nl + 1) * sizeof(int), st));
-    k_p2p_ranges<0><<<nblk(nl), 64, 0, st>>>(nl, ts.leaves, pl->loff[0], pl->litem[0], pl->loff[2], pl->litem[2],
+    k_p2p_ranges<0><<<nblk(nl, 64), 64, 0, st>>>(nl, ts.leaves, pl->loff[0], pl->litem[0], pl->loff[2], pl->litem[2],
                                              pl->flags, pl->sbeg, pl->send, cnt, nullptr, nullptr);
     IBIM_LAUNCH_CHECK();
     std::vector<int> hc(nl);
@@ -677,7 +699,7 @@ int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, std::vector<void *> *tmp_owne
     int rc;
     if ((rc = alloc(&roff, (size_t)nl + 1)) || (rc = alloc(&ranges, (size_t)std::max(1, ho[nl])))) return rc;
     IBIM_HIP_CHECK(hipMemcpyAsync(roff, ho.data(), ho.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    k_p2p_ranges<1><<<nblk(nl), 64, 0, st>>>(nl, ts.leaves, pl->loff[0], pl->litem[0], pl->loff[2], pl->litem[2],
+    k_p2p_ranges<1><<<nblk(nl, 64), 64, 0, st>>>(nl, ts.leaves, pl->loff[0], pl->litem[0], pl->loff[2], pl->litem[2],
                                              pl->flags, pl->sbeg, pl->send, nullptr, roff, ranges);
     IBIM_LAUNCH_CHECK();
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
@@ -725,15 +747,15 @@ int build_chebyshev(ibim_plan *pl) {
     return IBIM_OK;
 }
 
-int fmm_forward(ibim_plan *pl, const TargetSet &ts, double *F, int nt) {
+int fmm_forward(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mixed) {
     switch (pl->p) {
-    case 2: return fmm_forward_p<2>(pl, ts, F, nt);
-    case 3: return fmm_forward_p<3>(pl, ts, F, nt);
-    case 4: return fmm_forward_p<4>(pl, ts, F, nt);
-    case 5: return fmm_forward_p<5>(pl, ts, F, nt);
-    case 6: return fmm_forward_p<6>(pl, ts, F, nt);
-    case 7: return fmm_forward_p<7>(pl, ts, F, nt);
-    case 8: return fmm_forward_p<8>(pl, ts, F, nt);
+    case 2: return fmm_forward_p<2>(pl, ts, F, nt, mixed);
+    case 3: return fmm_forward_p<3>(pl, ts, F, nt, mixed);
+    case 4: return fmm_forward_p<4>(pl, ts, F, nt, mixed);
+    case 5: return fmm_forward_p<5>(pl, ts, F, nt, mixed);
+    case 6: return fmm_forward_p<6>(pl, ts, F, nt, mixed);
+    case 7: return fmm_forward_p<7>(pl, ts, F, nt, mixed);
+    case 8: return fmm_forward_p<8>(pl, ts, F, nt, mixed);
     }
     set_error("unsupported Chebyshev order");
     return IBIM_ERR_ARG;
@@ -774,10 +796,25 @@ TargetSet source_targets(ibim_plan *pl) {
 
 // single kernel FMM: which charge field feeds it, which output field holds
 // it and with which sign (see pair_accum for the fused combination)
+static const int kFin[16] = {0, 1, 2, 3, 0, 1, 2, 0, 1, 2, 0, 1, 2, 3, 3, 3};
+static const int kFout[16] = {0, 0, 0, 0, 1, 2, 3, 2, 3, 1, 2, 3, 1, 1, 2, 3};
+static const double kSgn[16] = {1, 1, 1, -1, 1, 1, 1, 1, 1, 1, 1, 1, 1, -1, -1, -1};
+
+int single_charge_device(ibim_plan *pl, int kid, const double *d_q) {
+    k_single_charge<<<nblk(pl->n), 256, 0, pl->stream>>>(pl->n, pl->perm, d_q, kFin[kid], pl->Q);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
+int scatter_field_device(ibim_plan *pl, int kid, const int *perm, const double *F, int nt, double *d_out) {
+    k_scatter_field<<<nblk(nt), 256, 0, pl->stream>>>(nt, perm, F, kFout[kid], kSgn[kid], d_out);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
 int kernel_fmm_device(ibim_plan *pl, int kid, const double *d_q, double *d_out) {
-    static const int fin[16] = {0, 1, 2, 3, 0, 1, 2, 0, 1, 2, 0, 1, 2, 3, 3, 3};
-    static const int fout[16] = {0, 0, 0, 0, 1, 2, 3, 2, 3, 1, 2, 3, 1, 1, 2, 3};
-    static const double sgn[16] = {1, 1, 1, -1, 1, 1, 1, 1, 1, 1, 1, 1, 1, -1, -1, -1};
+    const int *fin = kFin, *fout = kFout;
+    const double *sgn = kSgn;
     if (kid < 0 || kid > 15) {
         set_error("kernel_id out of range");
         return IBIM_ERR_ARG;

@@ -117,6 +117,58 @@ __device__ __forceinline__ void pair_accum(T dx, T dy, T dz, T qx, T qy, T qz, T
     fz += a3 * qz + dz * s;
 }
 
+// FP32 form of pair_accum for the mixed-precision near field: the caller
+// forms d = s - t in FP64 (no cancellation of absolute coordinates) and
+// rounds it; reciprocal square root and exp use the hardware FP32 units.
+__device__ __forceinline__ void pair_accum_f32(float dx, float dy, float dz, float qx, float qy, float qz, float qp,
+                                               const PBConst &k, float &f0, float &fx, float &fy, float &fz) {
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (k.mode == IBIM_KERNEL_POLY) {
+        const float dq = dx * qx + dy * qy + dz * qz;
+        f0 = dq - (0.5f + 0.25f * r2) * qp;
+        const float s = 0.5f * dq + 0.75f * qp;
+        fx = qx + dx * s;
+        fy = qy + dy * s;
+        fz = qz + dz * s;
+        return;
+    }
+    const float ir = rsqrtf(r2);
+    const float r = r2 * ir;
+    if (!(r >= (float)k.vr)) {  // also catches r2 == 0
+        f0 = -(float)k.k2reg * qp;
+        fx = fy = fz = 0.0f;
+        return;
+    }
+    const float ir2 = ir * ir, ir3 = ir2 * ir;
+    const float t = (float)k.kappa * r;
+    const float e = __expf(-t);
+    const float et1 = e * (t + 1.0f);
+    const float c4 = (float)k.inv4pi * ir3;
+    const float g1 = ((float)k.eps * et1 - 1.0f) * c4;
+    const float k2 = (1.0f - e) * ir * (float)k.inv4pi;
+    const float a3 = (1.0f - et1) * ir3;
+    const float b3 = (e * ((t + 3.0f) * t + 3.0f) - 3.0f) * ir3 * ir2;
+    const float g4 = ((float)k.inv_eps * et1 - 1.0f) * c4;
+    const float dq = dx * qx + dy * qy + dz * qz;
+    f0 = g1 * dq - k2 * qp;
+    const float s = b3 * dq + g4 * qp;
+    fx = a3 * qx + dx * s;
+    fy = a3 * qy + dy * s;
+    fz = a3 * qz + dz * s;
+}
+
+// Kahan-compensated FP32 accumulator
+struct KahanF {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float v) {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+};
+
 // S_p(u, xbar_m) for m < P at one standardized coordinate u (bbfmm.h:408-419)
 template <int P>
 __device__ __forceinline__ void sp_values(double u, const ChebTab &tab, double *out) {
@@ -135,7 +187,10 @@ __device__ __forceinline__ void sp_values(double u, const ChebTab &tab, double *
 }
 
 // stage timers: HIP events recorded on the plan's stream around each stage
-enum Stage { kStP2M = 0, kStM2M, kStM2L, kStL2L, kStP2P, kStMatvec, kStOrth, kStRhs, kStPolarize, kStM2LX, kStCount };
+enum Stage {
+    kStP2M = 0, kStM2M, kStM2L, kStL2L, kStP2P, kStMatvec, kStOrth, kStRhs, kStPolarize, kStM2LX, kStMatvecMixed,
+    kStCount
+};
 
 struct StageTimer {
     bool on = false;
@@ -231,6 +286,7 @@ struct ibim_plan {
     double *gm_V = nullptr;              // (restart+1) * 2n
     double *gm_w = nullptr, *gm_r = nullptr, *gm_x = nullptr, *gm_b = nullptr;
     double *gm_part = nullptr;           // partial sums
+    double *gm_rr = nullptr, *gm_d = nullptr;  // refinement residual and correction
     double *gm_small = nullptr;          // H, s, cs, sn, scalars on device
 
     ibim::StageTimer timer;
@@ -238,6 +294,7 @@ struct ibim_plan {
     // V-list M2L operators: n_cls classes of MRP x KRP (row major), FP64
     int n_cls = 0, MRP = 0, KRP = 0;
     double *ops = nullptr;
+    float *ops32 = nullptr;              // FP32 copy for the mixed-precision matvec
     ibim::VPairs vp;                     // V pairs for the points as their own targets
     const int *p2p_roff = nullptr;       // merged near-field ranges of the non-empty leaves
     const int2 *p2p_ranges = nullptr;
@@ -269,7 +326,7 @@ int dalloc(ibim_plan *pl, T **ptr, size_t count) {
 // stages implemented in the .hip files
 int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const double *d_w);
 int build_chebyshev(ibim_plan *pl);
-int fmm_forward(ibim_plan *pl, const TargetSet &ts, double *F, int nt);
+int fmm_forward(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mixed = false);
 int form_charges(ibim_plan *pl, const double *d_x);
 int combine(ibim_plan *pl, const double *d_x, double *d_y);
 int make_target_set(ibim_plan *pl, int m, const double *d_txyz, TargetSet &ts, std::vector<void *> &tmp,
@@ -281,7 +338,8 @@ int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int
 int compute_stats(ibim_plan *pl);
 int build_m2l_operators(ibim_plan *pl);
 int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vector<void *> *tmp_owner);
-int m2l_gemm(ibim_plan *pl, const VPairs &vp);
+int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed);
+int build_m2l_operators_f32(ibim_plan *pl);
 int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, std::vector<void *> *tmp_owner);
 
 // scoped stage timing on the plan's stream (no-op when profiling is off)

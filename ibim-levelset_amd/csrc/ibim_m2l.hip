@@ -159,18 +159,41 @@ __device__ __forceinline__ int flip_node(int s, int m) {
     return (k * P + j) * P + i;
 }
 
-// one (class, 64 pairs, 64 rows) tile per block
-template <int P>
+// one (class, 64 pairs, 64 rows) tile per block; T = double (FP64 MFMA) or
+// float (FP32 MFMA, FP32 operators, multipoles rounded to FP32 on load)
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // result register q of lane l holds row (l >> 4) + 4q
+    static __device__ __forceinline__ int row(int l, int q) { return (l >> 4) + 4 * q; }
+};
+template <>
+struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // result register q of lane l holds row 4 (l >> 4) + q
+    static __device__ __forceinline__ int row(int l, int q) { return 4 * (l >> 4) + q; }
+};
+
+template <int P, typename T>
 __global__ __launch_bounds__(256) void k_m2l_gemm(const int *__restrict__ blk_cls, const int *__restrict__ blk_j0,
                                                   const int *__restrict__ cls_off, const int *__restrict__ pA,
                                                   const int *__restrict__ pB, const unsigned char *__restrict__ pm,
-                                                  const double *__restrict__ ops, int MRP, int KRP,
+                                                  const T *__restrict__ ops, int MRP, int KRP,
                                                   const double *__restrict__ M, double *__restrict__ L) {
     constexpr int R = P * P * P;
     constexpr int R4 = 4 * R;
     constexpr int KC = 32;
-    __shared__ double sA[64][KC + 1];
-    __shared__ double sB[KC][64 + 1];
+    typedef typename Mfma<T>::v4 v4;
+    __shared__ T sA[64][KC + 1];
+    __shared__ T sB[KC][64 + 1];
     __shared__ int sPA[64], sPB[64], sPM[64];
     const int tid = threadIdx.x;
     const int c = blk_cls[blockIdx.x];
@@ -187,12 +210,12 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(const int *__restrict__ blk_cl
     __syncthreads();
     const int w = tid >> 6, l = tid & 63;
     const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
-    d4 acc[2][2];
+    v4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-    const double *Aop = ops + (size_t)c * MRP * KRP + (size_t)row0 * KRP;
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4{T(0), T(0), T(0), T(0)};
+    const T *Aop = ops + (size_t)c * MRP * KRP + (size_t)row0 * KRP;
     for (int kc = 0; kc < KRP; kc += KC) {
         for (int e = tid; e < 64 * KC; e += 256) {
             const int rr = e / KC, kk = e % KC;
@@ -208,12 +231,12 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(const int *__restrict__ blk_cl
                 v = M[(size_t)a * R4 + f * R + flip_node<P>(s, m)];
                 if (f < 3 && ((m >> f) & 1)) v = -v;
             }
-            sB[kk][jj] = v;
+            sB[kk][jj] = (T)v;
         }
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < KC; kk += 4) {
-            double af[2], bf[2];
+            T af[2], bf[2];
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi) af[mi] = sA[wr + mi * 16 + (l & 15)][kk + (l >> 4)];
 #pragma unroll
@@ -221,8 +244,7 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(const int *__restrict__ blk_cl
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-                for (int ni = 0; ni < 2; ++ni)
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[mi], bf[ni], acc[mi][ni], 0, 0, 0);
+                for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = Mfma<T>::op(af[mi], bf[ni], acc[mi][ni]);
         }
         __syncthreads();
     }
@@ -236,15 +258,19 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(const int *__restrict__ blk_cl
             const int m = sPM[col];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                // v_mfma_f64_16x16x4f64 result: lane l, register q holds row (l >> 4) + 4q, column l & 15
-                const int row = row0 + wr + mi * 16 + (l >> 4) + 4 * q;
+                const int row = row0 + wr + mi * 16 + Mfma<T>::row(l, q);
                 if (row >= R4) continue;
                 const int g = row / R, t = row % R;
-                double v = acc[mi][ni][q];
+                double v = (double)acc[mi][ni][q];
                 if (g > 0 && ((m >> (g - 1)) & 1)) v = -v;
                 unsafeAtomicAdd(&L[(size_t)b * R4 + g * R + flip_node<P>(t, m)], v);
             }
         }
+}
+
+__global__ void k_to_float(const double *__restrict__ a, int64_t n, float *__restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
 }
 
 template <int P>
@@ -261,11 +287,15 @@ int build_ops_p(ibim_plan *pl) {
 }
 
 template <int P>
-int gemm_p(ibim_plan *pl, const VPairs &vp) {
+int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
     if (vp.nblocks == 0) return IBIM_OK;
     dim3 grid(vp.nblocks, pl->MRP / 64);
-    k_m2l_gemm<P><<<grid, 256, 0, pl->stream>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, vp.mask, pl->ops,
-                                                pl->MRP, pl->KRP, pl->M, pl->L);
+    if (mixed && pl->ops32)
+        k_m2l_gemm<P, float><<<grid, 256, 0, pl->stream>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, vp.mask,
+                                                           pl->ops32, pl->MRP, pl->KRP, pl->M, pl->L);
+    else
+        k_m2l_gemm<P, double><<<grid, 256, 0, pl->stream>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, vp.mask,
+                                                            pl->ops, pl->MRP, pl->KRP, pl->M, pl->L);
     IBIM_LAUNCH_CHECK();
     return IBIM_OK;
 }
@@ -392,17 +422,28 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
     return IBIM_OK;
 }
 
-int m2l_gemm(ibim_plan *pl, const VPairs &vp) {
+int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed) {
     switch (pl->p) {
-    case 2: return gemm_p<2>(pl, vp);
-    case 3: return gemm_p<3>(pl, vp);
-    case 4: return gemm_p<4>(pl, vp);
-    case 5: return gemm_p<5>(pl, vp);
-    case 6: return gemm_p<6>(pl, vp);
-    case 7: return gemm_p<7>(pl, vp);
-    case 8: return gemm_p<8>(pl, vp);
+    case 2: return gemm_p<2>(pl, vp, mixed);
+    case 3: return gemm_p<3>(pl, vp, mixed);
+    case 4: return gemm_p<4>(pl, vp, mixed);
+    case 5: return gemm_p<5>(pl, vp, mixed);
+    case 6: return gemm_p<6>(pl, vp, mixed);
+    case 7: return gemm_p<7>(pl, vp, mixed);
+    case 8: return gemm_p<8>(pl, vp, mixed);
     }
     return IBIM_ERR_ARG;
+}
+
+// FP32 copy of the operators for the mixed-precision matvec
+int build_m2l_operators_f32(ibim_plan *pl) {
+    if (pl->ops32 || !pl->ops || pl->n_cls == 0) return IBIM_OK;
+    const int64_t n = (int64_t)pl->n_cls * pl->MRP * pl->KRP;
+    int rc = dalloc(pl, &pl->ops32, (size_t)n);
+    if (rc) return rc;
+    k_to_float<<<nblk(n), 256, 0, pl->stream>>>(pl->ops, n, pl->ops32);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
 }
 
 }  // namespace ibim

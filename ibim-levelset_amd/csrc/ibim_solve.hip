@@ -15,7 +15,7 @@
 namespace ibim {
 
 int kernel_fmm_device(ibim_plan *pl, int kid, const double *d_q, double *d_out);
-int matvec_device(ibim_plan *pl, const double *d_x, double *d_y);
+int matvec_device(ibim_plan *pl, const double *d_x, double *d_y, bool mixed);
 
 namespace {
 
@@ -358,7 +358,11 @@ int make_target_set(ibim_plan *pl, int m, const double *d_txyz, TargetSet &ts, s
     return IBIM_OK;
 }
 
-int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_x, double *d_out) {
+int single_charge_device(ibim_plan *pl, int kid, const double *d_q);
+int scatter_field_device(ibim_plan *pl, int kid, const int *perm, const double *F, int nt, double *d_out);
+
+// polarizeMap (kid < 0) or one kernel FMM (kid = 0..15) at m target points
+int target_fmm_device(ibim_plan *pl, int kid, int m, const double *d_axyz, const double *d_in, double *d_out) {
     StageScope sc(pl, kStPolarize);
     std::vector<void *> tmp;
     TargetSet ts;
@@ -380,14 +384,15 @@ int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_
         ts.vpairs = &vp;
     }
     if (rc == IBIM_OK) rc = build_p2p_ranges(pl, ts, &tmp);
-    if (rc == IBIM_OK) rc = form_charges(pl, d_x);
+    if (rc == IBIM_OK) rc = kid < 0 ? form_charges(pl, d_in) : single_charge_device(pl, kid, d_in);
     if (rc == IBIM_OK) rc = fmm_forward(pl, ts, F, m);
     if (rc == IBIM_OK) {
-        k_scatter_neg_f0<<<nblk(m), 256, 0, pl->stream>>>(m, tperm, F, d_out);
+        if (kid < 0) k_scatter_neg_f0<<<nblk(m), 256, 0, pl->stream>>>(m, tperm, F, d_out);
+        else rc = scatter_field_device(pl, kid, tperm, F, m, d_out);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
         if (e != hipSuccess) {
-            set_error(std::string("polarize: ") + hipGetErrorString(e));
+            set_error(std::string("target FMM: ") + hipGetErrorString(e));
             rc = IBIM_ERR_HIP;
         }
     } else {
@@ -395,6 +400,10 @@ int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_
     }
     for (void *p : tmp) (void)hipFree(p);
     return rc;
+}
+
+int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_x, double *d_out) {
+    return target_fmm_device(pl, -1, m, d_axyz, d_x, d_out);
 }
 
 static int gmres_alloc(ibim_plan *pl, int restart) {
@@ -411,9 +420,10 @@ static int gmres_alloc(ibim_plan *pl, int restart) {
     return IBIM_OK;
 }
 
-// restarted GMRES, gmres.cpp:50-164, all vectors in HBM
-int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int maxit, double tol,
-                 ibim_solve_info *info) {
+// restarted GMRES, gmres.cpp:50-164, all vectors in HBM; `mixed` selects
+// the mixed-precision matvec for every operator application
+static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart, int maxit, double tol,
+                      ibim_solve_info *info, bool mixed) {
     if (restart < 1 || restart > 120 || maxit < 0) {
         set_error("gmres: restart must be in 1..120 and maxit >= 0");
         return IBIM_ERR_ARG;
@@ -439,7 +449,7 @@ int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int
     double normb = 0.0;
     IBIM_HIP_CHECK(hipMemcpyAsync(&normb, scal + 2, sizeof(double), hipMemcpyDeviceToHost, st));
     // r = b - A x
-    if ((rc = matvec_device(pl, d_x, p))) return rc;
+    if ((rc = matvec_device(pl, d_x, p, mixed))) return rc;
     nmv++;
     k_residual<<<kRG, kRB, 0, st>>>(d_b, p, r, n2, partA);
     k_sqrt_sum<<<1, kRB, 0, st>>>(partA, scal);
@@ -461,7 +471,7 @@ int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int
             IBIM_LAUNCH_CHECK();
             for (i = 0; i < m && j <= maxit; i++, j++) {
                 double *vi = V + (size_t)i * ldv;
-                if ((rc = matvec_device(pl, vi, w))) return rc;
+                if ((rc = matvec_device(pl, vi, w, mixed))) return rc;
                 nmv++;
                 inner++;
                 double *hcol = H + (size_t)i * ld;
@@ -491,7 +501,7 @@ int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int
             if (done) break;
             k_update<<<kRG, kRB, 0, st>>>(d_x, V, ldv, n2, H, ld, s, i - 1);
             IBIM_LAUNCH_CHECK();
-            if ((rc = matvec_device(pl, d_x, p))) return rc;
+            if ((rc = matvec_device(pl, d_x, p, mixed))) return rc;
             nmv++;
             k_residual<<<kRG, kRB, 0, st>>>(d_b, p, r, n2, partA);
             k_sqrt_sum<<<1, kRB, 0, st>>>(partA, scal);
@@ -515,4 +525,85 @@ int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int
     return ret;
 }
 
+}  // namespace ibim
+
+namespace ibim {
+namespace {
+__global__ void k_axpy1(double *__restrict__ x, const double *__restrict__ d, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += d[i];
+}
+}  // namespace
+
+// Mixed precision: iterative refinement.  The residual r = b - A x is formed
+// with the FP64 matvec; each correction solves A d = r with GMRES on the
+// mixed-precision matvec (FP32 near and far field, FP64 Krylov basis,
+// Hessenberg, rotations) to a relative tolerance no tighter than what the
+// FP32 operator can deliver, and x += d.  Converged when |b - A x| / |b| <
+// tol with the FP64 operator, so the answer is that of the FP64 solve.
+static int gmres_refine(ibim_plan *pl, const double *d_b, double *d_x, int restart, int maxit, double tol,
+                        ibim_solve_info *info) {
+    hipStream_t st = pl->stream;
+    const int n2 = 2 * pl->n;
+    int rc;
+    if (!pl->gm_rr && ((rc = dalloc(pl, &pl->gm_rr, (size_t)n2)) || (rc = dalloc(pl, &pl->gm_d, (size_t)n2))))
+        return rc;
+    if ((rc = gmres_alloc(pl, restart))) return rc;
+    double *scal = pl->gm_small;  // borrowed scratch for the two norms below (before any inner solve)
+    double *part = pl->gm_part + 3 * kRG;
+    auto norm = [&](const double *v, double *out) -> int {
+        k_dot_part<<<kRG, kRB, 0, st>>>(v, v, n2, part);
+        k_sqrt_sum<<<1, kRB, 0, st>>>(part, scal);
+        IBIM_LAUNCH_CHECK();
+        IBIM_HIP_CHECK(hipMemcpyAsync(out, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+        IBIM_HIP_CHECK(hipStreamSynchronize(st));
+        return IBIM_OK;
+    };
+    double normb = 0.0, rn = 0.0;
+    if ((rc = norm(d_b, &normb))) return rc;
+    if (normb == 0.0) normb = 1.0;
+    int nmv = 0, inner = 0, outer = 0;
+    auto residual = [&]() -> int {  // gm_rr = b - A64 x, rn = |gm_rr|
+        int r2 = matvec_device(pl, d_x, pl->gm_x, false);
+        if (r2) return r2;
+        ++nmv;
+        k_residual<<<kRG, kRB, 0, st>>>(d_b, pl->gm_x, pl->gm_rr, n2, part);
+        k_sqrt_sum<<<1, kRB, 0, st>>>(part, scal);
+        IBIM_LAUNCH_CHECK();
+        IBIM_HIP_CHECK(hipMemcpyAsync(&rn, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+        IBIM_HIP_CHECK(hipStreamSynchronize(st));
+        return IBIM_OK;
+    };
+    if ((rc = residual())) return rc;
+    const double floor_mixed = 1e-6;  // relative accuracy the FP32 matvec supports
+    while (rn / normb >= tol && inner < maxit && outer < 50) {
+        const double inner_tol = std::max(floor_mixed, 0.5 * tol * normb / rn);
+        IBIM_HIP_CHECK(hipMemsetAsync(pl->gm_d, 0, (size_t)n2 * sizeof(double), st));
+        ibim_solve_info ii{};
+        rc = gmres_core(pl, pl->gm_rr, pl->gm_d, restart, maxit - inner, inner_tol, &ii, true);
+        if (rc < 0) return rc;
+        inner += ii.iterations;
+        nmv += ii.matvecs;
+        k_axpy1<<<(n2 + 255) / 256, 256, 0, st>>>(d_x, pl->gm_d, n2);
+        IBIM_LAUNCH_CHECK();
+        ++outer;
+        const double before = rn;
+        if ((rc = residual())) return rc;
+        if (!(rn < before)) break;  // no progress: stop instead of looping
+    }
+    if (info) {
+        info->iterations = inner;
+        info->matvecs = nmv;
+        info->rel_residual = rn / normb;
+        info->converged = rn / normb < tol ? 1 : 0;
+        info->outer_steps = outer;
+    }
+    return rn / normb < tol ? IBIM_OK : IBIM_NOT_CONVERGED;
+}
+
+int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int maxit, double tol,
+                 ibim_solve_info *info) {
+    if (pl->prm.precision == IBIM_PREC_MIXED) return gmres_refine(pl, d_b, d_x, restart, maxit, tol, info);
+    return gmres_core(pl, d_b, d_x, restart, maxit, tol, info, false);
+}
 }  // namespace ibim

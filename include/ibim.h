@@ -121,7 +121,8 @@ int64_t ibim_plan_footprint(const ibim_plan *plan);
 #define IBIM_STAGE_RHS 7
 #define IBIM_STAGE_POLARIZE 8
 #define IBIM_STAGE_M2L_X 9    /* X list, direct, when the V list runs as GEMM */
-#define IBIM_STAGE_COUNT 10
+#define IBIM_STAGE_MATVEC_MIXED 10 /* one mixed-precision matvec */
+#define IBIM_STAGE_COUNT 11
 int ibim_plan_profile(ibim_plan *plan, int enable);
 int ibim_plan_stage_times(ibim_plan *plan, double *ms, int64_t *launches, int reset);
 
@@ -133,6 +134,10 @@ int ibim_matvec_device(ibim_plan *plan, const double *d_x, double *d_y);
 /* one black-box FMM: potential[t] = sum_s K(src_s, tgt_t) charge[s] for one
  * of the 16 kernels (kernel_id = IBIM_K*), far field by Chebyshev FMM */
 int ibim_kernel_fmm(ibim_plan *plan, int kernel_id, const double *charge, double *potential);
+/* the same at m separate target points (AoS), as bbfmm::kernel with a target
+ * set other than the sources (polarizeMap, electric.cpp:468-478) */
+int ibim_kernel_fmm_targets(ibim_plan *plan, int kernel_id, int64_t m, const double *target_xyz,
+                            const double *charge, double *potential);
 
 /* ---- either side of the solve -------------------------------------------- */
 /* right-hand side (length 2n) from m atoms (AoS centres, charges) */

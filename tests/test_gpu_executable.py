@@ -1,0 +1,50 @@
+"""The drop-in executable `levelset <input.cfg>` (main.cpp:14-79 of the
+reference) end to end on the GPU: PQR -> level set -> tube nodes -> PB solve
+-> polarized energy, for the Born ion (closed-form energy)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "ibim-levelset_amd", "bin", "levelset")
+BORN = 1.0 / (8 * np.pi * 2.0) * (1 / 80.0 - 1.0)
+
+
+def run(tmp_path, cfg_lines):
+    cfg = tmp_path / "input.cfg"
+    cfg.write_text("\n".join(cfg_lines) + "\n")
+    out = subprocess.run([EXE, str(cfg)], capture_output=True, text=True, timeout=240, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr + out.stdout
+    return out.stdout
+
+
+def test_born_ion_energy(tmp_path, gpu):
+    pqr = tmp_path / "ion.pqr"
+    pqr.write_text("ATOM      1  NA  ION     1       0.000   0.000   0.000  1.0000 2.0000\n")
+    stdout = run(tmp_path, [
+        "# Born ion, Angstrom grid", f"pqr_file = {pqr}", f"energy_file = {tmp_path / 'out.energy'}",
+        f"potent_file = {tmp_path / 'out.potent'}", "grid_h = 0.125", "lvlset_thickness = 2",
+        "lvlset_thres = 1e-4", "gmres_restart = 20", "gmres_maxiter = 200", "gmres_tol = 1e-8",
+        "fmm_np = 4", "fmm_maxpoint = 64", "fmm_maxlevel = 10", "tau = 1.0", "solvent_kappa = 0.0",
+        "solvent_dE = 80.0", "solvent_dI = 1.0"])
+    m = re.search(r"polarized energy:\s+(\S+)", stdout)
+    assert m, stdout
+    energy = float(m.group(1))
+    assert energy == pytest.approx(BORN, rel=0.03), stdout
+    assert "CONVERGED" in stdout
+    n = int(re.search(r"POINTS NUM\s+(\d+)", stdout).group(1))
+    assert len(open(tmp_path / "out.potent").read().split()) == 2 * n
+    phi = float(open(tmp_path / "out.energy").read().split()[0])
+    assert 0.5 * phi == pytest.approx(energy, rel=1e-5)  # files carry 6 significant digits, as the reference writes them
+
+
+def test_missing_pqr_fails_loudly(tmp_path, gpu):
+    cfg = tmp_path / "bad.cfg"
+    cfg.write_text("pqr_file = /nonexistent.pqr\ngrid_h = 0.25\n")
+    out = subprocess.run([EXE, str(cfg)], capture_output=True, text=True, timeout=60)
+    assert out.returncode != 0 and "pqr_file" in out.stderr

@@ -1,0 +1,62 @@
+"""Mixed precision (FP32 near and far field inside FP64 refinement) against
+the FP64 path on the same plan geometry.  Bars: the mixed matvec within 1e-5
+relative of FP64; the refined solve meets the FP64 residual tolerance with
+the FP64 operator, and its solvation energy is within 1e-6 relative of the
+FP64 solve's."""
+import numpy as np
+import pytest
+
+import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+PB = dict(kappa=0.1257, eps_in=2.0, eps_out=80.0)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def sphere_case(ibim, gpu):
+    pts, nrm, w = G.sphere_tube(a=2.0, h=0.125, lo=-4.0, npts=64)
+    return pts, nrm, w
+
+
+def test_mixed_matvec_close_to_fp64(ibim, sphere_case):
+    pts, nrm, w = sphere_case
+    x = np.random.default_rng(0).uniform(-1, 1, 2 * len(pts))
+    with ibim.Plan(pts, nrm, w, p=5, max_points=64, vacant_radius=0.125, **PB) as plan:
+        y64 = plan.matvec(x)
+        plan.set_precision(ibim.PREC_MIXED)
+        y32 = plan.matvec(x)
+    err = rel(y32, y64)
+    print(f"mixed matvec rel err {err:.2e}")
+    assert err < 1e-5
+    assert err > 0  # the mixed path really ran
+
+
+@pytest.mark.parametrize("kappa", [0.0, 0.1257])
+def test_mixed_solve_energy_matches_fp64(ibim, sphere_case, kappa):
+    pts, nrm, w = sphere_case
+    atoms = np.array([[0.0, 0.0, 0.0], [0.4, -0.3, 0.2]])
+    qa = np.array([1.0, -0.5])
+    out = {}
+    for prec in (ibim.PREC_FP64, ibim.PREC_MIXED):
+        with ibim.Plan(pts, nrm, w, p=5, max_points=64, vacant_radius=0.125, precision=prec, kappa=kappa,
+                       eps_in=1.0, eps_out=80.0) as plan:
+            b = plan.rhs(atoms, qa)
+            x, rc, info = plan.gmres(b, restart=20, maxit=300, tol=1e-8)
+            assert rc == 0, info
+            # the residual is measured with the FP64 operator in both cases
+            plan.set_precision(ibim.PREC_FP64)
+            res = np.linalg.norm(b - plan.matvec(x)) / np.linalg.norm(b)
+            phi = plan.polarize(atoms, x)
+        out[prec] = (x, 0.5 * float(phi @ qa), info, res)
+    (x64, e64, i64, r64), (x32, e32, i32, r32) = out[ibim.PREC_FP64], out[ibim.PREC_MIXED]
+    print(f"kappa={kappa}: E64={e64:.10e} Emix={e32:.10e} its {i64['iterations']} / {i32['iterations']} "
+          f"outer {i32['outer_steps']} res {r64:.1e} / {r32:.1e}")
+    assert r64 < 1e-8 and r32 < 1e-8
+    assert i32["outer_steps"] >= 1
+    assert abs(e32 - e64) / abs(e64) < 1e-6
+    assert rel(x32, x64) < 1e-6
