@@ -82,6 +82,23 @@ class SurfaceInfo(C.Structure):
         return out
 
 
+class Request(C.Structure):
+    _fields_ = [("n_atoms", C.c_int64), ("atom_xyz", C.POINTER(C.c_double)), ("radii", C.POINTER(C.c_double)),
+                ("charges", C.POINTER(C.c_double)), ("geom", GeomParams), ("fmm", Params), ("restart", C.c_int),
+                ("maxit", C.c_int), ("tol", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int), ("energy", C.c_double), ("n_points", C.c_int64), ("info", SolveInfo),
+                ("t_submit", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double),
+                ("footprint_est", C.c_int64), ("footprint", C.c_int64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "info"}
+        d["info"] = self.info.as_dict()
+        return d
+
+
 def build(jobs=8):
     """Compile the HIP library (hipcc cross-compiles gfx950 without a GPU)."""
     subprocess.run(["make", "-s", f"-j{jobs}", "-C", HERE], check=True)
@@ -134,6 +151,14 @@ def lib():
     L.ibim_surface_destroy.argtypes = [vp]
     L.ibim_surface_destroy.restype = None
     L.ibim_synthetic_globule.argtypes = [i64, C.c_uint64, C.c_double, d, d, d]
+    L.ibim_server_create.argtypes = [C.c_int, i64, pp]
+    L.ibim_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_int64)]
+    L.ibim_server_wait.argtypes = [vp, C.c_int64, C.POINTER(Result)]
+    L.ibim_server_now.argtypes = [vp]
+    L.ibim_server_now.restype = C.c_double
+    L.ibim_server_destroy.argtypes = [vp]
+    L.ibim_server_destroy.restype = None
+    L.ibim_kernel_fmm_targets.argtypes = [vp, C.c_int, i64, d, d, d]
     L.ibim_last_error.restype = C.c_char_p
     L.ibim_version.restype = C.c_char_p
     _lib = L
@@ -393,3 +418,44 @@ def synthetic_globule(m, seed, confine_radius):
     q = np.zeros(m)
     _check(lib().ibim_synthetic_globule(m, seed, confine_radius, _d(xyz), _d(r), _d(q)), "ibim_synthetic_globule")
     return xyz, r, q
+
+
+# ------------------------------------------------------------- serving --
+class Server:
+    """Continuous batching of PB solves: `slots` concurrent requests, admission
+    by estimated device footprint within `mem_budget` bytes (0 = 90% of free)."""
+
+    def __init__(self, slots=4, mem_budget=0):
+        self._h = C.c_void_p()
+        _check(lib().ibim_server_create(slots, mem_budget, C.byref(self._h)), "ibim_server_create")
+        self._keep = {}
+
+    def submit(self, atom_xyz, radii, charges, h=0.35, thickness=2, probe=1.4, restart=40, maxit=2000, tol=1e-8,
+               **fmm):
+        axyz = _f64(atom_xyz, (-1, 3))
+        rad = _f64(radii)
+        q = _f64(charges)
+        req = Request(len(axyz), _d(axyz), _d(rad), _d(q), geom_params(h, thickness=thickness, probe=probe),
+                      params(**fmm), restart, maxit, tol)
+        rid = C.c_int64()
+        _check(lib().ibim_server_submit(self._h, C.byref(req), C.byref(rid)), "ibim_server_submit")
+        return int(rid.value)
+
+    def wait(self, rid):
+        res = Result()
+        _check(lib().ibim_server_wait(self._h, rid, C.byref(res)), "ibim_server_wait")
+        return res.as_dict()
+
+    def now(self):
+        return float(lib().ibim_server_now(self._h))
+
+    def close(self):
+        if self._h:
+            lib().ibim_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

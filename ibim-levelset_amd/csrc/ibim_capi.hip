@@ -145,13 +145,6 @@ int ibim_plan_create_device(const ibim_params *prm, int64_t n, const double *d_x
         return IBIM_ERR_HIP;
     }
     pl->owns_stream = true;
-    // the plan's stream must see the caller's writes of the inputs
-    e = hipDeviceSynchronize();
-    if (e != hipSuccess) {
-        set_error(std::string("hipDeviceSynchronize: ") + hipGetErrorString(e));
-        ibim_plan_destroy(pl);
-        return IBIM_ERR_HIP;
-    }
     if ((rc = build_chebyshev(pl)) || (rc = build_tree(pl, d_xyz, d_nrm, d_w)) || (rc = build_m2l_operators(pl)) ||
         (rc = build_vpairs(pl, pl->need_local, pl->vp, nullptr))) {
         ibim_plan_destroy(pl);

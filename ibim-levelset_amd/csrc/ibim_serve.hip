@@ -1,0 +1,256 @@
+// ibim_serve.hip -- continuous batching of PB solves on one GPU.
+//
+// Workers (one host thread per slot) pull requests from a FIFO queue.  A
+// request is admitted when a slot is free and its estimated device footprint
+// fits beside the running requests within the memory budget; otherwise it
+// waits (and runs alone if it could never fit).  Each request builds its own
+// level set, tube nodes and plan on the worker's streams, so requests of
+// different sizes overlap on the device; everything a solve needs stays in
+// HBM until it finishes.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cmath>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "ibim_internal.h"
+
+struct ibim_server {
+    struct Job {
+        int64_t id;
+        ibim_request req;
+        std::vector<double> xyz, rad, q;
+        int64_t est = 0;
+        ibim_result res{};
+        bool done = false;
+    };
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<int64_t> queue;
+    std::map<int64_t, Job> jobs;
+    std::vector<std::thread> workers;
+    int64_t next_id = 0;
+    int64_t budget = 0, in_use = 0;
+    int running = 0;
+    bool stop = false;
+    int device = 0;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace ibim {
+namespace {
+
+// device bytes a request will hold: the level-set grids (4 arrays while the
+// surface is built) or the plan, whichever is larger, plus the tube nodes.
+// Tube points are bounded by the atoms' probe-expanded sphere area times the
+// tube's (2 thickness) layers per unit area, capped by the grid's surface;
+// each point costs ~ (4 r fields x 2 expansions per ~20 points per box +
+// vectors + GMRES basis) bytes.
+int64_t estimate_footprint(const ibim_request &r, const std::vector<double> &xyz, const std::vector<double> &rad) {
+    const double h = r.geom.h;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, area = 0.0;
+    for (int64_t a = 0; a < r.n_atoms; ++a) {
+        const double rr = rad[a] + r.geom.probe;
+        area += 4.0 * M_PI * rr * rr;
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], xyz[3 * a + d] - rr);
+            hi[d] = std::max(hi[d], xyz[3 * a + d] + rr);
+        }
+    }
+    double ext[3], grid = 1.0;
+    for (int d = 0; d < 3; ++d) {
+        ext[d] = hi[d] - lo[d] + 2 * r.geom.pad;
+        grid *= ext[d] / h + 1;
+    }
+    const double box_area = 2 * (ext[0] * ext[1] + ext[1] * ext[2] + ext[0] * ext[2]);
+    const double npts = std::min(area, 3.0 * box_area) * (2.0 * r.geom.thickness) / (h * h);
+    const int p = r.fmm.p, rr3 = p * p * p;
+    const double per_point = 2.0 * 4 * rr3 * 8 / 12.0 + 8.0 * (2 * (r.restart + 1) + 40);
+    const double ops = (r.fmm.precision == IBIM_PREC_MIXED ? 1.5 : 1.0) * 12 * 56 * pow(4.0 * rr3, 2) * 8;
+    return (int64_t)(std::max(grid * 8 * 4.2, npts * per_point + ops) + npts * 56);
+}
+
+void run_job(ibim_server::Job &j) {
+    ibim_result &res = j.res;
+    ibim_surface *surf = nullptr;
+    ibim_plan *plan = nullptr;
+    double *d_b = nullptr, *d_x = nullptr, *d_at = nullptr, *d_phi = nullptr;
+    const int64_t m = j.req.n_atoms;
+    int rc = ibim_surface_from_atoms(&j.req.geom, m, j.xyz.data(), j.rad.data(), &surf);
+    const double *sx = nullptr, *sn = nullptr, *sw = nullptr;
+    ibim_surface_info inf{};
+    if (rc == IBIM_OK) rc = ibim_surface_get_info(surf, &inf);
+    if (rc == IBIM_OK) rc = ibim_surface_device(surf, &sx, &sn, &sw);
+    if (rc == IBIM_OK) rc = ibim_plan_create_device(&j.req.fmm, inf.n_nodes, sx, sn, sw, &plan);
+    const size_t n2 = 2 * (size_t)inf.n_nodes;
+    if (rc == IBIM_OK) {
+        if (hipMalloc(&d_b, n2 * sizeof(double)) != hipSuccess || hipMalloc(&d_x, n2 * sizeof(double)) != hipSuccess ||
+            hipMalloc(&d_at, 4 * (size_t)m * sizeof(double)) != hipSuccess ||
+            hipMalloc(&d_phi, (size_t)m * sizeof(double)) != hipSuccess) {
+            set_error("hipMalloc (request vectors)");
+            rc = IBIM_ERR_NOMEM;
+        }
+    }
+    hipStream_t st = nullptr;
+    if (rc == IBIM_OK && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) rc = IBIM_ERR_HIP;
+    if (rc == IBIM_OK) {
+        std::vector<double> packed(4 * (size_t)m);
+        std::copy(j.xyz.begin(), j.xyz.end(), packed.begin());
+        std::copy(j.q.begin(), j.q.end(), packed.begin() + 3 * m);
+        if (hipMemcpyAsync(d_at, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice, st) !=
+                hipSuccess ||
+            hipMemsetAsync(d_x, 0, n2 * sizeof(double), st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+            rc = IBIM_ERR_HIP;
+    }
+    if (rc == IBIM_OK) rc = ibim_rhs_device(plan, m, d_at, d_at + 3 * m, d_b);
+    if (rc == IBIM_OK) rc = ibim_gmres_device(plan, d_b, d_x, j.req.restart, j.req.maxit, j.req.tol, &res.info);
+    const int solve_rc = rc;
+    if (rc >= 0) rc = ibim_polarize_device(plan, m, d_at, d_x, d_phi);
+    if (rc >= 0) {
+        std::vector<double> phi((size_t)m);
+        if (hipMemcpyAsync(phi.data(), d_phi, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            rc = IBIM_ERR_HIP;
+        } else {
+            double e = 0.0;
+            for (int64_t a = 0; a < m; ++a) e += phi[a] * j.q[a];
+            res.energy = 0.5 * e;
+            rc = solve_rc;
+        }
+    }
+    res.status = rc;
+    res.n_points = inf.n_nodes;
+    res.footprint = (plan ? ibim_plan_footprint(plan) : 0) + (int64_t)(7 * (size_t)inf.n_nodes * sizeof(double));
+    for (double *p : {d_b, d_x, d_at, d_phi})
+        if (p) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+    ibim_plan_destroy(plan);
+    ibim_surface_destroy(surf);
+}
+
+void worker(ibim_server *s) {
+    (void)hipSetDevice(s->device);
+    for (;;) {
+        int64_t id = -1;
+        ibim_server::Job *job = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            for (;;) {
+                if (!s->queue.empty()) {
+                    ibim_server::Job &head = s->jobs[s->queue.front()];
+                    if (s->running == 0 || s->in_use + head.est <= s->budget) {
+                        id = s->queue.front();
+                        s->queue.pop_front();
+                        job = &head;
+                        s->in_use += head.est;
+                        s->running++;
+                        break;
+                    }
+                }
+                if (s->stop && s->queue.empty()) return;
+                s->cv_work.wait(lk);
+            }
+            job->res.t_start = s->now();
+        }
+        run_job(*job);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            job->res.t_end = s->now();
+            job->done = true;
+            s->in_use -= job->est;
+            s->running--;
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+        (void)id;
+    }
+}
+
+}  // namespace
+}  // namespace ibim
+
+using namespace ibim;
+
+extern "C" {
+
+int ibim_server_create(int slots, int64_t mem_budget, ibim_server **out) {
+    if (!out || slots < 1 || slots > 64 || mem_budget < 0) {
+        set_error("bad argument to ibim_server_create (1..64 slots)");
+        return IBIM_ERR_ARG;
+    }
+    ibim_server *s = new ibim_server();
+    IBIM_HIP_CHECK(hipGetDevice(&s->device));
+    size_t fr = 0, tot = 0;
+    IBIM_HIP_CHECK(hipMemGetInfo(&fr, &tot));
+    s->budget = mem_budget > 0 ? mem_budget : (int64_t)(0.9 * (double)fr);
+    for (int i = 0; i < slots; ++i) s->workers.emplace_back(worker, s);
+    *out = s;
+    return IBIM_OK;
+}
+
+int ibim_server_submit(ibim_server *s, const ibim_request *r, int64_t *id) {
+    if (!s || !r || !id || r->n_atoms < 1 || !r->atom_xyz || !r->radii || !r->charges) {
+        set_error("bad argument to ibim_server_submit");
+        return IBIM_ERR_ARG;
+    }
+    ibim_server::Job j;
+    j.req = *r;
+    j.xyz.assign(r->atom_xyz, r->atom_xyz + 3 * r->n_atoms);
+    j.rad.assign(r->radii, r->radii + r->n_atoms);
+    j.q.assign(r->charges, r->charges + r->n_atoms);
+    j.req.atom_xyz = nullptr;
+    j.req.radii = nullptr;
+    j.req.charges = nullptr;
+    j.est = estimate_footprint(j.req, j.xyz, j.rad);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        j.id = s->next_id++;
+        j.res.t_submit = s->now();
+        j.res.footprint_est = j.est;
+        *id = j.id;
+        s->queue.push_back(j.id);
+        s->jobs.emplace(j.id, std::move(j));
+    }
+    s->cv_work.notify_all();
+    return IBIM_OK;
+}
+
+int ibim_server_wait(ibim_server *s, int64_t id, ibim_result *out) {
+    if (!s || !out) {
+        set_error("bad argument to ibim_server_wait");
+        return IBIM_ERR_ARG;
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) {
+        set_error("unknown request id");
+        return IBIM_ERR_ARG;
+    }
+    s->cv_done.wait(lk, [&] { return it->second.done; });
+    *out = it->second.res;
+    s->jobs.erase(it);
+    return IBIM_OK;
+}
+
+double ibim_server_now(ibim_server *s) { return s ? s->now() : 0.0; }
+
+void ibim_server_destroy(ibim_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers) t.join();
+    delete s;
+}
+
+}  // extern "C"

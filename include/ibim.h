@@ -10,7 +10,9 @@
  *   - Plain pointers and sizes only.  Functions without a `_device` suffix
  *     take host arrays and return when the results are in them.  `_device`
  *     variants take device pointers (HBM) and are ordered on the plan's
- *     stream; call ibim_plan_sync() before reading their results.
+ *     stream; call ibim_plan_sync() before reading their results.  Device
+ *     inputs must be complete when a call is made (synchronise the stream
+ *     that wrote them first): the plan's stream does not wait for others.
  *   - Points are AoS: xyz[3*i+0..2]; normals the same; one weight per point.
  *   - Return 0 (IBIM_OK) on success, a negative IBIM_ERR_* otherwise;
  *     ibim_last_error() describes the last failure of the calling thread.
@@ -219,6 +221,49 @@ void ibim_surface_destroy(ibim_surface *s);
  * Arrays are sized m (xyz 3m). */
 int ibim_synthetic_globule(int64_t m, uint64_t seed, double confine_radius, double *xyz, double *radii,
                            double *charges);
+
+/* ---- serving: continuous batching of PB solves -------------------------------
+ * A request is one molecule: the level set and tube nodes are built, then
+ * the plan, right-hand side, GMRES and reaction potential, and the result is
+ * the solvation energy.  Up to `slots` requests run at once, each on its own
+ * HIP streams with its state resident in HBM; a request is admitted when a
+ * slot is free and its estimated device footprint fits next to the running
+ * ones within `mem_budget` bytes (0: 90% of the free device memory at
+ * creation).  Requests start in submission order; a request that does not fit
+ * waits, and runs alone if it would not fit even then.  This replaces the
+ * reference's one-molecule-per-process executable (main.cpp:14-79) for
+ * streams of molecules. */
+typedef struct ibim_server ibim_server;
+
+typedef struct ibim_request {
+    int64_t n_atoms;
+    const double *atom_xyz;   /* 3 per atom; copied at submission */
+    const double *radii;
+    const double *charges;
+    ibim_geom_params geom;    /* grid spacing etc.; dims 0 = fitted to the atoms */
+    ibim_params fmm;          /* p, leaf, kappa, eps, vacant radius, precision */
+    int restart, maxit;
+    double tol;
+} ibim_request;
+
+typedef struct ibim_result {
+    int status;               /* IBIM_OK, IBIM_NOT_CONVERGED or an error code */
+    double energy;            /* (1/2) sum q phi_reac */
+    int64_t n_points;         /* tube points */
+    ibim_solve_info info;
+    double t_submit, t_start, t_end;  /* seconds on the server's steady clock */
+    int64_t footprint_est, footprint; /* device bytes: admission estimate, measured plan + surface */
+} ibim_result;
+
+int ibim_server_create(int slots, int64_t mem_budget, ibim_server **out);
+/* copies the request; returns its id (>= 0) in *id */
+int ibim_server_submit(ibim_server *s, const ibim_request *req, int64_t *id);
+/* blocks until request `id` has finished */
+int ibim_server_wait(ibim_server *s, int64_t id, ibim_result *out);
+/* seconds on the server's clock (for arrival schedules) */
+double ibim_server_now(ibim_server *s);
+/* finishes every queued request, then stops the workers */
+void ibim_server_destroy(ibim_server *s);
 
 const char *ibim_last_error(void);
 const char *ibim_version(void);
