@@ -98,6 +98,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision solves")
+    ap.add_argument("--leaf", type=int, default=None, help="override the leaf capacity")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (solves of different molecules "
@@ -110,7 +111,9 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
     ibim = load_ibim()
-    cfg = CONFIGS[args.config]
+    cfg = dict(CONFIGS[args.config])
+    if args.leaf:
+        cfg["leaf"] = args.leaf
 
     # ---- setup (untimed): molecule, level set, tube nodes -- all in HBM
     xyz, rad, qa = ibim.synthetic_globule(cfg["atoms"], cfg["seed"], cfg["confine"])
@@ -217,7 +220,7 @@ def main():
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{args.config}: synthetic 500-atom globule, h=0.25 A, p=5, leaf 64, "
+        "config": {"workload": f"{args.config}: synthetic 500-atom globule, h=0.25 A, p=5, leaf {cfg['leaf']}, "
                                f"kappa=0.1257, eps 2/80, GMRES(40) tol 1e-8, FP64",
                    "tube_points": n, "atoms": m, "parallelism": "single GPU"},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),

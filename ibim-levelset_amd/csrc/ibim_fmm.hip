@@ -443,34 +443,30 @@ __global__ void k_p2p_ranges(int nleaves, const int *__restrict__ tleaves, const
     }
 }
 
-// ----------------------------------------------------- P2P + L2P, v2 ---
-// One wavefront per target leaf.  The wave's 64 lanes are tp targets x
-// (64/tp) source slices (tp = next power of two >= targets in the chunk);
-// each lane walks its slice of the leaf's merged source ranges, the slices
-// are summed with cross-lane shuffles, and the first slice adds the far
-// field (L2P) from the leaf's local expansion staged in LDS.
-template <int P, bool MIXED>
-__global__ __launch_bounds__(256) void k_p2p_wave(
+// ----------------------------------------------------------- P2P, v2 ---
+// One wavefront (one 64-thread block) per target leaf.  The leaf's merged
+// near-field source ranges are streamed in batches of 64 sources: each lane
+// loads one source (coordinates and the 4 charges) into LDS, then the 64
+// lanes -- tp targets x (64/tp) source slices, tp = next power of two >=
+// targets in the chunk -- evaluate the batch from LDS, two sources per step
+// for instruction-level parallelism.  Slices are summed with cross-lane
+// shuffles.  MIXED: FP32 kernel evaluation of the FP64 coordinate
+// difference, Kahan-compensated FP32 sums.  Writes the near field into F;
+// k_l2p adds the far field.
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_p2p_wave(
     int nleaves, const int *__restrict__ tleaves, const int *__restrict__ tbeg, const int *__restrict__ tend,
     const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
     const double *__restrict__ sx, const double *__restrict__ sy, const double *__restrict__ sz,
-    const double *__restrict__ Q, int ns, const int *__restrict__ roff, const int2 *__restrict__ ranges,
-    const double *__restrict__ ctr, const int *__restrict__ level, const double *__restrict__ lrad, ChebTab tab,
-    PBConst kc, const double *__restrict__ L, double *__restrict__ F, int nt) {
-    constexpr int R = P * P * P;
-    __shared__ double sL[4][4 * R];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int li = blockIdx.x * 4 + wv;
-    const bool active = li < nleaves;
-    const int b = active ? tleaves[li] : 0;
-    double *myL = sL[wv];
-    if (active)
-        for (int e = lane; e < 4 * R; e += 64) myL[e] = L[(size_t)b * 4 * R + e];
-    __syncthreads();
-    if (!active) return;  // no block barrier below
-    const double cx = ctr[3 * b], cy = ctr[3 * b + 1], cz = ctr[3 * b + 2];
-    const double *rd = lrad + 3 * level[b];
+    const double *__restrict__ Q, int ns, const int *__restrict__ roff, const int2 *__restrict__ ranges, PBConst kc,
+    double *__restrict__ F, int nt) {
+    __shared__ double bx[64], by[64], bz[64], bq0[64], bq1[64], bq2[64], bq3[64];
+    const int lane = threadIdx.x;
+    const int li = blockIdx.x;
+    const int b = tleaves[li];
     const int r0 = roff[li], r1 = roff[li + 1];
+    int total = 0;
+    for (int r = r0; r < r1; ++r) total += ranges[r].y - ranges[r].x;
     const int t_beg = tbeg[b], t_end = tend[b];
     for (int t0 = t_beg; t0 < t_end; t0 += 64) {
         const int ntc = min(64, t_end - t0);
@@ -483,30 +479,68 @@ __global__ __launch_bounds__(256) void k_p2p_wave(
         const double px = valid ? tx[t] : 0.0, py = valid ? ty[t] : 0.0, pz = valid ? tz[t] : 0.0;
         double f0 = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
         KahanF k0, k1, k2, k3;
-        int base = 0, j = sl;
-        for (int r = r0; r < r1; ++r) {
-            const int2 rg = ranges[r];
-            const int len = rg.y - rg.x;
-            while (j - base < len) {
-                const int s = rg.x + (j - base);
-                if (valid) {
+        int cr = r0, cbase = 0;  // loading cursor of this lane: source number j = batch + lane
+        for (int b0 = 0; b0 < total; b0 += 64) {
+            const int j = b0 + lane;
+            if (j < total) {
+                while (j - cbase >= ranges[cr].y - ranges[cr].x) {
+                    cbase += ranges[cr].y - ranges[cr].x;
+                    ++cr;
+                }
+                const int s = ranges[cr].x + (j - cbase);
+                bx[lane] = sx[s];
+                by[lane] = sy[s];
+                bz[lane] = sz[s];
+                bq0[lane] = Q[s];
+                bq1[lane] = Q[(size_t)ns + s];
+                bq2[lane] = Q[2 * (size_t)ns + s];
+                bq3[lane] = Q[3 * (size_t)ns + s];
+            }
+            __syncthreads();
+            const int cnt = min(64, total - b0);
+            if (valid) {
+                int k = sl;
+                for (; k + nsl < cnt; k += 2 * nsl) {
+                    const int kb = k + nsl;
                     if (MIXED) {
-                        float g0, gx, gy, gz;
-                        pair_accum_f32((float)(sx[s] - px), (float)(sy[s] - py), (float)(sz[s] - pz), (float)Q[s],
-                                       (float)Q[(size_t)ns + s], (float)Q[2 * (size_t)ns + s],
-                                       (float)Q[3 * (size_t)ns + s], kc, g0, gx, gy, gz);
-                        k0.add(g0);
-                        k1.add(gx);
-                        k2.add(gy);
-                        k3.add(gz);
+                        float a0, ax, ay, az, c0, cx, cy, cz;
+                        pair_accum_f32((float)(bx[k] - px), (float)(by[k] - py), (float)(bz[k] - pz), (float)bq0[k],
+                                       (float)bq1[k], (float)bq2[k], (float)bq3[k], kc, a0, ax, ay, az);
+                        pair_accum_f32((float)(bx[kb] - px), (float)(by[kb] - py), (float)(bz[kb] - pz),
+                                       (float)bq0[kb], (float)bq1[kb], (float)bq2[kb], (float)bq3[kb], kc, c0, cx, cy,
+                                       cz);
+                        k0.add(a0 + c0);
+                        k1.add(ax + cx);
+                        k2.add(ay + cy);
+                        k3.add(az + cz);
                     } else {
-                        pair_accum<double>(sx[s] - px, sy[s] - py, sz[s] - pz, Q[s], Q[(size_t)ns + s],
-                                           Q[2 * (size_t)ns + s], Q[3 * (size_t)ns + s], kc, f0, fx, fy, fz);
+                        double a0 = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
+                        pair_accum<double>(bx[k] - px, by[k] - py, bz[k] - pz, bq0[k], bq1[k], bq2[k], bq3[k], kc, a0,
+                                           ax, ay, az);
+                        pair_accum<double>(bx[kb] - px, by[kb] - py, bz[kb] - pz, bq0[kb], bq1[kb], bq2[kb], bq3[kb],
+                                           kc, f0, fx, fy, fz);
+                        f0 += a0;
+                        fx += ax;
+                        fy += ay;
+                        fz += az;
                     }
                 }
-                j += nsl;
+                if (k < cnt) {
+                    if (MIXED) {
+                        float a0, ax, ay, az;
+                        pair_accum_f32((float)(bx[k] - px), (float)(by[k] - py), (float)(bz[k] - pz), (float)bq0[k],
+                                       (float)bq1[k], (float)bq2[k], (float)bq3[k], kc, a0, ax, ay, az);
+                        k0.add(a0);
+                        k1.add(ax);
+                        k2.add(ay);
+                        k3.add(az);
+                    } else {
+                        pair_accum<double>(bx[k] - px, by[k] - py, bz[k] - pz, bq0[k], bq1[k], bq2[k], bq3[k], kc, f0,
+                                           fx, fy, fz);
+                    }
+                }
             }
-            base += len;
+            __syncthreads();
         }
         if (MIXED) {
             f0 = k0.value();
@@ -521,31 +555,54 @@ __global__ __launch_bounds__(256) void k_p2p_wave(
             fz += __shfl_xor(fz, o);
         }
         if (valid && sl == 0) {
-            double Sx[P], Sy[P], Sz[P];
-            sp_values<P>((px - cx) / rd[0], tab, Sx);
-            sp_values<P>((py - cy) / rd[1], tab, Sy);
-            sp_values<P>((pz - cz) / rd[2], tab, Sz);
-            double g0 = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
-#pragma unroll
-            for (int l = 0; l < P; ++l)
-#pragma unroll
-                for (int jj = 0; jj < P; ++jj) {
-                    const double wyz = Sy[jj] * Sz[l];
-#pragma unroll
-                    for (int i = 0; i < P; ++i) {
-                        const int m = l * P * P + jj * P + i;
-                        const double w = Sx[i] * wyz;
-                        g0 += w * myL[m];
-                        gx += w * myL[R + m];
-                        gy += w * myL[2 * R + m];
-                        gz += w * myL[3 * R + m];
-                    }
-                }
-            F[t] = f0 + g0;
-            F[(size_t)nt + t] = fx + gx;
-            F[2 * (size_t)nt + t] = fy + gy;
-            F[3 * (size_t)nt + t] = fz + gz;
+            F[t] = f0;
+            F[(size_t)nt + t] = fx;
+            F[2 * (size_t)nt + t] = fy;
+            F[3 * (size_t)nt + t] = fz;
         }
+    }
+}
+
+// L2P (bbfmm.h:600): F += S L_leaf at each target, one block per target
+// leaf, the leaf's local expansion staged in LDS
+template <int P>
+__global__ __launch_bounds__(64) void k_l2p(const int *__restrict__ tleaves, const int *__restrict__ tbeg,
+                                            const int *__restrict__ tend, const double *__restrict__ tx,
+                                            const double *__restrict__ ty, const double *__restrict__ tz,
+                                            const double *__restrict__ ctr, const int *__restrict__ level,
+                                            const double *__restrict__ lrad, ChebTab tab,
+                                            const double *__restrict__ L, double *__restrict__ F, int nt) {
+    constexpr int R = P * P * P;
+    __shared__ double sL[4 * R];
+    const int b = tleaves[blockIdx.x];
+    for (int e = threadIdx.x; e < 4 * R; e += 64) sL[e] = L[(size_t)b * 4 * R + e];
+    __syncthreads();
+    const double cx = ctr[3 * b], cy = ctr[3 * b + 1], cz = ctr[3 * b + 2];
+    const double *rd = lrad + 3 * level[b];
+    for (int t = tbeg[b] + threadIdx.x; t < tend[b]; t += 64) {
+        double Sx[P], Sy[P], Sz[P];
+        sp_values<P>((tx[t] - cx) / rd[0], tab, Sx);
+        sp_values<P>((ty[t] - cy) / rd[1], tab, Sy);
+        sp_values<P>((tz[t] - cz) / rd[2], tab, Sz);
+        double g0 = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+        for (int l = 0; l < P; ++l)
+#pragma unroll
+            for (int jj = 0; jj < P; ++jj) {
+                const double wyz = Sy[jj] * Sz[l];
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    const int m = l * P * P + jj * P + i;
+                    const double w = Sx[i] * wyz;
+                    g0 += w * sL[m];
+                    gx += w * sL[R + m];
+                    gy += w * sL[2 * R + m];
+                    gz += w * sL[3 * R + m];
+                }
+            }
+        F[t] += g0;
+        F[(size_t)nt + t] += gx;
+        F[2 * (size_t)nt + t] += gy;
+        F[3 * (size_t)nt + t] += gz;
     }
 }
 
@@ -639,14 +696,19 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     }
     {
     StageScope sc(pl, kStP2P);
-    if (ts.nleaves > 0 && ts.roff && mixed)
-        k_p2p_wave<P, true><<<(ts.nleaves + 3) / 4, 256, 0, st>>>(
-            ts.nleaves, ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q, pl->n, ts.roff,
-            ts.ranges, pl->ctr, pl->level, pl->d_lvl_rad, pl->tab, pl->kc, pl->L, F, nt);
-    else if (ts.nleaves > 0 && ts.roff)
-        k_p2p_wave<P, false><<<(ts.nleaves + 3) / 4, 256, 0, st>>>(
-            ts.nleaves, ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q, pl->n, ts.roff,
-            ts.ranges, pl->ctr, pl->level, pl->d_lvl_rad, pl->tab, pl->kc, pl->L, F, nt);
+    if (ts.nleaves > 0 && ts.roff) {
+        if (mixed)
+            k_p2p_wave<true><<<ts.nleaves, 64, 0, st>>>(ts.nleaves, ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z,
+                                                        pl->sx, pl->sy, pl->sz, pl->Q, pl->n, ts.roff, ts.ranges,
+                                                        pl->kc, F, nt);
+        else
+            k_p2p_wave<false><<<ts.nleaves, 64, 0, st>>>(ts.nleaves, ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z,
+                                                         pl->sx, pl->sy, pl->sz, pl->Q, pl->n, ts.roff, ts.ranges,
+                                                         pl->kc, F, nt);
+        IBIM_LAUNCH_CHECK();
+        k_l2p<P><<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->ctr, pl->level,
+                                            pl->d_lvl_rad, pl->tab, pl->L, F, nt);
+    }
     else if (ts.nleaves > 0)
         k_p2p_l2p<P><<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->sx, pl->sy,
                                                 pl->sz, pl->Q, pl->n, pl->loff[0], pl->litem[0], pl->loff[2],

@@ -219,7 +219,7 @@ struct VPairs {
     int npairs = 0;
     int *A = nullptr, *B = nullptr;     // source and target box of each pair, sorted by class
     unsigned char *mask = nullptr;      // reflection bits of the offset
-    std::vector<int> cls_off;           // host: pairs of class c in [cls_off[c], cls_off[c+1])
+    std::vector<int> cls_off;           // host: pairs of key c = 8 class + mask in [cls_off[c], cls_off[c+1])
     int *cls_off_d = nullptr;
     int nblocks = 0;                    // GEMM blocks of 64 pairs
     int *blk_cls = nullptr, *blk_j0 = nullptr;
@@ -295,6 +295,7 @@ struct ibim_plan {
     int n_cls = 0, MRP = 0, KRP = 0;
     double *ops = nullptr;
     float *ops32 = nullptr;              // FP32 copy for the mixed-precision matvec
+    int *gtab = nullptr, *stab = nullptr; // gather / scatter tables of the 8 reflections
     ibim::VPairs vp;                     // V pairs for the points as their own targets
     const int *p2p_roff = nullptr;       // merged near-field ranges of the non-empty leaves
     const int2 *p2p_ranges = nullptr;

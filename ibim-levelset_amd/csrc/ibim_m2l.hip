@@ -87,7 +87,7 @@ __global__ void k_vpairs(int nnodes, const unsigned char *__restrict__ need, con
                 pA[j] = a;
                 pB[j] = b;
                 pm[j] = (unsigned char)m;
-                pkey[j] = (lev - 2) * kNCanon + ci;
+                pkey[j] = ((lev - 2) * kNCanon + ci) * 8 + m;  // grouped by class, then reflection
             }
             ++c;
         }
@@ -182,88 +182,146 @@ struct Mfma<float> {
     static __device__ __forceinline__ int row(int l, int q) { return 4 * (l >> 4) + q; }
 };
 
+// Gather table of one reflection mask m: column k = f*r + s of the operator
+// reads multipole element f*r + flip_m(s) of the source box, negated for the
+// reflected x/y/z fields; the scatter uses the same permutation on the rows
+// with the output-field signs.  gtab[m][k] = index | (negate << 31).
+template <int P>
+__global__ void k_gather_table(int *__restrict__ gtab, int *__restrict__ stab) {
+    constexpr int R = P * P * P;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 8 * 4 * R) return;
+    const int m = e / (4 * R), k = e % (4 * R);
+    const int f = k / R, s = k % R;
+    const int idx = f * R + flip_node<P>(s, m);
+    const bool neg_in = f < 3 && ((m >> f) & 1);
+    const bool neg_out = f > 0 && ((m >> (f - 1)) & 1);
+    gtab[e] = idx | (neg_in ? (int)0x80000000 : 0);
+    stab[e] = idx | (neg_out ? (int)0x80000000 : 0);
+}
+
+// One (class, mask, 64 pairs, 64 operator rows) tile per block, 4 waves of
+// 32 x 32.  Computed as Y^T = X^T K^T so that a result register holds 16
+// consecutive operator rows of one pair: the atomic adds into L_B of one
+// instruction hit 16 neighbouring elements of one box.  The next K-chunk of
+// the operator and of the gathered multipoles is loaded into registers while
+// the current one is multiplied from LDS.
 template <int P, typename T>
-__global__ __launch_bounds__(256) void k_m2l_gemm(const int *__restrict__ blk_cls, const int *__restrict__ blk_j0,
-                                                  const int *__restrict__ cls_off, const int *__restrict__ pA,
-                                                  const int *__restrict__ pB, const unsigned char *__restrict__ pm,
-                                                  const T *__restrict__ ops, int MRP, int KRP,
-                                                  const double *__restrict__ M, double *__restrict__ L) {
+__global__ __launch_bounds__(256) void k_m2l_gemm(const int *__restrict__ blk_key, const int *__restrict__ blk_j0,
+                                                  const int *__restrict__ key_off, const int *__restrict__ pA,
+                                                  const int *__restrict__ pB, const int *__restrict__ gtab,
+                                                  const int *__restrict__ stab, const T *__restrict__ ops, int MRP,
+                                                  int KRP, const double *__restrict__ M, double *__restrict__ L,
+                                                  int ntiles, int per_xcd) {
     constexpr int R = P * P * P;
     constexpr int R4 = 4 * R;
     constexpr int KC = 32;
     typedef typename Mfma<T>::v4 v4;
-    __shared__ T sA[64][KC + 1];
-    __shared__ T sB[KC][64 + 1];
-    __shared__ int sPA[64], sPB[64], sPM[64];
+    __shared__ T sA[64][KC + 1];   // operator rows x k
+    __shared__ T sB[64][KC + 1];   // pairs x k
+    __shared__ int sPA[64], sPB[64];
+    __shared__ int sS[64];
+    // Workgroups go to the 8 XCDs round robin; renumber so that each XCD
+    // gets one contiguous range of tiles.  Tiles are ordered (class, mask,
+    // pair tile, row tile): the 8 row tiles of a pair tile, and the pair
+    // tiles of one operator class, then run on one XCD and share its L2.
+    const int logical = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const int nrt = MRP / 64;
+    if (logical >= ntiles * nrt) return;
+    const int tile = logical / nrt;
     const int tid = threadIdx.x;
-    const int c = blk_cls[blockIdx.x];
-    const int j0 = blk_j0[blockIdx.x];
-    const int jend = cls_off[c + 1];
-    const int row0 = blockIdx.y * 64;
+    const int key = blk_key[tile];
+    const int cls = key >> 3, m = key & 7;
+    const int j0 = blk_j0[tile];
+    const int jend = key_off[key + 1];
+    const int row0 = (logical % nrt) * 64;
     if (tid < 64) {
         const int j = j0 + tid;
         const bool ok = j < jend;
         sPA[tid] = ok ? pA[j] : -1;
         sPB[tid] = ok ? pB[j] : -1;
-        sPM[tid] = ok ? (int)pm[j] : 0;
+        const int row = row0 + tid;
+        sS[tid] = row < R4 ? stab[m * R4 + row] : 0;
     }
-    __syncthreads();
     const int w = tid >> 6, l = tid & 63;
-    const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
+    const int wr = (w & 1) * 32;  // operator rows of this wave
+    const int wc = (w >> 1) * 32; // pairs of this wave
     v4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = v4{T(0), T(0), T(0), T(0)};
-    const T *Aop = ops + (size_t)c * MRP * KRP + (size_t)row0 * KRP;
-    for (int kc = 0; kc < KRP; kc += KC) {
-        for (int e = tid; e < 64 * KC; e += 256) {
+    const T *Aop = ops + (size_t)cls * MRP * KRP + (size_t)row0 * KRP;
+    const int *gt = gtab + m * R4;
+    // each thread moves 8 elements of each tile per chunk: element e = tid + 256 i,
+    // row/pair e / KC, k e % KC (consecutive threads read consecutive k)
+    T ra[8], rb[8];
+    auto load_chunk = [&](int kc) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int e = tid + 256 * i;
             const int rr = e / KC, kk = e % KC;
-            sA[rr][kk] = Aop[(size_t)rr * KRP + kc + kk];
+            ra[i] = Aop[(size_t)rr * KRP + kc + kk];
         }
-        for (int e = tid; e < 64 * KC; e += 256) {
-            const int kk = e / 64, jj = e % 64;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int e = tid + 256 * i;
+            const int jj = e / KC, kk = e % KC;
             const int k = kc + kk;
-            double v = 0.0;
             const int a = sPA[jj];
+            double v = 0.0;
             if (a >= 0 && k < R4) {
-                const int f = k / R, s = k % R, m = sPM[jj];
-                v = M[(size_t)a * R4 + f * R + flip_node<P>(s, m)];
-                if (f < 3 && ((m >> f) & 1)) v = -v;
+                const int g = gt[k];
+                v = M[(size_t)a * R4 + (g & 0x7fffffff)];
+                if (g < 0) v = -v;
             }
-            sB[kk][jj] = (T)v;
+            rb[i] = (T)v;
+        }
+    };
+    __syncthreads();  // sPA, sPB, sS
+    load_chunk(0);
+    for (int kc = 0; kc < KRP; kc += KC) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int e = tid + 256 * i;
+            sA[e / KC][e % KC] = ra[i];
+            sB[e / KC][e % KC] = rb[i];
         }
         __syncthreads();
+        if (kc + KC < KRP) load_chunk(kc + KC);
 #pragma unroll
         for (int kk = 0; kk < KC; kk += 4) {
-            T af[2], bf[2];
+            T xf[2], kf[2];
 #pragma unroll
-            for (int mi = 0; mi < 2; ++mi) af[mi] = sA[wr + mi * 16 + (l & 15)][kk + (l >> 4)];
+            for (int pi = 0; pi < 2; ++pi) xf[pi] = sB[wc + pi * 16 + (l & 15)][kk + (l >> 4)];
 #pragma unroll
-            for (int ni = 0; ni < 2; ++ni) bf[ni] = sB[kk + (l >> 4)][wc + ni * 16 + (l & 15)];
+            for (int ri = 0; ri < 2; ++ri) kf[ri] = sA[wr + ri * 16 + (l & 15)][kk + (l >> 4)];
 #pragma unroll
-            for (int mi = 0; mi < 2; ++mi)
+            for (int pi = 0; pi < 2; ++pi)
 #pragma unroll
-                for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = Mfma<T>::op(af[mi], bf[ni], acc[mi][ni]);
+                for (int ri = 0; ri < 2; ++ri) acc[pi][ri] = Mfma<T>::op(xf[pi], kf[ri], acc[pi][ri]);
         }
         __syncthreads();
     }
+    // result of tile (pi, ri): register q of lane l holds pair Mfma::row(l, q)
+    // and operator row l & 15
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
+    for (int pi = 0; pi < 2; ++pi)
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni) {
-            const int col = wc + ni * 16 + (l & 15);
-            const int b = sPB[col];
-            if (b < 0) continue;
-            const int m = sPM[col];
+        for (int ri = 0; ri < 2; ++ri) {
+            const int lrow = wr + ri * 16 + (l & 15);
+            const int row = row0 + lrow;
+            if (row >= R4) continue;
+            const int st = sS[lrow];
+            const int dst = st & 0x7fffffff;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int row = row0 + wr + mi * 16 + Mfma<T>::row(l, q);
-                if (row >= R4) continue;
-                const int g = row / R, t = row % R;
-                double v = (double)acc[mi][ni][q];
-                if (g > 0 && ((m >> (g - 1)) & 1)) v = -v;
-                unsafeAtomicAdd(&L[(size_t)b * R4 + g * R + flip_node<P>(t, m)], v);
+                const int col = wc + pi * 16 + Mfma<T>::row(l, q);
+                const int b = sPB[col];
+                if (b < 0) continue;
+                double v = (double)acc[pi][ri][q];
+                if (st < 0) v = -v;
+                unsafeAtomicAdd(&L[(size_t)b * R4 + dst], v);
             }
         }
 }
@@ -279,6 +337,9 @@ int build_ops_p(ibim_plan *pl) {
     const int ncls = pl->n_cls;
     if (ncls == 0) return IBIM_OK;
     const int64_t total = (int64_t)R * R * ncls;
+    int rc;
+    if ((rc = dalloc(pl, &pl->gtab, 8 * 4 * R)) || (rc = dalloc(pl, &pl->stab, 8 * 4 * R))) return rc;
+    k_gather_table<P><<<nblk(8 * 4 * R), 256, 0, pl->stream>>>(pl->gtab, pl->stab);
     IBIM_HIP_CHECK(hipMemsetAsync(pl->ops, 0, (size_t)ncls * pl->MRP * pl->KRP * sizeof(double), pl->stream));
     k_build_ops<P><<<nblk(total), 256, 0, pl->stream>>>(2, ncls, pl->d_lvl_rad, pl->tab, pl->kc, pl->MRP, pl->KRP,
                                                         pl->ops);
@@ -289,13 +350,17 @@ int build_ops_p(ibim_plan *pl) {
 template <int P>
 int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
     if (vp.nblocks == 0) return IBIM_OK;
-    dim3 grid(vp.nblocks, pl->MRP / 64);
+    const int total = vp.nblocks * (pl->MRP / 64);
+    const int per_xcd = (total + 7) / 8;
+    const int grid = 8 * per_xcd;
     if (mixed && pl->ops32)
-        k_m2l_gemm<P, float><<<grid, 256, 0, pl->stream>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, vp.mask,
-                                                           pl->ops32, pl->MRP, pl->KRP, pl->M, pl->L);
+        k_m2l_gemm<P, float><<<grid, 256, 0, pl->stream>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, pl->gtab,
+                                                           pl->stab, pl->ops32, pl->MRP, pl->KRP, pl->M, pl->L,
+                                                           vp.nblocks, per_xcd);
     else
-        k_m2l_gemm<P, double><<<grid, 256, 0, pl->stream>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, vp.mask,
-                                                            pl->ops, pl->MRP, pl->KRP, pl->M, pl->L);
+        k_m2l_gemm<P, double><<<grid, 256, 0, pl->stream>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B,
+                                                            pl->gtab, pl->stab, pl->ops, pl->MRP, pl->KRP, pl->M,
+                                                            pl->L, vp.nblocks, per_xcd);
     IBIM_LAUNCH_CHECK();
     return IBIM_OK;
 }
@@ -381,7 +446,7 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
         IBIM_HIP_CHECK(hipStreamSynchronize(st));
         (void)hipFree(tmp);
         int endbit = 1;
-        while ((1 << endbit) <= pl->n_cls) ++endbit;
+        while ((1 << endbit) <= 8 * pl->n_cls) ++endbit;
         bytes = 0;
         IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key0, key1, iota, order, np, 0, endbit, st));
         IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
@@ -396,20 +461,21 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
     std::vector<int> hkey(np);
     if (np > 0) IBIM_HIP_CHECK(hipMemcpyAsync(hkey.data(), key1, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, st));
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
-    vp.cls_off.assign(pl->n_cls + 1, 0);
+    const int nkeys = 8 * pl->n_cls;  // (class, reflection mask)
+    vp.cls_off.assign(nkeys + 1, 0);
     for (int j = 0; j < np; ++j) vp.cls_off[hkey[j] + 1]++;
-    for (int c = 0; c < pl->n_cls; ++c) vp.cls_off[c + 1] += vp.cls_off[c];
+    for (int c = 0; c < nkeys; ++c) vp.cls_off[c + 1] += vp.cls_off[c];
     std::vector<int> bc, bj;
-    for (int c = 0; c < pl->n_cls; ++c)
+    for (int c = 0; c < nkeys; ++c)
         for (int j = vp.cls_off[c]; j < vp.cls_off[c + 1]; j += 64) {
             bc.push_back(c);
             bj.push_back(j);
         }
     vp.nblocks = (int)bc.size();
-    if ((rc = alloc(&vp.cls_off_d, pl->n_cls + 1)) || (rc = alloc(&vp.blk_cls, std::max(1, vp.nblocks))) ||
+    if ((rc = alloc(&vp.cls_off_d, nkeys + 1)) || (rc = alloc(&vp.blk_cls, std::max(1, vp.nblocks))) ||
         (rc = alloc(&vp.blk_j0, std::max(1, vp.nblocks))))
         return rc;
-    IBIM_HIP_CHECK(hipMemcpyAsync(vp.cls_off_d, vp.cls_off.data(), (pl->n_cls + 1) * sizeof(int),
+    IBIM_HIP_CHECK(hipMemcpyAsync(vp.cls_off_d, vp.cls_off.data(), (nkeys + 1) * sizeof(int),
                                   hipMemcpyHostToDevice, st));
     if (vp.nblocks > 0) {
         IBIM_HIP_CHECK(hipMemcpyAsync(vp.blk_cls, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice, st));

@@ -448,9 +448,20 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
     IBIM_LAUNCH_CHECK();
     double normb = 0.0;
     IBIM_HIP_CHECK(hipMemcpyAsync(&normb, scal + 2, sizeof(double), hipMemcpyDeviceToHost, st));
-    // r = b - A x
-    if ((rc = matvec_device(pl, d_x, p, mixed))) return rc;
-    nmv++;
+    // r = b - A x; for x = 0 this is b exactly (the matvec of 0 is 0), so the
+    // operator is not applied
+    double xnorm = 0.0;
+    k_dot_part<<<kRG, kRB, 0, st>>>(d_x, d_x, n2, partB);
+    k_sqrt_sum<<<1, kRB, 0, st>>>(partB, scal + 3);
+    IBIM_LAUNCH_CHECK();
+    IBIM_HIP_CHECK(hipMemcpyAsync(&xnorm, scal + 3, sizeof(double), hipMemcpyDeviceToHost, st));
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    if (xnorm != 0.0) {
+        if ((rc = matvec_device(pl, d_x, p, mixed))) return rc;
+        nmv++;
+    } else {
+        IBIM_HIP_CHECK(hipMemsetAsync(p, 0, (size_t)n2 * sizeof(double), st));
+    }
     k_residual<<<kRG, kRB, 0, st>>>(d_b, p, r, n2, partA);
     k_sqrt_sum<<<1, kRB, 0, st>>>(partA, scal);
     IBIM_LAUNCH_CHECK();
@@ -563,10 +574,18 @@ static int gmres_refine(ibim_plan *pl, const double *d_b, double *d_x, int resta
     if ((rc = norm(d_b, &normb))) return rc;
     if (normb == 0.0) normb = 1.0;
     int nmv = 0, inner = 0, outer = 0;
+    double xnorm = 0.0;
+    if ((rc = norm(d_x, &xnorm))) return rc;
+    bool x_zero = xnorm == 0.0;
     auto residual = [&]() -> int {  // gm_rr = b - A64 x, rn = |gm_rr|
-        int r2 = matvec_device(pl, d_x, pl->gm_x, false);
-        if (r2) return r2;
-        ++nmv;
+        if (x_zero) {  // A 0 = 0 exactly
+            IBIM_HIP_CHECK(hipMemsetAsync(pl->gm_x, 0, (size_t)n2 * sizeof(double), st));
+            x_zero = false;
+        } else {
+            int r2 = matvec_device(pl, d_x, pl->gm_x, false);
+            if (r2) return r2;
+            ++nmv;
+        }
         k_residual<<<kRG, kRB, 0, st>>>(d_b, pl->gm_x, pl->gm_rr, n2, part);
         k_sqrt_sum<<<1, kRB, 0, st>>>(part, scal);
         IBIM_LAUNCH_CHECK();
