@@ -156,16 +156,12 @@ int ibim_plan_create_device(const ibim_params *prm, int64_t n, const double *d_x
     }
     {
         TargetSet ts = source_targets(pl);
-        if ((rc = build_p2p_ranges(pl, ts, nullptr))) {
+        if ((rc = build_p2p_ranges(pl, ts, pl->p2p, nullptr))) {
             ibim_plan_destroy(pl);
             return rc;
         }
         pl->p2p_roff = ts.roff;
         pl->p2p_ranges = ts.ranges;
-    }
-    if (false) {
-        ibim_plan_destroy(pl);
-        return rc;
     }
     const size_t fields = (size_t)pl->nnodes * 4 * pl->r;
     if ((rc = dalloc(pl, &pl->M, fields)) || (rc = dalloc(pl, &pl->L, fields)) ||

@@ -18,6 +18,9 @@
 //         sources (bbfmm.h:577-595)
 //   L2P   pot += S L_leaf at the standardized target coordinate (bbfmm.h:600)
 #include <cmath>
+#include <type_traits>
+
+#include <hipcub/hipcub.hpp>
 
 #include "ibim_internal.h"
 
@@ -443,124 +446,200 @@ __global__ void k_p2p_ranges(int nleaves, const int *__restrict__ tleaves, const
     }
 }
 
-// ----------------------------------------------------------- P2P, v2 ---
-// One wavefront (one 64-thread block) per target leaf.  The leaf's merged
-// near-field source ranges are streamed in batches of 64 sources: each lane
-// loads one source (coordinates and the 4 charges) into LDS, then the 64
-// lanes -- tp targets x (64/tp) source slices, tp = next power of two >=
-// targets in the chunk -- evaluate the batch from LDS, two sources per step
-// for instruction-level parallelism.  Slices are summed with cross-lane
-// shuffles.  MIXED: FP32 kernel evaluation of the FP64 coordinate
-// difference, Kahan-compensated FP32 sums.  Writes the near field into F;
-// k_l2p adds the far field.
+// ------------------------------------------------------- P2P work units ---
+// The near field of a target leaf is its merged source ranges (U list and
+// the W list's subtree sources).  Those lists differ by orders of magnitude
+// between leaves, so one block per leaf leaves a long tail of a few busy
+// blocks.  The work is therefore cut into units of at most 64 targets x
+// kP2PChunk sources, launched heaviest first; a leaf whose sources span
+// several units writes one partial field per unit, and k_p2p_reduce sums
+// them in unit order (deterministic).
+constexpr int kP2PChunk = 1024;
+
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<double> { using type = double4; };
+
+// Each lane of the 64-thread block is (target mt, source slice sl):
+// nsl = 64 / ntc slices of ntc targets, so 64 - nsl*ntc lanes idle.  Sources
+// are staged in LDS 64 at a time (one per lane).  MIXED: coordinates relative
+// to the unit's first target are formed in FP64 and rounded to FP32, the
+// kernels are evaluated in FP32, each lane's FP32 sums over one LDS batch
+// (at most 64 terms) are accumulated in FP64.
 template <bool MIXED>
-__global__ __launch_bounds__(64) void k_p2p_wave(
-    int nleaves, const int *__restrict__ tleaves, const int *__restrict__ tbeg, const int *__restrict__ tend,
-    const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
-    const double *__restrict__ sx, const double *__restrict__ sy, const double *__restrict__ sz,
-    const double *__restrict__ Q, int ns, const int *__restrict__ roff, const int2 *__restrict__ ranges, PBConst kc,
-    double *__restrict__ F, int nt) {
-    __shared__ double bx[64], by[64], bz[64], bq0[64], bq1[64], bq2[64], bq3[64];
+__global__ __launch_bounds__(64) void k_p2p_unit(
+    const int4 *__restrict__ ua, const int4 *__restrict__ ub, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz, const double *__restrict__ sx,
+    const double *__restrict__ sy, const double *__restrict__ sz, const double *__restrict__ Q, int ns,
+    const int2 *__restrict__ ranges, PBConst kc, double *__restrict__ F, int nt, double *__restrict__ part) {
+    using T = typename std::conditional<MIXED, float, double>::type;
+    using T4 = typename Vec4<T>::type;
+    __shared__ T4 s0[64], s1[64];  // {x, y, z, q_psi}, {q_x, q_y, q_z, -}
+    __shared__ double red[4][64];
     const int lane = threadIdx.x;
-    const int li = blockIdx.x;
-    const int b = tleaves[li];
-    const int r0 = roff[li], r1 = roff[li + 1];
-    int total = 0;
-    for (int r = r0; r < r1; ++r) total += ranges[r].y - ranges[r].x;
-    const int t_beg = tbeg[b], t_end = tend[b];
-    for (int t0 = t_beg; t0 < t_end; t0 += 64) {
-        const int ntc = min(64, t_end - t0);
-        int tp = 1;
-        while (tp < ntc) tp <<= 1;
-        const int nsl = 64 / tp;
-        const int mt = lane & (tp - 1), sl = lane / tp;
-        const bool valid = mt < ntc;
-        const int t = t0 + mt;
-        const double px = valid ? tx[t] : 0.0, py = valid ? ty[t] : 0.0, pz = valid ? tz[t] : 0.0;
-        double f0 = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
-        KahanF k0, k1, k2, k3;
-        int cr = r0, cbase = 0;  // loading cursor of this lane: source number j = batch + lane
-        for (int b0 = 0; b0 < total; b0 += 64) {
-            const int j = b0 + lane;
-            if (j < total) {
-                while (j - cbase >= ranges[cr].y - ranges[cr].x) {
-                    cbase += ranges[cr].y - ranges[cr].x;
-                    ++cr;
-                }
-                const int s = ranges[cr].x + (j - cbase);
-                bx[lane] = sx[s];
-                by[lane] = sy[s];
-                bz[lane] = sz[s];
-                bq0[lane] = Q[s];
-                bq1[lane] = Q[(size_t)ns + s];
-                bq2[lane] = Q[2 * (size_t)ns + s];
-                bq3[lane] = Q[3 * (size_t)ns + s];
+    const int4 a = ua[blockIdx.x], c = ub[blockIdx.x];
+    const int t0 = a.y, ntc = a.z, slot = a.w, total = c.z;
+    const int nsl = 64 / ntc;
+    const int mt = lane % ntc, sl = lane / ntc;
+    const bool active = sl < nsl;
+    const int t = t0 + mt;
+    const double ox = MIXED ? tx[t0] : 0.0, oy = MIXED ? ty[t0] : 0.0, oz = MIXED ? tz[t0] : 0.0;
+    const T px = (T)(tx[t] - ox), py = (T)(ty[t] - oy), pz = (T)(tz[t] - oz);
+    double f0 = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    int cr = c.x, cbase = -c.y;  // this lane's loading cursor: source j sits at ranges[cr].x + j - cbase
+    for (int b0 = 0; b0 < total; b0 += 64) {
+        const int j = b0 + lane;
+        if (j < total) {
+            while (j - cbase >= ranges[cr].y - ranges[cr].x) {
+                cbase += ranges[cr].y - ranges[cr].x;
+                ++cr;
             }
-            __syncthreads();
-            const int cnt = min(64, total - b0);
-            if (valid) {
-                int k = sl;
-                for (; k + nsl < cnt; k += 2 * nsl) {
-                    const int kb = k + nsl;
-                    if (MIXED) {
-                        float a0, ax, ay, az, c0, cx, cy, cz;
-                        pair_accum_f32((float)(bx[k] - px), (float)(by[k] - py), (float)(bz[k] - pz), (float)bq0[k],
-                                       (float)bq1[k], (float)bq2[k], (float)bq3[k], kc, a0, ax, ay, az);
-                        pair_accum_f32((float)(bx[kb] - px), (float)(by[kb] - py), (float)(bz[kb] - pz),
-                                       (float)bq0[kb], (float)bq1[kb], (float)bq2[kb], (float)bq3[kb], kc, c0, cx, cy,
-                                       cz);
-                        k0.add(a0 + c0);
-                        k1.add(ax + cx);
-                        k2.add(ay + cy);
-                        k3.add(az + cz);
-                    } else {
-                        double a0 = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
-                        pair_accum<double>(bx[k] - px, by[k] - py, bz[k] - pz, bq0[k], bq1[k], bq2[k], bq3[k], kc, a0,
-                                           ax, ay, az);
-                        pair_accum<double>(bx[kb] - px, by[kb] - py, bz[kb] - pz, bq0[kb], bq1[kb], bq2[kb], bq3[kb],
-                                           kc, f0, fx, fy, fz);
-                        f0 += a0;
-                        fx += ax;
-                        fy += ay;
-                        fz += az;
-                    }
-                }
-                if (k < cnt) {
-                    if (MIXED) {
-                        float a0, ax, ay, az;
-                        pair_accum_f32((float)(bx[k] - px), (float)(by[k] - py), (float)(bz[k] - pz), (float)bq0[k],
-                                       (float)bq1[k], (float)bq2[k], (float)bq3[k], kc, a0, ax, ay, az);
-                        k0.add(a0);
-                        k1.add(ax);
-                        k2.add(ay);
-                        k3.add(az);
-                    } else {
-                        pair_accum<double>(bx[k] - px, by[k] - py, bz[k] - pz, bq0[k], bq1[k], bq2[k], bq3[k], kc, f0,
-                                           fx, fy, fz);
-                    }
-                }
+            const int s = ranges[cr].x + (j - cbase);
+            T4 u, v;
+            u.x = (T)(sx[s] - ox);
+            u.y = (T)(sy[s] - oy);
+            u.z = (T)(sz[s] - oz);
+            u.w = (T)Q[3 * (size_t)ns + s];
+            v.x = (T)Q[s];
+            v.y = (T)Q[(size_t)ns + s];
+            v.z = (T)Q[2 * (size_t)ns + s];
+            v.w = (T)0;
+            s0[lane] = u;
+            s1[lane] = v;
+        }
+        __syncthreads();
+        const int cnt = min(64, total - b0);
+        if (active) {
+            T g0 = 0, gx = 0, gy = 0, gz = 0, h0 = 0, hx = 0, hy = 0, hz = 0;
+            int k = sl;
+            for (; k + nsl < cnt; k += 2 * nsl) {
+                const T4 u = s0[k], v = s1[k], u2 = s0[k + nsl], v2 = s1[k + nsl];
+                pair_accum<T>(u.x - px, u.y - py, u.z - pz, v.x, v.y, v.z, u.w, kc, g0, gx, gy, gz);
+                pair_accum<T>(u2.x - px, u2.y - py, u2.z - pz, v2.x, v2.y, v2.z, u2.w, kc, h0, hx, hy, hz);
             }
-            __syncthreads();
+            if (k < cnt) {
+                const T4 u = s0[k], v = s1[k];
+                pair_accum<T>(u.x - px, u.y - py, u.z - pz, v.x, v.y, v.z, u.w, kc, g0, gx, gy, gz);
+            }
+            f0 += (double)(g0 + h0);
+            fx += (double)(gx + hx);
+            fy += (double)(gy + hy);
+            fz += (double)(gz + hz);
         }
-        if (MIXED) {
-            f0 = k0.value();
-            fx = k1.value();
-            fy = k2.value();
-            fz = k3.value();
+        __syncthreads();
+    }
+    red[0][lane] = f0;
+    red[1][lane] = fx;
+    red[2][lane] = fy;
+    red[3][lane] = fz;
+    __syncthreads();
+    if (lane < ntc) {
+        double v[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            double acc = red[f][lane];
+            for (int q = 1; q < nsl; ++q) acc += red[f][q * ntc + lane];
+            v[f] = acc;
         }
-        for (int o = tp; o < 64; o <<= 1) {
-            f0 += __shfl_xor(f0, o);
-            fx += __shfl_xor(fx, o);
-            fy += __shfl_xor(fy, o);
-            fz += __shfl_xor(fz, o);
-        }
-        if (valid && sl == 0) {
-            F[t] = f0;
-            F[(size_t)nt + t] = fx;
-            F[2 * (size_t)nt + t] = fy;
-            F[3 * (size_t)nt + t] = fz;
+        if (slot < 0) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) F[f * (size_t)nt + t] = v[f];
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) part[(size_t)slot * 256 + f * 64 + lane] = v[f];
         }
     }
+}
+
+// F of the targets of a leaf whose sources were split over several units:
+// the sum of the units' partial fields, in unit order
+__global__ __launch_bounds__(64) void k_p2p_reduce(const int *__restrict__ tleaves, const int *__restrict__ tbeg,
+                                                   const int *__restrict__ tend, const int2 *__restrict__ lsplit,
+                                                   const double *__restrict__ part, double *__restrict__ F, int nt) {
+    const int2 s = lsplit[blockIdx.x];
+    if (s.y <= 1) return;
+    const int b = tleaves[blockIdx.x];
+    const int nl = tend[b] - tbeg[b];
+    for (int tc = 0; 64 * tc < nl; ++tc) {
+        const int i = 64 * tc + threadIdx.x;
+        if (i >= nl) continue;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            double acc = 0.0;
+            for (int sc = 0; sc < s.y; ++sc) acc += part[(size_t)(s.x + tc * s.y + sc) * 256 + f * 64 + threadIdx.x];
+            F[f * (size_t)nt + tbeg[b] + i] = acc;
+        }
+    }
+}
+
+__device__ __forceinline__ int range_sources(const int *roff, const int2 *ranges, int i) {
+    int ns = 0;
+    for (int r = roff[i]; r < roff[i + 1]; ++r) ns += ranges[r].y - ranges[r].x;
+    return ns;
+}
+
+// units and partial slots per target leaf
+__global__ void k_p2p_count(int nl, const int *__restrict__ tleaves, const int *__restrict__ tbeg,
+                            const int *__restrict__ tend, const int *__restrict__ roff,
+                            const int2 *__restrict__ ranges, int *__restrict__ ucnt, int *__restrict__ pcnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    const int b = tleaves[i];
+    const int ns = range_sources(roff, ranges, i);
+    const int nct = (tend[b] - tbeg[b] + 63) / 64;
+    const int ncs = max(1, (ns + kP2PChunk - 1) / kP2PChunk);
+    ucnt[i] = nct * ncs;
+    pcnt[i] = ncs > 1 ? nct * ncs : 0;
+}
+
+// unit records, their cost (per-lane pair evaluations), and the per-leaf
+// split table read by k_p2p_reduce
+__global__ void k_p2p_fill(int nl, const int *__restrict__ tleaves, const int *__restrict__ tbeg,
+                           const int *__restrict__ tend, const int *__restrict__ roff,
+                           const int2 *__restrict__ ranges, const int *__restrict__ uoff,
+                           const int *__restrict__ poff, int4 *__restrict__ ua, int4 *__restrict__ ub,
+                           int *__restrict__ cost, int *__restrict__ idx, int2 *__restrict__ lsplit) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    const int b = tleaves[i];
+    const int ns = range_sources(roff, ranges, i);
+    const int ntl = tend[b] - tbeg[b];
+    const int nct = (ntl + 63) / 64;
+    const int ncs = max(1, (ns + kP2PChunk - 1) / kP2PChunk);
+    lsplit[i] = make_int2(ncs > 1 ? poff[i] : -1, ncs);
+    int u = uoff[i];
+    int r = roff[i], off = 0;  // cursor at the first source of the chunk
+    for (int sc = 0; sc < ncs; ++sc) {
+        const int cnt = max(0, min(kP2PChunk, ns - sc * kP2PChunk));
+        for (int tc = 0; tc < nct; ++tc) {
+            const int ntc = min(64, ntl - 64 * tc);
+            ua[u] = make_int4(i, tbeg[b] + 64 * tc, ntc, ncs > 1 ? poff[i] + tc * ncs + sc : -1);
+            ub[u] = make_int4(r, off, cnt, 0);
+            cost[u] = cnt / (64 / ntc) + 1;
+            idx[u] = u;
+            ++u;
+        }
+        int adv = cnt;
+        while (adv > 0) {
+            const int left = ranges[r].y - ranges[r].x - off;
+            if (adv < left) {
+                off += adv;
+                adv = 0;
+            } else {
+                adv -= left;
+                ++r;
+                off = 0;
+            }
+        }
+    }
+}
+
+__global__ void k_p2p_order(int n, const int *__restrict__ idx, const int4 *__restrict__ ua,
+                            const int4 *__restrict__ ub, int4 *__restrict__ oa, int4 *__restrict__ ob) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    oa[k] = ua[idx[k]];
+    ob[k] = ub[idx[k]];
 }
 
 // L2P (bbfmm.h:600): F += S L_leaf at each target, one block per target
@@ -696,16 +775,17 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     }
     {
     StageScope sc(pl, kStP2P);
-    if (ts.nleaves > 0 && ts.roff) {
+    if (ts.nleaves > 0 && ts.p2p) {
+        const P2PWork &w = *ts.p2p;
         if (mixed)
-            k_p2p_wave<true><<<ts.nleaves, 64, 0, st>>>(ts.nleaves, ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z,
-                                                        pl->sx, pl->sy, pl->sz, pl->Q, pl->n, ts.roff, ts.ranges,
-                                                        pl->kc, F, nt);
+            k_p2p_unit<true><<<w.nunits, 64, 0, st>>>(w.ua, w.ub, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q,
+                                                      pl->n, ts.ranges, pl->kc, F, nt, w.part);
         else
-            k_p2p_wave<false><<<ts.nleaves, 64, 0, st>>>(ts.nleaves, ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z,
-                                                         pl->sx, pl->sy, pl->sz, pl->Q, pl->n, ts.roff, ts.ranges,
-                                                         pl->kc, F, nt);
+            k_p2p_unit<false><<<w.nunits, 64, 0, st>>>(w.ua, w.ub, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q,
+                                                       pl->n, ts.ranges, pl->kc, F, nt, w.part);
         IBIM_LAUNCH_CHECK();
+        if (w.npart)
+            k_p2p_reduce<<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, w.lsplit, w.part, F, nt);
         k_l2p<P><<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->ctr, pl->level,
                                             pl->d_lvl_rad, pl->tab, pl->L, F, nt);
     }
@@ -722,19 +802,19 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
 }  // namespace
 
 // merged near-field source ranges of every target leaf of a target set
-int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, std::vector<void *> *tmp_owner) {
+int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, P2PWork &work, std::vector<void *> *tmp_owner) {
     ts.roff = nullptr;
     ts.ranges = nullptr;
+    ts.p2p = nullptr;
+    work = P2PWork{};
     if (ts.nleaves == 0) return IBIM_OK;
     hipStream_t st = pl->stream;
     const int nl = ts.nleaves;
-    int *cnt = nullptr, *roff = nullptr;
-    int2 *ranges = nullptr;
     auto alloc = [&](auto **p, size_t n) -> int {
         if (tmp_owner) {
             hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(**p));
             if (e != hipSuccess) {
-                set_error("hipMalloc (p2p ranges)");
+                set_error("hipMalloc (near-field work)");
                 return IBIM_ERR_NOMEM;
             }
             tmp_owner->push_back((void *)*p);
@@ -742,7 +822,27 @@ int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, std::vector<void *> *tmp_owne
         }
         return dalloc(pl, p, n);
     };
-    IBIM_HIP_CHECK(hipMalloc(&cnt, ((size_t)nl + 1) * sizeof(int)));
+    // build-time scratch, freed on return
+    std::vector<void *> scratch;
+    struct Free {
+        std::vector<void *> &v;
+        ~Free() {
+            for (void *p : v) (void)hipFree(p);
+        }
+    } free_scratch{scratch};
+    auto salloc = [&](auto **p, size_t n) -> int {
+        hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(**p));
+        if (e != hipSuccess) {
+            set_error("hipMalloc (near-field work scratch)");
+            return IBIM_ERR_NOMEM;
+        }
+        scratch.push_back((void *)*p);
+        return IBIM_OK;
+    };
+    int rc;
+    int *cnt = nullptr, *roff = nullptr;
+    int2 *ranges = nullptr;
+    if ((rc = salloc(&cnt, (size_t)nl + 1))) return rc;
     IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, ((size_t)nl + 1) * sizeof(int), st));
     k_p2p_ranges<0><<<nblk(nl, 64), 64, 0, st>>>(nl, ts.leaves, pl->loff[0], pl->litem[0], pl->loff[2], pl->litem[2],
                                              pl->flags, pl->sbeg, pl->send, cnt, nullptr, nullptr);
@@ -752,22 +852,58 @@ int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, std::vector<void *> *tmp_owne
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
     std::vector<int> ho(nl + 1, 0);
     for (int i = 0; i < nl; ++i) {
-        if (hc[i] < 0) {  // more than 512 near-field boxes: keep the per-box kernel
-            (void)hipFree(cnt);
-            return IBIM_OK;
-        }
+        if (hc[i] < 0) return IBIM_OK;  // more than 512 near-field boxes: keep the per-box kernel
         ho[i + 1] = ho[i] + hc[i];
     }
-    int rc;
     if ((rc = alloc(&roff, (size_t)nl + 1)) || (rc = alloc(&ranges, (size_t)std::max(1, ho[nl])))) return rc;
     IBIM_HIP_CHECK(hipMemcpyAsync(roff, ho.data(), ho.size() * sizeof(int), hipMemcpyHostToDevice, st));
     k_p2p_ranges<1><<<nblk(nl, 64), 64, 0, st>>>(nl, ts.leaves, pl->loff[0], pl->litem[0], pl->loff[2], pl->litem[2],
                                              pl->flags, pl->sbeg, pl->send, nullptr, roff, ranges);
     IBIM_LAUNCH_CHECK();
+
+    // work units: count, scan, fill, order by cost (descending)
+    int *ucnt = nullptr, *pcnt = nullptr, *uoff = nullptr, *poff = nullptr;
+    if ((rc = salloc(&ucnt, (size_t)nl + 1)) || (rc = salloc(&pcnt, (size_t)nl + 1)) ||
+        (rc = salloc(&uoff, (size_t)nl + 1)) || (rc = salloc(&poff, (size_t)nl + 1)))
+        return rc;
+    IBIM_HIP_CHECK(hipMemsetAsync(ucnt + nl, 0, sizeof(int), st));
+    IBIM_HIP_CHECK(hipMemsetAsync(pcnt + nl, 0, sizeof(int), st));
+    k_p2p_count<<<nblk(nl, 64), 64, 0, st>>>(nl, ts.leaves, ts.tbeg, ts.tend, roff, ranges, ucnt, pcnt);
+    IBIM_LAUNCH_CHECK();
+    size_t bytes = 0, b2 = 0;
+    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ucnt, uoff, nl + 1, st));
+    void *tmp = nullptr;
+    if ((rc = salloc((char **)&tmp, bytes))) return rc;
+    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, ucnt, uoff, nl + 1, st));
+    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, pcnt, poff, nl + 1, st));
+    int tot[2];
+    IBIM_HIP_CHECK(hipMemcpyAsync(&tot[0], uoff + nl, sizeof(int), hipMemcpyDeviceToHost, st));
+    IBIM_HIP_CHECK(hipMemcpyAsync(&tot[1], poff + nl, sizeof(int), hipMemcpyDeviceToHost, st));
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
-    (void)hipFree(cnt);
+    const int nu = tot[0];
+    int4 *ua = nullptr, *ub = nullptr, *sa = nullptr, *sb = nullptr;
+    int *cost = nullptr, *idx = nullptr, *cost2 = nullptr, *idx2 = nullptr;
+    if ((rc = salloc(&ua, nu)) || (rc = salloc(&ub, nu)) || (rc = salloc(&cost, nu)) || (rc = salloc(&idx, nu)) ||
+        (rc = salloc(&cost2, nu)) || (rc = salloc(&idx2, nu)) || (rc = alloc(&sa, nu)) || (rc = alloc(&sb, nu)) ||
+        (rc = alloc(&work.lsplit, nl)) || (rc = alloc(&work.part, std::max<size_t>(1, (size_t)tot[1] * 256))))
+        return rc;
+    k_p2p_fill<<<nblk(nl, 64), 64, 0, st>>>(nl, ts.leaves, ts.tbeg, ts.tend, roff, ranges, uoff, poff, ua, ub, cost,
+                                           idx, work.lsplit);
+    IBIM_LAUNCH_CHECK();
+    IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b2, cost, cost2, idx, idx2, nu, 0, 31, st));
+    void *tmp2 = nullptr;
+    if ((rc = salloc((char **)&tmp2, b2))) return rc;
+    IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(tmp2, b2, cost, cost2, idx, idx2, nu, 0, 31, st));
+    k_p2p_order<<<nblk(nu, 256), 256, 0, st>>>(nu, idx2, ua, ub, sa, sb);
+    IBIM_LAUNCH_CHECK();
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    work.nunits = nu;
+    work.ua = sa;
+    work.ub = sb;
+    work.npart = (size_t)tot[1];
     ts.roff = roff;
     ts.ranges = ranges;
+    ts.p2p = &work;
     return IBIM_OK;
 }
 
@@ -853,6 +989,7 @@ TargetSet source_targets(ibim_plan *pl) {
     ts.vpairs = pl->ops ? &pl->vp : nullptr;
     ts.roff = pl->p2p_roff;
     ts.ranges = pl->p2p_ranges;
+    ts.p2p = pl->p2p.ua ? &pl->p2p : nullptr;
     return ts;
 }
 

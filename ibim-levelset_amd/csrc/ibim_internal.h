@@ -118,25 +118,26 @@ __device__ __forceinline__ void pair_accum(T dx, T dy, T dz, T qx, T qy, T qz, T
 }
 
 // FP32 form of pair_accum for the mixed-precision near field: the caller
-// forms d = s - t in FP64 (no cancellation of absolute coordinates) and
-// rounds it; reciprocal square root and exp use the hardware FP32 units.
-__device__ __forceinline__ void pair_accum_f32(float dx, float dy, float dz, float qx, float qy, float qz, float qp,
-                                               const PBConst &k, float &f0, float &fx, float &fy, float &fz) {
+// rounds an FP64-formed coordinate difference; reciprocal square root and
+// exp use the hardware FP32 units.
+template <>
+__device__ __forceinline__ void pair_accum<float>(float dx, float dy, float dz, float qx, float qy, float qz,
+                                                  float qp, const PBConst &k, float &f0, float &fx, float &fy,
+                                                  float &fz) {
     const float r2 = dx * dx + dy * dy + dz * dz;
     if (k.mode == IBIM_KERNEL_POLY) {
         const float dq = dx * qx + dy * qy + dz * qz;
-        f0 = dq - (0.5f + 0.25f * r2) * qp;
+        f0 += dq - (0.5f + 0.25f * r2) * qp;
         const float s = 0.5f * dq + 0.75f * qp;
-        fx = qx + dx * s;
-        fy = qy + dy * s;
-        fz = qz + dz * s;
+        fx += qx + dx * s;
+        fy += qy + dy * s;
+        fz += qz + dz * s;
         return;
     }
     const float ir = rsqrtf(r2);
     const float r = r2 * ir;
     if (!(r >= (float)k.vr)) {  // also catches r2 == 0
-        f0 = -(float)k.k2reg * qp;
-        fx = fy = fz = 0.0f;
+        f0 -= (float)k.k2reg * qp;
         return;
     }
     const float ir2 = ir * ir, ir3 = ir2 * ir;
@@ -150,24 +151,12 @@ __device__ __forceinline__ void pair_accum_f32(float dx, float dy, float dz, flo
     const float b3 = (e * ((t + 3.0f) * t + 3.0f) - 3.0f) * ir3 * ir2;
     const float g4 = ((float)k.inv_eps * et1 - 1.0f) * c4;
     const float dq = dx * qx + dy * qy + dz * qz;
-    f0 = g1 * dq - k2 * qp;
+    f0 += g1 * dq - k2 * qp;
     const float s = b3 * dq + g4 * qp;
-    fx = a3 * qx + dx * s;
-    fy = a3 * qy + dy * s;
-    fz = a3 * qz + dz * s;
+    fx += a3 * qx + dx * s;
+    fy += a3 * qy + dy * s;
+    fz += a3 * qz + dz * s;
 }
-
-// Kahan-compensated FP32 accumulator
-struct KahanF {
-    float s = 0.0f, c = 0.0f;
-    __device__ __forceinline__ void add(float v) {
-        const float y = v - c;
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
-};
 
 // S_p(u, xbar_m) for m < P at one standardized coordinate u (bbfmm.h:408-419)
 template <int P>
@@ -238,6 +227,18 @@ struct TargetSet {
     const VPairs *vpairs = nullptr;                         // V interactions into those nodes
     const int *roff = nullptr;                              // per target leaf: merged near-field
     const int2 *ranges = nullptr;                           //   source ranges [roff[i], roff[i+1])
+    const struct P2PWork *p2p = nullptr;                    // near-field work units over those ranges
+};
+
+// Near-field work units of a target set (at most 64 targets x kP2PChunk
+// sources each, heaviest first) and the partial fields of split leaves
+struct P2PWork {
+    int nunits = 0;
+    int4 *ua = nullptr;     // {target leaf index, first target, targets, partial slot or -1}
+    int4 *ub = nullptr;     // {first range, offset in it, sources, 0}
+    int2 *lsplit = nullptr; // per target leaf: {first partial slot, source chunks}
+    double *part = nullptr; // 256 doubles (4 fields x 64 targets) per partial slot
+    size_t npart = 0;
 };
 
 }  // namespace ibim
@@ -299,6 +300,7 @@ struct ibim_plan {
     ibim::VPairs vp;                     // V pairs for the points as their own targets
     const int *p2p_roff = nullptr;       // merged near-field ranges of the non-empty leaves
     const int2 *p2p_ranges = nullptr;
+    ibim::P2PWork p2p;                   // near-field work units of the plan's own points
 
     ibim::ChebTab tab{};
     ibim::PBConst kc{};
@@ -341,7 +343,7 @@ int build_m2l_operators(ibim_plan *pl);
 int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vector<void *> *tmp_owner);
 int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed);
 int build_m2l_operators_f32(ibim_plan *pl);
-int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, std::vector<void *> *tmp_owner);
+int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, P2PWork &work, std::vector<void *> *tmp_owner);
 
 // scoped stage timing on the plan's stream (no-op when profiling is off)
 struct StageScope {

@@ -383,7 +383,8 @@ int target_fmm_device(ibim_plan *pl, int kid, int m, const double *d_axyz, const
         rc = build_vpairs(pl, ts.need_local, vp, &tmp);
         ts.vpairs = &vp;
     }
-    if (rc == IBIM_OK) rc = build_p2p_ranges(pl, ts, &tmp);
+    P2PWork work;
+    if (rc == IBIM_OK) rc = build_p2p_ranges(pl, ts, work, &tmp);
     if (rc == IBIM_OK) rc = kid < 0 ? form_charges(pl, d_in) : single_charge_device(pl, kid, d_in);
     if (rc == IBIM_OK) rc = fmm_forward(pl, ts, F, m);
     if (rc == IBIM_OK) {
