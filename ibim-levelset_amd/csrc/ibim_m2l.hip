@@ -253,29 +253,24 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(const int *__restrict__ blk_ke
         for (int b = 0; b < 2; ++b) acc[a][b] = v4{T(0), T(0), T(0), T(0)};
     const T *Aop = ops + (size_t)cls * MRP * KRP + (size_t)row0 * KRP;
     const int *gt = gtab + m * R4;
-    // each thread moves 8 elements of each tile per chunk: element e = tid + 256 i,
-    // row/pair e / KC, k e % KC (consecutive threads read consecutive k)
+    // each thread moves 8 elements of each tile per chunk: k = kc + (tid & 31)
+    // of rows (tid >> 5) + 8 i.  Loads are unconditional (clamped indices,
+    // masked values) so that all 16 of them are in flight together.
+    const int kk = tid & (KC - 1), r8 = tid >> 5;
     T ra[8], rb[8];
     auto load_chunk = [&](int kc) {
+        const int k = kc + kk;
+        const int g = gt[min(k, R4 - 1)];
+        const int gi = g & 0x7fffffff;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) ra[i] = Aop[(size_t)(r8 + 8 * i) * KRP + k];
+        double v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = M[(size_t)max(sPA[r8 + 8 * i], 0) * R4 + gi];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int e = tid + 256 * i;
-            const int rr = e / KC, kk = e % KC;
-            ra[i] = Aop[(size_t)rr * KRP + kc + kk];
-        }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int e = tid + 256 * i;
-            const int jj = e / KC, kk = e % KC;
-            const int k = kc + kk;
-            const int a = sPA[jj];
-            double v = 0.0;
-            if (a >= 0 && k < R4) {
-                const int g = gt[k];
-                v = M[(size_t)a * R4 + (g & 0x7fffffff)];
-                if (g < 0) v = -v;
-            }
-            rb[i] = (T)v;
+            const bool ok = sPA[r8 + 8 * i] >= 0 && k < R4;
+            rb[i] = ok ? (T)(g < 0 ? -v[i] : v[i]) : T(0);
         }
     };
     __syncthreads();  // sPA, sPB, sS
@@ -283,9 +278,8 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(const int *__restrict__ blk_ke
     for (int kc = 0; kc < KRP; kc += KC) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int e = tid + 256 * i;
-            sA[e / KC][e % KC] = ra[i];
-            sB[e / KC][e % KC] = rb[i];
+            sA[r8 + 8 * i][kk] = ra[i];
+            sB[r8 + 8 * i][kk] = rb[i];
         }
         __syncthreads();
         if (kc + KC < KRP) load_chunk(kc + KC);
