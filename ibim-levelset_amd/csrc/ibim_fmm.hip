@@ -231,8 +231,11 @@ __global__ __launch_bounds__(256) void k_l2l(int lo, const unsigned char *__rest
 // ------------------------------------------------------ M2L, direct form ---
 // L_B = sum over non-empty A in V(B) and X(B) of K(grid_B, grid_A) M_A, the
 // kernel matrix evaluated at the tensor Chebyshev grids of both boxes
-// (kernelEvalChebyshev, bbfmm.h:456-479).
-template <int P, int BS>
+// (kernelEvalChebyshev, bbfmm.h:456-479).  T = float (mixed precision):
+// node coordinates relative to B's centre are formed in FP64 and rounded,
+// the kernels are evaluated in FP32, and each source box's FP32 sums are
+// accumulated in FP64.
+template <int P, int BS, typename T>
 __global__ __launch_bounds__(BS) void k_m2l_direct(int lists, const int *__restrict__ work, const int *__restrict__ voff,
                                                    const int *__restrict__ vit, const int *__restrict__ xoff,
                                                    const int *__restrict__ xit, const int *__restrict__ flags,
@@ -241,8 +244,9 @@ __global__ __launch_bounds__(BS) void k_m2l_direct(int lists, const int *__restr
                                                    const double *__restrict__ M, double *__restrict__ L) {
     constexpr int R = P * P * P;
     constexpr int PER = (R + BS - 1) / BS;
-    __shared__ double spx[R], spy[R], spz[R];
-    __shared__ double sm[4][R];
+    constexpr bool F64 = std::is_same<T, double>::value;
+    __shared__ T spx[R], spy[R], spz[R];
+    __shared__ T sm[4][R];
     __shared__ double sxb[P];
     const int id = work[blockIdx.x];
     const int tid = threadIdx.x;
@@ -258,14 +262,17 @@ __global__ __launch_bounds__(BS) void k_m2l_direct(int lists, const int *__restr
     }
     __syncthreads();
     const double *rb = lrad + 3 * level[id];
-    double tx[PER], ty[PER], tz[PER], acc[PER][4];
+    // FP64: absolute coordinates, as the reference; FP32: relative to B's centre
+    const double ox = F64 ? 0.0 : ctr[3 * id], oy = F64 ? 0.0 : ctr[3 * id + 1], oz = F64 ? 0.0 : ctr[3 * id + 2];
+    T tx[PER], ty[PER], tz[PER];
+    double acc[PER][4];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int m = tid + BS * k;
         const int i = m % P, j = (m / P) % P, l = (m / (P * P)) % P;
-        tx[k] = ctr[3 * id] + rb[0] * sxb[i];
-        ty[k] = ctr[3 * id + 1] + rb[1] * sxb[j];
-        tz[k] = ctr[3 * id + 2] + rb[2] * sxb[l];
+        tx[k] = (T)(ctr[3 * id] + rb[0] * sxb[i] - ox);
+        ty[k] = (T)(ctr[3 * id + 1] + rb[1] * sxb[j] - oy);
+        tz[k] = (T)(ctr[3 * id + 2] + rb[2] * sxb[l] - oz);
 #pragma unroll
         for (int f = 0; f < 4; ++f) acc[k][f] = 0.0;
     }
@@ -277,27 +284,38 @@ __global__ __launch_bounds__(BS) void k_m2l_direct(int lists, const int *__restr
             const int a = it[q];
             if (flags[a] & kEmpty) continue;
             const double *ra = lrad + 3 * level[a];
-            const double ax = ctr[3 * a], ay = ctr[3 * a + 1], az = ctr[3 * a + 2];
+            const double ax = ctr[3 * a] - ox, ay = ctr[3 * a + 1] - oy, az = ctr[3 * a + 2] - oz;
             for (int e = tid; e < R; e += BS) {
                 const int i = e % P, j = (e / P) % P, l = e / (P * P);
-                spx[e] = ax + ra[0] * sxb[i];
-                spy[e] = ay + ra[1] * sxb[j];
-                spz[e] = az + ra[2] * sxb[l];
+                spx[e] = (T)(ax + ra[0] * sxb[i]);
+                spy[e] = (T)(ay + ra[1] * sxb[j]);
+                spz[e] = (T)(az + ra[2] * sxb[l]);
             }
             const double *Ma = M + (size_t)a * 4 * R;
-            for (int e = tid; e < 4 * R; e += BS) sm[e / R][e % R] = Ma[e];
+            for (int e = tid; e < 4 * R; e += BS) sm[e / R][e % R] = (T)Ma[e];
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
                 if (tid + BS * k < R) {
-                    double f0 = acc[k][0], fx = acc[k][1], fy = acc[k][2], fz = acc[k][3];
-                    for (int s = 0; s < R; ++s)
-                        pair_accum<double>(spx[s] - tx[k], spy[s] - ty[k], spz[s] - tz[k], sm[0][s], sm[1][s],
-                                           sm[2][s], sm[3][s], kc, f0, fx, fy, fz);
-                    acc[k][0] = f0;
-                    acc[k][1] = fx;
-                    acc[k][2] = fy;
-                    acc[k][3] = fz;
+                    if constexpr (F64) {
+                        double f0 = acc[k][0], fx = acc[k][1], fy = acc[k][2], fz = acc[k][3];
+                        for (int s = 0; s < R; ++s)
+                            pair_accum<double>(spx[s] - tx[k], spy[s] - ty[k], spz[s] - tz[k], sm[0][s], sm[1][s],
+                                               sm[2][s], sm[3][s], kc, f0, fx, fy, fz);
+                        acc[k][0] = f0;
+                        acc[k][1] = fx;
+                        acc[k][2] = fy;
+                        acc[k][3] = fz;
+                    } else {
+                        T f0 = 0, fx = 0, fy = 0, fz = 0;
+                        for (int s = 0; s < R; ++s)
+                            pair_accum<T>(spx[s] - tx[k], spy[s] - ty[k], spz[s] - tz[k], sm[0][s], sm[1][s],
+                                          sm[2][s], sm[3][s], kc, f0, fx, fy, fz);
+                        acc[k][0] += (double)f0;
+                        acc[k][1] += (double)fx;
+                        acc[k][2] += (double)fy;
+                        acc[k][3] += (double)fz;
+                    }
                 }
             }
             __syncthreads();
@@ -754,10 +772,16 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     StageScope sc(pl, ts.vpairs ? kStM2LX : kStM2L);
     // X list (cross-level) directly; V list as GEMMs when the pairs are grouped
     const int lists = ts.vpairs ? 2 : 3;
-    if (ts.n_m2l_work > 0)
-        k_m2l_direct<P, BS><<<ts.n_m2l_work, BS, 0, st>>>(lists, ts.m2l_work, pl->loff[1], pl->litem[1], pl->loff[3],
-                                                          pl->litem[3], pl->flags, pl->ctr, pl->level,
-                                                          pl->d_lvl_rad, pl->tab, pl->kc, pl->M, pl->L);
+    if (ts.n_m2l_work > 0) {
+        if (mixed && ts.vpairs)
+            k_m2l_direct<P, BS, float><<<ts.n_m2l_work, BS, 0, st>>>(
+                lists, ts.m2l_work, pl->loff[1], pl->litem[1], pl->loff[3], pl->litem[3], pl->flags, pl->ctr,
+                pl->level, pl->d_lvl_rad, pl->tab, pl->kc, pl->M, pl->L);
+        else
+            k_m2l_direct<P, BS, double><<<ts.n_m2l_work, BS, 0, st>>>(
+                lists, ts.m2l_work, pl->loff[1], pl->litem[1], pl->loff[3], pl->litem[3], pl->flags, pl->ctr,
+                pl->level, pl->d_lvl_rad, pl->tab, pl->kc, pl->M, pl->L);
+    }
     IBIM_LAUNCH_CHECK();
     }
     if (ts.vpairs) {

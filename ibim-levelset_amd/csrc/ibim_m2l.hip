@@ -207,7 +207,9 @@ __global__ void k_gather_table(int *__restrict__ gtab, int *__restrict__ stab) {
 // the operator and of the gathered multipoles is loaded into registers while
 // the current one is multiplied from LDS.
 template <int P, typename T>
-__global__ __launch_bounds__(256) void k_m2l_gemm(const int *__restrict__ blk_key, const int *__restrict__ blk_j0,
+// four waves per SIMD: the register budget (at most 128) keeps the chunk
+// prefetch without spilling
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_m2l_gemm(const int *__restrict__ blk_key, const int *__restrict__ blk_j0,
                                                   const int *__restrict__ key_off, const int *__restrict__ pA,
                                                   const int *__restrict__ pB, const int *__restrict__ gtab,
                                                   const int *__restrict__ stab, const T *__restrict__ ops, int MRP,

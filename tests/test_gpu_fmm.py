@@ -230,3 +230,27 @@ def test_bad_arguments_fail_loudly(ibim, gpu):
     with ibim.Plan(pts, nrm, w, p=4, vacant_radius=0.1) as plan:
         with pytest.raises(ibim.IbimError):
             plan.kernel_fmm(16, np.zeros(len(pts)))
+
+
+def test_near_field_split_into_units(ibim, oracle, gpu):
+    """Leaves holding more than 64 targets and more than 1024 near-field
+    sources (max_level caps the tree at 64 leaves of ~125 points): each
+    leaf's near field is cut into several work units whose partial fields are
+    summed, in FP64 and in the mixed path."""
+    rng = np.random.default_rng(31)
+    n = 8000
+    pts = rng.uniform(-1, 1, size=(n, 3))
+    nrm = rng.normal(size=pts.shape)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    w = np.full(n, 1e-3)
+    x = rng.uniform(-1, 1, size=2 * n)
+    with ibim.Plan(pts, nrm, w, p=4, max_points=64, max_level=2, **PB) as plan:
+        t = plan.tree()
+        leaves = t["leaf"].astype(bool)
+        assert t["nsrc"][leaves].max() > 64
+        g = plan.matvec(x)
+        plan.set_precision(ibim.PREC_MIXED)
+        gm = plan.matvec(x)
+    o = oracle.dmapping(pts, nrm, w, x, p=4, max_point=64, max_level=2, **PB)
+    assert rel(g, o) < TOL, rel(g, o)
+    assert rel(gm, o) < 1e-5, rel(gm, o)
