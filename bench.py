@@ -1,27 +1,36 @@
 #!/usr/bin/env python3
-"""Benchmark: PB solves/s on one MI355X.
+"""Benchmark: PB solves/s and latency on one MI355X, mixed precision vs FP64.
 
 Workload (default): the synthetic 500-atom globule -- seeded self-avoiding
 walk of 3.8 A steps in a 20 A sphere, radii U[1.2,2.0] A, charges U[-0.5,0.5]
 shifted to net -2 e -- meshed at h = 0.25 A (tube width 2h, probe 1.4 A),
 eps_in = 2, eps_out = 80, kappa = 0.1257 /A, Chebyshev order p = 5, leaf
-capacity 64, max level 10, tau = 1, GMRES(40) to 1e-8, FP64 throughout.
+capacity 192 (the fastest FP64 matvec of a 128..384 sweep), max level 10,
+tau = 1, GMRES(40) to 1e-8.
 
 One step = one complete PB solve with every input already in HBM (surface
-nodes, normals, weights, atoms): octree + interaction lists, right-hand side,
-GMRES with the FMM matvec, reaction potential at the atoms and the
-solvation energy.  The level-set meshing runs once before the timed region.
+nodes, normals, weights, atoms): octree + interaction lists + M2L operators,
+right-hand side, GMRES with the FMM matvec, reaction potential at the atoms
+and the solvation energy.  The level-set meshing runs once before the timed
+region.  The solve is timed in FP64 throughout and in the mixed form (FP32
+FMM inside FP64 iterative refinement); `value` is the mixed solves/s, valid
+when its energy is within 1e-6 of the FP64 solve's (`energy_within_1e-6`).
 
-Prints one JSON line.  The roofline entry is computed live from HIP events
-recorded on the library's stream around its dominant stage; the cpu_baseline
-entry times the CPU oracle (a restatement of the reference algorithm) on a
-bounded sample of the same surface.
+A served leg then pushes a burst of seeded globules of the same size through
+the continuous-batching server (geometry included per request) and reports
+its solves/s and p50/p99 latency.
+
+Prints one JSON line.  The roofline entry is computed from HIP events that
+the library records on its own stream around each FMM stage during the timed
+mixed solves (the V-list M2L GEMM launch is the dominant kernel); its traffic
+figure comes from the committed rocprofv3 counter summary when present.  The
+cpu_baseline entry times the CPU oracle (a restatement of the reference
+algorithm) on a bounded sample of the same surface.
 """
 import argparse
 import json
 import os
 import platform
-import subprocess
 import sys
 import time
 
@@ -29,12 +38,13 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
-PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
-PEAK_HBM_GBS = 8000.0
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}   # MI355X dense MFMA
+METRIC = "PB solves/s + p50/p99 latency, mixed vs FP64 GPU at <=1e-6 rel energy error"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "m2l_gemm_f32_globule500_traffic.json")
 
 CONFIGS = {
     # name: molecule (atoms, seed, confinement), grid h, p, leaf, kappa, eps_in, eps_out
-    "globule500": dict(atoms=500, seed=500, confine=20.0, h=0.25, p=5, leaf=64, max_level=10, kappa=0.1257,
+    "globule500": dict(atoms=500, seed=500, confine=20.0, h=0.25, p=5, leaf=192, max_level=10, kappa=0.1257,
                        eps_in=2.0, eps_out=80.0, tau=1.0, restart=40, tol=1e-8),
 }
 
@@ -60,19 +70,18 @@ def cpu_model():
     return platform.processor()
 
 
-def run_cpu_baseline(cfg, pts, nrm, w, matvecs_per_solve, budget_s, threads):
+def run_cpu_baseline(cfg, pts, nrm, w, matvecs_per_solve, threads):
     """Time the CPU oracle on a bounded sample: one of the 16 kernel FMMs (K2,
     tree build included, as the reference does per kernel) on the surface
     points of one octant of the molecule, then scale to the full solve:
     16 kernels per matvec, points/sample points (the FMM is O(N)), and the
-    solve's matvec count plus 4 FMMs for the energy."""
+    FP64 solve's matvec count plus 4 FMMs for the energy."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     c = np.median(pts, axis=0)
     sel = np.all(pts > c, axis=1)
     sample = pts[sel]
     q = np.random.default_rng(0).normal(size=len(sample))
-    env_threads = os.environ.get("OMP_NUM_THREADS")
     t0 = time.perf_counter()
     tree = O.Tree(sample, max_point=cfg["leaf"], max_level=cfg["max_level"])
     tree.fmm(cfg["p"], "K2", q, kappa=cfg["kappa"], eps_in=cfg["eps_in"], eps_out=cfg["eps_out"],
@@ -83,10 +92,14 @@ def run_cpu_baseline(cfg, pts, nrm, w, matvecs_per_solve, budget_s, threads):
     t_matvec = 16 * t_fmm * scale
     t_solve = (matvecs_per_solve + 0.25) * t_matvec
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle K2 FMM (tree build + up/down pass) on the {len(sample)} surface points of one octant "
-                       f"({t_fmm:.2f} s on {threads} threads, {cpu_model()}); scaled x16 kernels x{scale:.2f} points "
-                       f"x({matvecs_per_solve}+0.25) matvecs per solve"),
-            "matvec_s_est": t_matvec, "omp_threads_env": env_threads}
+            "sample": (f"oracle K2 FMM (tree build + up/down pass, OpenMP) on the {len(sample)} surface points of "
+                       f"one octant ({t_fmm:.2f} s on {threads} threads, {cpu_model()}); scaled x16 kernels "
+                       f"x{scale:.2f} points x({matvecs_per_solve}+0.25) matvecs per FP64 solve"),
+            "matvec_s_est": t_matvec}
+
+
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
 
 
 def main():
@@ -96,9 +109,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="globule500", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-budget", type=float, default=30.0)
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision solves")
+    ap.add_argument("--serve", type=int, default=8, help="requests in the served burst (0 = skip)")
+    ap.add_argument("--slots", type=int, default=2, help="server slots of the served burst")
     ap.add_argument("--leaf", type=int, default=None, help="override the leaf capacity")
+    ap.add_argument("--matvecs-only", type=int, default=0, metavar="K",
+                    help="counter-collection mode: one 256 MiB device copy (calibration), then K mixed and K FP64 "
+                         "matvecs on the workload's plan; prints a short JSON line, not the benchmark line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (solves of different molecules "
@@ -132,8 +148,28 @@ def main():
     import ctypes as C
     P = lambda t: C.c_void_p(t.data_ptr())
     m = cfg["atoms"]
+    stage_acc = {}
 
-    def solve(profile=False, precision=ibim.PREC_FP64):
+    if args.matvecs_only:
+        cal = torch.ones(32 << 20, dtype=torch.float64, device=dev)  # 256 MiB
+        cal2 = torch.empty_like(cal)
+        cal2.copy_(cal)
+        plan = ibim.Plan(device_ptrs=ptrs, p=cfg["p"], max_points=cfg["leaf"], max_level=cfg["max_level"],
+                         kappa=cfg["kappa"], eps_in=cfg["eps_in"], eps_out=cfg["eps_out"],
+                         vacant_radius=cfg["tau"] * cfg["h"], precision=ibim.PREC_MIXED)
+        d_x.uniform_(-1, 1)
+        for prec in (ibim.PREC_MIXED, ibim.PREC_FP64):
+            plan.set_precision(prec)
+            for _ in range(args.matvecs_only):
+                plan.matvec_device(P(d_x), P(d_b))
+        plan.sync()
+        print(json.dumps({"mode": "matvecs-only", "copy_bytes": 2 * cal.numel() * 8, "matvecs": args.matvecs_only,
+                          "tree": plan.stats()}))
+        plan.close()
+        surf.close()
+        return
+
+    def solve(precision, profile=False):
         plan = ibim.Plan(device_ptrs=ptrs, p=cfg["p"], max_points=cfg["leaf"], max_level=cfg["max_level"],
                          kappa=cfg["kappa"], eps_in=cfg["eps_in"], eps_out=cfg["eps_out"],
                          vacant_radius=cfg["tau"] * cfg["h"], precision=precision)
@@ -141,95 +177,108 @@ def main():
             plan.profile(True)
         plan.rhs_device(m, P(d_atoms), P(d_q), P(d_b))
         d_x.zero_()
-        torch.cuda.synchronize()
         rc, info = plan.gmres_device(P(d_b), P(d_x), restart=cfg["restart"], maxit=2000, tol=cfg["tol"])
         plan.polarize_device(m, P(d_atoms), P(d_x), P(d_phi))
         plan.sync()
         energy = 0.5 * float(torch.dot(d_phi, d_q).item())
-        stages = plan.stage_times() if profile else None
-        stats = plan.stats() if profile else None
+        if profile:
+            for k, (ms, cnt) in plan.stage_times().items():
+                a = stage_acc.setdefault(k, [0.0, 0])
+                a[0] += ms
+                a[1] += cnt
+        stats = plan.stats()
         foot = plan.footprint()
         plan.close()
-        return rc, info, energy, stages, stats, foot
+        return rc, info, energy, stats, foot
 
-    for _ in range(args.warmup):
-        solve()
-    torch.cuda.synchronize()
-    lat = []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        rc, info, energy, _, _, foot = solve()
-        lat.append(time.perf_counter() - ts)
-    torch.cuda.synchronize()
-    total = time.perf_counter() - t_start
-
-    # the mixed-precision form of the same solve (FP32 far and near field
-    # inside FP64 iterative refinement), timed the same way
-    mixed = None
-    if not args.no_mixed:
+    def leg(precision, profile=False):
         for _ in range(args.warmup):
-            solve(precision=ibim.PREC_MIXED)
+            solve(precision)
         torch.cuda.synchronize()
-        lat_m = []
-        tm0 = time.perf_counter()
+        lat = []
+        t_start = time.perf_counter()
         for _ in range(args.steps):
             ts = time.perf_counter()
-            rc_m, info_m, energy_m, _, _, _ = solve(precision=ibim.PREC_MIXED)
-            lat_m.append(time.perf_counter() - ts)
-        total_m = time.perf_counter() - tm0
-        lat_m = sorted(1e3 * x for x in lat_m)
-        mixed = {"solves_per_s": args.steps / total_m, "speedup_vs_fp64": (args.steps / total_m) / (args.steps / total),
-                 "energy": energy_m, "rel_energy_err_vs_fp64": abs(energy_m - energy) / abs(energy),
-                 "latency_ms": {"p50": float(np.percentile(lat_m, 50)), "p99": float(np.percentile(lat_m, 99))},
-                 "solve": info_m}
+            rc, info, energy, stats, foot = solve(precision, profile)
+            lat.append(1e3 * (time.perf_counter() - ts))
+        torch.cuda.synchronize()
+        total = time.perf_counter() - t_start
+        return {"solves_per_s": args.steps / total, "ms_per_solve": 1e3 * total / args.steps,
+                "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "all": lat}, "energy": energy,
+                "solve": dict(info, rc=rc), "plan_bytes": foot}, stats
 
-    # one more solve with stage timers on (outside the timed region)
-    rc_p, info_p, energy_p, stages, stats, _ = solve(profile=True)
+    fp64, stats = leg(ibim.PREC_FP64)
+    mixed, _ = leg(ibim.PREC_MIXED, profile=True)
+    rel_err = abs(mixed["energy"] - fp64["energy"]) / abs(fp64["energy"])
+    energy_ok = bool(rel_err <= 1e-6 and mixed["solve"]["converged"] == 1 and fp64["solve"]["converged"] == 1)
+
+    # ---- served burst: seeded globules of the same size through the server
+    serving = None
+    if args.serve > 0:
+        mols = [ibim.synthetic_globule(cfg["atoms"], cfg["seed"] + 1 + i, cfg["confine"]) for i in range(args.serve)]
+        fmm = dict(p=cfg["p"], max_points=cfg["leaf"], max_level=cfg["max_level"], kappa=cfg["kappa"],
+                   eps_in=cfg["eps_in"], eps_out=cfg["eps_out"], vacant_radius=cfg["tau"] * cfg["h"],
+                   precision=ibim.PREC_MIXED)
+        with ibim.Server(slots=args.slots) as srv:
+            t0 = time.perf_counter()
+            ids = [srv.submit(a, r, q, h=cfg["h"], restart=cfg["restart"], tol=cfg["tol"], **fmm) for a, r, q in mols]
+            res = [srv.wait(i) for i in ids]
+            wall = time.perf_counter() - t0
+        lat = [1e3 * (r["t_end"] - r["t_submit"]) for r in res]
+        serving = {"requests": args.serve, "slots": args.slots, "precision": "mixed",
+                   "solves_per_s": args.serve / wall, "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
+                   "all_ok": all(r["status"] == 0 and r["info"]["converged"] == 1 for r in res),
+                   "note": "burst submitted at t=0; latency = completion - submission, geometry (level set, tube) "
+                           "and plan build included per request"}
+
+    # ---- roofline of the dominant kernel in the timed mixed solves
     r = cfg["p"] ** 3
-    nmv = info_p["matvecs"]
-    p2p_flops = 60.0 * stats["p2p_pairs"]
-    # V-list GEMM: 32 r^2 flops per V interaction; the X list runs in its own launch
-    v_flops = 32.0 * r * r * stats["v_pairs"]
-    per_launch = {"M2L": (stages["M2L"][0] / max(stages["M2L"][1], 1), v_flops),
-                  "P2P": (stages["P2P"][0] / max(stages["P2P"][1], 1), p2p_flops)}
-    dom = max(per_launch, key=lambda k: per_launch[k][0])
-    ms_launch, flops = per_launch[dom]
-    achieved = flops / (ms_launch * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": dom,
-                "ms_per_launch": ms_launch, "algorithmic_flops_per_launch": flops,
-                "note": "FP64; M2L counted as 32 r^2 flops per V/X interaction, P2P as 60 flops per pair"}
-    stage_ms = {k: v[0] for k, v in stages.items()}
-    lat_ms = sorted(1e3 * x for x in lat)
+    per_launch = {k: v[0] / v[1] for k, v in stage_acc.items() if v[1] > 0 and k not in ("MATVEC", "MATVEC_MIXED")}
+    dom = max(per_launch, key=per_launch.get)
+    ms_gemm = per_launch["M2L"]
+    flops = 32.0 * r * r * stats["v_pairs"]  # 2 x (4r) x (4r) per V interaction
+    achieved = flops / (ms_gemm * 1e-3) / 1e12
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        try:
+            traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
+                "frac": achieved / PEAK_TFLOPS["f32"], "traffic": traffic, "kernel": "k_m2l_gemm<5,float>",
+                "dominant_stage": dom, "ms_per_launch": ms_gemm, "launches": stage_acc["M2L"][1],
+                "algorithmic_flops_per_launch": flops,
+                "note": "FP32 MFMA V-list M2L: 32 r^2 flops per V interaction (r = p^3), v_pairs interactions per "
+                        "launch; time from HIP events on the plan stream over the timed mixed solves"}
+    stage_ms = {k: v[0] / args.steps for k, v in stage_acc.items()}
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
         pts, nrm, w = surf.arrays()
         try:
-            cpu = run_cpu_baseline(cfg, pts, nrm, w, nmv, args.cpu_budget, threads)
+            cpu = run_cpu_baseline(cfg, pts, nrm, w, fp64["solve"]["matvecs"], threads)
         except Exception as e:  # report, do not hide
             cpu = {"error": repr(e)}
     surf.close()
 
-    value = args.steps / total
+    value = mixed["solves_per_s"]
     out = {
-        "metric": "PB solves/s (FP64 GPU; p50/p99 latency reported)",
+        "metric": METRIC,
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "ms_per_step": mixed["ms_per_solve"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32/f64", "data": "synthetic",
         "config": {"workload": f"{args.config}: synthetic 500-atom globule, h=0.25 A, p=5, leaf {cfg['leaf']}, "
-                               f"kappa=0.1257, eps 2/80, GMRES(40) tol 1e-8, FP64",
+                               f"kappa=0.1257, eps 2/80, GMRES(40) tol 1e-8; value = mixed precision (FP32 FMM "
+                               f"inside FP64 refinement), FP64 solve timed beside it",
                    "tube_points": n, "atoms": m, "parallelism": "single GPU"},
-        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
-                       "all": lat_ms},
-        "solve": {"iterations": info["iterations"], "matvecs": info["matvecs"], "rel_residual": info["rel_residual"],
-                  "converged": info["converged"], "energy": energy, "plan_bytes": foot},
-        "matvecs_per_s": info["matvecs"] * value, "tube_points_per_s": n * info["matvecs"] * value,
-        "stage_ms_per_solve": stage_ms, "tree": stats, "mesh_s": t_mesh,
-        "mixed": mixed, "roofline": roofline, "cpu_baseline": cpu,
+        "latency_ms": {"p50": mixed["latency_ms"]["p50"], "p99": mixed["latency_ms"]["p99"]},
+        "mixed": mixed, "fp64": fp64, "mixed_speedup_vs_fp64": mixed["solves_per_s"] / fp64["solves_per_s"],
+        "rel_energy_err_vs_fp64": rel_err, "energy_within_1e-6": energy_ok,
+        "fp64_matvecs_per_s": fp64["solve"]["matvecs"] * fp64["solves_per_s"],
+        "fp64_tube_points_per_s": n * fp64["solve"]["matvecs"] * fp64["solves_per_s"],
+        "serving": serving, "stage_ms_per_mixed_solve": stage_ms, "tree": stats, "mesh_s": t_mesh,
+        "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
