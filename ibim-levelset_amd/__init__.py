@@ -28,7 +28,7 @@ KERNELS = ["K1x", "K1y", "K1z", "K2", "K3xx", "K3yy", "K3zz", "K3xy", "K3yz", "K
            "K3yx", "K3zy", "K3xz", "K4x", "K4y", "K4z"]
 KID = {k: i for i, k in enumerate(KERNELS)}
 
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_F16 = 0, 1, 2
 KERNEL_PB, KERNEL_POLY = 0, 1
 IBIM_OK, IBIM_NOT_CONVERGED = 0, 1
 STAGES = ["P2M", "M2M", "M2L", "L2L", "P2P", "MATVEC", "ORTH", "RHS", "POLARIZE", "M2L_X", "MATVEC_MIXED"]

@@ -56,7 +56,7 @@ static int check_params(const ibim_params *q, int64_t n) {
         set_error("vacant_radius (tau*h) must be > 0 for the PB kernels");
         return IBIM_ERR_ARG;
     }
-    if (q->precision != IBIM_PREC_FP64 && q->precision != IBIM_PREC_MIXED) {
+    if (q->precision != IBIM_PREC_FP64 && q->precision != IBIM_PREC_MIXED && q->precision != IBIM_PREC_MIXED_F16) {
         set_error("unknown precision");
         return IBIM_ERR_ARG;
     }
@@ -150,7 +150,8 @@ int ibim_plan_create_device(const ibim_params *prm, int64_t n, const double *d_x
         ibim_plan_destroy(pl);
         return rc;
     }
-    if (pl->prm.precision == IBIM_PREC_MIXED && (rc = build_m2l_operators_f32(pl))) {
+    if (pl->prm.precision != IBIM_PREC_FP64 &&
+        (rc = pl->prm.precision == IBIM_PREC_MIXED ? build_m2l_operators_f32(pl) : build_m2l_operators_f16(pl))) {
         ibim_plan_destroy(pl);
         return rc;
     }
@@ -217,12 +218,13 @@ int ibim_plan_sync(ibim_plan *pl) {
 }
 
 int ibim_plan_set_precision(ibim_plan *pl, int precision) {
-    if (!pl || (precision != IBIM_PREC_FP64 && precision != IBIM_PREC_MIXED)) {
+    if (!pl || (precision != IBIM_PREC_FP64 && precision != IBIM_PREC_MIXED && precision != IBIM_PREC_MIXED_F16)) {
         set_error("bad plan or precision");
         return IBIM_ERR_ARG;
     }
     pl->prm.precision = precision;
     if (precision == IBIM_PREC_MIXED) return build_m2l_operators_f32(pl);
+    if (precision == IBIM_PREC_MIXED_F16) return build_m2l_operators_f16(pl);
     return IBIM_OK;
 }
 
@@ -270,7 +272,7 @@ int ibim_matvec_device(ibim_plan *pl, const double *d_x, double *d_y) {
         set_error("null argument");
         return IBIM_ERR_ARG;
     }
-    return matvec_device(pl, d_x, d_y, pl->prm.precision == IBIM_PREC_MIXED);
+    return matvec_device(pl, d_x, d_y, pl->prm.precision != IBIM_PREC_FP64);
 }
 
 int ibim_matvec(ibim_plan *pl, const double *x, double *y) {
@@ -280,7 +282,7 @@ int ibim_matvec(ibim_plan *pl, const double *x, double *y) {
     }
     const size_t n2 = 2 * (size_t)pl->n;
     return with_host_vectors(pl, n2, x, n2, y, [&](const double *dx, double *dy) {
-        return matvec_device(pl, dx, dy, pl->prm.precision == IBIM_PREC_MIXED);
+        return matvec_device(pl, dx, dy, pl->prm.precision != IBIM_PREC_FP64);
     });
 }
 

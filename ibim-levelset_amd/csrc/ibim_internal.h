@@ -296,6 +296,13 @@ struct ibim_plan {
     int n_cls = 0, MRP = 0, KRP = 0;
     double *ops = nullptr;
     float *ops32 = nullptr;              // FP32 copy for the mixed-precision matvec
+    // FP16 M2L (IBIM_PREC_MIXED_F16): operators with one power-of-two scale
+    // per row, multipoles with one per box, FP32 accumulation buffer
+    _Float16 *ops16 = nullptr;
+    int *esig = nullptr;                 // per (class, row) exponent
+    _Float16 *Mh = nullptr;              // nnodes * 4r, per box scaled
+    int *ebox = nullptr;                 // per box exponent
+    float *Lf = nullptr;                 // nnodes * 4r
     int *gtab = nullptr, *stab = nullptr; // gather / scatter tables of the 8 reflections
     ibim::VPairs vp;                     // V pairs for the points as their own targets
     const int *p2p_roff = nullptr;       // merged near-field ranges of the non-empty leaves
@@ -343,6 +350,7 @@ int build_m2l_operators(ibim_plan *pl);
 int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vector<void *> *tmp_owner);
 int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed);
 int build_m2l_operators_f32(ibim_plan *pl);
+int build_m2l_operators_f16(ibim_plan *pl);
 int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, P2PWork &work, std::vector<void *> *tmp_owner);
 
 // scoped stage timing on the plan's stream (no-op when profiling is off)

@@ -343,20 +343,197 @@ int build_ops_p(ibim_plan *pl) {
     return IBIM_OK;
 }
 
+// ------------------------------------------------- FP16 V-list GEMM ---
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// e such that every |v| of a set with maximum vmax satisfies |v| 2^-e < 1
+// (0 for an all-zero set)
+__device__ __forceinline__ int scale_exp(double vmax) {
+    if (!(vmax > 0.0)) return 0;
+    int e;
+    (void)frexp(vmax, &e);
+    return e;
+}
+
+// one block of 64 per row (class, row) of the FP64 operators
+__global__ __launch_bounds__(64) void k_ops_to_half(const double *__restrict__ ops, int KRP,
+                                                    _Float16 *__restrict__ ops16, int *__restrict__ esig) {
+    const size_t row = blockIdx.x;
+    const double *src = ops + row * KRP;
+    double mx = 0.0;
+    for (int k = threadIdx.x; k < KRP; k += 64) mx = fmax(mx, fabs(src[k]));
+    for (int o = 32; o; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    const int e = scale_exp(mx);
+    if (threadIdx.x == 0) esig[row] = e;
+    for (int k = threadIdx.x; k < KRP; k += 64) ops16[row * KRP + k] = (_Float16)ldexp(src[k], -e);
+}
+
+// per box: FP16 multipoles with one power-of-two scale (every matvec)
+__global__ __launch_bounds__(64) void k_m_to_half(const double *__restrict__ M, int R4, _Float16 *__restrict__ Mh,
+                                                  int *__restrict__ ebox) {
+    const size_t a = blockIdx.x;
+    const double *src = M + a * R4;
+    double mx = 0.0;
+    for (int k = threadIdx.x; k < R4; k += 64) mx = fmax(mx, fabs(src[k]));
+    for (int o = 32; o; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    const int e = scale_exp(mx);
+    if (threadIdx.x == 0) ebox[a] = e;
+    for (int k = threadIdx.x; k < R4; k += 64) Mh[a * R4 + k] = (_Float16)ldexp(src[k], -e);
+}
+
+__global__ void k_add_float(int64_t n, const float *__restrict__ a, double *__restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] += (double)a[i];
+}
+
+// The V-list GEMM of k_m2l_gemm on FP16 MFMA (v_mfma_f32_16x16x32_f16, FP32
+// accumulation): same (class, mask, 64 pairs, 64 operator rows) tiles and
+// gather/scatter tables.  Operator rows carry a power-of-two scale 2^esig,
+// source multipoles one per box (2^ebox); both are applied to the FP32
+// result, which is added atomically into the FP32 buffer Lf.
+template <int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_m2l_gemm_h(
+    const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
+    const int *__restrict__ pA, const int *__restrict__ pB, const int *__restrict__ gtab, const int *__restrict__ stab,
+    const _Float16 *__restrict__ ops16, const int *__restrict__ esig, int MRP, int KRP,
+    const _Float16 *__restrict__ Mh, const int *__restrict__ ebox, float *__restrict__ Lf, int ntiles, int per_xcd) {
+    constexpr int R = P * P * P;
+    constexpr int R4 = 4 * R;
+    constexpr int KC = 64, LDK = KC + 8;
+    __shared__ __attribute__((aligned(16))) _Float16 sA[64][LDK];  // operator rows x k
+    __shared__ __attribute__((aligned(16))) _Float16 sB[64][LDK];  // pairs x k
+    __shared__ int sG[R4];                                           // gather table of the mask
+    __shared__ int sPA[64], sPB[64], sEA[64], sS[64], sE[64];
+    const int logical = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const int nrt = MRP / 64;
+    if (logical >= ntiles * nrt) return;
+    const int tile = logical / nrt;
+    const int tid = threadIdx.x;
+    const int key = blk_key[tile];
+    const int cls = key >> 3, m = key & 7;
+    const int j0 = blk_j0[tile];
+    const int jend = key_off[key + 1];
+    const int row0 = (logical % nrt) * 64;
+    if (tid < 64) {
+        const int j = j0 + tid;
+        const bool ok = j < jend;
+        const int a = ok ? pA[j] : -1;
+        sPA[tid] = a;
+        sPB[tid] = ok ? pB[j] : -1;
+        sEA[tid] = a >= 0 ? ebox[a] : 0;
+        const int row = row0 + tid;
+        sS[tid] = row < R4 ? stab[m * R4 + row] : 0;
+        sE[tid] = esig[(size_t)cls * MRP + row];
+    }
+    for (int k = tid; k < R4; k += 256) sG[k] = gtab[m * R4 + k];
+    const int w = tid >> 6, l = tid & 63;
+    const int wr = (w & 1) * 32;   // operator rows of this wave
+    const int wc = (w >> 1) * 32;  // pairs of this wave
+    f4v acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+    const _Float16 *Aop = ops16 + ((size_t)cls * MRP + row0) * KRP;
+    // per chunk, thread t moves operator row t >> 2, k = kc + 16 (t & 3) .. +15,
+    // and the gathered multipoles of pairs (t >> 3) and (t >> 3) + 32 at
+    // k = kc + 8 (t & 7) .. +7
+    const int orow = tid >> 2, ok16 = 16 * (tid & 3);
+    const int xg = tid & 7, xp = tid >> 3;
+    h8 ra0, ra1, rb0, rb1;
+    __syncthreads();  // sPA, sG, ...
+    const int a0 = sPA[xp], a1 = sPA[xp + 32];
+    const size_t m0 = (size_t)max(a0, 0) * R4, m1 = (size_t)max(a1, 0) * R4;
+    auto load_chunk = [&](int kc) {
+        const h8 *src = (const h8 *)(Aop + (size_t)orow * KRP + kc + ok16);
+        ra0 = src[0];
+        ra1 = src[1];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = kc + 8 * xg + e;
+            const int g = sG[min(k, R4 - 1)];
+            const int gi = g & 0x7fffffff;
+            _Float16 x0 = Mh[m0 + gi], x1 = Mh[m1 + gi];
+            if (g < 0) {
+                x0 = -x0;
+                x1 = -x1;
+            }
+            rb0[e] = (k < R4 && a0 >= 0) ? x0 : (_Float16)0;
+            rb1[e] = (k < R4 && a1 >= 0) ? x1 : (_Float16)0;
+        }
+    };
+    load_chunk(0);
+    for (int kc = 0; kc < KRP; kc += KC) {
+        *(h8 *)&sA[orow][ok16] = ra0;
+        *(h8 *)&sA[orow][ok16 + 8] = ra1;
+        *(h8 *)&sB[xp][8 * xg] = rb0;
+        *(h8 *)&sB[xp + 32][8 * xg] = rb1;
+        __syncthreads();
+        if (kc + KC < KRP) load_chunk(kc + KC);
+#pragma unroll
+        for (int ks = 0; ks < KC; ks += 32) {
+            h8 xf[2], kf[2];
+#pragma unroll
+            for (int pi = 0; pi < 2; ++pi) xf[pi] = *(const h8 *)&sB[wc + pi * 16 + (l & 15)][ks + 8 * (l >> 4)];
+#pragma unroll
+            for (int ri = 0; ri < 2; ++ri) kf[ri] = *(const h8 *)&sA[wr + ri * 16 + (l & 15)][ks + 8 * (l >> 4)];
+#pragma unroll
+            for (int pi = 0; pi < 2; ++pi)
+#pragma unroll
+                for (int ri = 0; ri < 2; ++ri)
+                    acc[pi][ri] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xf[pi], kf[ri], acc[pi][ri], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // result of tile (pi, ri): register q of lane l holds pair 4 (l >> 4) + q
+    // and operator row l & 15
+#pragma unroll
+    for (int pi = 0; pi < 2; ++pi)
+#pragma unroll
+        for (int ri = 0; ri < 2; ++ri) {
+            const int lrow = wr + ri * 16 + (l & 15);
+            const int row = row0 + lrow;
+            if (row >= R4) continue;
+            const int st = sS[lrow];
+            const int dst = st & 0x7fffffff;
+            const int er = sE[lrow];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int col = wc + pi * 16 + 4 * (l >> 4) + q;
+                const int b = sPB[col];
+                if (b < 0) continue;
+                float v = ldexpf(acc[pi][ri][q], er + sEA[col]);
+                if (st < 0) v = -v;
+                unsafeAtomicAdd(&Lf[(size_t)b * R4 + dst], v);
+            }
+        }
+}
+
 template <int P>
 int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
     if (vp.nblocks == 0) return IBIM_OK;
+    constexpr int R4 = 4 * P * P * P;
     const int total = vp.nblocks * (pl->MRP / 64);
     const int per_xcd = (total + 7) / 8;
     const int grid = 8 * per_xcd;
-    if (mixed && pl->ops32)
-        k_m2l_gemm<P, float><<<grid, 256, 0, pl->stream>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, pl->gtab,
-                                                           pl->stab, pl->ops32, pl->MRP, pl->KRP, pl->M, pl->L,
-                                                           vp.nblocks, per_xcd);
+    hipStream_t st = pl->stream;
+    if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->ops16) {
+        const int64_t n = (int64_t)pl->nnodes * R4;
+        IBIM_HIP_CHECK(hipMemsetAsync(pl->Lf, 0, (size_t)n * sizeof(float), st));
+        k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, pl->Mh, pl->ebox);
+        k_m2l_gemm_h<P><<<grid, 256, 0, st>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, pl->gtab, pl->stab,
+                                              pl->ops16, pl->esig, pl->MRP, pl->KRP, pl->Mh, pl->ebox, pl->Lf,
+                                              vp.nblocks, per_xcd);
+        k_add_float<<<nblk(n), 256, 0, st>>>(n, pl->Lf, pl->L);
+    } else if (mixed && pl->ops32)
+        k_m2l_gemm<P, float><<<grid, 256, 0, st>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, pl->gtab,
+                                                   pl->stab, pl->ops32, pl->MRP, pl->KRP, pl->M, pl->L, vp.nblocks,
+                                                   per_xcd);
     else
-        k_m2l_gemm<P, double><<<grid, 256, 0, pl->stream>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B,
-                                                            pl->gtab, pl->stab, pl->ops, pl->MRP, pl->KRP, pl->M,
-                                                            pl->L, vp.nblocks, per_xcd);
+        k_m2l_gemm<P, double><<<grid, 256, 0, st>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, pl->gtab,
+                                                    pl->stab, pl->ops, pl->MRP, pl->KRP, pl->M, pl->L, vp.nblocks,
+                                                    per_xcd);
     IBIM_LAUNCH_CHECK();
     return IBIM_OK;
 }
@@ -495,6 +672,22 @@ int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed) {
     case 8: return gemm_p<8>(pl, vp, mixed);
     }
     return IBIM_ERR_ARG;
+}
+
+// FP16 operators, their row exponents and the per-matvec FP16 multipole and
+// FP32 accumulation buffers of the IBIM_PREC_MIXED_F16 matvec
+int build_m2l_operators_f16(ibim_plan *pl) {
+    if (pl->ops16 || !pl->ops || pl->n_cls == 0) return IBIM_OK;
+    const int64_t rows = (int64_t)pl->n_cls * pl->MRP;
+    const int64_t nf = (int64_t)pl->nnodes * 4 * pl->r;
+    int rc;
+    if ((rc = dalloc(pl, &pl->ops16, (size_t)(rows * pl->KRP))) || (rc = dalloc(pl, &pl->esig, (size_t)rows)) ||
+        (rc = dalloc(pl, &pl->Mh, (size_t)nf)) || (rc = dalloc(pl, &pl->ebox, (size_t)pl->nnodes)) ||
+        (rc = dalloc(pl, &pl->Lf, (size_t)nf)))
+        return rc;
+    k_ops_to_half<<<(unsigned)rows, 64, 0, pl->stream>>>(pl->ops, pl->KRP, pl->ops16, pl->esig);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
 }
 
 // FP32 copy of the operators for the mixed-precision matvec

@@ -74,7 +74,7 @@ int64_t estimate_footprint(const ibim_request &r, const std::vector<double> &xyz
     const double npts = std::min(area, 3.0 * box_area) * (2.0 * r.geom.thickness) / (h * h);
     const int p = r.fmm.p, rr3 = p * p * p;
     const double per_point = 2.0 * 4 * rr3 * 8 / 12.0 + 8.0 * (2 * (r.restart + 1) + 40);
-    const double ops = (r.fmm.precision == IBIM_PREC_MIXED ? 1.5 : 1.0) * 12 * 56 * pow(4.0 * rr3, 2) * 8;
+    const double ops = (r.fmm.precision != IBIM_PREC_FP64 ? 1.5 : 1.0) * 12 * 56 * pow(4.0 * rr3, 2) * 8;
     return (int64_t)(std::max(grid * 8 * 4.2, npts * per_point + ops) + npts * 56);
 }
 

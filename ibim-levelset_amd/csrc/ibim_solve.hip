@@ -623,7 +623,7 @@ static int gmres_refine(ibim_plan *pl, const double *d_b, double *d_x, int resta
 
 int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int maxit, double tol,
                  ibim_solve_info *info) {
-    if (pl->prm.precision == IBIM_PREC_MIXED) return gmres_refine(pl, d_b, d_x, restart, maxit, tol, info);
+    if (pl->prm.precision != IBIM_PREC_FP64) return gmres_refine(pl, d_b, d_x, restart, maxit, tol, info);
     return gmres_core(pl, d_b, d_x, restart, maxit, tol, info, false);
 }
 }  // namespace ibim

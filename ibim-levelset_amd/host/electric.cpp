@@ -22,7 +22,8 @@ double electric(Grid &, levelset &ls, Surface &surf, Molecule &mol, double resca
     prm.eps_out = cfg.getd("solvent_dE", 80.0);
     prm.eps_in = cfg.getd("solvent_dI", 1.0);
     prm.vacant_radius = cfg.getd("tau", 1.0) * dx;  // electric.cpp:58
-    prm.precision = cfg.gets("precision", "fp64") == "mixed" ? IBIM_PREC_MIXED : IBIM_PREC_FP64;
+    const std::string prec = cfg.gets("precision", "fp64");
+    prm.precision = prec == "mixed" ? IBIM_PREC_MIXED : prec == "mixed16" ? IBIM_PREC_MIXED_F16 : IBIM_PREC_FP64;
     prm.kernel_mode = IBIM_KERNEL_PB;
 
     const double *dxyz, *dnrm, *dw;
