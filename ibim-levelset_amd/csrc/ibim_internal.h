@@ -212,6 +212,9 @@ struct VPairs {
     int *cls_off_d = nullptr;
     int nblocks = 0;                    // GEMM blocks of 64 pairs
     int *blk_cls = nullptr, *blk_j0 = nullptr;
+    int *tptr = nullptr, *tlist = nullptr; // pairs of target node b: tlist[tptr[b] .. tptr[b+1])
+    int hnblocks = 0;                   // FP16 GEMM blocks (32 pairs for p <= 5, 16 for p = 6)
+    int *hblk_cls = nullptr, *hblk_j0 = nullptr;
 };
 
 // a set of targets sorted into the leaves of a plan's tree
@@ -302,7 +305,7 @@ struct ibim_plan {
     int *esig = nullptr;                 // per (class, row) exponent
     _Float16 *Mh = nullptr;              // nnodes * 4r, per box scaled
     int *ebox = nullptr;                 // per box exponent
-    float *Lf = nullptr;                 // nnodes * 4r
+    float *Yv = nullptr;                 // V interaction results, v pairs * 4r
     int *gtab = nullptr, *stab = nullptr; // gather / scatter tables of the 8 reflections
     ibim::VPairs vp;                     // V pairs for the points as their own targets
     const int *p2p_roff = nullptr;       // merged near-field ranges of the non-empty leaves

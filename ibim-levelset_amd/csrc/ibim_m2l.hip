@@ -382,132 +382,158 @@ __global__ __launch_bounds__(64) void k_m_to_half(const double *__restrict__ M, 
     for (int k = threadIdx.x; k < R4; k += 64) Mh[a * R4 + k] = (_Float16)ldexp(src[k], -e);
 }
 
-__global__ void k_add_float(int64_t n, const float *__restrict__ a, double *__restrict__ b) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] += (double)a[i];
-}
-
-// The V-list GEMM of k_m2l_gemm on FP16 MFMA (v_mfma_f32_16x16x32_f16, FP32
-// accumulation): same (class, mask, 64 pairs, 64 operator rows) tiles and
-// gather/scatter tables.  Operator rows carry a power-of-two scale 2^esig,
-// source multipoles one per box (2^ebox); both are applied to the FP32
-// result, which is added atomically into the FP32 buffer Lf.
-template <int P>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_m2l_gemm_h(
-    const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
-    const int *__restrict__ pA, const int *__restrict__ pB, const int *__restrict__ gtab, const int *__restrict__ stab,
-    const _Float16 *__restrict__ ops16, const int *__restrict__ esig, int MRP, int KRP,
-    const _Float16 *__restrict__ Mh, const int *__restrict__ ebox, float *__restrict__ Lf, int ntiles, int per_xcd) {
+// The V-list GEMM on FP16 MFMA (v_mfma_f32_16x16x32_f16, FP32 accumulation).
+// One block takes NP pairs of one (class, mask) key and ALL operator rows:
+// the pairs' gathered, scaled FP16 multipoles stay in LDS for the whole
+// block while the operator streams through in NR-row x KC-k chunks.
+// Operator rows carry a power-of-two scale 2^esig, source multipoles one per
+// box (2^ebox); both are applied to the FP32 result, stored unpermuted and
+// without atomics as Y[pair][row]; k_v_reduce then sums every target's pairs
+// into L.  Wave w computes rows [w NR/4, (w+1) NR/4) of a row tile for all NP
+// pairs.  LDS stays under 64 KB: (NP, NR, KC) = (32, 128, 64) for p <= 5,
+// (16, 256, 32) for p = 6.
+template <int P, int NP, int KC>
+__global__ __launch_bounds__(256) void k_m2l_gemm_h(const int *__restrict__ blk_key, const int *__restrict__ blk_j0,
+                                                    const int *__restrict__ key_off, const int *__restrict__ pA,
+                                                    const int *__restrict__ gtab, const _Float16 *__restrict__ ops16,
+                                                    const int *__restrict__ esig, int MRP, const _Float16 *__restrict__ Mh,
+                                                    const int *__restrict__ ebox, float *__restrict__ Y, int nblocks,
+                                                    int per_xcd) {
     constexpr int R = P * P * P;
     constexpr int R4 = 4 * R;
-    constexpr int KC = 64, LDK = KC + 8;
-    __shared__ __attribute__((aligned(16))) _Float16 sA[64][LDK];  // operator rows x k
-    __shared__ __attribute__((aligned(16))) _Float16 sB[64][LDK];  // pairs x k
-    __shared__ int sG[R4];                                           // gather table of the mask
-    __shared__ int sPA[64], sPB[64], sEA[64], sS[64], sE[64];
-    const int logical = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    const int nrt = MRP / 64;
-    if (logical >= ntiles * nrt) return;
-    const int tile = logical / nrt;
+    constexpr int KP = (R4 + 63) / 64 * 64;  // operator row length (padded, zero beyond R4)
+    constexpr int NR = 4096 / NP;
+    constexpr int LDA = KC + 8, LDX = KP + 8;
+    constexpr int OPT = NR * KC / 8 / 256;   // operator h8 per thread per chunk
+    constexpr int PS = NP / 16, RS = NR / 64;  // 16x16 subtiles of a wave: pairs x rows
+    static_assert(KP % KC == 0 && OPT >= 1, "tile shape");
+    __shared__ __attribute__((aligned(16))) _Float16 sX[NP][LDX];
+    __shared__ __attribute__((aligned(16))) _Float16 sA[NR][LDA];
+    __shared__ int sG[R4];
+    __shared__ int sPA[NP], sEA[NP];
+    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (tile >= nblocks) return;
     const int tid = threadIdx.x;
     const int key = blk_key[tile];
     const int cls = key >> 3, m = key & 7;
     const int j0 = blk_j0[tile];
     const int jend = key_off[key + 1];
-    const int row0 = (logical % nrt) * 64;
-    if (tid < 64) {
+    if (tid < NP) {
         const int j = j0 + tid;
-        const bool ok = j < jend;
-        const int a = ok ? pA[j] : -1;
+        const int a = j < jend ? pA[j] : -1;
         sPA[tid] = a;
-        sPB[tid] = ok ? pB[j] : -1;
         sEA[tid] = a >= 0 ? ebox[a] : 0;
-        const int row = row0 + tid;
-        sS[tid] = row < R4 ? stab[m * R4 + row] : 0;
-        sE[tid] = esig[(size_t)cls * MRP + row];
     }
     for (int k = tid; k < R4; k += 256) sG[k] = gtab[m * R4 + k];
-    const int w = tid >> 6, l = tid & 63;
-    const int wr = (w & 1) * 32;   // operator rows of this wave
-    const int wc = (w >> 1) * 32;  // pairs of this wave
-    f4v acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
-    const _Float16 *Aop = ops16 + ((size_t)cls * MRP + row0) * KRP;
-    // per chunk, thread t moves operator row t >> 2, k = kc + 16 (t & 3) .. +15,
-    // and the gathered multipoles of pairs (t >> 3) and (t >> 3) + 32 at
-    // k = kc + 8 (t & 7) .. +7
-    const int orow = tid >> 2, ok16 = 16 * (tid & 3);
-    const int xg = tid & 7, xp = tid >> 3;
-    h8 ra0, ra1, rb0, rb1;
-    __syncthreads();  // sPA, sG, ...
-    const int a0 = sPA[xp], a1 = sPA[xp + 32];
-    const size_t m0 = (size_t)max(a0, 0) * R4, m1 = (size_t)max(a1, 0) * R4;
-    auto load_chunk = [&](int kc) {
-        const h8 *src = (const h8 *)(Aop + (size_t)orow * KRP + kc + ok16);
-        ra0 = src[0];
-        ra1 = src[1];
+    __syncthreads();
+    // gather: item = (pair, 8 consecutive k)
+    for (int it = tid; it < NP * (KP / 8); it += 256) {
+        const int pr = it / (KP / 8), k0 = 8 * (it % (KP / 8));
+        const int a = sPA[pr];
+        const size_t mb = (size_t)max(a, 0) * R4;
+        h8 v;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const int k = kc + 8 * xg + e;
+            const int k = k0 + e;
             const int g = sG[min(k, R4 - 1)];
-            const int gi = g & 0x7fffffff;
-            _Float16 x0 = Mh[m0 + gi], x1 = Mh[m1 + gi];
-            if (g < 0) {
-                x0 = -x0;
-                x1 = -x1;
-            }
-            rb0[e] = (k < R4 && a0 >= 0) ? x0 : (_Float16)0;
-            rb1[e] = (k < R4 && a1 >= 0) ? x1 : (_Float16)0;
+            _Float16 x = Mh[mb + (g & 0x7fffffff)];
+            if (g < 0) x = -x;
+            v[e] = (k < R4 && a >= 0) ? x : (_Float16)0;
+        }
+        *(h8 *)&sX[pr][k0] = v;
+    }
+    const int w = tid >> 6, l = tid & 63;
+    const int wr = w * (NR / 4);
+    const _Float16 *Aop = ops16 + (size_t)cls * MRP * KP;
+    const int nrt = (MRP + NR - 1) / NR, nkc = KP / KC;
+    const int nch = nrt * nkc;
+    // chunk c = (row tile c / nkc, k chunk c % nkc); thread t moves h8 number t + 256 i
+    h8 ra[OPT];
+    auto load_chunk = [&](int c) {
+        const int rt = c / nkc, kc = (c % nkc) * KC;
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const int v = tid + 256 * i;
+            const int row = rt * NR + v / (KC / 8), kk = 8 * (v % (KC / 8));
+            const h8 *src = (const h8 *)(Aop + (size_t)min(row, MRP - 1) * KP + kc + kk);
+            ra[i] = row < MRP ? *src : h8{};
         }
     };
+    f4v acc[PS][RS];
     load_chunk(0);
-    for (int kc = 0; kc < KRP; kc += KC) {
-        *(h8 *)&sA[orow][ok16] = ra0;
-        *(h8 *)&sA[orow][ok16 + 8] = ra1;
-        *(h8 *)&sB[xp][8 * xg] = rb0;
-        *(h8 *)&sB[xp + 32][8 * xg] = rb1;
+    for (int c = 0; c < nch; ++c) {
+        const int rt = c / nkc, kc = (c % nkc) * KC;
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const int v = tid + 256 * i;
+            *(h8 *)&sA[v / (KC / 8)][8 * (v % (KC / 8))] = ra[i];
+        }
         __syncthreads();
-        if (kc + KC < KRP) load_chunk(kc + KC);
+        if (c + 1 < nch) load_chunk(c + 1);
+        if (kc == 0) {
+#pragma unroll
+            for (int a = 0; a < PS; ++a)
+#pragma unroll
+                for (int b = 0; b < RS; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+        }
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 32) {
-            h8 xf[2], kf[2];
+            h8 xf[PS], kf[RS];
 #pragma unroll
-            for (int pi = 0; pi < 2; ++pi) xf[pi] = *(const h8 *)&sB[wc + pi * 16 + (l & 15)][ks + 8 * (l >> 4)];
+            for (int pi = 0; pi < PS; ++pi) xf[pi] = *(const h8 *)&sX[pi * 16 + (l & 15)][kc + ks + 8 * (l >> 4)];
 #pragma unroll
-            for (int ri = 0; ri < 2; ++ri) kf[ri] = *(const h8 *)&sA[wr + ri * 16 + (l & 15)][ks + 8 * (l >> 4)];
+            for (int ri = 0; ri < RS; ++ri) kf[ri] = *(const h8 *)&sA[wr + ri * 16 + (l & 15)][ks + 8 * (l >> 4)];
 #pragma unroll
-            for (int pi = 0; pi < 2; ++pi)
+            for (int pi = 0; pi < PS; ++pi)
 #pragma unroll
-                for (int ri = 0; ri < 2; ++ri)
+                for (int ri = 0; ri < RS; ++ri)
                     acc[pi][ri] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xf[pi], kf[ri], acc[pi][ri], 0, 0, 0);
+        }
+        if (kc + KC == KP) {
+            // row tile done: register q of lane l holds pair 4 (l >> 4) + q, row l & 15
+#pragma unroll
+            for (int ri = 0; ri < RS; ++ri) {
+                const int row = rt * NR + wr + ri * 16 + (l & 15);
+                if (row >= R4) continue;
+                const int er = esig[(size_t)cls * MRP + row];
+#pragma unroll
+                for (int pi = 0; pi < PS; ++pi)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int col = pi * 16 + 4 * (l >> 4) + q;
+                        if (j0 + col >= jend) continue;
+                        Y[(size_t)(j0 + col) * R4 + row] = ldexpf(acc[pi][ri][q], er + sEA[col]);
+                    }
+            }
         }
         __syncthreads();
     }
-    // result of tile (pi, ri): register q of lane l holds pair 4 (l >> 4) + q
-    // and operator row l & 15
-#pragma unroll
-    for (int pi = 0; pi < 2; ++pi)
-#pragma unroll
-        for (int ri = 0; ri < 2; ++ri) {
-            const int lrow = wr + ri * 16 + (l & 15);
-            const int row = row0 + lrow;
-            if (row >= R4) continue;
-            const int st = sS[lrow];
-            const int dst = st & 0x7fffffff;
-            const int er = sE[lrow];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int col = wc + pi * 16 + 4 * (l >> 4) + q;
-                const int b = sPB[col];
-                if (b < 0) continue;
-                float v = ldexpf(acc[pi][ri][q], er + sEA[col]);
-                if (st < 0) v = -v;
-                unsafeAtomicAdd(&Lf[(size_t)b * R4 + dst], v);
-            }
+}
+
+// L_b += sum over the V pairs (a, b) of the pair's result, scattered into
+// b's expansion with the pair's reflection (the inverse of a reflection is
+// itself): one block per target node
+template <int R4>
+__global__ __launch_bounds__(256) void k_v_reduce(const int *__restrict__ work, const int *__restrict__ tptr,
+                                                  const int *__restrict__ tlist, const unsigned char *__restrict__ mask,
+                                                  const int *__restrict__ stab, const float *__restrict__ Y,
+                                                  double *__restrict__ L) {
+    __shared__ int sS[8 * R4];
+    const int b = work[blockIdx.x];
+    const int q0 = tptr[b], q1 = tptr[b + 1];
+    if (q0 == q1) return;
+    for (int e = threadIdx.x; e < 8 * R4; e += 256) sS[e] = stab[e];
+    __syncthreads();
+    for (int e = threadIdx.x; e < R4; e += 256) {
+        float acc = 0.f;
+        for (int q = q0; q < q1; ++q) {
+            const int j = tlist[q];
+            const int s = sS[mask[j] * R4 + e];
+            const float v = Y[(size_t)j * R4 + (s & 0x7fffffff)];
+            acc += s < 0 ? -v : v;
         }
+        L[(size_t)b * R4 + e] += (double)acc;
+    }
 }
 
 template <int P>
@@ -518,15 +544,25 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
     const int per_xcd = (total + 7) / 8;
     const int grid = 8 * per_xcd;
     hipStream_t st = pl->stream;
-    if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->ops16) {
-        const int64_t n = (int64_t)pl->nnodes * R4;
-        IBIM_HIP_CHECK(hipMemsetAsync(pl->Lf, 0, (size_t)n * sizeof(float), st));
-        k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, pl->Mh, pl->ebox);
-        k_m2l_gemm_h<P><<<grid, 256, 0, st>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, pl->gtab, pl->stab,
-                                              pl->ops16, pl->esig, pl->MRP, pl->KRP, pl->Mh, pl->ebox, pl->Lf,
-                                              vp.nblocks, per_xcd);
-        k_add_float<<<nblk(n), 256, 0, st>>>(n, pl->Lf, pl->L);
-    } else if (mixed && pl->ops32)
+    if constexpr (P <= 6) {
+        if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->ops16 && vp.tptr && pl->Yv &&
+            vp.npairs <= pl->vp.npairs) {
+            constexpr int NP = P <= 5 ? 32 : 16;
+            constexpr int KC = P <= 5 ? 64 : 32;
+            k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, pl->Mh, pl->ebox);
+            if (vp.hnblocks > 0) {
+                const int pxcd = (vp.hnblocks + 7) / 8;
+                k_m2l_gemm_h<P, NP, KC><<<8 * pxcd, 256, 0, st>>>(vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A,
+                                                                  pl->gtab, pl->ops16, pl->esig, pl->MRP, pl->Mh,
+                                                                  pl->ebox, pl->Yv, vp.hnblocks, pxcd);
+                k_v_reduce<R4><<<pl->n_m2l_work, 256, 0, st>>>(pl->m2l_work, vp.tptr, vp.tlist, vp.mask, pl->stab,
+                                                               pl->Yv, pl->L);
+            }
+            IBIM_LAUNCH_CHECK();
+            return IBIM_OK;
+        }
+    }
+    if (mixed && pl->ops32)
         k_m2l_gemm<P, float><<<grid, 256, 0, st>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, pl->gtab,
                                                    pl->stab, pl->ops32, pl->MRP, pl->KRP, pl->M, pl->L, vp.nblocks,
                                                    per_xcd);
@@ -562,6 +598,17 @@ int build_m2l_operators(ibim_plan *pl) {
 }
 
 // V pairs of the boxes flagged in `need`, grouped by operator class
+namespace {
+__global__ void k_count_targets(int np, const int *__restrict__ B, int *__restrict__ cnt) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < np) atomicAdd(&cnt[B[j]], 1);
+}
+__global__ void k_iota_pairs(int np, int *__restrict__ v) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < np) v[j] = j;
+}
+}  // namespace
+
 int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vector<void *> *tmp_owner) {
     hipStream_t st = pl->stream;
     const int nn = pl->nnodes;
@@ -654,6 +701,50 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
         IBIM_HIP_CHECK(hipMemcpyAsync(vp.blk_cls, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice, st));
         IBIM_HIP_CHECK(hipMemcpyAsync(vp.blk_j0, bj.data(), bj.size() * sizeof(int), hipMemcpyHostToDevice, st));
     }
+    // FP16 GEMM block table (blocks of 32 pairs for p <= 5, 16 for p = 6)
+    if (pl->p <= 6) {
+        const int hnp = pl->p <= 5 ? 32 : 16;
+        std::vector<int> hc, hj;
+        for (int c = 0; c < nkeys; ++c)
+            for (int j = vp.cls_off[c]; j < vp.cls_off[c + 1]; j += hnp) {
+                hc.push_back(c);
+                hj.push_back(j);
+            }
+        vp.hnblocks = (int)hc.size();
+        if ((rc = alloc(&vp.hblk_cls, std::max(1, vp.hnblocks))) || (rc = alloc(&vp.hblk_j0, std::max(1, vp.hnblocks))))
+            return rc;
+        if (vp.hnblocks > 0) {
+            IBIM_HIP_CHECK(hipMemcpyAsync(vp.hblk_cls, hc.data(), hc.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            IBIM_HIP_CHECK(hipMemcpyAsync(vp.hblk_j0, hj.data(), hj.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        }
+    }
+    // pairs of each target node, in pair order: tlist[tptr[b] .. tptr[b+1])
+    if ((rc = alloc(&vp.tptr, (size_t)nn + 1)) || (rc = alloc(&vp.tlist, std::max(1, np)))) return rc;
+    IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, ((size_t)nn + 1) * sizeof(int), st));
+    if (np > 0) {
+        k_count_targets<<<nblk(np), 256, 0, st>>>(np, vp.B, cnt);
+        IBIM_LAUNCH_CHECK();
+    }
+    bytes = 0;
+    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt, vp.tptr, nn + 1, st));
+    IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
+    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, cnt, vp.tptr, nn + 1, st));
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipFree(tmp);
+    if (np > 0) {
+        int endbit = 1;
+        while ((1 << endbit) <= nn) ++endbit;
+        // key0/key1 and iota/order are free again: B copy -> key0, sorted keys -> key1, values iota -> tlist
+        IBIM_HIP_CHECK(hipMemcpyAsync(key0, vp.B, (size_t)np * sizeof(int), hipMemcpyDeviceToDevice, st));
+        k_iota_pairs<<<nblk(np), 256, 0, st>>>(np, iota);
+        IBIM_LAUNCH_CHECK();
+        bytes = 0;
+        IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key0, key1, iota, vp.tlist, np, 0, endbit, st));
+        IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
+        IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, key0, key1, iota, vp.tlist, np, 0, endbit, st));
+        IBIM_HIP_CHECK(hipStreamSynchronize(st));
+        (void)hipFree(tmp);
+    }
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
     for (void *p : {(void *)cnt, (void *)off, (void *)a0, (void *)b0, (void *)m0, (void *)key0, (void *)key1,
                     (void *)iota, (void *)order})
@@ -675,15 +766,17 @@ int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed) {
 }
 
 // FP16 operators, their row exponents and the per-matvec FP16 multipole and
-// FP32 accumulation buffers of the IBIM_PREC_MIXED_F16 matvec
+// V-result buffers of the IBIM_PREC_MIXED_F16 matvec (the plan's V pairs
+// must exist)
 int build_m2l_operators_f16(ibim_plan *pl) {
     if (pl->ops16 || !pl->ops || pl->n_cls == 0) return IBIM_OK;
     const int64_t rows = (int64_t)pl->n_cls * pl->MRP;
     const int64_t nf = (int64_t)pl->nnodes * 4 * pl->r;
     int rc;
+    if (pl->p > 6) return build_m2l_operators_f32(pl);  // the FP16 GEMM's LDS tiles stop at p = 6
     if ((rc = dalloc(pl, &pl->ops16, (size_t)(rows * pl->KRP))) || (rc = dalloc(pl, &pl->esig, (size_t)rows)) ||
         (rc = dalloc(pl, &pl->Mh, (size_t)nf)) || (rc = dalloc(pl, &pl->ebox, (size_t)pl->nnodes)) ||
-        (rc = dalloc(pl, &pl->Lf, (size_t)nf)))
+        (rc = dalloc(pl, &pl->Yv, std::max<size_t>(1, (size_t)pl->vp.npairs * 4 * pl->r))))
         return rc;
     k_ops_to_half<<<(unsigned)rows, 64, 0, pl->stream>>>(pl->ops, pl->KRP, pl->ops16, pl->esig);
     IBIM_LAUNCH_CHECK();
