@@ -470,8 +470,8 @@ __global__ void k_p2p_ranges(int nleaves, const int *__restrict__ tleaves, const
 // between leaves, so one block per leaf leaves a long tail of a few busy
 // blocks.  The work is therefore cut into units of at most 64 targets x
 // kP2PChunk sources, launched heaviest first; a leaf whose sources span
-// several units writes one partial field per unit, and k_p2p_reduce sums
-// them in unit order (deterministic).
+// several units writes one partial field per unit, and k_l2p sums them in
+// unit order (deterministic).
 constexpr int kP2PChunk = 1024;
 
 template <typename T> struct Vec4;
@@ -569,27 +569,6 @@ __global__ __launch_bounds__(64) void k_p2p_unit(
     }
 }
 
-// F of the targets of a leaf whose sources were split over several units:
-// the sum of the units' partial fields, in unit order
-__global__ __launch_bounds__(64) void k_p2p_reduce(const int *__restrict__ tleaves, const int *__restrict__ tbeg,
-                                                   const int *__restrict__ tend, const int2 *__restrict__ lsplit,
-                                                   const double *__restrict__ part, double *__restrict__ F, int nt) {
-    const int2 s = lsplit[blockIdx.x];
-    if (s.y <= 1) return;
-    const int b = tleaves[blockIdx.x];
-    const int nl = tend[b] - tbeg[b];
-    for (int tc = 0; 64 * tc < nl; ++tc) {
-        const int i = 64 * tc + threadIdx.x;
-        if (i >= nl) continue;
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            double acc = 0.0;
-            for (int sc = 0; sc < s.y; ++sc) acc += part[(size_t)(s.x + tc * s.y + sc) * 256 + f * 64 + threadIdx.x];
-            F[f * (size_t)nt + tbeg[b] + i] = acc;
-        }
-    }
-}
-
 __device__ __forceinline__ int range_sources(const int *roff, const int2 *ranges, int i) {
     int ns = 0;
     for (int r = roff[i]; r < roff[i + 1]; ++r) ns += ranges[r].y - ranges[r].x;
@@ -611,7 +590,7 @@ __global__ void k_p2p_count(int nl, const int *__restrict__ tleaves, const int *
 }
 
 // unit records, their cost (per-lane pair evaluations), and the per-leaf
-// split table read by k_p2p_reduce
+// split table read by k_l2p
 __global__ void k_p2p_fill(int nl, const int *__restrict__ tleaves, const int *__restrict__ tbeg,
                            const int *__restrict__ tend, const int *__restrict__ roff,
                            const int2 *__restrict__ ranges, const int *__restrict__ uoff,
@@ -661,17 +640,21 @@ __global__ void k_p2p_order(int n, const int *__restrict__ idx, const int4 *__re
 }
 
 // L2P (bbfmm.h:600): F += S L_leaf at each target, one block per target
-// leaf, the leaf's local expansion staged in LDS
+// leaf, the leaf's local expansion staged in LDS.  For a leaf whose near
+// field was split over several work units (lsplit[leaf].y > 1) the near
+// field is first summed from the units' partial fields, in unit order.
 template <int P>
 __global__ __launch_bounds__(64) void k_l2p(const int *__restrict__ tleaves, const int *__restrict__ tbeg,
                                             const int *__restrict__ tend, const double *__restrict__ tx,
                                             const double *__restrict__ ty, const double *__restrict__ tz,
                                             const double *__restrict__ ctr, const int *__restrict__ level,
                                             const double *__restrict__ lrad, ChebTab tab,
-                                            const double *__restrict__ L, double *__restrict__ F, int nt) {
+                                            const double *__restrict__ L, const int2 *__restrict__ lsplit,
+                                            const double *__restrict__ part, double *__restrict__ F, int nt) {
     constexpr int R = P * P * P;
     __shared__ double sL[4 * R];
     const int b = tleaves[blockIdx.x];
+    const int2 sp = lsplit ? lsplit[blockIdx.x] : make_int2(-1, 1);
     for (int e = threadIdx.x; e < 4 * R; e += 64) sL[e] = L[(size_t)b * 4 * R + e];
     __syncthreads();
     const double cx = ctr[3 * b], cy = ctr[3 * b + 1], cz = ctr[3 * b + 2];
@@ -696,10 +679,27 @@ __global__ __launch_bounds__(64) void k_l2p(const int *__restrict__ tleaves, con
                     gz += w * sL[3 * R + m];
                 }
             }
-        F[t] += g0;
-        F[(size_t)nt + t] += gx;
-        F[2 * (size_t)nt + t] += gy;
-        F[3 * (size_t)nt + t] += gz;
+        double n0, nx, ny, nz;  // near field
+        if (sp.y > 1) {
+            const int i = t - tbeg[b];
+            n0 = nx = ny = nz = 0.0;
+            for (int sc = 0; sc < sp.y; ++sc) {
+                const double *pp = part + (size_t)(sp.x + (i >> 6) * sp.y + sc) * 256 + (i & 63);
+                n0 += pp[0];
+                nx += pp[64];
+                ny += pp[128];
+                nz += pp[192];
+            }
+        } else {
+            n0 = F[t];
+            nx = F[(size_t)nt + t];
+            ny = F[2 * (size_t)nt + t];
+            nz = F[3 * (size_t)nt + t];
+        }
+        F[t] = n0 + g0;
+        F[(size_t)nt + t] = nx + gx;
+        F[2 * (size_t)nt + t] = ny + gy;
+        F[3 * (size_t)nt + t] = nz + gz;
     }
 }
 
@@ -808,10 +808,9 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
             k_p2p_unit<false><<<w.nunits, 64, 0, st>>>(w.ua, w.ub, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q,
                                                        pl->n, ts.ranges, pl->kc, F, nt, w.part);
         IBIM_LAUNCH_CHECK();
-        if (w.npart)
-            k_p2p_reduce<<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, w.lsplit, w.part, F, nt);
         k_l2p<P><<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->ctr, pl->level,
-                                            pl->d_lvl_rad, pl->tab, pl->L, F, nt);
+                                            pl->d_lvl_rad, pl->tab, pl->L, w.npart ? w.lsplit : nullptr, w.part,
+                                            F, nt);
     }
     else if (ts.nleaves > 0)
         k_p2p_l2p<P><<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->sx, pl->sy,

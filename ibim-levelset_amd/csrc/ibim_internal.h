@@ -65,6 +65,17 @@ struct PBConst {
     int mode;         // IBIM_KERNEL_*
 };
 
+// 1/x for x > 0 normal: the hardware reciprocal refined by two Newton steps
+// (within an ulp of the quotient; no special-value handling needed here)
+__device__ __forceinline__ double recip(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+__device__ __forceinline__ float recip(float x) { return 1.0f / x; }
+
 // --------------------------------------------------------------------------
 // The 16 kernels fused: one source s with weighted charges (qx, qy, qz, qp)
 // acting on one target t, d = s - t.  Accumulates
@@ -93,11 +104,11 @@ __device__ __forceinline__ void pair_accum(T dx, T dy, T dz, T qx, T qy, T qz, T
     }
     T r2 = dx * dx + dy * dy + dz * dz;
     T r = sqrt(r2);
-    if (r < T(k.vr)) {
-        f0 -= T(k.k2reg) * qp;
-        return;
-    }
-    T ir = T(1) / r;
+    // below the vacant radius only K2 = k2reg acts; evaluated without a
+    // branch (the kernel terms are computed at r = 1 and discarded)
+    const bool reg = r < T(k.vr);
+    if (reg) r = T(1);
+    T ir = recip(r);
     T ir2 = ir * ir;
     T ir3 = ir2 * ir;
     T t = T(k.kappa) * r;
@@ -110,6 +121,13 @@ __device__ __forceinline__ void pair_accum(T dx, T dy, T dz, T qx, T qy, T qz, T
     T b3 = (e * ((t + T(3)) * t + T(3)) - T(3)) * ir3 * ir2;
     T g4 = (T(k.inv_eps) * et1 - T(1)) * c4;
     T dq = dx * qx + dy * qy + dz * qz;
+    if (reg) {
+        g1 = T(0);
+        k2 = T(k.k2reg);
+        a3 = T(0);
+        b3 = T(0);
+        g4 = T(0);
+    }
     f0 += g1 * dq - k2 * qp;
     T s = b3 * dq + g4 * qp;
     fx += a3 * qx + dx * s;

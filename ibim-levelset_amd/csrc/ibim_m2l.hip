@@ -395,7 +395,8 @@ __global__ __launch_bounds__(64) void k_m_to_half(const double *__restrict__ M, 
 template <int P, int NP, int KC>
 __global__ __launch_bounds__(256) void k_m2l_gemm_h(const int *__restrict__ blk_key, const int *__restrict__ blk_j0,
                                                     const int *__restrict__ key_off, const int *__restrict__ pA,
-                                                    const int *__restrict__ gtab, const _Float16 *__restrict__ ops16,
+                                                    const int *__restrict__ gtab, const int *__restrict__ stab,
+                                                    const _Float16 *__restrict__ ops16,
                                                     const int *__restrict__ esig, int MRP, const _Float16 *__restrict__ Mh,
                                                     const int *__restrict__ ebox, float *__restrict__ Y, int nblocks,
                                                     int per_xcd) {
@@ -490,19 +491,23 @@ __global__ __launch_bounds__(256) void k_m2l_gemm_h(const int *__restrict__ blk_
                     acc[pi][ri] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xf[pi], kf[ri], acc[pi][ri], 0, 0, 0);
         }
         if (kc + KC == KP) {
-            // row tile done: register q of lane l holds pair 4 (l >> 4) + q, row l & 15
+            // row tile done: register q of lane l holds pair 4 (l >> 4) + q,
+            // row l & 15; stored at the row's place in the target expansion
 #pragma unroll
             for (int ri = 0; ri < RS; ++ri) {
                 const int row = rt * NR + wr + ri * 16 + (l & 15);
                 if (row >= R4) continue;
                 const int er = esig[(size_t)cls * MRP + row];
+                const int st = stab[m * R4 + row];
+                const int dst = st & 0x7fffffff;
 #pragma unroll
                 for (int pi = 0; pi < PS; ++pi)
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int col = pi * 16 + 4 * (l >> 4) + q;
                         if (j0 + col >= jend) continue;
-                        Y[(size_t)(j0 + col) * R4 + row] = ldexpf(acc[pi][ri][q], er + sEA[col]);
+                        const float v = ldexpf(acc[pi][ri][q], er + sEA[col]);
+                        Y[(size_t)(j0 + col) * R4 + dst] = st < 0 ? -v : v;
                     }
             }
         }
@@ -510,29 +515,26 @@ __global__ __launch_bounds__(256) void k_m2l_gemm_h(const int *__restrict__ blk_
     }
 }
 
-// L_b += sum over the V pairs (a, b) of the pair's result, scattered into
-// b's expansion with the pair's reflection (the inverse of a reflection is
-// itself): one block per target node
+// L_b += sum over the V pairs (a, b) of the pair's result: one block per
+// target node, four pairs' rows in flight per step
 template <int R4>
 __global__ __launch_bounds__(256) void k_v_reduce(const int *__restrict__ work, const int *__restrict__ tptr,
-                                                  const int *__restrict__ tlist, const unsigned char *__restrict__ mask,
-                                                  const int *__restrict__ stab, const float *__restrict__ Y,
+                                                  const int *__restrict__ tlist, const float *__restrict__ Y,
                                                   double *__restrict__ L) {
-    __shared__ int sS[8 * R4];
     const int b = work[blockIdx.x];
     const int q0 = tptr[b], q1 = tptr[b + 1];
     if (q0 == q1) return;
-    for (int e = threadIdx.x; e < 8 * R4; e += 256) sS[e] = stab[e];
-    __syncthreads();
     for (int e = threadIdx.x; e < R4; e += 256) {
-        float acc = 0.f;
-        for (int q = q0; q < q1; ++q) {
-            const int j = tlist[q];
-            const int s = sS[mask[j] * R4 + e];
-            const float v = Y[(size_t)j * R4 + (s & 0x7fffffff)];
-            acc += s < 0 ? -v : v;
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+        int q = q0;
+        for (; q + 3 < q1; q += 4) {
+            a0 += Y[(size_t)tlist[q] * R4 + e];
+            a1 += Y[(size_t)tlist[q + 1] * R4 + e];
+            a2 += Y[(size_t)tlist[q + 2] * R4 + e];
+            a3 += Y[(size_t)tlist[q + 3] * R4 + e];
         }
-        L[(size_t)b * R4 + e] += (double)acc;
+        for (; q < q1; ++q) a0 += Y[(size_t)tlist[q] * R4 + e];
+        L[(size_t)b * R4 + e] += (double)((a0 + a1) + (a2 + a3));
     }
 }
 
@@ -553,10 +555,9 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
             if (vp.hnblocks > 0) {
                 const int pxcd = (vp.hnblocks + 7) / 8;
                 k_m2l_gemm_h<P, NP, KC><<<8 * pxcd, 256, 0, st>>>(vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A,
-                                                                  pl->gtab, pl->ops16, pl->esig, pl->MRP, pl->Mh,
-                                                                  pl->ebox, pl->Yv, vp.hnblocks, pxcd);
-                k_v_reduce<R4><<<pl->n_m2l_work, 256, 0, st>>>(pl->m2l_work, vp.tptr, vp.tlist, vp.mask, pl->stab,
-                                                               pl->Yv, pl->L);
+                                                                  pl->gtab, pl->stab, pl->ops16, pl->esig, pl->MRP,
+                                                                  pl->Mh, pl->ebox, pl->Yv, vp.hnblocks, pxcd);
+                k_v_reduce<R4><<<pl->n_m2l_work, 256, 0, st>>>(pl->m2l_work, vp.tptr, vp.tlist, pl->Yv, pl->L);
             }
             IBIM_LAUNCH_CHECK();
             return IBIM_OK;
