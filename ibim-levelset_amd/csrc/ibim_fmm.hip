@@ -421,46 +421,63 @@ __global__ __launch_bounds__(64) void k_p2p_l2p(
 // of the sorted point order, sorted and merged where they touch (Morton
 // order makes neighbouring boxes' ranges contiguous).
 template <int PASS>
-__global__ void k_p2p_ranges(int nleaves, const int *__restrict__ tleaves, const int *__restrict__ uoff,
-                             const int *__restrict__ uit, const int *__restrict__ woff, const int *__restrict__ wit,
-                             const int *__restrict__ flags, const int *__restrict__ sbeg, const int *__restrict__ send,
-                             int *__restrict__ cnt, const int *__restrict__ roff, int2 *__restrict__ ranges) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nleaves) return;
-    const int b = tleaves[i];
+__global__ __launch_bounds__(64) void k_p2p_ranges(int nleaves, const int *__restrict__ tleaves,
+                                                   const int *__restrict__ uoff, const int *__restrict__ uit,
+                                                   const int *__restrict__ woff, const int *__restrict__ wit,
+                                                   const int *__restrict__ flags, const int *__restrict__ sbeg,
+                                                   const int *__restrict__ send, int *__restrict__ cnt,
+                                                   const int *__restrict__ roff, int2 *__restrict__ ranges) {
+    // one wave per target leaf: collect the non-empty U and W boxes' source
+    // ranges (ballot compaction), sort them by begin (rank sort: begins of
+    // disjoint non-empty ranges are distinct) and merge touching ones
     constexpr int CAP = 512;
-    int2 loc[CAP];
+    __shared__ int2 loc[CAP], srt[CAP];
+    __shared__ int runs[CAP + 1];
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int b = tleaves[i];
     int n = 0;
-    bool overflow = false;
     for (int lst = 0; lst < 2; ++lst) {
         const int *off = lst == 0 ? uoff : woff;
         const int *it = lst == 0 ? uit : wit;
-        for (int q = off[b]; q < off[b + 1]; ++q) {
-            const int a = it[q];
-            if (flags[a] & kEmpty) continue;
-            if (n < CAP) loc[n++] = make_int2(sbeg[a], send[a]);
-            else overflow = true;
+        for (int q0 = off[b]; q0 < off[b + 1]; q0 += 64) {
+            const int q = q0 + lane;
+            const int a = q < off[b + 1] ? it[q] : -1;
+            const bool v = a >= 0 && !(flags[a] & kEmpty);
+            const unsigned long long m = __ballot(v);
+            const int pos = n + __popcll(m & ((1ull << lane) - 1ull));
+            if (v && pos < CAP) loc[pos] = make_int2(sbeg[a], send[a]);
+            n += __popcll(m);
         }
     }
-    // insertion sort by begin, then merge touching ranges
-    for (int x = 1; x < n; ++x) {
-        int2 v = loc[x];
-        int y = x - 1;
-        while (y >= 0 && loc[y].x > v.x) {
-            loc[y + 1] = loc[y];
-            --y;
-        }
-        loc[y + 1] = v;
+    if (n > CAP) {  // keep the per-box kernel for this target set
+        if (PASS == 0 && lane == 0) cnt[i] = -1;
+        return;
     }
-    int m = 0;
-    for (int x = 0; x < n; ++x) {
-        if (m > 0 && loc[m - 1].y == loc[x].x) loc[m - 1].y = loc[x].y;
-        else loc[m++] = loc[x];
+    __syncthreads();
+    for (int e = lane; e < n; e += 64) {
+        const int x = loc[e].x;
+        int r = 0;
+        for (int f = 0; f < n; ++f) r += loc[f].x < x;
+        srt[r] = loc[e];
     }
+    __syncthreads();
+    // element k starts a run unless it continues the previous range
+    int nr = 0;
+    for (int k0 = 0; k0 < n; k0 += 64) {
+        const int k = k0 + lane;
+        const bool st = k < n && (k == 0 || srt[k - 1].y != srt[k].x);
+        const unsigned long long m = __ballot(st);
+        if (st) runs[nr + __popcll(m & ((1ull << lane) - 1ull))] = k;
+        nr += __popcll(m);
+    }
+    if (lane == 0) runs[nr] = n;
+    __syncthreads();
     if (PASS == 0) {
-        cnt[i] = overflow ? -1 : m;
+        if (lane == 0) cnt[i] = nr;
     } else {
-        for (int x = 0; x < m; ++x) ranges[roff[i] + x] = loc[x];
+        for (int r = lane; r < nr; r += 64)
+            ranges[roff[i] + r] = make_int2(srt[runs[r]].x, srt[runs[r + 1] - 1].y);
     }
 }
 
@@ -639,8 +656,8 @@ __global__ void k_p2p_order(int n, const int *__restrict__ idx, const int4 *__re
     ob[k] = ub[idx[k]];
 }
 
-// L2P (bbfmm.h:600): F += S L_leaf at each target, one block per target
-// leaf, the leaf's local expansion staged in LDS.  For a leaf whose near
+// L2P (bbfmm.h:600): F += S L_leaf at each target, one block (one wave) per
+// target leaf.  For a leaf whose near
 // field was split over several work units (lsplit[leaf].y > 1) the near
 // field is first summed from the units' partial fields, in unit order.
 template <int P>
@@ -652,11 +669,10 @@ __global__ __launch_bounds__(64) void k_l2p(const int *__restrict__ tleaves, con
                                             const double *__restrict__ L, const int2 *__restrict__ lsplit,
                                             const double *__restrict__ part, double *__restrict__ F, int nt) {
     constexpr int R = P * P * P;
-    __shared__ double sL[4 * R];
     const int b = tleaves[blockIdx.x];
     const int2 sp = lsplit ? lsplit[blockIdx.x] : make_int2(-1, 1);
-    for (int e = threadIdx.x; e < 4 * R; e += 64) sL[e] = L[(size_t)b * 4 * R + e];
-    __syncthreads();
+    // the leaf's expansion is read at wave-uniform addresses: scalar loads
+    const double *sL = L + (size_t)b * 4 * R;
     const double cx = ctr[3 * b], cy = ctr[3 * b + 1], cz = ctr[3 * b + 2];
     const double *rd = lrad + 3 * level[b];
     for (int t = tbeg[b] + threadIdx.x; t < tend[b]; t += 64) {
@@ -867,7 +883,7 @@ int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, P2PWork &work, std::vector<vo
     int2 *ranges = nullptr;
     if ((rc = salloc(&cnt, (size_t)nl + 1))) return rc;
     IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, ((size_t)nl + 1) * sizeof(int), st));
-    k_p2p_ranges<0><<<nblk(nl, 64), 64, 0, st>>>(nl, ts.leaves, pl->loff[0], pl->litem[0], pl->loff[2], pl->litem[2],
+    k_p2p_ranges<0><<<nl, 64, 0, st>>>(nl, ts.leaves, pl->loff[0], pl->litem[0], pl->loff[2], pl->litem[2],
                                              pl->flags, pl->sbeg, pl->send, cnt, nullptr, nullptr);
     IBIM_LAUNCH_CHECK();
     std::vector<int> hc(nl);
@@ -880,7 +896,7 @@ int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, P2PWork &work, std::vector<vo
     }
     if ((rc = alloc(&roff, (size_t)nl + 1)) || (rc = alloc(&ranges, (size_t)std::max(1, ho[nl])))) return rc;
     IBIM_HIP_CHECK(hipMemcpyAsync(roff, ho.data(), ho.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    k_p2p_ranges<1><<<nblk(nl, 64), 64, 0, st>>>(nl, ts.leaves, pl->loff[0], pl->litem[0], pl->loff[2], pl->litem[2],
+    k_p2p_ranges<1><<<nl, 64, 0, st>>>(nl, ts.leaves, pl->loff[0], pl->litem[0], pl->loff[2], pl->litem[2],
                                              pl->flags, pl->sbeg, pl->send, nullptr, roff, ranges);
     IBIM_LAUNCH_CHECK();
 
