@@ -235,7 +235,7 @@ __global__ __launch_bounds__(256) void k_l2l(int lo, const unsigned char *__rest
 // node coordinates relative to B's centre are formed in FP64 and rounded,
 // the kernels are evaluated in FP32, and each source box's FP32 sums are
 // accumulated in FP64.
-template <int P, int BS, typename T>
+template <int P, int BS, typename T, bool POLY>
 __global__ __launch_bounds__(BS) void k_m2l_direct(int lists, const int *__restrict__ work, const int *__restrict__ voff,
                                                    const int *__restrict__ vit, const int *__restrict__ xoff,
                                                    const int *__restrict__ xit, const int *__restrict__ flags,
@@ -300,8 +300,8 @@ __global__ __launch_bounds__(BS) void k_m2l_direct(int lists, const int *__restr
                     if constexpr (F64) {
                         double f0 = acc[k][0], fx = acc[k][1], fy = acc[k][2], fz = acc[k][3];
                         for (int s = 0; s < R; ++s)
-                            pair_accum<double>(spx[s] - tx[k], spy[s] - ty[k], spz[s] - tz[k], sm[0][s], sm[1][s],
-                                               sm[2][s], sm[3][s], kc, f0, fx, fy, fz);
+                            pair_accum_m<POLY, double>(spx[s] - tx[k], spy[s] - ty[k], spz[s] - tz[k], sm[0][s],
+                                                       sm[1][s], sm[2][s], sm[3][s], kc, f0, fx, fy, fz);
                         acc[k][0] = f0;
                         acc[k][1] = fx;
                         acc[k][2] = fy;
@@ -309,8 +309,8 @@ __global__ __launch_bounds__(BS) void k_m2l_direct(int lists, const int *__restr
                     } else {
                         T f0 = 0, fx = 0, fy = 0, fz = 0;
                         for (int s = 0; s < R; ++s)
-                            pair_accum<T>(spx[s] - tx[k], spy[s] - ty[k], spz[s] - tz[k], sm[0][s], sm[1][s],
-                                          sm[2][s], sm[3][s], kc, f0, fx, fy, fz);
+                            pair_accum_m<POLY, T>(spx[s] - tx[k], spy[s] - ty[k], spz[s] - tz[k], sm[0][s], sm[1][s],
+                                                  sm[2][s], sm[3][s], kc, f0, fx, fy, fz);
                         acc[k][0] += (double)f0;
                         acc[k][1] += (double)fx;
                         acc[k][2] += (double)fy;
@@ -501,7 +501,7 @@ template <> struct Vec4<double> { using type = double4; };
 // to the unit's first target are formed in FP64 and rounded to FP32, the
 // kernels are evaluated in FP32, each lane's FP32 sums over one LDS batch
 // (at most 64 terms) are accumulated in FP64.
-template <bool MIXED>
+template <bool MIXED, bool POLY>
 __global__ __launch_bounds__(64) void k_p2p_unit(
     const int4 *__restrict__ ua, const int4 *__restrict__ ub, const double *__restrict__ tx,
     const double *__restrict__ ty, const double *__restrict__ tz, const double *__restrict__ sx,
@@ -549,12 +549,12 @@ __global__ __launch_bounds__(64) void k_p2p_unit(
             int k = sl;
             for (; k + nsl < cnt; k += 2 * nsl) {
                 const T4 u = s0[k], v = s1[k], u2 = s0[k + nsl], v2 = s1[k + nsl];
-                pair_accum<T>(u.x - px, u.y - py, u.z - pz, v.x, v.y, v.z, u.w, kc, g0, gx, gy, gz);
-                pair_accum<T>(u2.x - px, u2.y - py, u2.z - pz, v2.x, v2.y, v2.z, u2.w, kc, h0, hx, hy, hz);
+                pair_accum_m<POLY, T>(u.x - px, u.y - py, u.z - pz, v.x, v.y, v.z, u.w, kc, g0, gx, gy, gz);
+                pair_accum_m<POLY, T>(u2.x - px, u2.y - py, u2.z - pz, v2.x, v2.y, v2.z, u2.w, kc, h0, hx, hy, hz);
             }
             if (k < cnt) {
                 const T4 u = s0[k], v = s1[k];
-                pair_accum<T>(u.x - px, u.y - py, u.z - pz, v.x, v.y, v.z, u.w, kc, g0, gx, gy, gz);
+                pair_accum_m<POLY, T>(u.x - px, u.y - py, u.z - pz, v.x, v.y, v.z, u.w, kc, g0, gx, gy, gz);
             }
             f0 += (double)(g0 + h0);
             fx += (double)(gx + hx);
@@ -789,14 +789,14 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     // X list (cross-level) directly; V list as GEMMs when the pairs are grouped
     const int lists = ts.vpairs ? 2 : 3;
     if (ts.n_m2l_work > 0) {
-        if (mixed && ts.vpairs)
-            k_m2l_direct<P, BS, float><<<ts.n_m2l_work, BS, 0, st>>>(
-                lists, ts.m2l_work, pl->loff[1], pl->litem[1], pl->loff[3], pl->litem[3], pl->flags, pl->ctr,
-                pl->level, pl->d_lvl_rad, pl->tab, pl->kc, pl->M, pl->L);
-        else
-            k_m2l_direct<P, BS, double><<<ts.n_m2l_work, BS, 0, st>>>(
-                lists, ts.m2l_work, pl->loff[1], pl->litem[1], pl->loff[3], pl->litem[3], pl->flags, pl->ctr,
-                pl->level, pl->d_lvl_rad, pl->tab, pl->kc, pl->M, pl->L);
+        auto launch = [&](auto kern) {
+            kern<<<ts.n_m2l_work, BS, 0, st>>>(lists, ts.m2l_work, pl->loff[1], pl->litem[1], pl->loff[3],
+                                               pl->litem[3], pl->flags, pl->ctr, pl->level, pl->d_lvl_rad, pl->tab,
+                                               pl->kc, pl->M, pl->L);
+        };
+        const bool poly = pl->kc.mode == IBIM_KERNEL_POLY;
+        if (mixed && ts.vpairs) launch(poly ? k_m2l_direct<P, BS, float, true> : k_m2l_direct<P, BS, float, false>);
+        else launch(poly ? k_m2l_direct<P, BS, double, true> : k_m2l_direct<P, BS, double, false>);
     }
     IBIM_LAUNCH_CHECK();
     }
@@ -817,12 +817,13 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     StageScope sc(pl, kStP2P);
     if (ts.nleaves > 0 && ts.p2p) {
         const P2PWork &w = *ts.p2p;
-        if (mixed)
-            k_p2p_unit<true><<<w.nunits, 64, 0, st>>>(w.ua, w.ub, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q,
-                                                      pl->n, ts.ranges, pl->kc, F, nt, w.part);
-        else
-            k_p2p_unit<false><<<w.nunits, 64, 0, st>>>(w.ua, w.ub, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q,
-                                                       pl->n, ts.ranges, pl->kc, F, nt, w.part);
+        auto launch = [&](auto kern) {
+            kern<<<w.nunits, 64, 0, st>>>(w.ua, w.ub, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q, pl->n,
+                                          ts.ranges, pl->kc, F, nt, w.part);
+        };
+        const bool poly = pl->kc.mode == IBIM_KERNEL_POLY;
+        if (mixed) launch(poly ? k_p2p_unit<true, true> : k_p2p_unit<true, false>);
+        else launch(poly ? k_p2p_unit<false, true> : k_p2p_unit<false, false>);
         IBIM_LAUNCH_CHECK();
         k_l2p<P><<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->ctr, pl->level,
                                             pl->d_lvl_rad, pl->tab, pl->L, w.npart ? w.lsplit : nullptr, w.part,

@@ -87,25 +87,31 @@ __device__ __forceinline__ float recip(float x) { return 1.0f / x; }
 //   K2  = (1 - e) / (4 pi r)
 //   K3ab = delta_ab (1 - e (t+1)) / r^3 + (e (t^2 + 3t + 3) - 3) d_a d_b / r^5
 //   K4a = -d_a (e (t+1) / eps - 1) / (4 pi r^3)
-// and below the vacant radius K1 = K3 = K4 = 0, K2 = k2reg.
+// and below the vacant radius K1 = K3 = K4 = 0, K2 = k2reg.  pair_accum
+// dispatches on the kernel family at run time, pair_accum_m at compile time.
+
+// polynomial surrogate kernels (IBIM_KERNEL_POLY): K1a = d_a,
+// K2 = 0.5 + 0.25 r^2, K3ab = delta_ab + 0.5 d_a d_b, K4a = -0.75 d_a
 template <typename T>
-__device__ __forceinline__ void pair_accum(T dx, T dy, T dz, T qx, T qy, T qz, T qp, const PBConst &k,
-                                           T &f0, T &fx, T &fy, T &fz) {
-    if (k.mode == IBIM_KERNEL_POLY) {
-        // K1a = d_a, K2 = 0.5 + 0.25 r^2, K3ab = delta_ab + 0.5 d_a d_b, K4a = -0.75 d_a
-        T r2 = dx * dx + dy * dy + dz * dz;
-        T dq = dx * qx + dy * qy + dz * qz;
-        f0 += dq - (T(0.5) + T(0.25) * r2) * qp;
-        T s = T(0.5) * dq + T(0.75) * qp;
-        fx += qx + dx * s;
-        fy += qy + dy * s;
-        fz += qz + dz * s;
-        return;
-    }
+__device__ __forceinline__ void pair_accum_poly(T dx, T dy, T dz, T qx, T qy, T qz, T qp, T &f0, T &fx, T &fy,
+                                                T &fz) {
+    const T r2 = dx * dx + dy * dy + dz * dz;
+    const T dq = dx * qx + dy * qy + dz * qz;
+    f0 += dq - (T(0.5) + T(0.25) * r2) * qp;
+    const T s = T(0.5) * dq + T(0.75) * qp;
+    fx += qx + dx * s;
+    fy += qy + dy * s;
+    fz += qz + dz * s;
+}
+
+// the PB kernels.  Below the vacant radius only K2 = k2reg acts; that case
+// is evaluated without a branch (the kernel terms are computed at a safe r
+// and discarded).
+template <typename T>
+__device__ __forceinline__ void pair_accum_pb(T dx, T dy, T dz, T qx, T qy, T qz, T qp, const PBConst &k, T &f0,
+                                              T &fx, T &fy, T &fz) {
     T r2 = dx * dx + dy * dy + dz * dz;
     T r = sqrt(r2);
-    // below the vacant radius only K2 = k2reg acts; evaluated without a
-    // branch (the kernel terms are computed at r = 1 and discarded)
     const bool reg = r < T(k.vr);
     if (reg) r = T(1);
     T ir = recip(r);
@@ -135,36 +141,27 @@ __device__ __forceinline__ void pair_accum(T dx, T dy, T dz, T qx, T qy, T qz, T
     fz += a3 * qz + dz * s;
 }
 
-// FP32 form of pair_accum for the mixed-precision near field: the caller
-// rounds an FP64-formed coordinate difference; reciprocal square root and
-// exp use the hardware FP32 units.
+// FP32 form for the mixed-precision near field and X list: the caller rounds
+// an FP64-formed coordinate difference; reciprocal square root and exp use
+// the hardware FP32 units.  At r = 0 or below the vacant radius 1/r and r
+// are replaced by 0, which zeroes every term but K2.
 template <>
-__device__ __forceinline__ void pair_accum<float>(float dx, float dy, float dz, float qx, float qy, float qz,
-                                                  float qp, const PBConst &k, float &f0, float &fx, float &fy,
-                                                  float &fz) {
+__device__ __forceinline__ void pair_accum_pb<float>(float dx, float dy, float dz, float qx, float qy, float qz,
+                                                     float qp, const PBConst &k, float &f0, float &fx, float &fy,
+                                                     float &fz) {
     const float r2 = dx * dx + dy * dy + dz * dz;
-    if (k.mode == IBIM_KERNEL_POLY) {
-        const float dq = dx * qx + dy * qy + dz * qz;
-        f0 += dq - (0.5f + 0.25f * r2) * qp;
-        const float s = 0.5f * dq + 0.75f * qp;
-        fx += qx + dx * s;
-        fy += qy + dy * s;
-        fz += qz + dz * s;
-        return;
-    }
-    const float ir = rsqrtf(r2);
-    const float r = r2 * ir;
-    if (!(r >= (float)k.vr)) {  // also catches r2 == 0
-        f0 -= (float)k.k2reg * qp;
-        return;
-    }
+    const float ir0 = rsqrtf(r2);
+    const float r0 = r2 * ir0;
+    const bool reg = !(r0 >= (float)k.vr);  // also catches r2 == 0
+    const float ir = reg ? 0.0f : ir0;
+    const float r = reg ? 0.0f : r0;
     const float ir2 = ir * ir, ir3 = ir2 * ir;
     const float t = (float)k.kappa * r;
     const float e = __expf(-t);
     const float et1 = e * (t + 1.0f);
     const float c4 = (float)k.inv4pi * ir3;
     const float g1 = ((float)k.eps * et1 - 1.0f) * c4;
-    const float k2 = (1.0f - e) * ir * (float)k.inv4pi;
+    const float k2 = reg ? (float)k.k2reg : (1.0f - e) * ir * (float)k.inv4pi;
     const float a3 = (1.0f - et1) * ir3;
     const float b3 = (e * ((t + 3.0f) * t + 3.0f) - 3.0f) * ir3 * ir2;
     const float g4 = ((float)k.inv_eps * et1 - 1.0f) * c4;
@@ -174,6 +171,21 @@ __device__ __forceinline__ void pair_accum<float>(float dx, float dy, float dz, 
     fx += a3 * qx + dx * s;
     fy += a3 * qy + dy * s;
     fz += a3 * qz + dz * s;
+}
+
+template <typename T>
+__device__ __forceinline__ void pair_accum(T dx, T dy, T dz, T qx, T qy, T qz, T qp, const PBConst &k, T &f0,
+                                           T &fx, T &fy, T &fz) {
+    if (k.mode == IBIM_KERNEL_POLY) pair_accum_poly<T>(dx, dy, dz, qx, qy, qz, qp, f0, fx, fy, fz);
+    else pair_accum_pb<T>(dx, dy, dz, qx, qy, qz, qp, k, f0, fx, fy, fz);
+}
+
+// one pair with the kernel family fixed at compile time (hot loops)
+template <bool POLY, typename T>
+__device__ __forceinline__ void pair_accum_m(T dx, T dy, T dz, T qx, T qy, T qz, T qp, const PBConst &k, T &f0,
+                                             T &fx, T &fy, T &fz) {
+    if constexpr (POLY) pair_accum_poly<T>(dx, dy, dz, qx, qy, qz, qp, f0, fx, fy, fz);
+    else pair_accum_pb<T>(dx, dy, dz, qx, qy, qz, qp, k, f0, fx, fy, fz);
 }
 
 // S_p(u, xbar_m) for m < P at one standardized coordinate u (bbfmm.h:408-419)
@@ -231,7 +243,7 @@ struct VPairs {
     int nblocks = 0;                    // GEMM blocks of 64 pairs
     int *blk_cls = nullptr, *blk_j0 = nullptr;
     int *tptr = nullptr, *tlist = nullptr; // pairs of target node b: tlist[tptr[b] .. tptr[b+1])
-    int hnblocks = 0;                   // FP16 GEMM blocks (32 pairs for p <= 5, 16 for p = 6)
+    int hnblocks = 0;                   // FP16 GEMM blocks (128 pairs for p <= 5, 16 for p = 6)
     int *hblk_cls = nullptr, *hblk_j0 = nullptr;
 };
 
@@ -319,9 +331,11 @@ struct ibim_plan {
     float *ops32 = nullptr;              // FP32 copy for the mixed-precision matvec
     // FP16 M2L (IBIM_PREC_MIXED_F16): operators with one power-of-two scale
     // per row, multipoles with one per box, FP32 accumulation buffer
-    _Float16 *ops16 = nullptr;
-    int *esig = nullptr;                 // per (class, row) exponent
-    _Float16 *Mh = nullptr;              // nnodes * 4r, per box scaled
+    _Float16 *ops16 = nullptr;           // p <= 5: one unfolded operator per (class, mask) key
+                                         // with pairs (slot keyslot[key]); p = 6: one per class
+    int *esig = nullptr;                 // per (operator, row) exponent
+    int *keyslot = nullptr;
+    _Float16 *Mh = nullptr;              // nnodes * (4r rounded up to 64), per box scaled
     int *ebox = nullptr;                 // per box exponent
     float *Yv = nullptr;                 // V interaction results, v pairs * 4r
     int *gtab = nullptr, *stab = nullptr; // gather / scatter tables of the 8 reflections

@@ -370,8 +370,9 @@ __global__ __launch_bounds__(64) void k_ops_to_half(const double *__restrict__ o
 }
 
 // per box: FP16 multipoles with one power-of-two scale (every matvec)
-__global__ __launch_bounds__(64) void k_m_to_half(const double *__restrict__ M, int R4, _Float16 *__restrict__ Mh,
-                                                  int *__restrict__ ebox) {
+// rows of Mh are KP = R4 rounded up to 64 long, zero beyond R4
+__global__ __launch_bounds__(64) void k_m_to_half(const double *__restrict__ M, int R4, int KP,
+                                                  _Float16 *__restrict__ Mh, int *__restrict__ ebox) {
     const size_t a = blockIdx.x;
     const double *src = M + a * R4;
     double mx = 0.0;
@@ -379,7 +380,44 @@ __global__ __launch_bounds__(64) void k_m_to_half(const double *__restrict__ M, 
     for (int o = 32; o; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
     const int e = scale_exp(mx);
     if (threadIdx.x == 0) ebox[a] = e;
-    for (int k = threadIdx.x; k < R4; k += 64) Mh[a * R4 + k] = (_Float16)ldexp(src[k], -e);
+    for (int k = threadIdx.x; k < KP; k += 64) Mh[a * KP + k] = k < R4 ? (_Float16)ldexp(src[k], -e) : (_Float16)0;
+}
+
+// one block of 64 per row i of an unfolded FP16 operator: slot s holds the
+// key (class c, reflection mask m) and U[i][j] = so * si * K_c[s(i)][g(j)]
+// 2^-esig, where g and s are the mask's gather and scatter permutations
+// (reflections, their own inverses) and so, si the output / input signs,
+// so that L_b += U M_a needs neither permutation
+__global__ __launch_bounds__(64) void k_unfold_half(const int *__restrict__ slot_key, const double *__restrict__ ops,
+                                                    int MRP, int KRP, int R4, const int *__restrict__ gtab,
+                                                    const int *__restrict__ stab, _Float16 *__restrict__ ops16u,
+                                                    int *__restrict__ esig) {
+    const int i = blockIdx.x, sl = blockIdx.y;
+    const int key = slot_key[sl], c = key >> 3, m = key & 7;
+    _Float16 *dst = ops16u + ((size_t)sl * MRP + i) * KRP;
+    if (i >= R4) {
+        for (int j = threadIdx.x; j < KRP; j += 64) dst[j] = (_Float16)0;
+        if (threadIdx.x == 0) esig[(size_t)sl * MRP + i] = 0;
+        return;
+    }
+    const int row = stab[m * R4 + i] & 0x7fffffff;
+    const bool nout = stab[m * R4 + row] < 0;
+    const double *src = ops + ((size_t)c * MRP + row) * KRP;
+    double mx = 0.0;
+    for (int k = threadIdx.x; k < R4; k += 64) mx = fmax(mx, fabs(src[k]));
+    for (int o = 32; o; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    const int e = scale_exp(mx);
+    if (threadIdx.x == 0) esig[(size_t)sl * MRP + i] = e;
+    for (int j = threadIdx.x; j < KRP; j += 64) {
+        _Float16 v = (_Float16)0;
+        if (j < R4) {
+            const int k = gtab[m * R4 + j] & 0x7fffffff;
+            const bool nin = gtab[m * R4 + k] < 0;
+            const double x = ldexp(src[k], -e);
+            v = (_Float16)(nout != nin ? -x : x);
+        }
+        dst[j] = v;
+    }
 }
 
 // The V-list GEMM on FP16 MFMA (v_mfma_f32_16x16x32_f16, FP32 accumulation).
@@ -431,7 +469,7 @@ __global__ __launch_bounds__(256) void k_m2l_gemm_h(const int *__restrict__ blk_
     for (int it = tid; it < NP * (KP / 8); it += 256) {
         const int pr = it / (KP / 8), k0 = 8 * (it % (KP / 8));
         const int a = sPA[pr];
-        const size_t mb = (size_t)max(a, 0) * R4;
+        const size_t mb = (size_t)max(a, 0) * KP;
         h8 v;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -515,6 +553,112 @@ __global__ __launch_bounds__(256) void k_m2l_gemm_h(const int *__restrict__ blk_
     }
 }
 
+// The FP16 V-list GEMM for p <= 5 on unfolded operators (one per (class,
+// mask) key with pairs, k_unfold_half): a block takes 128 pairs of one key,
+// wave w the pairs [32 w, 32 w + 32).  Each wave loads its pairs' MFMA
+// A-fragments (8 consecutive k of one pair = 16 B of Mh) for all k straight
+// into registers once; the operator streams through LDS in 64-row x KC-k
+// chunks shared by the four waves (register prefetch of the next chunk), so
+// one pass over the operator serves 128 pairs.  Row-tile results, scaled by
+// 2^(esig + ebox), are stored in target order as Y[pair][row].
+template <int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_m2l_gemm_hu(
+    const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
+    const int *__restrict__ pA, const int *__restrict__ keyslot, const _Float16 *__restrict__ ops16u,
+    const int *__restrict__ esig, int MRP, const _Float16 *__restrict__ Mh, const int *__restrict__ ebox,
+    float *__restrict__ Y, int nblocks, int per_xcd) {
+    constexpr int R = P * P * P;
+    constexpr int R4 = 4 * R;
+    constexpr int KP = (R4 + 63) / 64 * 64;
+    constexpr int KS = KP / 32;                  // MFMA k-steps
+    constexpr int KC = KP < 128 ? KP : 128;      // k per operator chunk
+    constexpr int NKC = KP / KC;
+    constexpr int LDA = KC + 8;
+    constexpr int OPT = 64 * KC / 8 / 256;       // operator h8 per thread per chunk
+    static_assert(KP % KC == 0 && OPT >= 1, "tile shape");
+    __shared__ __attribute__((aligned(16))) _Float16 sA[64][LDA];
+    __shared__ int sEA[128];
+    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (tile >= nblocks) return;
+    const int tid = threadIdx.x;
+    const int key = blk_key[tile];
+    const int j0 = blk_j0[tile];
+    const int jend = key_off[key + 1];
+    const int sl = keyslot[key];
+    const int w = tid >> 6, l = tid & 63;
+    if (tid < 128) {
+        const int j = j0 + tid;
+        sEA[tid] = j < jend ? ebox[pA[j]] : 0;
+    }
+    // A-fragments of this wave's 32 pairs, all k
+    h8 xr[KS][2];
+#pragma unroll
+    for (int pi = 0; pi < 2; ++pi) {
+        const int j = j0 + 32 * w + 16 * pi + (l & 15);
+        const bool ok = j < jend;
+        const _Float16 *src = Mh + (size_t)(ok ? pA[j] : 0) * KP + 8 * (l >> 4);
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) xr[ks][pi] = ok ? *(const h8 *)(src + 32 * ks) : h8{};
+    }
+    const _Float16 *Aop = ops16u + (size_t)sl * MRP * KP;
+    const int nrt = MRP / 64;
+    h8 ra[OPT];
+    auto load_chunk = [&](int rt, int kc) {
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const int v = tid + 256 * i;
+            ra[i] = *(const h8 *)(Aop + (size_t)(rt * 64 + v / (KC / 8)) * KP + kc * KC + 8 * (v % (KC / 8)));
+        }
+    };
+    load_chunk(0, 0);
+    for (int rt = 0; rt < nrt; ++rt) {
+        f4v acc[2][4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < NKC; ++kc) {
+            __syncthreads();  // previous chunk consumed (and sEA written)
+#pragma unroll
+            for (int i = 0; i < OPT; ++i) {
+                const int v = tid + 256 * i;
+                *(h8 *)&sA[v / (KC / 8)][8 * (v % (KC / 8))] = ra[i];
+            }
+            __syncthreads();
+            if (kc + 1 < NKC) load_chunk(rt, kc + 1);
+            else if (rt + 1 < nrt) load_chunk(rt + 1, 0);
+#pragma unroll
+            for (int ks = 0; ks < KC / 32; ++ks) {
+                h8 kf[4];
+#pragma unroll
+                for (int ri = 0; ri < 4; ++ri) kf[ri] = *(const h8 *)&sA[16 * ri + (l & 15)][32 * ks + 8 * (l >> 4)];
+#pragma unroll
+                for (int pi = 0; pi < 2; ++pi)
+#pragma unroll
+                    for (int ri = 0; ri < 4; ++ri)
+                        acc[pi][ri] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xr[kc * (KC / 32) + ks][pi], kf[ri],
+                                                                             acc[pi][ri], 0, 0, 0);
+            }
+        }
+        // register q of lane l holds pair 4 (l >> 4) + q, row l & 15
+#pragma unroll
+        for (int ri = 0; ri < 4; ++ri) {
+            const int row = rt * 64 + 16 * ri + (l & 15);
+            if (row >= R4) continue;
+            const int er = esig[(size_t)sl * MRP + row];
+#pragma unroll
+            for (int pi = 0; pi < 2; ++pi)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int col = 32 * w + 16 * pi + 4 * (l >> 4) + q;
+                    if (j0 + col >= jend) continue;
+                    Y[(size_t)(j0 + col) * R4 + row] = ldexpf(acc[pi][ri][q], er + sEA[col]);
+                }
+        }
+    }
+}
+
 // L_b += sum over the V pairs (a, b) of the pair's result: one block per
 // target node, four pairs' rows in flight per step
 template <int R4>
@@ -549,14 +693,21 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
     if constexpr (P <= 6) {
         if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->ops16 && vp.tptr && pl->Yv &&
             vp.npairs <= pl->vp.npairs) {
-            constexpr int NP = P <= 5 ? 32 : 16;
-            constexpr int KC = P <= 5 ? 64 : 32;
-            k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, pl->Mh, pl->ebox);
+            constexpr int NP = 16;  // p = 6 blocks
+            constexpr int KC = 32;
+            constexpr int KP = (R4 + 63) / 64 * 64;
+            k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, KP, pl->Mh, pl->ebox);
             if (vp.hnblocks > 0) {
                 const int pxcd = (vp.hnblocks + 7) / 8;
-                k_m2l_gemm_h<P, NP, KC><<<8 * pxcd, 256, 0, st>>>(vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A,
-                                                                  pl->gtab, pl->stab, pl->ops16, pl->esig, pl->MRP,
-                                                                  pl->Mh, pl->ebox, pl->Yv, vp.hnblocks, pxcd);
+                if constexpr (P <= 5)
+                    k_m2l_gemm_hu<P><<<8 * pxcd, 256, 0, st>>>(vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A,
+                                                               pl->keyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
+                                                               pl->ebox, pl->Yv, vp.hnblocks, pxcd);
+                else
+                    k_m2l_gemm_h<P, NP, KC><<<8 * pxcd, 256, 0, st>>>(vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A,
+                                                                      pl->gtab, pl->stab, pl->ops16, pl->esig,
+                                                                      pl->MRP, pl->Mh, pl->ebox, pl->Yv, vp.hnblocks,
+                                                                      pxcd);
                 k_v_reduce<R4><<<pl->n_m2l_work, 256, 0, st>>>(pl->m2l_work, vp.tptr, vp.tlist, pl->Yv, pl->L);
             }
             IBIM_LAUNCH_CHECK();
@@ -702,9 +853,9 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
         IBIM_HIP_CHECK(hipMemcpyAsync(vp.blk_cls, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice, st));
         IBIM_HIP_CHECK(hipMemcpyAsync(vp.blk_j0, bj.data(), bj.size() * sizeof(int), hipMemcpyHostToDevice, st));
     }
-    // FP16 GEMM block table (blocks of 32 pairs for p <= 5, 16 for p = 6)
+    // FP16 GEMM block table (blocks of 128 pairs for p <= 5, 16 for p = 6)
     if (pl->p <= 6) {
-        const int hnp = pl->p <= 5 ? 32 : 16;
+        const int hnp = pl->p <= 5 ? 128 : 16;
         std::vector<int> hc, hj;
         for (int c = 0; c < nkeys; ++c)
             for (int j = vp.cls_off[c]; j < vp.cls_off[c + 1]; j += hnp) {
@@ -771,16 +922,47 @@ int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed) {
 // must exist)
 int build_m2l_operators_f16(ibim_plan *pl) {
     if (pl->ops16 || !pl->ops || pl->n_cls == 0) return IBIM_OK;
-    const int64_t rows = (int64_t)pl->n_cls * pl->MRP;
-    const int64_t nf = (int64_t)pl->nnodes * 4 * pl->r;
-    int rc;
     if (pl->p > 6) return build_m2l_operators_f32(pl);  // the FP16 GEMM's LDS tiles stop at p = 6
-    if ((rc = dalloc(pl, &pl->ops16, (size_t)(rows * pl->KRP))) || (rc = dalloc(pl, &pl->esig, (size_t)rows)) ||
-        (rc = dalloc(pl, &pl->Mh, (size_t)nf)) || (rc = dalloc(pl, &pl->ebox, (size_t)pl->nnodes)) ||
-        (rc = dalloc(pl, &pl->Yv, std::max<size_t>(1, (size_t)pl->vp.npairs * 4 * pl->r))))
+    const int R4 = 4 * pl->r;
+    const int KP = (R4 + 63) / 64 * 64;
+    int rc;
+    if ((rc = dalloc(pl, &pl->Mh, (size_t)pl->nnodes * KP)) || (rc = dalloc(pl, &pl->ebox, (size_t)pl->nnodes)) ||
+        (rc = dalloc(pl, &pl->Yv, std::max<size_t>(1, (size_t)pl->vp.npairs * R4))))
         return rc;
-    k_ops_to_half<<<(unsigned)rows, 64, 0, pl->stream>>>(pl->ops, pl->KRP, pl->ops16, pl->esig);
-    IBIM_LAUNCH_CHECK();
+    if (pl->p == 6) {  // folded operators, one FP16 copy per class
+        const int64_t rows = (int64_t)pl->n_cls * pl->MRP;
+        if ((rc = dalloc(pl, &pl->ops16, (size_t)(rows * pl->KRP))) || (rc = dalloc(pl, &pl->esig, (size_t)rows)))
+            return rc;
+        k_ops_to_half<<<(unsigned)rows, 64, 0, pl->stream>>>(pl->ops, pl->KRP, pl->ops16, pl->esig);
+        IBIM_LAUNCH_CHECK();
+        return IBIM_OK;
+    }
+    // unfolded operators for the (class, mask) keys that have pairs
+    const std::vector<int> &off = pl->vp.cls_off;
+    std::vector<int> keyslot(off.size() > 0 ? off.size() - 1 : 0, -1), slot_key;
+    for (size_t k = 0; k + 1 < off.size(); ++k)
+        if (off[k + 1] > off[k]) {
+            keyslot[k] = (int)slot_key.size();
+            slot_key.push_back((int)k);
+        }
+    const int ns = (int)slot_key.size();
+    int *d_slot_key = nullptr;
+    if ((rc = dalloc(pl, &pl->keyslot, std::max<size_t>(1, keyslot.size()))) ||
+        (rc = dalloc(pl, &d_slot_key, std::max(1, ns))) ||
+        (rc = dalloc(pl, &pl->ops16, std::max<size_t>(1, (size_t)ns * pl->MRP * pl->KRP))) ||
+        (rc = dalloc(pl, &pl->esig, std::max<size_t>(1, (size_t)ns * pl->MRP))))
+        return rc;
+    if (!keyslot.empty())
+        IBIM_HIP_CHECK(hipMemcpyAsync(pl->keyslot, keyslot.data(), keyslot.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, pl->stream));
+    if (ns > 0) {
+        IBIM_HIP_CHECK(hipMemcpyAsync(d_slot_key, slot_key.data(), ns * sizeof(int), hipMemcpyHostToDevice,
+                                      pl->stream));
+        k_unfold_half<<<dim3(pl->MRP, ns), 64, 0, pl->stream>>>(d_slot_key, pl->ops, pl->MRP, pl->KRP, R4, pl->gtab,
+                                                                pl->stab, pl->ops16, pl->esig);
+        IBIM_LAUNCH_CHECK();
+    }
+    IBIM_HIP_CHECK(hipStreamSynchronize(pl->stream));
     return IBIM_OK;
 }
 
