@@ -252,12 +252,13 @@ def main():
     # ---- roofline of the dominant kernel in the timed mixed solves
     r = cfg["p"] ** 3
     totals = {k: v[0] for k, v in stage_acc.items() if v[1] > 0 and k not in ("MATVEC", "MATVEC_MIXED")}
+    # M2L_MIXED / P2P_MIXED: the inner (mixed) matvecs; M2L / P2P: the FP64 outer residuals
     dom = max(totals, key=totals.get)  # the stage with the most time per solve
-    ms_gemm = stage_acc["M2L"][0] / stage_acc["M2L"][1]
+    ms_gemm = stage_acc["M2L_MIXED"][0] / stage_acc["M2L_MIXED"][1]
     flops = 32.0 * r * r * stats["v_pairs"]  # 2 x (4r) x (4r) per V interaction
     achieved = flops / (ms_gemm * 1e-3) / 1e12
     kern, peak = {"mixed_f32": ("k_m2l_gemm<5,float> (FP32 MFMA)", PEAK_TFLOPS["f32"]),
-                  "mixed_f16": ("k_m2l_gemm_h<5> + k_m_to_half + k_v_reduce (FP16 MFMA)", PEAK_TFLOPS["f16"])}[best]
+                  "mixed_f16": ("k_m_to_half + k_m2l_gemm_hu<5> + k_v_reduce (FP16 MFMA)", PEAK_TFLOPS["f16"])}[best]
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         try:
@@ -266,11 +267,11 @@ def main():
             traffic = None
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": traffic, "kernel": kern,
-                "dominant_stage": dom, "ms_per_launch": ms_gemm, "launches": stage_acc["M2L"][1],
+                "dominant_stage": dom, "ms_per_launch": ms_gemm, "launches": stage_acc["M2L_MIXED"][1],
                 "algorithmic_flops_per_launch": flops,
-                "note": "V-list M2L stage of the inner (mixed) matvec: 32 r^2 flops per V interaction (r = p^3), "
-                        "v_pairs interactions per launch; time from HIP events on the plan stream over the timed "
-                        "mixed solves, all kernels of the stage included"}
+                "note": "V-list M2L stage of the inner (mixed) matvecs: 32 r^2 flops per V interaction (r = p^3), "
+                        "v_pairs interactions per launch; time from HIP events on the plan stream around the stage "
+                        "over the timed mixed solves, all kernels of the stage included"}
     stage_ms = {k: v[0] / args.steps for k, v in stage_acc.items()}
 
     cpu = None

@@ -31,7 +31,8 @@ KID = {k: i for i, k in enumerate(KERNELS)}
 PREC_FP64, PREC_MIXED, PREC_MIXED_F16 = 0, 1, 2
 KERNEL_PB, KERNEL_POLY = 0, 1
 IBIM_OK, IBIM_NOT_CONVERGED = 0, 1
-STAGES = ["P2M", "M2M", "M2L", "L2L", "P2P", "MATVEC", "ORTH", "RHS", "POLARIZE", "M2L_X", "MATVEC_MIXED"]
+STAGES = ["P2M", "M2M", "M2L", "L2L", "P2P", "MATVEC", "ORTH", "RHS", "POLARIZE", "M2L_X", "MATVEC_MIXED", "M2L_MIXED",
+          "P2P_MIXED"]
 
 
 class IbimError(RuntimeError):

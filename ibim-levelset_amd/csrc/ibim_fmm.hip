@@ -801,7 +801,7 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     IBIM_LAUNCH_CHECK();
     }
     if (ts.vpairs) {
-        StageScope sc(pl, kStM2L);
+        StageScope sc(pl, mixed ? kStM2LMixed : kStM2L);
         int rc = m2l_gemm(pl, *ts.vpairs, mixed);
         if (rc) return rc;
     }
@@ -814,7 +814,7 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     IBIM_LAUNCH_CHECK();
     }
     {
-    StageScope sc(pl, kStP2P);
+    StageScope sc(pl, mixed ? kStP2PMixed : kStP2P);
     if (ts.nleaves > 0 && ts.p2p) {
         const P2PWork &w = *ts.p2p;
         auto launch = [&](auto kern) {

@@ -208,6 +208,7 @@ __device__ __forceinline__ void sp_values(double u, const ChebTab &tab, double *
 // stage timers: HIP events recorded on the plan's stream around each stage
 enum Stage {
     kStP2M = 0, kStM2M, kStM2L, kStL2L, kStP2P, kStMatvec, kStOrth, kStRhs, kStPolarize, kStM2LX, kStMatvecMixed,
+    kStM2LMixed, kStP2PMixed,
     kStCount
 };
 

@@ -126,7 +126,9 @@ int64_t ibim_plan_footprint(const ibim_plan *plan);
 #define IBIM_STAGE_POLARIZE 8
 #define IBIM_STAGE_M2L_X 9    /* X list, direct, when the V list runs as GEMM */
 #define IBIM_STAGE_MATVEC_MIXED 10 /* one mixed-precision matvec */
-#define IBIM_STAGE_COUNT 11
+#define IBIM_STAGE_M2L_MIXED 11    /* V list of a mixed-precision matvec (counted here, not in M2L) */
+#define IBIM_STAGE_P2P_MIXED 12    /* near field + L2P of a mixed-precision matvec (not in P2P) */
+#define IBIM_STAGE_COUNT 13
 int ibim_plan_profile(ibim_plan *plan, int enable);
 int ibim_plan_stage_times(ibim_plan *plan, double *ms, int64_t *launches, int reset);
 
