@@ -581,7 +581,7 @@ __global__ __launch_bounds__(64) void k_p2p_unit(
             for (int f = 0; f < 4; ++f) F[f * (size_t)nt + t] = v[f];
         } else {
 #pragma unroll
-            for (int f = 0; f < 4; ++f) part[(size_t)slot * 256 + f * 64 + lane] = v[f];
+            for (int f = 0; f < 4; ++f) part[(size_t)slot * 256 + f * 64 + c.w + lane] = v[f];
         }
     }
 }
@@ -592,6 +592,29 @@ __device__ __forceinline__ int range_sources(const int *roff, const int2 *ranges
     return ns;
 }
 
+// Target chunks of a leaf: full chunks of 64, then the remainder r as one
+// chunk if r <= 32, else as 32 + (r - 32), so that a unit's lanes (targets x
+// floor(64 / targets) source slices) stay mostly busy.  A leaf's partial
+// fields are laid out by 64-target blocks; a remainder chunk's lanes start at
+// offset 0 or 32 of its block.
+__device__ __forceinline__ int p2p_target_chunks(int nt) {
+    const int q = nt / 64, r = nt % 64;
+    return q + (r == 0 ? 0 : (r > 32 ? 2 : 1));
+}
+__device__ __forceinline__ void p2p_target_chunk(int nt, int tc, int &first, int &count) {
+    const int q = nt / 64, r = nt % 64;
+    if (tc < q) {
+        first = 64 * tc;
+        count = 64;
+    } else if (r > 32) {
+        first = 64 * q + (tc == q ? 0 : 32);
+        count = tc == q ? 32 : r - 32;
+    } else {
+        first = 64 * q;
+        count = r;
+    }
+}
+
 // units and partial slots per target leaf
 __global__ void k_p2p_count(int nl, const int *__restrict__ tleaves, const int *__restrict__ tbeg,
                             const int *__restrict__ tend, const int *__restrict__ roff,
@@ -600,10 +623,10 @@ __global__ void k_p2p_count(int nl, const int *__restrict__ tleaves, const int *
     if (i >= nl) return;
     const int b = tleaves[i];
     const int ns = range_sources(roff, ranges, i);
-    const int nct = (tend[b] - tbeg[b] + 63) / 64;
+    const int nt = tend[b] - tbeg[b];
     const int ncs = max(1, (ns + kP2PChunk - 1) / kP2PChunk);
-    ucnt[i] = nct * ncs;
-    pcnt[i] = ncs > 1 ? nct * ncs : 0;
+    ucnt[i] = p2p_target_chunks(nt) * ncs;
+    pcnt[i] = ncs > 1 ? (nt + 63) / 64 * ncs : 0;
 }
 
 // unit records, their cost (per-lane pair evaluations), and the per-leaf
@@ -618,7 +641,7 @@ __global__ void k_p2p_fill(int nl, const int *__restrict__ tleaves, const int *_
     const int b = tleaves[i];
     const int ns = range_sources(roff, ranges, i);
     const int ntl = tend[b] - tbeg[b];
-    const int nct = (ntl + 63) / 64;
+    const int nct = p2p_target_chunks(ntl);
     const int ncs = max(1, (ns + kP2PChunk - 1) / kP2PChunk);
     lsplit[i] = make_int2(ncs > 1 ? poff[i] : -1, ncs);
     int u = uoff[i];
@@ -626,9 +649,11 @@ __global__ void k_p2p_fill(int nl, const int *__restrict__ tleaves, const int *_
     for (int sc = 0; sc < ncs; ++sc) {
         const int cnt = max(0, min(kP2PChunk, ns - sc * kP2PChunk));
         for (int tc = 0; tc < nct; ++tc) {
-            const int ntc = min(64, ntl - 64 * tc);
-            ua[u] = make_int4(i, tbeg[b] + 64 * tc, ntc, ncs > 1 ? poff[i] + tc * ncs + sc : -1);
-            ub[u] = make_int4(r, off, cnt, 0);
+            int first, ntc;
+            p2p_target_chunk(ntl, tc, first, ntc);
+            const int blk = first / 64;  // 64-target block of the partial-field layout
+            ua[u] = make_int4(i, tbeg[b] + first, ntc, ncs > 1 ? poff[i] + blk * ncs + sc : -1);
+            ub[u] = make_int4(r, off, cnt, first % 64);
             cost[u] = cnt / (64 / ntc) + 1;
             idx[u] = u;
             ++u;

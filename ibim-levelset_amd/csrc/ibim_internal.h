@@ -269,7 +269,7 @@ struct TargetSet {
 struct P2PWork {
     int nunits = 0;
     int4 *ua = nullptr;     // {target leaf index, first target, targets, partial slot or -1}
-    int4 *ub = nullptr;     // {first range, offset in it, sources, 0}
+    int4 *ub = nullptr;     // {first range, offset in it, sources, first lane in the partial block}
     int2 *lsplit = nullptr; // per target leaf: {first partial slot, source chunks}
     double *part = nullptr; // 256 doubles (4 fields x 64 targets) per partial slot
     size_t npart = 0;
