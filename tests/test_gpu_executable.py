@@ -23,15 +23,18 @@ def run(tmp_path, cfg_lines):
     return out.stdout
 
 
+def born_cfg(tmp_path, pqr, precision):
+    return ["# Born ion, Angstrom grid", f"pqr_file = {pqr}", f"energy_file = {tmp_path / 'out.energy'}",
+            f"potent_file = {tmp_path / 'out.potent'}", "grid_h = 0.125", "lvlset_thickness = 2",
+            "lvlset_thres = 1e-4", "gmres_restart = 20", "gmres_maxiter = 200", "gmres_tol = 1e-8",
+            "fmm_np = 4", "fmm_maxpoint = 64", "fmm_maxlevel = 10", "tau = 1.0", "solvent_kappa = 0.0",
+            "solvent_dE = 80.0", "solvent_dI = 1.0", f"precision = {precision}"]
+
+
 def test_born_ion_energy(tmp_path, gpu):
     pqr = tmp_path / "ion.pqr"
     pqr.write_text("ATOM      1  NA  ION     1       0.000   0.000   0.000  1.0000 2.0000\n")
-    stdout = run(tmp_path, [
-        "# Born ion, Angstrom grid", f"pqr_file = {pqr}", f"energy_file = {tmp_path / 'out.energy'}",
-        f"potent_file = {tmp_path / 'out.potent'}", "grid_h = 0.125", "lvlset_thickness = 2",
-        "lvlset_thres = 1e-4", "gmres_restart = 20", "gmres_maxiter = 200", "gmres_tol = 1e-8",
-        "fmm_np = 4", "fmm_maxpoint = 64", "fmm_maxlevel = 10", "tau = 1.0", "solvent_kappa = 0.0",
-        "solvent_dE = 80.0", "solvent_dI = 1.0"])
+    stdout = run(tmp_path, born_cfg(tmp_path, pqr, "fp64"))
     m = re.search(r"polarized energy:\s+(\S+)", stdout)
     assert m, stdout
     energy = float(m.group(1))
@@ -41,6 +44,22 @@ def test_born_ion_energy(tmp_path, gpu):
     assert len(open(tmp_path / "out.potent").read().split()) == 2 * n
     phi = float(open(tmp_path / "out.energy").read().split()[0])
     assert 0.5 * phi == pytest.approx(energy, rel=1e-5)  # files carry 6 significant digits, as the reference writes them
+
+
+@pytest.mark.parametrize("precision", ["mixed", "mixed16"])
+def test_born_ion_mixed_precision_matches_fp64(tmp_path, gpu, precision):
+    """The executable's `precision` key: the mixed forms give the FP64 energy
+    within 1e-6 relative; the energy is printed as the reference prints it
+    (7 significant digits), so two printed values may differ by up to 1e-6
+    more from rounding: bar 2e-6."""
+    pqr = tmp_path / "ion.pqr"
+    pqr.write_text("ATOM      1  NA  ION     1       0.000   0.000   0.000  1.0000 2.0000\n")
+    e = {}
+    for prec in ("fp64", precision):
+        out = run(tmp_path, born_cfg(tmp_path, pqr, prec))
+        assert "CONVERGED" in out, out
+        e[prec] = float(re.search(r"polarized energy:\s+(\S+)", out).group(1))
+    assert abs(e[precision] - e["fp64"]) <= 2e-6 * abs(e["fp64"]), e
 
 
 def test_missing_pqr_fails_loudly(tmp_path, gpu):

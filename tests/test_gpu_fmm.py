@@ -109,7 +109,7 @@ def test_all_16_kernels_shell(ibim, oracle, gpu):
             assert rel(g, o) < TOL, (k, rel(g, o))
 
 
-@pytest.mark.parametrize("p", [3, 4, 6])
+@pytest.mark.parametrize("p", [3, 4, 6, 8])
 def test_polynomial_kernels_exact(ibim, oracle, gpu, p):
     """With polynomial kernels the Chebyshev FMM is exact, so the GPU FMM must
     equal the direct sum over all pairs: pins tree, lists and all operators."""

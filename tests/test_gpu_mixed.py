@@ -28,18 +28,21 @@ def sphere_case(ibim, gpu):
 MATVEC_BAR = {"mixed": 1e-5, "mixed16": 1e-3}
 
 
+# p = 5 runs the unfolded-operator FP16 GEMM, p = 6 the folded one, p = 8 the
+# FP32 GEMM (the FP16 tiles stop at p = 6), p = 3 the smallest tiles
+@pytest.mark.parametrize("p", [3, 5, 6, 8])
 @pytest.mark.parametrize("mode", ["mixed", "mixed16"])
-def test_mixed_matvec_close_to_fp64(ibim, sphere_case, mode):
+def test_mixed_matvec_close_to_fp64(ibim, sphere_case, mode, p):
     pts, nrm, w = sphere_case
     prec = {"mixed": ibim.PREC_MIXED, "mixed16": ibim.PREC_MIXED_F16}[mode]
     x = np.random.default_rng(0).uniform(-1, 1, 2 * len(pts))
-    with ibim.Plan(pts, nrm, w, p=5, max_points=64, vacant_radius=0.125, **PB) as plan:
+    with ibim.Plan(pts, nrm, w, p=p, max_points=64, vacant_radius=0.125, **PB) as plan:
         y64 = plan.matvec(x)
         plan.set_precision(prec)
         y32 = plan.matvec(x)
         far = plan.stats()["v_pairs"]
     err = rel(y32, y64)
-    print(f"{mode} matvec rel err {err:.2e} ({far} V interactions)")
+    print(f"{mode} p={p} matvec rel err {err:.2e} ({far} V interactions)")
     assert far > 0
     assert err < MATVEC_BAR[mode]
     assert err > 0  # the mixed path really ran

@@ -17,22 +17,28 @@ def molecules(ibim):
     return out
 
 
-def test_same_alone_and_in_the_mix(ibim, gpu):
+@pytest.mark.parametrize("precision", ["fp64", "mixed16"])
+def test_same_alone_and_in_the_mix(ibim, gpu, precision):
+    """Bar: energy within 1e-10 relative alone vs in the mix, in FP64 and in
+    the FP16-M2L mixed form (whose FP64 outer residual uses the FP64 GEMM's
+    atomics, so its iterates may differ at rounding level: iteration counts
+    within one)."""
+    fmm = dict(FMM, precision={"fp64": ibim.PREC_FP64, "mixed16": ibim.PREC_MIXED_F16}[precision])
     mols = molecules(ibim)
     alone = []
     with ibim.Server(slots=1) as srv:
         for xyz, r, q in mols:
-            res = srv.wait(srv.submit(xyz, r, q, h=0.5, tol=1e-8, **FMM))
+            res = srv.wait(srv.submit(xyz, r, q, h=0.5, tol=1e-8, **fmm))
             assert res["status"] == 0, res
             alone.append(res)
     with ibim.Server(slots=4) as srv:
-        ids = [srv.submit(xyz, r, q, h=0.5, tol=1e-8, **FMM) for xyz, r, q in mols]
+        ids = [srv.submit(xyz, r, q, h=0.5, tol=1e-8, **fmm) for xyz, r, q in mols]
         mixed = [srv.wait(i) for i in ids]
     for a, b in zip(alone, mixed):
         assert b["status"] == 0
         assert a["n_points"] == b["n_points"]
         assert abs(a["energy"] - b["energy"]) <= 1e-10 * abs(a["energy"]), (a["energy"], b["energy"])
-        assert a["info"]["iterations"] == b["info"]["iterations"]
+        assert abs(a["info"]["iterations"] - b["info"]["iterations"]) <= (0 if precision == "fp64" else 1)
     # with 4 slots some requests ran at the same time
     spans = sorted((r["t_start"], r["t_end"]) for r in mixed)
     assert any(spans[k + 1][0] < spans[k][1] for k in range(len(spans) - 1))
