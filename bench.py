@@ -48,6 +48,9 @@ CONFIGS = {
     # name: molecule (atoms, seed, confinement), grid h, p, leaf, kappa, eps_in, eps_out
     "globule500": dict(atoms=500, seed=500, confine=20.0, h=0.25, p=5, leaf=192, max_level=10, kappa=0.1257,
                        eps_in=2.0, eps_out=80.0, tau=1.0, restart=40, tol=1e-8),
+    # the 5000-atom globule in a 45 A sphere at p = 6 (not the default workload)
+    "globule5000": dict(atoms=5000, seed=5000, confine=45.0, h=0.25, p=6, leaf=256, max_level=10, kappa=0.1257,
+                        eps_in=2.0, eps_out=80.0, tau=1.0, restart=40, tol=1e-8),
 }
 
 
@@ -104,6 +107,48 @@ def pct(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
 
 
+def serve_stream(ibim, n, slots, precision, cfg):
+    """A request stream like the serving configuration, scaled down: n
+    globules with atom counts exp(U[ln 100, ln 2000]) (seed 4), molecule i from
+    seed 10000 + i in a sphere of radius 2.6 m^(1/3) A, h = 0.35 A, the
+    workload's FMM settings.  A burst measures the saturated rate mu; the
+    same molecules then arrive as a Poisson process at 0.7 mu."""
+    import threading
+    rng = np.random.default_rng(4)
+    sizes = np.exp(rng.uniform(np.log(100), np.log(2000), n)).astype(int)
+    mols = [ibim.synthetic_globule(int(m), 10000 + i, 2.6 * m ** (1 / 3)) for i, m in enumerate(sizes)]
+    fmm = dict(p=cfg["p"], max_points=cfg["leaf"], max_level=cfg["max_level"], kappa=cfg["kappa"],
+               eps_in=cfg["eps_in"], eps_out=cfg["eps_out"], vacant_radius=cfg["tau"] * 0.35, precision=precision)
+    out = {"requests": n, "slots": slots, "atoms": [int(m) for m in sizes], "h": 0.35}
+    with ibim.Server(slots=slots) as srv:
+        t0 = time.perf_counter()
+        ids = [srv.submit(a, r, q, h=0.35, restart=cfg["restart"], tol=cfg["tol"], **fmm) for a, r, q in mols]
+        res = [srv.wait(i) for i in ids]
+        mu = n / (time.perf_counter() - t0)
+        out["burst"] = {"solves_per_s": mu, "all_ok": all(r["status"] == 0 for r in res),
+                        "tube_points": [int(r["n_points"]) for r in res]}
+        lam = 0.7 * mu
+        gaps = np.random.default_rng(5).exponential(1.0 / lam, n)
+        ids = []
+
+        def producer():
+            for (a, r, q), g in zip(mols, gaps):
+                time.sleep(g)
+                ids.append(srv.submit(a, r, q, h=0.35, restart=cfg["restart"], tol=cfg["tol"], **fmm))
+
+        t1 = time.perf_counter()
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join()
+        res = [srv.wait(i) for i in ids]
+        wall = time.perf_counter() - t1
+    lat = [1e3 * (r["t_end"] - r["t_submit"]) for r in res]
+    out["poisson"] = {"rate_per_s": lam, "solves_per_s": n / wall, "latency_ms": {"p50": pct(lat, 50),
+                                                                                  "p99": pct(lat, 99)},
+                      "all_ok": all(r["status"] == 0 and r["info"]["converged"] == 1 for r in res)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +158,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--serve", type=int, default=8, help="requests in the served burst (0 = skip)")
     ap.add_argument("--slots", type=int, default=2, help="server slots of the served burst")
+    ap.add_argument("--stream", type=int, default=0, metavar="N",
+                    help="also serve a stream of N seeded globules of 100..2000 atoms (h = 0.35 A): a burst for the "
+                         "saturated rate, then Poisson arrivals at 70%% of it")
     ap.add_argument("--leaf", type=int, default=None, help="override the leaf capacity")
     ap.add_argument("--matvecs-only", type=int, default=0, metavar="K",
                     help="counter-collection mode: one 256 MiB device copy (calibration), then K matvecs in each "
@@ -249,6 +297,8 @@ def main():
                    "note": "burst submitted at t=0; latency = completion - submission, geometry (level set, tube) "
                            "and plan build included per request"}
 
+    stream = serve_stream(ibim, args.stream, args.slots, PRECS[best], cfg) if args.stream > 0 else None
+
     # ---- roofline of the dominant kernel in the timed mixed solves
     r = cfg["p"] ** 3
     totals = {k: v[0] for k, v in stage_acc.items() if v[1] > 0 and k not in ("MATVEC", "MATVEC_MIXED")}
@@ -257,10 +307,14 @@ def main():
     ms_gemm = stage_acc["M2L_MIXED"][0] / stage_acc["M2L_MIXED"][1]
     flops = 32.0 * r * r * stats["v_pairs"]  # 2 x (4r) x (4r) per V interaction
     achieved = flops / (ms_gemm * 1e-3) / 1e12
-    kern, peak = {"mixed_f32": ("k_m2l_gemm<5,float> (FP32 MFMA)", PEAK_TFLOPS["f32"]),
-                  "mixed_f16": ("k_m_to_half + k_m2l_gemm_hu<5> + k_v_reduce (FP16 MFMA)", PEAK_TFLOPS["f16"])}[best]
+    p = cfg["p"]
+    gemm16 = f"k_m2l_gemm_hu<{p}>" if p <= 5 else f"k_m2l_gemm_h<{p}>" if p == 6 else f"k_m2l_gemm<{p},float>"
+    kern, peak = {"mixed_f32": (f"k_m2l_gemm<{p},float> (FP32 MFMA)", PEAK_TFLOPS["f32"]),
+                  "mixed_f16": (f"k_m_to_half + {gemm16} + k_v_reduce (FP16 MFMA)" if p <= 6 else
+                                f"{gemm16} (FP32 MFMA; the FP16 tiles stop at p = 6)",
+                                PEAK_TFLOPS["f16"] if p <= 6 else PEAK_TFLOPS["f32"])}[best]
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
+    if args.config == "globule500" and cfg["leaf"] == 192 and best == "mixed_f16" and os.path.exists(TRAFFIC_FILE):
         try:
             traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_launch")
         except (OSError, ValueError):
@@ -290,7 +344,8 @@ def main():
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": mixed["ms_per_solve"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32/f64", "data": "synthetic",
-        "config": {"workload": f"{args.config}: synthetic 500-atom globule, h=0.25 A, p=5, leaf {cfg['leaf']}, "
+        "config": {"workload": f"{args.config}: synthetic {cfg['atoms']}-atom globule, h={cfg['h']} A, p={cfg['p']}, "
+                               f"leaf {cfg['leaf']}, "
                                f"kappa=0.1257, eps 2/80, GMRES(40) tol 1e-8; value = the faster mixed-precision "
                                f"form meeting 1e-6 energy error ({best}: FP32 near field and X list, "
                                f"{'FP16' if best == 'mixed_f16' else 'FP32'} V-list M2L, inside FP64 iterative "
@@ -306,7 +361,7 @@ def main():
         "rel_energy_err_vs_fp64": rel_err, "energy_within_1e-6": energy_ok,
         "fp64_matvecs_per_s": fp64["solve"]["matvecs"] * fp64["solves_per_s"],
         "fp64_tube_points_per_s": n * fp64["solve"]["matvecs"] * fp64["solves_per_s"],
-        "serving": serving, "stage_ms_per_mixed_solve": stage_ms, "tree": stats, "mesh_s": t_mesh,
+        "serving": serving, "stream": stream, "stage_ms_per_mixed_solve": stage_ms, "tree": stats, "mesh_s": t_mesh,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

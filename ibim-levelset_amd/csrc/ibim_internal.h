@@ -244,7 +244,7 @@ struct VPairs {
     int nblocks = 0;                    // GEMM blocks of 64 pairs
     int *blk_cls = nullptr, *blk_j0 = nullptr;
     int *tptr = nullptr, *tlist = nullptr; // pairs of target node b: tlist[tptr[b] .. tptr[b+1])
-    int hnblocks = 0;                   // FP16 GEMM blocks (128 pairs for p <= 5, 16 for p = 6)
+    int hnblocks = 0;                   // FP16 GEMM blocks (128 pairs for p <= 5, 64 for p = 6)
     int *hblk_cls = nullptr, *hblk_j0 = nullptr;
 };
 
@@ -332,8 +332,8 @@ struct ibim_plan {
     float *ops32 = nullptr;              // FP32 copy for the mixed-precision matvec
     // FP16 M2L (IBIM_PREC_MIXED_F16): operators with one power-of-two scale
     // per row, multipoles with one per box, FP32 accumulation buffer
-    _Float16 *ops16 = nullptr;           // p <= 5: one unfolded operator per (class, mask) key
-                                         // with pairs (slot keyslot[key]); p = 6: one per class
+    _Float16 *ops16 = nullptr;           // one unfolded operator per (class, mask) key with pairs
+                                         // (slot keyslot[key])
     int *esig = nullptr;                 // per (operator, row) exponent
     int *keyslot = nullptr;
     _Float16 *Mh = nullptr;              // nnodes * (4r rounded up to 64), per box scaled

@@ -356,19 +356,6 @@ __device__ __forceinline__ int scale_exp(double vmax) {
     return e;
 }
 
-// one block of 64 per row (class, row) of the FP64 operators
-__global__ __launch_bounds__(64) void k_ops_to_half(const double *__restrict__ ops, int KRP,
-                                                    _Float16 *__restrict__ ops16, int *__restrict__ esig) {
-    const size_t row = blockIdx.x;
-    const double *src = ops + row * KRP;
-    double mx = 0.0;
-    for (int k = threadIdx.x; k < KRP; k += 64) mx = fmax(mx, fabs(src[k]));
-    for (int o = 32; o; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
-    const int e = scale_exp(mx);
-    if (threadIdx.x == 0) esig[row] = e;
-    for (int k = threadIdx.x; k < KRP; k += 64) ops16[row * KRP + k] = (_Float16)ldexp(src[k], -e);
-}
-
 // per box: FP16 multipoles with one power-of-two scale (every matvec)
 // rows of Mh are KP = R4 rounded up to 64 long, zero beyond R4
 __global__ __launch_bounds__(64) void k_m_to_half(const double *__restrict__ M, int R4, int KP,
@@ -420,148 +407,17 @@ __global__ __launch_bounds__(64) void k_unfold_half(const int *__restrict__ slot
     }
 }
 
-// The V-list GEMM on FP16 MFMA (v_mfma_f32_16x16x32_f16, FP32 accumulation).
-// One block takes NP pairs of one (class, mask) key and ALL operator rows:
-// the pairs' gathered, scaled FP16 multipoles stay in LDS for the whole
-// block while the operator streams through in NR-row x KC-k chunks.
-// Operator rows carry a power-of-two scale 2^esig, source multipoles one per
-// box (2^ebox); both are applied to the FP32 result, stored unpermuted and
-// without atomics as Y[pair][row]; k_v_reduce then sums every target's pairs
-// into L.  Wave w computes rows [w NR/4, (w+1) NR/4) of a row tile for all NP
-// pairs.  LDS stays under 64 KB: (NP, NR, KC) = (32, 128, 64) for p <= 5,
-// (16, 256, 32) for p = 6.
-template <int P, int NP, int KC>
-__global__ __launch_bounds__(256) void k_m2l_gemm_h(const int *__restrict__ blk_key, const int *__restrict__ blk_j0,
-                                                    const int *__restrict__ key_off, const int *__restrict__ pA,
-                                                    const int *__restrict__ gtab, const int *__restrict__ stab,
-                                                    const _Float16 *__restrict__ ops16,
-                                                    const int *__restrict__ esig, int MRP, const _Float16 *__restrict__ Mh,
-                                                    const int *__restrict__ ebox, float *__restrict__ Y, int nblocks,
-                                                    int per_xcd) {
-    constexpr int R = P * P * P;
-    constexpr int R4 = 4 * R;
-    constexpr int KP = (R4 + 63) / 64 * 64;  // operator row length (padded, zero beyond R4)
-    constexpr int NR = 4096 / NP;
-    constexpr int LDA = KC + 8, LDX = KP + 8;
-    constexpr int OPT = NR * KC / 8 / 256;   // operator h8 per thread per chunk
-    constexpr int PS = NP / 16, RS = NR / 64;  // 16x16 subtiles of a wave: pairs x rows
-    static_assert(KP % KC == 0 && OPT >= 1, "tile shape");
-    __shared__ __attribute__((aligned(16))) _Float16 sX[NP][LDX];
-    __shared__ __attribute__((aligned(16))) _Float16 sA[NR][LDA];
-    __shared__ int sG[R4];
-    __shared__ int sPA[NP], sEA[NP];
-    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (tile >= nblocks) return;
-    const int tid = threadIdx.x;
-    const int key = blk_key[tile];
-    const int cls = key >> 3, m = key & 7;
-    const int j0 = blk_j0[tile];
-    const int jend = key_off[key + 1];
-    if (tid < NP) {
-        const int j = j0 + tid;
-        const int a = j < jend ? pA[j] : -1;
-        sPA[tid] = a;
-        sEA[tid] = a >= 0 ? ebox[a] : 0;
-    }
-    for (int k = tid; k < R4; k += 256) sG[k] = gtab[m * R4 + k];
-    __syncthreads();
-    // gather: item = (pair, 8 consecutive k)
-    for (int it = tid; it < NP * (KP / 8); it += 256) {
-        const int pr = it / (KP / 8), k0 = 8 * (it % (KP / 8));
-        const int a = sPA[pr];
-        const size_t mb = (size_t)max(a, 0) * KP;
-        h8 v;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int k = k0 + e;
-            const int g = sG[min(k, R4 - 1)];
-            _Float16 x = Mh[mb + (g & 0x7fffffff)];
-            if (g < 0) x = -x;
-            v[e] = (k < R4 && a >= 0) ? x : (_Float16)0;
-        }
-        *(h8 *)&sX[pr][k0] = v;
-    }
-    const int w = tid >> 6, l = tid & 63;
-    const int wr = w * (NR / 4);
-    const _Float16 *Aop = ops16 + (size_t)cls * MRP * KP;
-    const int nrt = (MRP + NR - 1) / NR, nkc = KP / KC;
-    const int nch = nrt * nkc;
-    // chunk c = (row tile c / nkc, k chunk c % nkc); thread t moves h8 number t + 256 i
-    h8 ra[OPT];
-    auto load_chunk = [&](int c) {
-        const int rt = c / nkc, kc = (c % nkc) * KC;
-#pragma unroll
-        for (int i = 0; i < OPT; ++i) {
-            const int v = tid + 256 * i;
-            const int row = rt * NR + v / (KC / 8), kk = 8 * (v % (KC / 8));
-            const h8 *src = (const h8 *)(Aop + (size_t)min(row, MRP - 1) * KP + kc + kk);
-            ra[i] = row < MRP ? *src : h8{};
-        }
-    };
-    f4v acc[PS][RS];
-    load_chunk(0);
-    for (int c = 0; c < nch; ++c) {
-        const int rt = c / nkc, kc = (c % nkc) * KC;
-#pragma unroll
-        for (int i = 0; i < OPT; ++i) {
-            const int v = tid + 256 * i;
-            *(h8 *)&sA[v / (KC / 8)][8 * (v % (KC / 8))] = ra[i];
-        }
-        __syncthreads();
-        if (c + 1 < nch) load_chunk(c + 1);
-        if (kc == 0) {
-#pragma unroll
-            for (int a = 0; a < PS; ++a)
-#pragma unroll
-                for (int b = 0; b < RS; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
-        }
-#pragma unroll
-        for (int ks = 0; ks < KC; ks += 32) {
-            h8 xf[PS], kf[RS];
-#pragma unroll
-            for (int pi = 0; pi < PS; ++pi) xf[pi] = *(const h8 *)&sX[pi * 16 + (l & 15)][kc + ks + 8 * (l >> 4)];
-#pragma unroll
-            for (int ri = 0; ri < RS; ++ri) kf[ri] = *(const h8 *)&sA[wr + ri * 16 + (l & 15)][ks + 8 * (l >> 4)];
-#pragma unroll
-            for (int pi = 0; pi < PS; ++pi)
-#pragma unroll
-                for (int ri = 0; ri < RS; ++ri)
-                    acc[pi][ri] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xf[pi], kf[ri], acc[pi][ri], 0, 0, 0);
-        }
-        if (kc + KC == KP) {
-            // row tile done: register q of lane l holds pair 4 (l >> 4) + q,
-            // row l & 15; stored at the row's place in the target expansion
-#pragma unroll
-            for (int ri = 0; ri < RS; ++ri) {
-                const int row = rt * NR + wr + ri * 16 + (l & 15);
-                if (row >= R4) continue;
-                const int er = esig[(size_t)cls * MRP + row];
-                const int st = stab[m * R4 + row];
-                const int dst = st & 0x7fffffff;
-#pragma unroll
-                for (int pi = 0; pi < PS; ++pi)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int col = pi * 16 + 4 * (l >> 4) + q;
-                        if (j0 + col >= jend) continue;
-                        const float v = ldexpf(acc[pi][ri][q], er + sEA[col]);
-                        Y[(size_t)(j0 + col) * R4 + dst] = st < 0 ? -v : v;
-                    }
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// The FP16 V-list GEMM for p <= 5 on unfolded operators (one per (class,
-// mask) key with pairs, k_unfold_half): a block takes 128 pairs of one key,
-// wave w the pairs [32 w, 32 w + 32).  Each wave loads its pairs' MFMA
-// A-fragments (8 consecutive k of one pair = 16 B of Mh) for all k straight
-// into registers once; the operator streams through LDS in 64-row x KC-k
-// chunks shared by the four waves (register prefetch of the next chunk), so
-// one pass over the operator serves 128 pairs.  Row-tile results, scaled by
-// 2^(esig + ebox), are stored in target order as Y[pair][row].
-template <int P>
+// The FP16 V-list GEMM (v_mfma_f32_16x16x32_f16, FP32 accumulation) on
+// unfolded operators (one per (class, mask) key with pairs, k_unfold_half):
+// a block takes 4 PPW pairs of one key, wave w the pairs [PPW w, PPW w +
+// PPW), PPW = 32 for p <= 5 and 16 for p = 6 (register budget).  Each wave
+// loads its pairs' MFMA A-fragments (8 consecutive k of one pair = 16 B of Mh)
+// for all k straight into registers once; the operator streams through LDS
+// in 64-row x KC-k chunks shared by the four waves (register prefetch of the
+// next chunk), so one pass over the operator serves the block's pairs.
+// Row-tile results, scaled by 2^(esig + ebox), are stored in target order as
+// Y[pair][row]; k_v_reduce sums every target's pairs into L.
+template <int P, int PPW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_m2l_gemm_hu(
     const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
     const int *__restrict__ pA, const int *__restrict__ keyslot, const _Float16 *__restrict__ ops16u,
@@ -571,13 +427,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     constexpr int R4 = 4 * R;
     constexpr int KP = (R4 + 63) / 64 * 64;
     constexpr int KS = KP / 32;                  // MFMA k-steps
-    constexpr int KC = KP < 128 ? KP : 128;      // k per operator chunk
+    constexpr int KC = KP % 128 == 0 ? 128 : 64; // k per operator chunk
     constexpr int NKC = KP / KC;
     constexpr int LDA = KC + 8;
     constexpr int OPT = 64 * KC / 8 / 256;       // operator h8 per thread per chunk
+    constexpr int PS = PPW / 16;                 // 16-pair subtiles of a wave
+    constexpr int NPB = 4 * PPW;                 // pairs per block
     static_assert(KP % KC == 0 && OPT >= 1, "tile shape");
     __shared__ __attribute__((aligned(16))) _Float16 sA[64][LDA];
-    __shared__ int sEA[128];
+    __shared__ int sEA[NPB];
     const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= nblocks) return;
     const int tid = threadIdx.x;
@@ -586,15 +444,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int jend = key_off[key + 1];
     const int sl = keyslot[key];
     const int w = tid >> 6, l = tid & 63;
-    if (tid < 128) {
+    if (tid < NPB) {
         const int j = j0 + tid;
         sEA[tid] = j < jend ? ebox[pA[j]] : 0;
     }
-    // A-fragments of this wave's 32 pairs, all k
-    h8 xr[KS][2];
+    // A-fragments of this wave's pairs, all k
+    h8 xr[KS][PS];
 #pragma unroll
-    for (int pi = 0; pi < 2; ++pi) {
-        const int j = j0 + 32 * w + 16 * pi + (l & 15);
+    for (int pi = 0; pi < PS; ++pi) {
+        const int j = j0 + PPW * w + 16 * pi + (l & 15);
         const bool ok = j < jend;
         const _Float16 *src = Mh + (size_t)(ok ? pA[j] : 0) * KP + 8 * (l >> 4);
 #pragma unroll
@@ -612,9 +470,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     };
     load_chunk(0, 0);
     for (int rt = 0; rt < nrt; ++rt) {
-        f4v acc[2][4];
+        f4v acc[PS][4];
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < PS; ++a)
 #pragma unroll
             for (int b = 0; b < 4; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -634,7 +492,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
                 for (int ri = 0; ri < 4; ++ri) kf[ri] = *(const h8 *)&sA[16 * ri + (l & 15)][32 * ks + 8 * (l >> 4)];
 #pragma unroll
-                for (int pi = 0; pi < 2; ++pi)
+                for (int pi = 0; pi < PS; ++pi)
 #pragma unroll
                     for (int ri = 0; ri < 4; ++ri)
                         acc[pi][ri] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xr[kc * (KC / 32) + ks][pi], kf[ri],
@@ -648,10 +506,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             if (row >= R4) continue;
             const int er = esig[(size_t)sl * MRP + row];
 #pragma unroll
-            for (int pi = 0; pi < 2; ++pi)
+            for (int pi = 0; pi < PS; ++pi)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int col = 32 * w + 16 * pi + 4 * (l >> 4) + q;
+                    const int col = PPW * w + 16 * pi + 4 * (l >> 4) + q;
                     if (j0 + col >= jend) continue;
                     Y[(size_t)(j0 + col) * R4 + row] = ldexpf(acc[pi][ri][q], er + sEA[col]);
                 }
@@ -693,21 +551,13 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
     if constexpr (P <= 6) {
         if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->ops16 && vp.tptr && pl->Yv &&
             vp.npairs <= pl->vp.npairs) {
-            constexpr int NP = 16;  // p = 6 blocks
-            constexpr int KC = 32;
             constexpr int KP = (R4 + 63) / 64 * 64;
             k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, KP, pl->Mh, pl->ebox);
             if (vp.hnblocks > 0) {
                 const int pxcd = (vp.hnblocks + 7) / 8;
-                if constexpr (P <= 5)
-                    k_m2l_gemm_hu<P><<<8 * pxcd, 256, 0, st>>>(vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A,
-                                                               pl->keyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
-                                                               pl->ebox, pl->Yv, vp.hnblocks, pxcd);
-                else
-                    k_m2l_gemm_h<P, NP, KC><<<8 * pxcd, 256, 0, st>>>(vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A,
-                                                                      pl->gtab, pl->stab, pl->ops16, pl->esig,
-                                                                      pl->MRP, pl->Mh, pl->ebox, pl->Yv, vp.hnblocks,
-                                                                      pxcd);
+                k_m2l_gemm_hu<P, P <= 5 ? 32 : 16><<<8 * pxcd, 256, 0, st>>>(
+                    vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A, pl->keyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
+                    pl->ebox, pl->Yv, vp.hnblocks, pxcd);
                 k_v_reduce<R4><<<pl->n_m2l_work, 256, 0, st>>>(pl->m2l_work, vp.tptr, vp.tlist, pl->Yv, pl->L);
             }
             IBIM_LAUNCH_CHECK();
@@ -853,9 +703,9 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
         IBIM_HIP_CHECK(hipMemcpyAsync(vp.blk_cls, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice, st));
         IBIM_HIP_CHECK(hipMemcpyAsync(vp.blk_j0, bj.data(), bj.size() * sizeof(int), hipMemcpyHostToDevice, st));
     }
-    // FP16 GEMM block table (blocks of 128 pairs for p <= 5, 16 for p = 6)
+    // FP16 GEMM block table (blocks of 128 pairs for p <= 5, 64 for p = 6)
     if (pl->p <= 6) {
-        const int hnp = pl->p <= 5 ? 128 : 16;
+        const int hnp = pl->p <= 5 ? 128 : 64;
         std::vector<int> hc, hj;
         for (int c = 0; c < nkeys; ++c)
             for (int j = vp.cls_off[c]; j < vp.cls_off[c + 1]; j += hnp) {
@@ -929,14 +779,6 @@ int build_m2l_operators_f16(ibim_plan *pl) {
     if ((rc = dalloc(pl, &pl->Mh, (size_t)pl->nnodes * KP)) || (rc = dalloc(pl, &pl->ebox, (size_t)pl->nnodes)) ||
         (rc = dalloc(pl, &pl->Yv, std::max<size_t>(1, (size_t)pl->vp.npairs * R4))))
         return rc;
-    if (pl->p == 6) {  // folded operators, one FP16 copy per class
-        const int64_t rows = (int64_t)pl->n_cls * pl->MRP;
-        if ((rc = dalloc(pl, &pl->ops16, (size_t)(rows * pl->KRP))) || (rc = dalloc(pl, &pl->esig, (size_t)rows)))
-            return rc;
-        k_ops_to_half<<<(unsigned)rows, 64, 0, pl->stream>>>(pl->ops, pl->KRP, pl->ops16, pl->esig);
-        IBIM_LAUNCH_CHECK();
-        return IBIM_OK;
-    }
     // unfolded operators for the (class, mask) keys that have pairs
     const std::vector<int> &off = pl->vp.cls_off;
     std::vector<int> keyslot(off.size() > 0 ? off.size() - 1 : 0, -1), slot_key;
