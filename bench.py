@@ -210,6 +210,7 @@ def main():
                          kappa=cfg["kappa"], eps_in=cfg["eps_in"], eps_out=cfg["eps_out"],
                          vacant_radius=cfg["tau"] * cfg["h"], precision=ibim.PREC_MIXED)
         d_x.uniform_(-1, 1)
+        torch.cuda.synchronize()
         for prec in (ibim.PREC_MIXED_F16, ibim.PREC_MIXED, ibim.PREC_FP64):
             plan.set_precision(prec)
             for _ in range(args.matvecs_only):
@@ -229,6 +230,7 @@ def main():
             plan.profile(True)
         plan.rhs_device(m, P(d_atoms), P(d_q), P(d_b))
         d_x.zero_()
+        torch.cuda.synchronize()  # the library's stream does not wait for torch's
         rc, info = plan.gmres_device(P(d_b), P(d_x), restart=cfg["restart"], maxit=2000, tol=cfg["tol"])
         plan.polarize_device(m, P(d_atoms), P(d_x), P(d_phi))
         plan.sync()

@@ -146,7 +146,7 @@ int ibim_plan_create_device(const ibim_params *prm, int64_t n, const double *d_x
     }
     pl->owns_stream = true;
     if ((rc = build_chebyshev(pl)) || (rc = build_tree(pl, d_xyz, d_nrm, d_w)) || (rc = build_m2l_operators(pl)) ||
-        (rc = build_vpairs(pl, pl->need_local, pl->vp, nullptr))) {
+        (rc = build_vpairs(pl, pl->need_local, pl->vp, nullptr)) || (rc = build_xpairs(pl))) {
         ibim_plan_destroy(pl);
         return rc;
     }

@@ -811,9 +811,13 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     IBIM_HIP_CHECK(hipMemsetAsync(pl->L, 0, (size_t)pl->nnodes * 4 * R * sizeof(double), st));
     {
     StageScope sc(pl, ts.vpairs ? kStM2LX : kStM2L);
-    // X list (cross-level) directly; V list as GEMMs when the pairs are grouped
+    // X list (cross-level) as class GEMMs for the plan's own points, else
+    // directly; V list as GEMMs when the pairs are grouped
     const int lists = ts.vpairs ? 2 : 3;
-    if (ts.n_m2l_work > 0) {
+    if (ts.xgemm) {
+        int rc = m2l_gemm_x(pl, mixed);
+        if (rc) return rc;
+    } else if (ts.n_m2l_work > 0) {
         auto launch = [&](auto kern) {
             kern<<<ts.n_m2l_work, BS, 0, st>>>(lists, ts.m2l_work, pl->loff[1], pl->litem[1], pl->loff[3],
                                                pl->litem[3], pl->flags, pl->ctr, pl->level, pl->d_lvl_rad, pl->tab,
@@ -1052,6 +1056,7 @@ TargetSet source_targets(ibim_plan *pl) {
     ts.n_m2l_work = pl->n_m2l_work;
     ts.need_local = pl->need_local;
     ts.vpairs = pl->ops ? &pl->vp : nullptr;
+    ts.xgemm = ts.vpairs && pl->xp.npairs > 0 && pl->xops;
     ts.roff = pl->p2p_roff;
     ts.ranges = pl->p2p_ranges;
     ts.p2p = pl->p2p.ua ? &pl->p2p : nullptr;

@@ -259,6 +259,7 @@ struct TargetSet {
     int n_m2l_work = 0;
     const unsigned char *need_local = nullptr;              // per node
     const VPairs *vpairs = nullptr;                         // V interactions into those nodes
+    bool xgemm = false;                                     // X list through the plan's class GEMMs
     const int *roff = nullptr;                              // per target leaf: merged near-field
     const int2 *ranges = nullptr;                           //   source ranges [roff[i], roff[i+1])
     const struct P2PWork *p2p = nullptr;                    // near-field work units over those ranges
@@ -341,6 +342,12 @@ struct ibim_plan {
     float *Yv = nullptr;                 // V interaction results, v pairs * 4r
     int *gtab = nullptr, *stab = nullptr; // gather / scatter tables of the 8 reflections
     ibim::VPairs vp;                     // V pairs for the points as their own targets
+    // X list as GEMMs: pairs grouped by class (level of the target, level
+    // difference, offset), one operator per class (key = 8 class, no reflection)
+    ibim::VPairs xp;
+    int n_xcls = 0;
+    double *xops = nullptr;
+    float *xops32 = nullptr;
     const int *p2p_roff = nullptr;       // merged near-field ranges of the non-empty leaves
     const int2 *p2p_ranges = nullptr;
     ibim::P2PWork p2p;                   // near-field work units of the plan's own points
@@ -385,6 +392,8 @@ int compute_stats(ibim_plan *pl);
 int build_m2l_operators(ibim_plan *pl);
 int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vector<void *> *tmp_owner);
 int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed);
+int build_xpairs(ibim_plan *pl);
+int m2l_gemm_x(ibim_plan *pl, bool mixed);
 int build_m2l_operators_f32(ibim_plan *pl);
 int build_m2l_operators_f16(ibim_plan *pl);
 int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, P2PWork &work, std::vector<void *> *tmp_owner);

@@ -754,6 +754,255 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
     return IBIM_OK;
 }
 
+// ------------------------------------------------------ X list as GEMMs ---
+// An X interaction (target b, coarser non-empty source leaf a, X list of b)
+// evaluates K between b's and a's Chebyshev grids (kernelEvalChebyshev,
+// bbfmm.h:456-479), like a V interaction but across levels.  Its geometry is
+// fixed by the level of b, dl = level b - level a and the integer vector o =
+// (c_a - c_b) / r_b: per axis d = r_b (o - x_t) + r_a x_s.  As for the V
+// list, reflections fold o onto |o| (the mask records the negative
+// components, and the same gather/scatter tables apply), pairs are grouped by
+// (class, mask), and run through k_m2l_gemm with one operator per class.
+namespace {
+constexpr int kXOff = 8192;  // |o| < 8192 per axis
+
+__global__ void k_xpairs(int pass, int nnodes, const unsigned char *__restrict__ need, const int *__restrict__ xoff,
+                         const int *__restrict__ xit, const int *__restrict__ flags, const int *__restrict__ level,
+                         const double *__restrict__ ctr, const double *__restrict__ lrad, int *__restrict__ cnt,
+                         const int *__restrict__ off, int *__restrict__ pA, int *__restrict__ pB,
+                         unsigned char *__restrict__ pm, unsigned long long *__restrict__ pkey, int *__restrict__ bad) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nnodes) return;
+    int c = 0;
+    if (b != 0 && need[b]) {
+        const int lb = level[b];
+        const double *rd = lrad + 3 * lb;
+        for (int q = xoff[b]; q < xoff[b + 1]; ++q) {
+            const int a = xit[q];
+            if (flags[a] & kEmpty) continue;
+            if (pass == 1) {
+                // class key: level (5 bits), dl (6), |o| (3 x 13), then the mask (3)
+                unsigned long long key = ((unsigned long long)lb << 45) | ((unsigned long long)(lb - level[a]) << 39);
+                int m = 0;
+                for (int d = 0; d < 3; ++d) {
+                    const int o = (int)rint((ctr[3 * a + d] - ctr[3 * b + d]) / rd[d]);
+                    if (o <= -kXOff || o >= kXOff) *bad = 1;
+                    if (o < 0) m |= 1 << d;
+                    key |= (unsigned long long)(abs(o) & (kXOff - 1)) << (26 - 13 * d);
+                }
+                const int j = off[b] + c;
+                pA[j] = a;
+                pB[j] = b;
+                pm[j] = (unsigned char)m;
+                pkey[j] = (key << 3) | (unsigned long long)m;
+            }
+            ++c;
+        }
+    }
+    if (pass == 0) cnt[b] = c;
+}
+
+// operator entries of X class c: row g*r + t, column f*r + s
+template <int P>
+__global__ void k_build_xops(int ncls, const int *__restrict__ cls, const double *__restrict__ lrad, ChebTab tab,
+                             PBConst kc, int MRP, int KRP, double *__restrict__ ops) {
+    constexpr int R = P * P * P;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)R * R;
+    if (e >= per * ncls) return;
+    const int c = (int)(e / per);
+    const int t = (int)((e % per) / R), s = (int)(e % R);
+    const int lb = cls[5 * c], la = lb - cls[5 * c + 1];
+    const double *rb = lrad + 3 * lb, *ra = lrad + 3 * la;
+    const int ti[3] = {t % P, (t / P) % P, t / (P * P)};
+    const int si[3] = {s % P, (s / P) % P, s / (P * P)};
+    double d[3];
+    for (int q = 0; q < 3; ++q) {
+        double xs = 0.0, xt = 0.0;
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            if (i == si[q]) xs = tab.xbar[i];
+            if (i == ti[q]) xt = tab.xbar[i];
+        }
+        d[q] = rb[q] * (cls[5 * c + 2 + q] - xt) + ra[q] * xs;
+    }
+    double blk[4][4];
+    for (int f = 0; f < 4; ++f) {
+        double q[4] = {0, 0, 0, 0};
+        q[f] = 1.0;
+        double f0 = 0, fx = 0, fy = 0, fz = 0;
+        pair_accum<double>(d[0], d[1], d[2], q[0], q[1], q[2], q[3], kc, f0, fx, fy, fz);
+        blk[0][f] = f0;
+        blk[1][f] = fx;
+        blk[2][f] = fy;
+        blk[3][f] = fz;
+    }
+    double *op = ops + (size_t)c * MRP * KRP;
+    for (int g = 0; g < 4; ++g)
+        for (int f = 0; f < 4; ++f) op[(size_t)(g * R + t) * KRP + f * R + s] = blk[g][f];
+}
+
+template <int P>
+int build_xops_p(ibim_plan *pl, const int *d_cls) {
+    constexpr int R = P * P * P;
+    const int64_t total = (int64_t)R * R * pl->n_xcls;
+    IBIM_HIP_CHECK(hipMemsetAsync(pl->xops, 0, (size_t)pl->n_xcls * pl->MRP * pl->KRP * sizeof(double), pl->stream));
+    k_build_xops<P><<<nblk(total), 256, 0, pl->stream>>>(pl->n_xcls, d_cls, pl->d_lvl_rad, pl->tab, pl->kc, pl->MRP,
+                                                         pl->KRP, pl->xops);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
+template <int P>
+int gemm_x_p(ibim_plan *pl, bool mixed) {
+    const VPairs &xp = pl->xp;
+    if (xp.nblocks == 0) return IBIM_OK;
+    const int total = xp.nblocks * (pl->MRP / 64);
+    const int per_xcd = (total + 7) / 8;
+    if (mixed && pl->xops32)
+        k_m2l_gemm<P, float><<<8 * per_xcd, 256, 0, pl->stream>>>(xp.blk_cls, xp.blk_j0, xp.cls_off_d, xp.A, xp.B,
+                                                                  pl->gtab, pl->stab, pl->xops32, pl->MRP, pl->KRP,
+                                                                  pl->M, pl->L, xp.nblocks, per_xcd);
+    else
+        k_m2l_gemm<P, double><<<8 * per_xcd, 256, 0, pl->stream>>>(xp.blk_cls, xp.blk_j0, xp.cls_off_d, xp.A, xp.B,
+                                                                   pl->gtab, pl->stab, pl->xops, pl->MRP, pl->KRP,
+                                                                   pl->M, pl->L, xp.nblocks, per_xcd);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+}  // namespace
+
+// X pairs of the plan's own points, their classes and FP64 operators (run
+// after the V operators, which also build the gather tables)
+int build_xpairs(ibim_plan *pl) {
+    pl->xp = VPairs{};
+    pl->n_xcls = 0;
+    if (!pl->ops || !pl->gtab) return IBIM_OK;
+    hipStream_t st = pl->stream;
+    const int nn = pl->nnodes;
+    std::vector<void *> scratch;
+    struct Free {
+        std::vector<void *> &v;
+        ~Free() {
+            for (void *p : v) (void)hipFree(p);
+        }
+    } free_scratch{scratch};
+    auto salloc = [&](auto **p, size_t n) -> int {
+        hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(**p));
+        if (e != hipSuccess) {
+            set_error("hipMalloc (X pairs)");
+            return IBIM_ERR_NOMEM;
+        }
+        scratch.push_back((void *)*p);
+        return IBIM_OK;
+    };
+    int rc;
+    int *cnt = nullptr, *off = nullptr, *bad = nullptr;
+    if ((rc = salloc(&cnt, (size_t)nn + 1)) || (rc = salloc(&off, (size_t)nn + 1)) || (rc = salloc(&bad, 1))) return rc;
+    IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, ((size_t)nn + 1) * sizeof(int), st));
+    IBIM_HIP_CHECK(hipMemsetAsync(bad, 0, sizeof(int), st));
+    k_xpairs<<<nblk(nn), 256, 0, st>>>(0, nn, pl->need_local, pl->loff[3], pl->litem[3], pl->flags, pl->level, pl->ctr,
+                                       pl->d_lvl_rad, cnt, nullptr, nullptr, nullptr, nullptr, nullptr, bad);
+    IBIM_LAUNCH_CHECK();
+    size_t bytes = 0;
+    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt, off, nn + 1, st));
+    void *tmp = nullptr;
+    if ((rc = salloc((char **)&tmp, bytes))) return rc;
+    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, cnt, off, nn + 1, st));
+    int np = 0;
+    IBIM_HIP_CHECK(hipMemcpyAsync(&np, off + nn, sizeof(int), hipMemcpyDeviceToHost, st));
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    if (np == 0) return IBIM_OK;
+    int *a0 = nullptr, *b0 = nullptr, *iota = nullptr, *order = nullptr;
+    unsigned char *m0 = nullptr;
+    unsigned long long *k0 = nullptr, *k1 = nullptr;
+    if ((rc = salloc(&a0, np)) || (rc = salloc(&b0, np)) || (rc = salloc(&iota, np)) || (rc = salloc(&order, np)) ||
+        (rc = salloc(&k0, np)) || (rc = salloc(&k1, np)) || (rc = salloc(&m0, np)))
+        return rc;
+    k_xpairs<<<nblk(nn), 256, 0, st>>>(1, nn, pl->need_local, pl->loff[3], pl->litem[3], pl->flags, pl->level, pl->ctr,
+                                       pl->d_lvl_rad, nullptr, off, a0, b0, m0, k0, bad);
+    k_iota_pairs<<<nblk(np), 256, 0, st>>>(np, iota);
+    IBIM_LAUNCH_CHECK();
+    bytes = 0;
+    IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k0, k1, iota, order, np, 0, 53, st));
+    void *tmp2 = nullptr;
+    if ((rc = salloc((char **)&tmp2, bytes))) return rc;
+    IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp2, bytes, k0, k1, iota, order, np, 0, 53, st));
+    int hbad = 0;
+    std::vector<unsigned long long> hk(np);
+    IBIM_HIP_CHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+    IBIM_HIP_CHECK(hipMemcpyAsync(hk.data(), k1, (size_t)np * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    if (hbad) return IBIM_OK;  // offsets out of range: keep the direct X evaluation
+    // classes: runs of equal class keys (the key without its 3 mask bits)
+    std::vector<int> cls, cstart;
+    for (int j = 0; j < np; ++j)
+        if (j == 0 || (hk[j] >> 3) != (hk[j - 1] >> 3)) {
+            const unsigned long long k = hk[j] >> 3;
+            cls.push_back((int)(k >> 45));
+            cls.push_back((int)((k >> 39) & 63));
+            for (int d = 0; d < 3; ++d) cls.push_back((int)((k >> (26 - 13 * d)) & (kXOff - 1)));
+            cstart.push_back(j);
+        }
+    const int nc = (int)cstart.size();
+    VPairs &xp = pl->xp;
+    xp.npairs = np;
+    xp.cls_off.assign(8 * (size_t)nc + 1, 0);  // key 8 class + mask
+    for (int c = 0, j = 0; c < nc; ++c)
+        for (int k = 0; k < 8; ++k) {
+            const int end = c + 1 < nc ? cstart[c + 1] : np;
+            while (j < end && (int)(hk[j] & 7) == k) ++j;
+            xp.cls_off[8 * c + k + 1] = j;
+        }
+    std::vector<int> bc, bj;
+    for (int key = 0; key < 8 * nc; ++key)
+        for (int j = xp.cls_off[key]; j < xp.cls_off[key + 1]; j += 64) {
+            bc.push_back(key);
+            bj.push_back(j);
+        }
+    xp.nblocks = (int)bc.size();
+    int *d_cls = nullptr;
+    if ((rc = dalloc(pl, &xp.A, np)) || (rc = dalloc(pl, &xp.B, np)) || (rc = dalloc(pl, &xp.mask, np)) ||
+        (rc = dalloc(pl, &xp.cls_off_d, xp.cls_off.size())) || (rc = dalloc(pl, &xp.blk_cls, xp.nblocks)) ||
+        (rc = dalloc(pl, &xp.blk_j0, xp.nblocks)) || (rc = salloc(&d_cls, cls.size())) ||
+        (rc = dalloc(pl, &pl->xops, (size_t)nc * pl->MRP * pl->KRP)))
+        return rc;
+    k_permute_pairs<<<nblk(np), 256, 0, st>>>(np, order, a0, b0, m0, xp.A, xp.B, xp.mask);
+    IBIM_LAUNCH_CHECK();
+    IBIM_HIP_CHECK(hipMemcpyAsync(xp.cls_off_d, xp.cls_off.data(), xp.cls_off.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, st));
+    IBIM_HIP_CHECK(hipMemcpyAsync(xp.blk_cls, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    IBIM_HIP_CHECK(hipMemcpyAsync(xp.blk_j0, bj.data(), bj.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    IBIM_HIP_CHECK(hipMemcpyAsync(d_cls, cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    pl->n_xcls = nc;
+    switch (pl->p) {
+    case 2: rc = build_xops_p<2>(pl, d_cls); break;
+    case 3: rc = build_xops_p<3>(pl, d_cls); break;
+    case 4: rc = build_xops_p<4>(pl, d_cls); break;
+    case 5: rc = build_xops_p<5>(pl, d_cls); break;
+    case 6: rc = build_xops_p<6>(pl, d_cls); break;
+    case 7: rc = build_xops_p<7>(pl, d_cls); break;
+    case 8: rc = build_xops_p<8>(pl, d_cls); break;
+    default: rc = IBIM_ERR_ARG;
+    }
+    if (rc) return rc;
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    return IBIM_OK;
+}
+
+int m2l_gemm_x(ibim_plan *pl, bool mixed) {
+    switch (pl->p) {
+    case 2: return gemm_x_p<2>(pl, mixed);
+    case 3: return gemm_x_p<3>(pl, mixed);
+    case 4: return gemm_x_p<4>(pl, mixed);
+    case 5: return gemm_x_p<5>(pl, mixed);
+    case 6: return gemm_x_p<6>(pl, mixed);
+    case 7: return gemm_x_p<7>(pl, mixed);
+    case 8: return gemm_x_p<8>(pl, mixed);
+    }
+    return IBIM_ERR_ARG;
+}
+
 int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed) {
     switch (pl->p) {
     case 2: return gemm_p<2>(pl, vp, mixed);
@@ -770,7 +1019,10 @@ int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed) {
 // FP16 operators, their row exponents and the per-matvec FP16 multipole and
 // V-result buffers of the IBIM_PREC_MIXED_F16 matvec (the plan's V pairs
 // must exist)
+static int build_xops_f32(ibim_plan *pl);
+
 int build_m2l_operators_f16(ibim_plan *pl) {
+    if (int rc = build_xops_f32(pl)) return rc;
     if (pl->ops16 || !pl->ops || pl->n_cls == 0) return IBIM_OK;
     if (pl->p > 6) return build_m2l_operators_f32(pl);  // the FP16 GEMM's LDS tiles stop at p = 6
     const int R4 = 4 * pl->r;
@@ -808,8 +1060,20 @@ int build_m2l_operators_f16(ibim_plan *pl) {
     return IBIM_OK;
 }
 
+// FP32 copy of the X-list operators (both mixed forms)
+static int build_xops_f32(ibim_plan *pl) {
+    if (pl->xops32 || !pl->xops || pl->n_xcls == 0) return IBIM_OK;
+    const int64_t n = (int64_t)pl->n_xcls * pl->MRP * pl->KRP;
+    int rc = dalloc(pl, &pl->xops32, (size_t)n);
+    if (rc) return rc;
+    k_to_float<<<nblk(n), 256, 0, pl->stream>>>(pl->xops, n, pl->xops32);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
 // FP32 copy of the operators for the mixed-precision matvec
 int build_m2l_operators_f32(ibim_plan *pl) {
+    if (int rc = build_xops_f32(pl)) return rc;
     if (pl->ops32 || !pl->ops || pl->n_cls == 0) return IBIM_OK;
     const int64_t n = (int64_t)pl->n_cls * pl->MRP * pl->KRP;
     int rc = dalloc(pl, &pl->ops32, (size_t)n);
