@@ -310,7 +310,7 @@ def main():
     flops = 32.0 * r * r * stats["v_pairs"]  # 2 x (4r) x (4r) per V interaction
     achieved = flops / (ms_gemm * 1e-3) / 1e12
     p = cfg["p"]
-    gemm16 = f"k_m2l_gemm_hu<{p}>" if p <= 5 else f"k_m2l_gemm_h<{p}>" if p == 6 else f"k_m2l_gemm<{p},float>"
+    gemm16 = f"k_m2l_gemm_hu<{p},{32 if p <= 5 else 16}>" if p <= 6 else f"k_m2l_gemm<{p},float>"
     kern, peak = {"mixed_f32": (f"k_m2l_gemm<{p},float> (FP32 MFMA)", PEAK_TFLOPS["f32"]),
                   "mixed_f16": (f"k_m_to_half + {gemm16} + k_v_reduce (FP16 MFMA)" if p <= 6 else
                                 f"{gemm16} (FP32 MFMA; the FP16 tiles stop at p = 6)",
