@@ -339,7 +339,7 @@ struct ibim_plan {
     int *keyslot = nullptr;
     _Float16 *Mh = nullptr;              // nnodes * (4r rounded up to 64), per box scaled
     int *ebox = nullptr;                 // per box exponent
-    float *Yv = nullptr;                 // V interaction results, v pairs * 4r
+    float *Yv = nullptr;                 // V interaction results, v pairs * (4r rounded up to 32)
     int *gtab = nullptr, *stab = nullptr; // gather / scatter tables of the 8 reflections
     ibim::VPairs vp;                     // V pairs for the points as their own targets
     // X list as GEMMs: pairs grouped by class (level of the target, level

@@ -415,8 +415,11 @@ __global__ __launch_bounds__(64) void k_unfold_half(const int *__restrict__ slot
 // for all k straight into registers once; the operator streams through LDS
 // in 64-row x KC-k chunks shared by the four waves (register prefetch of the
 // next chunk), so one pass over the operator serves the block's pairs.
-// Row-tile results, scaled by 2^(esig + ebox), are stored in target order as
-// Y[pair][row]; k_v_reduce sums every target's pairs into L.
+// The operator chunk is the MFMA's A operand and the pairs its B operand, so
+// a lane's four accumulators are four consecutive rows of one pair: results,
+// scaled by 2^(esig + ebox), are stored 16 B per lane as Y[pair][row], rows
+// padded to YS = R4 rounded up to 32 (whole 128 B lines per 64-row tile);
+// k_v_reduce sums every target's pairs into L.
 template <int P, int PPW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_m2l_gemm_hu(
     const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
@@ -433,7 +436,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     constexpr int OPT = 64 * KC / 8 / 256;       // operator h8 per thread per chunk
     constexpr int PS = PPW / 16;                 // 16-pair subtiles of a wave
     constexpr int NPB = 4 * PPW;                 // pairs per block
-    static_assert(KP % KC == 0 && OPT >= 1, "tile shape");
+    constexpr int YS = (R4 + 31) / 32 * 32;      // row stride of Y
+    static_assert(KP % KC == 0 && OPT >= 1 && R4 % 4 == 0, "tile shape");
     __shared__ __attribute__((aligned(16))) _Float16 sA[64][LDA];
     __shared__ int sEA[NPB];
     const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
@@ -495,48 +499,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 for (int pi = 0; pi < PS; ++pi)
 #pragma unroll
                     for (int ri = 0; ri < 4; ++ri)
-                        acc[pi][ri] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xr[kc * (KC / 32) + ks][pi], kf[ri],
+                        acc[pi][ri] = __builtin_amdgcn_mfma_f32_16x16x32_f16(kf[ri], xr[kc * (KC / 32) + ks][pi],
                                                                              acc[pi][ri], 0, 0, 0);
             }
         }
-        // register q of lane l holds pair 4 (l >> 4) + q, row l & 15
+        // register q of lane l holds row 4 (l >> 4) + q of the row tile, pair
+        // l & 15 of the subtile; R4 is a multiple of 4, so a lane's four rows
+        // are all below R4 or all above
 #pragma unroll
         for (int ri = 0; ri < 4; ++ri) {
-            const int row = rt * 64 + 16 * ri + (l & 15);
+            const int row = rt * 64 + 16 * ri + 4 * (l >> 4);
             if (row >= R4) continue;
-            const int er = esig[(size_t)sl * MRP + row];
+            const int *er = esig + (size_t)sl * MRP + row;
+            const int e0 = er[0], e1 = er[1], e2 = er[2], e3 = er[3];
 #pragma unroll
-            for (int pi = 0; pi < PS; ++pi)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int col = PPW * w + 16 * pi + 4 * (l >> 4) + q;
-                    if (j0 + col >= jend) continue;
-                    Y[(size_t)(j0 + col) * R4 + row] = ldexpf(acc[pi][ri][q], er + sEA[col]);
-                }
+            for (int pi = 0; pi < PS; ++pi) {
+                const int col = PPW * w + 16 * pi + (l & 15);
+                if (j0 + col >= jend) continue;
+                const int ea = sEA[col];
+                const f4v a = acc[pi][ri];
+                *(f4v *)(Y + (size_t)(j0 + col) * YS + row) =
+                    f4v{ldexpf(a[0], e0 + ea), ldexpf(a[1], e1 + ea), ldexpf(a[2], e2 + ea), ldexpf(a[3], e3 + ea)};
+            }
         }
     }
 }
 
 // L_b += sum over the V pairs (a, b) of the pair's result: one block per
-// target node, four pairs' rows in flight per step
+// target node, a thread per four rows, four pairs' rows in flight per step
+// (fixed order: the sum is deterministic)
 template <int R4>
 __global__ __launch_bounds__(256) void k_v_reduce(const int *__restrict__ work, const int *__restrict__ tptr,
                                                   const int *__restrict__ tlist, const float *__restrict__ Y,
                                                   double *__restrict__ L) {
+    constexpr int YS = (R4 + 31) / 32 * 32;
     const int b = work[blockIdx.x];
     const int q0 = tptr[b], q1 = tptr[b + 1];
     if (q0 == q1) return;
-    for (int e = threadIdx.x; e < R4; e += 256) {
-        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    for (int e = 4 * threadIdx.x; e < R4; e += 4 * blockDim.x) {
+        f4v a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0, a2 = a0, a3 = a0;
         int q = q0;
         for (; q + 3 < q1; q += 4) {
-            a0 += Y[(size_t)tlist[q] * R4 + e];
-            a1 += Y[(size_t)tlist[q + 1] * R4 + e];
-            a2 += Y[(size_t)tlist[q + 2] * R4 + e];
-            a3 += Y[(size_t)tlist[q + 3] * R4 + e];
+            a0 += *(const f4v *)(Y + (size_t)tlist[q] * YS + e);
+            a1 += *(const f4v *)(Y + (size_t)tlist[q + 1] * YS + e);
+            a2 += *(const f4v *)(Y + (size_t)tlist[q + 2] * YS + e);
+            a3 += *(const f4v *)(Y + (size_t)tlist[q + 3] * YS + e);
         }
-        for (; q < q1; ++q) a0 += Y[(size_t)tlist[q] * R4 + e];
-        L[(size_t)b * R4 + e] += (double)((a0 + a1) + (a2 + a3));
+        for (; q < q1; ++q) a0 += *(const f4v *)(Y + (size_t)tlist[q] * YS + e);
+        const f4v t = (a0 + a1) + (a2 + a3);
+        double *dst = L + (size_t)b * R4 + e;
+        dst[0] += (double)t[0];
+        dst[1] += (double)t[1];
+        dst[2] += (double)t[2];
+        dst[3] += (double)t[3];
     }
 }
 
@@ -558,7 +573,8 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
                 k_m2l_gemm_hu<P, P <= 5 ? 32 : 16><<<8 * pxcd, 256, 0, st>>>(
                     vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A, pl->keyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
                     pl->ebox, pl->Yv, vp.hnblocks, pxcd);
-                k_v_reduce<R4><<<pl->n_m2l_work, 256, 0, st>>>(pl->m2l_work, vp.tptr, vp.tlist, pl->Yv, pl->L);
+                k_v_reduce<R4><<<pl->n_m2l_work, R4 / 4 <= 128 ? 128 : 256, 0, st>>>(pl->m2l_work, vp.tptr,
+                                                                                    vp.tlist, pl->Yv, pl->L);
             }
             IBIM_LAUNCH_CHECK();
             return IBIM_OK;
@@ -1029,7 +1045,7 @@ int build_m2l_operators_f16(ibim_plan *pl) {
     const int KP = (R4 + 63) / 64 * 64;
     int rc;
     if ((rc = dalloc(pl, &pl->Mh, (size_t)pl->nnodes * KP)) || (rc = dalloc(pl, &pl->ebox, (size_t)pl->nnodes)) ||
-        (rc = dalloc(pl, &pl->Yv, std::max<size_t>(1, (size_t)pl->vp.npairs * R4))))
+        (rc = dalloc(pl, &pl->Yv, std::max<size_t>(1, (size_t)pl->vp.npairs * ((R4 + 31) / 32 * 32)))))
         return rc;
     // unfolded operators for the (class, mask) keys that have pairs
     const std::vector<int> &off = pl->vp.cls_off;
