@@ -407,19 +407,41 @@ __global__ __launch_bounds__(64) void k_unfold_half(const int *__restrict__ slot
     }
 }
 
+typedef __attribute__((address_space(3))) void lds_void;
+
+// one direct global-to-LDS load: lane i's 16 B at p go to LDS byte address
+// base + 16 i (base wave-uniform).  Issued as inline assembly, so that the
+// compiler's wait insertion does not treat later LDS reads as dependent on
+// it: the caller orders the data with wait_vm and a barrier.
+__device__ __forceinline__ void lds_load16(const void *p, lds_void *base) {
+    const unsigned m = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)base);
+    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(p), "s"(m) : "memory", "m0");
+}
+
+// s_waitcnt for at most N outstanding vector memory operations of this wave
+// (expcnt and lgkmcnt not waited for)
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
 // The FP16 V-list GEMM (v_mfma_f32_16x16x32_f16, FP32 accumulation) on
 // unfolded operators (one per (class, mask) key with pairs, k_unfold_half):
 // a block takes 4 PPW pairs of one key, wave w the pairs [PPW w, PPW w +
 // PPW), PPW = 32 for p <= 5 and 16 for p = 6 (register budget).  Each wave
-// loads its pairs' MFMA A-fragments (8 consecutive k of one pair = 16 B of Mh)
-// for all k straight into registers once; the operator streams through LDS
-// in 64-row x KC-k chunks shared by the four waves (register prefetch of the
-// next chunk), so one pass over the operator serves the block's pairs.
-// The operator chunk is the MFMA's A operand and the pairs its B operand, so
-// a lane's four accumulators are four consecutive rows of one pair: results,
-// scaled by 2^(esig + ebox), are stored 16 B per lane as Y[pair][row], rows
-// padded to YS = R4 rounded up to 32 (whole 128 B lines per 64-row tile);
-// k_v_reduce sums every target's pairs into L.
+// loads its pairs' MFMA B-fragments (8 consecutive k of one pair = 16 B of
+// Mh) for all k straight into registers once.  The operator streams through
+// an LDS ring of NST chunks of 64 rows x KC k, filled by direct global-to-LDS
+// loads NST - 1 chunks ahead of use, so the HBM latency of the operator
+// (which all blocks of a key request at about the same time) is hidden; one
+// pass over the operator serves the block's pairs.  A chunk row's 16-byte
+// segments are stored XOR-swizzled by the row index, which keeps the MFMA
+// A-fragment reads free of bank conflicts.  A lane's four accumulators are
+// four consecutive rows of one pair: results, scaled by 2^(esig + ebox), are
+// stored 16 B per lane as Y[pair][row], rows padded to YS = R4 rounded up to
+// 32 (whole 128 B lines per 64-row tile); k_v_reduce sums every target's
+// pairs into L.
 template <int P, int PPW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_m2l_gemm_hu(
     const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
@@ -432,14 +454,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     constexpr int KS = KP / 32;                  // MFMA k-steps
     constexpr int KC = KP % 128 == 0 ? 128 : 64; // k per operator chunk
     constexpr int NKC = KP / KC;
-    constexpr int LDA = KC + 8;
-    constexpr int OPT = 64 * KC / 8 / 256;       // operator h8 per thread per chunk
+    constexpr int SEGS = KC / 8;                 // 16-byte segments per chunk row
+    constexpr int RPI = 64 / SEGS;               // chunk rows per LDS-load instruction (64 lanes x 16 B)
+    constexpr int DPW = SEGS / 4;                // LDS-load instructions per wave per chunk
+    constexpr int NST = 4;                       // chunks in the LDS ring
+    constexpr int CH = 64 * KC;                  // halves per chunk
     constexpr int PS = PPW / 16;                 // 16-pair subtiles of a wave
     constexpr int NPB = 4 * PPW;                 // pairs per block
     constexpr int YS = (R4 + 31) / 32 * 32;      // row stride of Y
-    static_assert(KP % KC == 0 && OPT >= 1 && R4 % 4 == 0, "tile shape");
-    __shared__ __attribute__((aligned(16))) _Float16 sA[64][LDA];
+    static_assert(KP % KC == 0 && DPW >= 1 && R4 % 4 == 0, "tile shape");
+    constexpr int MR = (R4 + 63) / 64 * 64;      // operator rows (padded)
+    __shared__ __attribute__((aligned(1024))) _Float16 sA[NST][CH];
     __shared__ int sEA[NPB];
+    __shared__ __attribute__((aligned(16))) int sE[MR];  // the operator's row exponents
     const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= nblocks) return;
     const int tid = threadIdx.x;
@@ -448,11 +475,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int jend = key_off[key + 1];
     const int sl = keyslot[key];
     const int w = tid >> 6, l = tid & 63;
+    const _Float16 *Aop = ops16u + (size_t)sl * MRP * KP;
+    const int nrt = MRP / 64;
+    const int NC = nrt * NKC;
+    // chunk c into ring slot c % NST: instruction i of wave w fills rows
+    // RPI (DPW w + i) .. + RPI - 1; lane l's 16 B land at row r = RPI (DPW w
+    // + i) + l / SEGS, position l % SEGS, and hold segment (l % SEGS) ^ (r %
+    // SEGS) of that row
+    auto issue = [&](int c) {
+        const int rt = c / NKC, kc = c - rt * NKC;
+        _Float16 *dst = sA[c % NST];
+#pragma unroll
+        for (int i = 0; i < DPW; ++i) {
+            const int r = RPI * (DPW * w + i) + l / SEGS;
+            const int sg = (l % SEGS) ^ (r % SEGS);
+            lds_load16(Aop + (size_t)(rt * 64 + r) * KP + kc * KC + 8 * sg, (lds_void *)(dst + (DPW * w + i) * 512));
+        }
+    };
+#pragma unroll
+    for (int c = 0; c < NST - 1; ++c)
+        if (c < NC) issue(c);
     if (tid < NPB) {
         const int j = j0 + tid;
         sEA[tid] = j < jend ? ebox[pA[j]] : 0;
     }
-    // A-fragments of this wave's pairs, all k
+    // (global loads inside the chunk loop would make the compiler wait for
+    // every outstanding load, the LDS ring's included)
+    for (int i = tid; i < MR; i += 256) sE[i] = esig[(size_t)sl * MRP + i];
+    // B-fragments of this wave's pairs, all k
     h8 xr[KS][PS];
 #pragma unroll
     for (int pi = 0; pi < PS; ++pi) {
@@ -462,17 +512,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) xr[ks][pi] = ok ? *(const h8 *)(src + 32 * ks) : h8{};
     }
-    const _Float16 *Aop = ops16u + (size_t)sl * MRP * KP;
-    const int nrt = MRP / 64;
-    h8 ra[OPT];
-    auto load_chunk = [&](int rt, int kc) {
-#pragma unroll
-        for (int i = 0; i < OPT; ++i) {
-            const int v = tid + 256 * i;
-            ra[i] = *(const h8 *)(Aop + (size_t)(rt * 64 + v / (KC / 8)) * KP + kc * KC + 8 * (v % (KC / 8)));
-        }
-    };
-    load_chunk(0, 0);
+    int c = 0;
     for (int rt = 0; rt < nrt; ++rt) {
         f4v acc[PS][4];
 #pragma unroll
@@ -480,21 +520,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int b = 0; b < 4; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kc = 0; kc < NKC; ++kc) {
-            __syncthreads();  // previous chunk consumed (and sEA written)
-#pragma unroll
-            for (int i = 0; i < OPT; ++i) {
-                const int v = tid + 256 * i;
-                *(h8 *)&sA[v / (KC / 8)][8 * (v % (KC / 8))] = ra[i];
-            }
-            __syncthreads();
-            if (kc + 1 < NKC) load_chunk(rt, kc + 1);
-            else if (rt + 1 < nrt) load_chunk(rt + 1, 0);
+        for (int kc = 0; kc < NKC; ++kc, ++c) {
+            // this wave's loads of chunk c are done once at most the loads of
+            // chunks c + 1 .. c + NST - 2 are outstanding (a lower bound on
+            // what was issued after them: stores may add more); the barrier
+            // then publishes every wave's part and retires the reads of chunk
+            // c - 1, whose ring slot the next issue refills
+            if (c + NST - 2 < NC) wait_vm<DPW * (NST - 2)>();
+            else wait_vm<0>();
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+            if (c + NST - 1 < NC) issue(c + NST - 1);
+            const _Float16 *sa = sA[c % NST];
 #pragma unroll
             for (int ks = 0; ks < KC / 32; ++ks) {
                 h8 kf[4];
+                const int sg = (4 * ks + (l >> 4)) ^ ((l & 15) % SEGS);
 #pragma unroll
-                for (int ri = 0; ri < 4; ++ri) kf[ri] = *(const h8 *)&sA[16 * ri + (l & 15)][32 * ks + 8 * (l >> 4)];
+                for (int ri = 0; ri < 4; ++ri) kf[ri] = *(const h8 *)(sa + (16 * ri + (l & 15)) * KC + 8 * sg);
 #pragma unroll
                 for (int pi = 0; pi < PS; ++pi)
 #pragma unroll
@@ -510,8 +555,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int ri = 0; ri < 4; ++ri) {
             const int row = rt * 64 + 16 * ri + 4 * (l >> 4);
             if (row >= R4) continue;
-            const int *er = esig + (size_t)sl * MRP + row;
-            const int e0 = er[0], e1 = er[1], e2 = er[2], e3 = er[3];
+            const int4 er = *(const int4 *)&sE[row];
+            const int e0 = er.x, e1 = er.y, e2 = er.z, e3 = er.w;
 #pragma unroll
             for (int pi = 0; pi < PS; ++pi) {
                 const int col = PPW * w + 16 * pi + (l & 15);
