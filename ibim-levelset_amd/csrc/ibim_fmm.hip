@@ -499,8 +499,8 @@ template <> struct Vec4<double> { using type = double4; };
 // nsl = 64 / ntc slices of ntc targets, so 64 - nsl*ntc lanes idle.  Sources
 // are staged in LDS 64 at a time (one per lane).  MIXED: coordinates relative
 // to the unit's first target are formed in FP64 and rounded to FP32, the
-// kernels are evaluated in FP32, each lane's FP32 sums over one LDS batch
-// (at most 64 terms) are accumulated in FP64.
+// kernels are evaluated in FP32 (PB kernels by pair_pb32), each lane's FP32
+// sums over one LDS batch (at most 64 terms) are accumulated in FP64.
 template <bool MIXED, bool POLY>
 __global__ __launch_bounds__(64) void k_p2p_unit(
     const int4 *__restrict__ ua, const int4 *__restrict__ ub, const double *__restrict__ tx,
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(64) void k_p2p_unit(
     const int2 *__restrict__ ranges, PBConst kc, double *__restrict__ F, int nt, double *__restrict__ part) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using T4 = typename Vec4<T>::type;
-    __shared__ T4 s0[64], s1[64];  // {x, y, z, q_psi}, {q_x, q_y, q_z, -}
+    __shared__ T4 s0[64], s1[64];  // {x, y, z, q_psi}, {q_x, q_y, q_z, q_psi / (4 pi)}
     __shared__ double red[4][64];
     const int lane = threadIdx.x;
     const int4 a = ua[blockIdx.x], c = ub[blockIdx.x];
@@ -520,6 +520,7 @@ __global__ __launch_bounds__(64) void k_p2p_unit(
     const int t = t0 + mt;
     const double ox = MIXED ? tx[t0] : 0.0, oy = MIXED ? ty[t0] : 0.0, oz = MIXED ? tz[t0] : 0.0;
     const T px = (T)(tx[t] - ox), py = (T)(ty[t] - oy), pz = (T)(tz[t] - oz);
+    const PBConst32 k32 = pb_const32(kc);
     double f0 = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     int cr = c.x, cbase = -c.y;  // this lane's loading cursor: source j sits at ranges[cr].x + j - cbase
     for (int b0 = 0; b0 < total; b0 += 64) {
@@ -538,13 +539,35 @@ __global__ __launch_bounds__(64) void k_p2p_unit(
             v.x = (T)Q[s];
             v.y = (T)Q[(size_t)ns + s];
             v.z = (T)Q[2 * (size_t)ns + s];
-            v.w = (T)0;
+            v.w = (T)(Q[3 * (size_t)ns + s] * kc.inv4pi);  // qs of pair_pb32
             s0[lane] = u;
             s1[lane] = v;
         }
         __syncthreads();
         const int cnt = min(64, total - b0);
-        if (active) {
+        if constexpr (MIXED && !POLY) {
+          if (active) {
+            // FP32 PB kernels: F0 = f0a / (4 pi) - f0b
+            float ga = 0, gb = 0, gx = 0, gy = 0, gz = 0, ha = 0, hb = 0, hx = 0, hy = 0, hz = 0;
+            int k = sl;
+            for (; k + nsl < cnt; k += 2 * nsl) {
+                const float4 u = s0[k], v = s1[k], u2 = s0[k + nsl], v2 = s1[k + nsl];
+                pair_pb32(u.x - (float)px, u.y - (float)py, u.z - (float)pz, v.x, v.y, v.z, v.w, k32, ga, gb, gx, gy,
+                          gz);
+                pair_pb32(u2.x - (float)px, u2.y - (float)py, u2.z - (float)pz, v2.x, v2.y, v2.z, v2.w, k32, ha, hb,
+                          hx, hy, hz);
+            }
+            if (k < cnt) {
+                const float4 u = s0[k], v = s1[k];
+                pair_pb32(u.x - (float)px, u.y - (float)py, u.z - (float)pz, v.x, v.y, v.z, v.w, k32, ga, gb, gx, gy,
+                          gz);
+            }
+            f0 += (double)(ga + ha) * kc.inv4pi - (double)(gb + hb);
+            fx += (double)(gx + hx);
+            fy += (double)(gy + hy);
+            fz += (double)(gz + hz);
+          }
+        } else if (active) {
             T g0 = 0, gx = 0, gy = 0, gz = 0, h0 = 0, hx = 0, hy = 0, hz = 0;
             int k = sl;
             for (; k + nsl < cnt; k += 2 * nsl) {

@@ -181,6 +181,53 @@ __device__ __forceinline__ void pair_accum(T dx, T dy, T dz, T qx, T qy, T qz, T
 }
 
 // one pair with the kernel family fixed at compile time (hot loops)
+// FP32 near-field form of the PB kernels with 1/(4 pi) taken out of the
+// sums.  Per source the caller supplies qs = qp / (4 pi) besides (qx, qy,
+// qz); the pair adds
+//   f0a += g1 dq,   f0b += k2 qs        so that F0 = f0a / (4 pi) - f0b
+//   fa  += a3 qa + da (b3 dq + g4 qs)
+// with g1 = (eps e (t+1) - 1) / r^3, g4 = (e (t+1) / eps - 1) / r^3, k2 =
+// (1 - e) / r (4 pi k2reg below the vacant radius), a3 = (1 - e (t+1)) / r^3
+// and b3 = (e (t^2 + 3t + 3) - 3) / r^5 = (e (t+1)(t+2) + e - 3) / r^5.
+// The hardware rsq and exp2 are used without range fix-ups: r^2 = 0 and r
+// below the vacant radius are masked (1/r and r replaced by 0), and exp2 of
+// the non-positive argument -kappa r log2(e) can only underflow to 0.
+struct PBConst32 {
+    float kappa, ck, eps, inv_eps, vr, k2reg4pi;  // ck = -kappa log2(e)
+};
+__host__ __device__ inline PBConst32 pb_const32(const PBConst &k) {
+    const double pi4 = 1.0 / k.inv4pi;
+    return PBConst32{(float)k.kappa, (float)(-k.kappa * 1.4426950408889634), (float)k.eps, (float)k.inv_eps,
+                     (float)k.vr, (float)(k.k2reg * pi4)};
+}
+__device__ __forceinline__ void pair_pb32(float dx, float dy, float dz, float qx, float qy, float qz, float qs,
+                                          const PBConst32 &k, float &f0a, float &f0b, float &fx, float &fy,
+                                          float &fz) {
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float ir0 = __builtin_amdgcn_rsqf(r2);
+    const float r0 = r2 * ir0;
+    const bool reg = !(r0 >= k.vr);  // also catches r2 == 0 (0 * inf)
+    const float ir = reg ? 0.0f : ir0;
+    const float r = reg ? 0.0f : r0;
+    const float ir2 = ir * ir, ir3 = ir2 * ir;
+    const float t = k.kappa * r;
+    const float e = __builtin_amdgcn_exp2f(k.ck * r);
+    const float et1 = fmaf(e, t, e);
+    const float m = et1 * ir3;
+    const float a3 = ir3 - m;
+    const float g1 = fmaf(k.eps, m, -ir3);
+    const float g4 = fmaf(k.inv_eps, m, -ir3);
+    const float k2 = reg ? k.k2reg4pi : (1.0f - e) * ir;
+    const float b3 = (fmaf(et1, t + 2.0f, e) - 3.0f) * ir3 * ir2;
+    const float dq = dx * qx + dy * qy + dz * qz;
+    f0a = fmaf(g1, dq, f0a);
+    f0b = fmaf(k2, qs, f0b);
+    const float s = fmaf(b3, dq, g4 * qs);
+    fx += a3 * qx + dx * s;
+    fy += a3 * qy + dy * s;
+    fz += a3 * qz + dz * s;
+}
+
 template <bool POLY, typename T>
 __device__ __forceinline__ void pair_accum_m(T dx, T dy, T dz, T qx, T qy, T qz, T qp, const PBConst &k, T &f0,
                                              T &fx, T &fy, T &fz) {
