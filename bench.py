@@ -307,9 +307,12 @@ def main():
     # M2L_MIXED / P2P_MIXED: the inner (mixed) matvecs; M2L / P2P: the FP64 outer residuals
     dom = max(totals, key=totals.get)  # the stage with the most time per solve
     ms_gemm = stage_acc["M2L_MIXED"][0] / stage_acc["M2L_MIXED"][1]
-    flops = 32.0 * r * r * stats["v_pairs"]  # 2 x (4r) x (4r) per V interaction
-    achieved = flops / (ms_gemm * 1e-3) / 1e12
     p = cfg["p"]
+    # the FP16 stage also evaluates the X list (as GEMMs on the same kernel)
+    x_in_stage = best == "mixed_f16" and p <= 6 and stats["x_pairs"] > 0
+    pairs = stats["v_pairs"] + (stats["x_pairs"] if x_in_stage else 0)
+    flops = 32.0 * r * r * pairs  # 2 x (4r) x (4r) per V or X interaction
+    achieved = flops / (ms_gemm * 1e-3) / 1e12
     gemm16 = f"k_m2l_gemm_hu<{p},{32 if p <= 5 else 16}>" if p <= 6 else f"k_m2l_gemm<{p},float>"
     kern, peak = {"mixed_f32": (f"k_m2l_gemm<{p},float> (FP32 MFMA)", PEAK_TFLOPS["f32"]),
                   "mixed_f16": (f"k_m_to_half + {gemm16} + k_v_reduce (FP16 MFMA)" if p <= 6 else
@@ -325,9 +328,10 @@ def main():
                 "frac": achieved / peak, "traffic": traffic, "kernel": kern,
                 "dominant_stage": dom, "ms_per_launch": ms_gemm, "launches": stage_acc["M2L_MIXED"][1],
                 "algorithmic_flops_per_launch": flops,
-                "note": "V-list M2L stage of the inner (mixed) matvecs: 32 r^2 flops per V interaction (r = p^3), "
-                        "v_pairs interactions per launch; time from HIP events on the plan stream around the stage "
-                        "over the timed mixed solves, all kernels of the stage included"}
+                "interactions_per_launch": pairs,
+                "note": "M2L stage of the inner (mixed) matvecs: 32 r^2 flops per interaction (r = p^3), V pairs "
+                        + ("and X pairs " if x_in_stage else "") + "per launch; time from HIP events on the plan "
+                        "stream around the stage over the timed mixed solves, all kernels of the stage included"}
     stage_ms = {k: v[0] / args.steps for k, v in stage_acc.items()}
 
     cpu = None

@@ -290,7 +290,6 @@ struct VPairs {
     int *cls_off_d = nullptr;
     int nblocks = 0;                    // GEMM blocks of 64 pairs
     int *blk_cls = nullptr, *blk_j0 = nullptr;
-    int *tptr = nullptr, *tlist = nullptr; // pairs of target node b: tlist[tptr[b] .. tptr[b+1])
     int hnblocks = 0;                   // FP16 GEMM blocks (128 pairs for p <= 5, 64 for p = 6)
     int *hblk_cls = nullptr, *hblk_j0 = nullptr;
 };
@@ -380,13 +379,15 @@ struct ibim_plan {
     float *ops32 = nullptr;              // FP32 copy for the mixed-precision matvec
     // FP16 M2L (IBIM_PREC_MIXED_F16): operators with one power-of-two scale
     // per row, multipoles with one per box, FP32 accumulation buffer
-    _Float16 *ops16 = nullptr;           // one unfolded operator per (class, mask) key with pairs
-                                         // (slot keyslot[key])
+    _Float16 *ops16 = nullptr;           // one unfolded operator per (class, mask) key with pairs, V keys
+                                         // (slot keyslot[key]) then X keys (slot xkeyslot[key])
     int *esig = nullptr;                 // per (operator, row) exponent
-    int *keyslot = nullptr;
+    int *keyslot = nullptr, *xkeyslot = nullptr;
+    bool x16 = false;                    // the X list runs in the FP16 stage
     _Float16 *Mh = nullptr;              // nnodes * (4r rounded up to 64), per box scaled
     int *ebox = nullptr;                 // per box exponent
-    float *Yv = nullptr;                 // V interaction results, v pairs * (4r rounded up to 32)
+    float *Yv = nullptr;                 // V then X interaction results, pairs * (4r rounded up to 32)
+    int *ytptr = nullptr, *ytlist = nullptr; // rows of Yv of target node b: ytlist[ytptr[b] .. ytptr[b+1])
     int *gtab = nullptr, *stab = nullptr; // gather / scatter tables of the 8 reflections
     ibim::VPairs vp;                     // V pairs for the points as their own targets
     // X list as GEMMs: pairs grouped by class (level of the target, level

@@ -609,18 +609,27 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
     const int grid = 8 * per_xcd;
     hipStream_t st = pl->stream;
     if constexpr (P <= 6) {
-        if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->ops16 && vp.tptr && pl->Yv &&
-            vp.npairs <= pl->vp.npairs) {
+        if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->ops16 && pl->ytptr && &vp == &pl->vp) {
             constexpr int KP = (R4 + 63) / 64 * 64;
+            constexpr int YS = (R4 + 31) / 32 * 32;
+            constexpr int PPW = P <= 5 ? 32 : 16;
             k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, KP, pl->Mh, pl->ebox);
+            // V pairs, then (x16) the X pairs into the rows after them
             if (vp.hnblocks > 0) {
                 const int pxcd = (vp.hnblocks + 7) / 8;
-                k_m2l_gemm_hu<P, P <= 5 ? 32 : 16><<<8 * pxcd, 256, 0, st>>>(
-                    vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A, pl->keyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
-                    pl->ebox, pl->Yv, vp.hnblocks, pxcd);
-                k_v_reduce<R4><<<pl->n_m2l_work, R4 / 4 <= 128 ? 128 : 256, 0, st>>>(pl->m2l_work, vp.tptr,
-                                                                                    vp.tlist, pl->Yv, pl->L);
+                k_m2l_gemm_hu<P, PPW><<<8 * pxcd, 256, 0, st>>>(vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A,
+                                                                pl->keyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
+                                                                pl->ebox, pl->Yv, vp.hnblocks, pxcd);
             }
+            const VPairs &xp = pl->xp;
+            if (pl->x16 && xp.hnblocks > 0) {
+                const int pxcd = (xp.hnblocks + 7) / 8;
+                k_m2l_gemm_hu<P, PPW><<<8 * pxcd, 256, 0, st>>>(
+                    xp.hblk_cls, xp.hblk_j0, xp.cls_off_d, xp.A, pl->xkeyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
+                    pl->ebox, pl->Yv + (size_t)vp.npairs * YS, xp.hnblocks, pxcd);
+            }
+            k_v_reduce<R4><<<pl->n_m2l_work, R4 / 4 <= 128 ? 128 : 256, 0, st>>>(pl->m2l_work, pl->ytptr, pl->ytlist,
+                                                                                pl->Yv, pl->L);
             IBIM_LAUNCH_CHECK();
             return IBIM_OK;
         }
@@ -781,33 +790,6 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
             IBIM_HIP_CHECK(hipMemcpyAsync(vp.hblk_j0, hj.data(), hj.size() * sizeof(int), hipMemcpyHostToDevice, st));
         }
     }
-    // pairs of each target node, in pair order: tlist[tptr[b] .. tptr[b+1])
-    if ((rc = alloc(&vp.tptr, (size_t)nn + 1)) || (rc = alloc(&vp.tlist, std::max(1, np)))) return rc;
-    IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, ((size_t)nn + 1) * sizeof(int), st));
-    if (np > 0) {
-        k_count_targets<<<nblk(np), 256, 0, st>>>(np, vp.B, cnt);
-        IBIM_LAUNCH_CHECK();
-    }
-    bytes = 0;
-    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt, vp.tptr, nn + 1, st));
-    IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
-    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, cnt, vp.tptr, nn + 1, st));
-    IBIM_HIP_CHECK(hipStreamSynchronize(st));
-    (void)hipFree(tmp);
-    if (np > 0) {
-        int endbit = 1;
-        while ((1 << endbit) <= nn) ++endbit;
-        // key0/key1 and iota/order are free again: B copy -> key0, sorted keys -> key1, values iota -> tlist
-        IBIM_HIP_CHECK(hipMemcpyAsync(key0, vp.B, (size_t)np * sizeof(int), hipMemcpyDeviceToDevice, st));
-        k_iota_pairs<<<nblk(np), 256, 0, st>>>(np, iota);
-        IBIM_LAUNCH_CHECK();
-        bytes = 0;
-        IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key0, key1, iota, vp.tlist, np, 0, endbit, st));
-        IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
-        IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, key0, key1, iota, vp.tlist, np, 0, endbit, st));
-        IBIM_HIP_CHECK(hipStreamSynchronize(st));
-        (void)hipFree(tmp);
-    }
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
     for (void *p : {(void *)cnt, (void *)off, (void *)a0, (void *)b0, (void *)m0, (void *)key0, (void *)key1,
                     (void *)iota, (void *)order})
@@ -918,6 +900,7 @@ template <int P>
 int gemm_x_p(ibim_plan *pl, bool mixed) {
     const VPairs &xp = pl->xp;
     if (xp.nblocks == 0) return IBIM_OK;
+    if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->x16) return IBIM_OK;  // in the FP16 V stage
     const int total = xp.nblocks * (pl->MRP / 64);
     const int per_xcd = (total + 7) / 8;
     if (mixed && pl->xops32)
@@ -1022,10 +1005,21 @@ int build_xpairs(ibim_plan *pl) {
             bj.push_back(j);
         }
     xp.nblocks = (int)bc.size();
+    std::vector<int> hc, hj;  // FP16 GEMM blocks, as for the V pairs
+    if (pl->p <= 6) {
+        const int hnp = pl->p <= 5 ? 128 : 64;
+        for (int key = 0; key < 8 * nc; ++key)
+            for (int j = xp.cls_off[key]; j < xp.cls_off[key + 1]; j += hnp) {
+                hc.push_back(key);
+                hj.push_back(j);
+            }
+    }
+    xp.hnblocks = (int)hc.size();
     int *d_cls = nullptr;
     if ((rc = dalloc(pl, &xp.A, np)) || (rc = dalloc(pl, &xp.B, np)) || (rc = dalloc(pl, &xp.mask, np)) ||
         (rc = dalloc(pl, &xp.cls_off_d, xp.cls_off.size())) || (rc = dalloc(pl, &xp.blk_cls, xp.nblocks)) ||
-        (rc = dalloc(pl, &xp.blk_j0, xp.nblocks)) || (rc = salloc(&d_cls, cls.size())) ||
+        (rc = dalloc(pl, &xp.blk_j0, xp.nblocks)) || (rc = dalloc(pl, &xp.hblk_cls, std::max(1, xp.hnblocks))) ||
+        (rc = dalloc(pl, &xp.hblk_j0, std::max(1, xp.hnblocks))) || (rc = salloc(&d_cls, cls.size())) ||
         (rc = dalloc(pl, &pl->xops, (size_t)nc * pl->MRP * pl->KRP)))
         return rc;
     k_permute_pairs<<<nblk(np), 256, 0, st>>>(np, order, a0, b0, m0, xp.A, xp.B, xp.mask);
@@ -1034,6 +1028,10 @@ int build_xpairs(ibim_plan *pl) {
                                   hipMemcpyHostToDevice, st));
     IBIM_HIP_CHECK(hipMemcpyAsync(xp.blk_cls, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice, st));
     IBIM_HIP_CHECK(hipMemcpyAsync(xp.blk_j0, bj.data(), bj.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    if (xp.hnblocks > 0) {
+        IBIM_HIP_CHECK(hipMemcpyAsync(xp.hblk_cls, hc.data(), hc.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        IBIM_HIP_CHECK(hipMemcpyAsync(xp.hblk_j0, hj.data(), hj.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    }
     IBIM_HIP_CHECK(hipMemcpyAsync(d_cls, cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice, st));
     pl->n_xcls = nc;
     switch (pl->p) {
@@ -1078,46 +1076,121 @@ int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed) {
 }
 
 // FP16 operators, their row exponents and the per-matvec FP16 multipole and
-// V-result buffers of the IBIM_PREC_MIXED_F16 matvec (the plan's V pairs
-// must exist)
+// interaction-result buffers of the IBIM_PREC_MIXED_F16 matvec (the plan's V
+// and X pairs must exist).  The X list joins the V list in this stage: its
+// (class, mask) keys get unfolded operators after the V keys', its results
+// the rows of Yv after the V pairs', and one per-target list over both (V
+// pairs first, each kind in pair order) drives the reduction.
 static int build_xops_f32(ibim_plan *pl);
 
 int build_m2l_operators_f16(ibim_plan *pl) {
-    if (int rc = build_xops_f32(pl)) return rc;
     if (pl->ops16 || !pl->ops || pl->n_cls == 0) return IBIM_OK;
     if (pl->p > 6) return build_m2l_operators_f32(pl);  // the FP16 GEMM's LDS tiles stop at p = 6
+    hipStream_t st = pl->stream;
     const int R4 = 4 * pl->r;
     const int KP = (R4 + 63) / 64 * 64;
+    const int YS = (R4 + 31) / 32 * 32;
+    const VPairs &vp = pl->vp, &xp = pl->xp;
+    const bool with_x = xp.npairs > 0 && pl->xops && xp.hnblocks > 0;
+    const int npv = vp.npairs, npx = with_x ? xp.npairs : 0, npa = npv + npx;
+    std::vector<void *> scratch;
+    struct Free {
+        std::vector<void *> &v;
+        ~Free() {
+            for (void *p : v) (void)hipFree(p);
+        }
+    } free_scratch{scratch};
+    auto salloc = [&](auto **p, size_t n) -> int {
+        hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(**p));
+        if (e != hipSuccess) {
+            set_error("hipMalloc (FP16 M2L set-up)");
+            return IBIM_ERR_NOMEM;
+        }
+        scratch.push_back((void *)*p);
+        return IBIM_OK;
+    };
     int rc;
     if ((rc = dalloc(pl, &pl->Mh, (size_t)pl->nnodes * KP)) || (rc = dalloc(pl, &pl->ebox, (size_t)pl->nnodes)) ||
-        (rc = dalloc(pl, &pl->Yv, std::max<size_t>(1, (size_t)pl->vp.npairs * ((R4 + 31) / 32 * 32)))))
+        (rc = dalloc(pl, &pl->Yv, std::max<size_t>(1, (size_t)npa * YS))))
         return rc;
-    // unfolded operators for the (class, mask) keys that have pairs
-    const std::vector<int> &off = pl->vp.cls_off;
-    std::vector<int> keyslot(off.size() > 0 ? off.size() - 1 : 0, -1), slot_key;
-    for (size_t k = 0; k + 1 < off.size(); ++k)
-        if (off[k + 1] > off[k]) {
-            keyslot[k] = (int)slot_key.size();
-            slot_key.push_back((int)k);
-        }
-    const int ns = (int)slot_key.size();
-    int *d_slot_key = nullptr;
+    // unfolded operators for the (class, mask) keys that have pairs: V keys
+    // in slots [0, nsv), X keys in [nsv, nsv + nsx)
+    auto slots = [](const std::vector<int> &off, int first, std::vector<int> &keyslot, std::vector<int> &slot_key) {
+        keyslot.assign(off.size() > 0 ? off.size() - 1 : 0, -1);
+        for (size_t k = 0; k + 1 < off.size(); ++k)
+            if (off[k + 1] > off[k]) {
+                keyslot[k] = first + (int)slot_key.size();
+                slot_key.push_back((int)k);
+            }
+    };
+    std::vector<int> keyslot, slot_key, xkeyslot, xslot_key;
+    slots(vp.cls_off, 0, keyslot, slot_key);
+    const int nsv = (int)slot_key.size();
+    if (with_x) slots(xp.cls_off, nsv, xkeyslot, xslot_key);
+    const int nsx = (int)xslot_key.size();
+    const int ns = nsv + nsx;
+    int *d_slot_key = nullptr, *d_xslot_key = nullptr;
     if ((rc = dalloc(pl, &pl->keyslot, std::max<size_t>(1, keyslot.size()))) ||
-        (rc = dalloc(pl, &d_slot_key, std::max(1, ns))) ||
+        (rc = dalloc(pl, &pl->xkeyslot, std::max<size_t>(1, xkeyslot.size()))) ||
+        (rc = salloc(&d_slot_key, (size_t)nsv)) || (rc = salloc(&d_xslot_key, (size_t)nsx)) ||
         (rc = dalloc(pl, &pl->ops16, std::max<size_t>(1, (size_t)ns * pl->MRP * pl->KRP))) ||
         (rc = dalloc(pl, &pl->esig, std::max<size_t>(1, (size_t)ns * pl->MRP))))
         return rc;
     if (!keyslot.empty())
         IBIM_HIP_CHECK(hipMemcpyAsync(pl->keyslot, keyslot.data(), keyslot.size() * sizeof(int),
-                                      hipMemcpyHostToDevice, pl->stream));
-    if (ns > 0) {
-        IBIM_HIP_CHECK(hipMemcpyAsync(d_slot_key, slot_key.data(), ns * sizeof(int), hipMemcpyHostToDevice,
-                                      pl->stream));
-        k_unfold_half<<<dim3(pl->MRP, ns), 64, 0, pl->stream>>>(d_slot_key, pl->ops, pl->MRP, pl->KRP, R4, pl->gtab,
-                                                                pl->stab, pl->ops16, pl->esig);
+                                      hipMemcpyHostToDevice, st));
+    if (!xkeyslot.empty())
+        IBIM_HIP_CHECK(hipMemcpyAsync(pl->xkeyslot, xkeyslot.data(), xkeyslot.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, st));
+    if (nsv > 0) {
+        IBIM_HIP_CHECK(hipMemcpyAsync(d_slot_key, slot_key.data(), nsv * sizeof(int), hipMemcpyHostToDevice, st));
+        k_unfold_half<<<dim3(pl->MRP, nsv), 64, 0, st>>>(d_slot_key, pl->ops, pl->MRP, pl->KRP, R4, pl->gtab,
+                                                         pl->stab, pl->ops16, pl->esig);
         IBIM_LAUNCH_CHECK();
     }
-    IBIM_HIP_CHECK(hipStreamSynchronize(pl->stream));
+    if (nsx > 0) {
+        IBIM_HIP_CHECK(hipMemcpyAsync(d_xslot_key, xslot_key.data(), nsx * sizeof(int), hipMemcpyHostToDevice, st));
+        k_unfold_half<<<dim3(pl->MRP, nsx), 64, 0, st>>>(d_xslot_key, pl->xops, pl->MRP, pl->KRP, R4, pl->gtab,
+                                                         pl->stab, pl->ops16 + (size_t)nsv * pl->MRP * pl->KRP,
+                                                         pl->esig + (size_t)nsv * pl->MRP);
+        IBIM_LAUNCH_CHECK();
+    }
+    // rows of Yv per target node: a stable sort of the target boxes of the
+    // V pairs followed by the X pairs
+    const int nn = pl->nnodes;
+    int *bkey = nullptr, *bsorted = nullptr, *iota = nullptr, *cnt = nullptr;
+    if ((rc = dalloc(pl, &pl->ytptr, (size_t)nn + 1)) || (rc = dalloc(pl, &pl->ytlist, std::max(1, npa))) ||
+        (rc = salloc(&bkey, (size_t)npa)) || (rc = salloc(&bsorted, (size_t)npa)) || (rc = salloc(&iota, (size_t)npa)) ||
+        (rc = salloc(&cnt, (size_t)nn + 1)))
+        return rc;
+    IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, ((size_t)nn + 1) * sizeof(int), st));
+    if (npa > 0) {
+        if (npv > 0) IBIM_HIP_CHECK(hipMemcpyAsync(bkey, vp.B, (size_t)npv * sizeof(int), hipMemcpyDeviceToDevice, st));
+        if (npx > 0)
+            IBIM_HIP_CHECK(hipMemcpyAsync(bkey + npv, xp.B, (size_t)npx * sizeof(int), hipMemcpyDeviceToDevice, st));
+        k_count_targets<<<nblk(npa), 256, 0, st>>>(npa, bkey, cnt);
+        k_iota_pairs<<<nblk(npa), 256, 0, st>>>(npa, iota);
+        IBIM_LAUNCH_CHECK();
+    }
+    size_t bytes = 0;
+    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt, pl->ytptr, nn + 1, st));
+    void *tmp = nullptr;
+    if ((rc = salloc((char **)&tmp, bytes))) return rc;
+    IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, cnt, pl->ytptr, nn + 1, st));
+    if (npa > 0) {
+        int endbit = 1;
+        while ((1 << endbit) <= nn) ++endbit;
+        bytes = 0;
+        IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, bkey, bsorted, iota, pl->ytlist, npa, 0,
+                                                          endbit, st));
+        void *tmp2 = nullptr;
+        if ((rc = salloc((char **)&tmp2, bytes))) return rc;
+        IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp2, bytes, bkey, bsorted, iota, pl->ytlist, npa, 0, endbit,
+                                                          st));
+    }
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    pl->x16 = nsx > 0;
+    if (!pl->x16) return build_xops_f32(pl);  // X list in FP32 GEMMs or direct
     return IBIM_OK;
 }
 
