@@ -386,7 +386,8 @@ struct ibim_plan {
     bool x16 = false;                    // the X list runs in the FP16 stage
     _Float16 *Mh = nullptr;              // nnodes * (4r rounded up to 64), per box scaled
     int *ebox = nullptr;                 // per box exponent
-    float *Yv = nullptr;                 // V then X interaction results, pairs * (4r rounded up to 32)
+    unsigned *Yv = nullptr;              // V then X interaction results as FP24 (3 dwords per 4 rows),
+                                         // pairs * (4r rounded up to 32) values
     int *ytptr = nullptr, *ytlist = nullptr; // rows of Yv of target node b: ytlist[ytptr[b] .. ytptr[b+1])
     int *gtab = nullptr, *stab = nullptr; // gather / scatter tables of the 8 reflections
     ibim::VPairs vp;                     // V pairs for the points as their own targets

@@ -346,6 +346,23 @@ int build_ops_p(ibim_plan *pl) {
 // ------------------------------------------------- FP16 V-list GEMM ---
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+// Pair results are kept as FP24: the upper 24 bits of the FP32 value,
+// rounded to nearest (relative error below 2^-16, far under the FP16
+// products' own), four consecutive rows packed into three dwords.
+__device__ __forceinline__ unsigned fp24(float x) { return (__float_as_uint(x) + 0x80u) >> 8; }
+__device__ __forceinline__ void store4_fp24(unsigned *p, float a, float b, float c, float d) {
+    const unsigned A = fp24(a), B = fp24(b), C = fp24(c), D = fp24(d);
+    p[0] = A | (B << 24);
+    p[1] = (B >> 8) | (C << 16);
+    p[2] = (C >> 16) | (D << 8);
+}
+__device__ __forceinline__ f4v load4_fp24(const unsigned *p) {
+    const unsigned x = p[0], y = p[1], z = p[2];
+    return f4v{__uint_as_float(x << 8), __uint_as_float(((x >> 24) | (y << 8)) << 8),
+               __uint_as_float(((y >> 16) | (z << 16)) << 8), __uint_as_float(z & 0xffffff00u)};
+}
 
 // e such that every |v| of a set with maximum vmax satisfies |v| 2^-e < 1
 // (0 for an all-zero set)
@@ -439,15 +456,15 @@ __device__ __forceinline__ void wait_vm() {
 // segments are stored XOR-swizzled by the row index, which keeps the MFMA
 // A-fragment reads free of bank conflicts.  A lane's four accumulators are
 // four consecutive rows of one pair: results, scaled by 2^(esig + ebox), are
-// stored 16 B per lane as Y[pair][row], rows padded to YS = R4 rounded up to
-// 32 (whole 128 B lines per 64-row tile); k_v_reduce sums every target's
-// pairs into L.
+// stored as FP24, 12 B per lane, as Y[pair][row], rows padded to YS = R4
+// rounded up to 32 (whole 128 B lines per 64-row tile); k_v_reduce sums every
+// target's pairs into L.
 template <int P, int PPW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_m2l_gemm_hu(
     const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
     const int *__restrict__ pA, const int *__restrict__ keyslot, const _Float16 *__restrict__ ops16u,
     const int *__restrict__ esig, int MRP, const _Float16 *__restrict__ Mh, const int *__restrict__ ebox,
-    float *__restrict__ Y, int nblocks, int per_xcd) {
+    unsigned *__restrict__ Y, int nblocks, int per_xcd) {
     constexpr int R = P * P * P;
     constexpr int R4 = 4 * R;
     constexpr int KP = (R4 + 63) / 64 * 64;
@@ -563,9 +580,142 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 if (j0 + col >= jend) continue;
                 const int ea = sEA[col];
                 const f4v a = acc[pi][ri];
-                *(f4v *)(Y + (size_t)(j0 + col) * YS + row) =
-                    f4v{ldexpf(a[0], e0 + ea), ldexpf(a[1], e1 + ea), ldexpf(a[2], e2 + ea), ldexpf(a[3], e3 + ea)};
+                store4_fp24(Y + ((size_t)(j0 + col) * YS + row) / 4 * 3, ldexpf(a[0], e0 + ea), ldexpf(a[1], e1 + ea),
+                            ldexpf(a[2], e2 + ea), ldexpf(a[3], e3 + ea));
             }
+        }
+    }
+}
+
+// The same GEMM on v_mfma_f32_32x32x16_f16, which on gfx950 does twice the
+// work of the 16x16x32 form in the same issue time: a wave holds the
+// B-fragments of 32 pairs (one 32-column tile) for all k and multiplies them
+// with the operator chunk as two 32-row A-tiles per 16-k step.  Operands: A
+// lane l holds rows l & 31, k 8 (l >> 5) .. + 7 of the step; B lane l holds
+// pair l & 31, the same k; accumulator j of lane l is row 8 (j >> 2) + 4 (l
+// >> 5) + (j & 3), pair l & 31.  For p <= 5 (the B-fragments of 32 pairs over
+// all k fit the register budget).
+template <int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_m2l_gemm_h32(
+    const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
+    const int *__restrict__ pA, const int *__restrict__ keyslot, const _Float16 *__restrict__ ops16u,
+    const int *__restrict__ esig, int MRP, const _Float16 *__restrict__ Mh, const int *__restrict__ ebox,
+    unsigned *__restrict__ Y, int nblocks, int per_xcd) {
+    constexpr int R = P * P * P;
+    constexpr int R4 = 4 * R;
+    constexpr int KP = (R4 + 63) / 64 * 64;
+    constexpr int KS = KP / 16;                  // MFMA k-steps
+    constexpr int KC = KP % 128 == 0 ? 128 : 64; // k per operator chunk
+    constexpr int NKC = KP / KC;
+    constexpr int SEGS = KC / 8;                 // 16-byte segments per chunk row
+    constexpr int RPI = 64 / SEGS;               // chunk rows per LDS-load instruction (64 lanes x 16 B)
+    constexpr int DPW = SEGS / 4;                // LDS-load instructions per wave per chunk
+    constexpr int NST = 4;                       // chunks in the LDS ring
+    constexpr int CH = 64 * KC;                  // halves per chunk
+    constexpr int NPB = 128;                     // pairs per block (32 per wave)
+    constexpr int YS = (R4 + 31) / 32 * 32;      // row stride of Y
+    constexpr int MR = (R4 + 63) / 64 * 64;      // operator rows (padded)
+    static_assert(KP % KC == 0 && DPW >= 1 && R4 % 4 == 0 && P <= 5, "tile shape");
+    __shared__ __attribute__((aligned(1024))) _Float16 sA[NST][CH];
+    __shared__ int sEA[NPB];
+    __shared__ __attribute__((aligned(16))) int sE[MR];  // the operator's row exponents
+    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (tile >= nblocks) return;
+    const int tid = threadIdx.x;
+    const int key = blk_key[tile];
+    const int j0 = blk_j0[tile];
+    const int jend = key_off[key + 1];
+    const int sl = keyslot[key];
+    const int w = tid >> 6, l = tid & 63;
+    const _Float16 *Aop = ops16u + (size_t)sl * MRP * KP;
+    const int nrt = MRP / 64;
+    const int NC = nrt * NKC;
+    // chunk c into ring slot c % NST, rows XOR-swizzled as in k_m2l_gemm_hu
+    auto issue = [&](int c) {
+        const int rt = c / NKC, kc = c - rt * NKC;
+        _Float16 *dst = sA[c % NST];
+#pragma unroll
+        for (int i = 0; i < DPW; ++i) {
+            const int r = RPI * (DPW * w + i) + l / SEGS;
+            const int sg = (l % SEGS) ^ (r % SEGS);
+            lds_load16(Aop + (size_t)(rt * 64 + r) * KP + kc * KC + 8 * sg, (lds_void *)(dst + (DPW * w + i) * 512));
+        }
+    };
+#pragma unroll
+    for (int c = 0; c < NST - 1; ++c)
+        if (c < NC) issue(c);
+    if (tid < NPB) {
+        const int j = j0 + tid;
+        sEA[tid] = j < jend ? ebox[pA[j]] : 0;
+    }
+    for (int i = tid; i < MR; i += 256) sE[i] = esig[(size_t)sl * MRP + i];
+    // B-fragments of this wave's 32 pairs, all k
+    h8 xr[KS];
+    {
+        const int j = j0 + 32 * w + (l & 31);
+        const bool ok = j < jend;
+        const _Float16 *src = Mh + (size_t)(ok ? pA[j] : 0) * KP + 8 * (l >> 5);
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) xr[ks] = ok ? *(const h8 *)(src + 16 * ks) : h8{};
+    }
+    int c = 0;
+    for (int rt = 0; rt < nrt; ++rt) {
+        f16v acc[2];
+        acc[0] = f16v{};
+        acc[1] = f16v{};
+#pragma unroll
+        for (int kc = 0; kc < NKC; ++kc, ++c) {
+            if (c + NST - 2 < NC) wait_vm<DPW * (NST - 2)>();
+            else wait_vm<0>();
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+            if (c + NST - 1 < NC) issue(c + NST - 1);
+            const _Float16 *sa = sA[c % NST];
+            // A-fragments double-buffered: step ks + 1 is read while step
+            // ks multiplies (the LDS latency would otherwise stall each step)
+            auto frag = [&](int ks, h8 *k) {
+                const int sg = (2 * ks + (l >> 5)) ^ ((l & 31) % SEGS);
+                k[0] = *(const h8 *)(sa + (l & 31) * KC + 8 * sg);
+                k[1] = *(const h8 *)(sa + (32 + (l & 31)) * KC + 8 * sg);
+            };
+            static_assert((KC / 16) % 2 == 0, "k-steps per chunk");
+            // (sched_barrier: the scheduler would sink the reads next to
+            // their use to save registers)
+            h8 ka[2], kb[2];
+            frag(0, ka);
+#pragma unroll
+            for (int ks = 0; ks < KC / 16; ks += 2) {
+                frag(ks + 1, kb);
+                __builtin_amdgcn_sched_barrier(0);
+                const h8 b0 = xr[kc * (KC / 16) + ks], b1 = xr[kc * (KC / 16) + ks + 1];
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ka[0], b0, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ka[1], b0, acc[1], 0, 0, 0);
+                if (ks + 2 < KC / 16) frag(ks + 2, ka);
+                __builtin_amdgcn_sched_barrier(0);
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(kb[0], b1, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(kb[1], b1, acc[1], 0, 0, 0);
+            }
+        }
+        // rows 8 q + 4 (l >> 5) .. + 3 of each 32-row half, pair l & 31; R4
+        // is a multiple of 4, so a lane's four rows are all below R4 or all
+        // above
+        const int col = 32 * w + (l & 31);
+        if (j0 + col < jend) {
+            const int ea = sEA[col];
+            unsigned *yp = Y + (size_t)(j0 + col) * YS / 4 * 3;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = rt * 64 + 32 * h + 8 * q + 4 * (l >> 5);
+                    if (row >= R4) continue;
+                    const int4 er = *(const int4 *)&sE[row];
+                    const f16v &a = acc[h];
+                    store4_fp24(yp + row / 4 * 3, ldexpf(a[4 * q], er.x + ea), ldexpf(a[4 * q + 1], er.y + ea),
+                                ldexpf(a[4 * q + 2], er.z + ea), ldexpf(a[4 * q + 3], er.w + ea));
+                }
         }
     }
 }
@@ -575,7 +725,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // (fixed order: the sum is deterministic)
 template <int R4>
 __global__ __launch_bounds__(256) void k_v_reduce(const int *__restrict__ work, const int *__restrict__ tptr,
-                                                  const int *__restrict__ tlist, const float *__restrict__ Y,
+                                                  const int *__restrict__ tlist, const unsigned *__restrict__ Y,
                                                   double *__restrict__ L) {
     constexpr int YS = (R4 + 31) / 32 * 32;
     const int b = work[blockIdx.x];
@@ -585,12 +735,12 @@ __global__ __launch_bounds__(256) void k_v_reduce(const int *__restrict__ work, 
         f4v a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0, a2 = a0, a3 = a0;
         int q = q0;
         for (; q + 3 < q1; q += 4) {
-            a0 += *(const f4v *)(Y + (size_t)tlist[q] * YS + e);
-            a1 += *(const f4v *)(Y + (size_t)tlist[q + 1] * YS + e);
-            a2 += *(const f4v *)(Y + (size_t)tlist[q + 2] * YS + e);
-            a3 += *(const f4v *)(Y + (size_t)tlist[q + 3] * YS + e);
+            a0 += load4_fp24(Y + ((size_t)tlist[q] * YS + e) / 4 * 3);
+            a1 += load4_fp24(Y + ((size_t)tlist[q + 1] * YS + e) / 4 * 3);
+            a2 += load4_fp24(Y + ((size_t)tlist[q + 2] * YS + e) / 4 * 3);
+            a3 += load4_fp24(Y + ((size_t)tlist[q + 3] * YS + e) / 4 * 3);
         }
-        for (; q < q1; ++q) a0 += *(const f4v *)(Y + (size_t)tlist[q] * YS + e);
+        for (; q < q1; ++q) a0 += load4_fp24(Y + ((size_t)tlist[q] * YS + e) / 4 * 3);
         const f4v t = (a0 + a1) + (a2 + a3);
         double *dst = L + (size_t)b * R4 + e;
         dst[0] += (double)t[0];
@@ -612,22 +762,21 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
         if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->ops16 && pl->ytptr && &vp == &pl->vp) {
             constexpr int KP = (R4 + 63) / 64 * 64;
             constexpr int YS = (R4 + 31) / 32 * 32;
-            constexpr int PPW = P <= 5 ? 32 : 16;
             k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, KP, pl->Mh, pl->ebox);
             // V pairs, then (x16) the X pairs into the rows after them
-            if (vp.hnblocks > 0) {
-                const int pxcd = (vp.hnblocks + 7) / 8;
-                k_m2l_gemm_hu<P, PPW><<<8 * pxcd, 256, 0, st>>>(vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A,
-                                                                pl->keyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
-                                                                pl->ebox, pl->Yv, vp.hnblocks, pxcd);
-            }
-            const VPairs &xp = pl->xp;
-            if (pl->x16 && xp.hnblocks > 0) {
-                const int pxcd = (xp.hnblocks + 7) / 8;
-                k_m2l_gemm_hu<P, PPW><<<8 * pxcd, 256, 0, st>>>(
-                    xp.hblk_cls, xp.hblk_j0, xp.cls_off_d, xp.A, pl->xkeyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
-                    pl->ebox, pl->Yv + (size_t)vp.npairs * YS, xp.hnblocks, pxcd);
-            }
+            auto gemm = [&](const VPairs &pp, const int *keyslot, unsigned *Y) {
+                const int pxcd = (pp.hnblocks + 7) / 8;
+                if constexpr (P <= 5)
+                    k_m2l_gemm_h32<P><<<8 * pxcd, 256, 0, st>>>(pp.hblk_cls, pp.hblk_j0, pp.cls_off_d, pp.A, keyslot,
+                                                                pl->ops16, pl->esig, pl->MRP, pl->Mh, pl->ebox, Y,
+                                                                pp.hnblocks, pxcd);
+                else
+                    k_m2l_gemm_hu<P, 16><<<8 * pxcd, 256, 0, st>>>(pp.hblk_cls, pp.hblk_j0, pp.cls_off_d, pp.A,
+                                                                   keyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
+                                                                   pl->ebox, Y, pp.hnblocks, pxcd);
+            };
+            if (vp.hnblocks > 0) gemm(vp, pl->keyslot, pl->Yv);
+            if (pl->x16 && pl->xp.hnblocks > 0) gemm(pl->xp, pl->xkeyslot, pl->Yv + (size_t)vp.npairs * YS / 4 * 3);
             k_v_reduce<R4><<<pl->n_m2l_work, R4 / 4 <= 128 ? 128 : 256, 0, st>>>(pl->m2l_work, pl->ytptr, pl->ytlist,
                                                                                 pl->Yv, pl->L);
             IBIM_LAUNCH_CHECK();
@@ -1111,7 +1260,7 @@ int build_m2l_operators_f16(ibim_plan *pl) {
     };
     int rc;
     if ((rc = dalloc(pl, &pl->Mh, (size_t)pl->nnodes * KP)) || (rc = dalloc(pl, &pl->ebox, (size_t)pl->nnodes)) ||
-        (rc = dalloc(pl, &pl->Yv, std::max<size_t>(1, (size_t)npa * YS))))
+        (rc = dalloc(pl, &pl->Yv, std::max<size_t>(1, (size_t)npa * YS / 4 * 3))))
         return rc;
     // unfolded operators for the (class, mask) keys that have pairs: V keys
     // in slots [0, nsv), X keys in [nsv, nsv + nsx)

@@ -75,12 +75,12 @@ int64_t estimate_footprint(const ibim_request &r, const std::vector<double> &xyz
     const int p = r.fmm.p, rr3 = p * p * p;
     const double per_point = 2.0 * 4 * rr3 * 8 / 12.0 + 8.0 * (2 * (r.restart + 1) + 40);
     const double ops = (r.fmm.precision != IBIM_PREC_FP64 ? 1.5 : 1.0) * 12 * 56 * pow(4.0 * rr3, 2) * 8;
-    // the FP16 V list keeps one FP32 row of 4 p^3 (rounded up to 32) per V
+    // the FP16 V list keeps one FP24 row of 4 p^3 (rounded up to 32) per V
     // interaction; about 340 / leaf capacity V interactions per tube point
     // (plus the X list's, a few per cent), and one unfolded FP16 operator
     // per (class, mask) key with pairs, V and X (a few thousand keys)
     const double yv = (r.fmm.precision == IBIM_PREC_MIXED_F16 && p <= 6)
-                          ? npts * 350.0 / std::max(1, r.fmm.max_points) * ((4 * rr3 + 31) / 32 * 32) * 4 +
+                          ? npts * 350.0 / std::max(1, r.fmm.max_points) * ((4 * rr3 + 31) / 32 * 32) * 3 +
                                 2500.0 * pow(4.0 * rr3, 2) * 2
                           : 0.0;
     return (int64_t)(std::max(grid * 8 * 4.2, npts * per_point + ops + yv) + npts * 56);
