@@ -261,7 +261,12 @@ def main():
                 "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "all": lat}, "energy": energy,
                 "solve": dict(info, rc=rc), "plan_bytes": foot}, stats
 
+    def progress(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    progress(f"{args.config}: {surf.n} tube points")
     fp64, stats = leg(ibim.PREC_FP64)
+    progress(f"fp64: {fp64['solves_per_s']:.3f} solves/s")
     forms = {}
     for name, prec in (("mixed_f32", ibim.PREC_MIXED), ("mixed_f16", ibim.PREC_MIXED_F16)):
         stage_acc.clear()
@@ -271,6 +276,7 @@ def main():
                                          and fp64["solve"]["converged"] == 1)
         res["stages"] = {k: list(v) for k, v in stage_acc.items()}
         forms[name] = res
+        progress(f"{name}: {res['solves_per_s']:.3f} solves/s, energy error {res['rel_energy_err_vs_fp64']:.1e}")
     ok = [k for k in forms if forms[k]["energy_within_1e-6"]]
     best = max(ok or forms, key=lambda k: forms[k]["solves_per_s"])
     mixed = forms[best]
@@ -313,7 +319,7 @@ def main():
     pairs = stats["v_pairs"] + (stats["x_pairs"] if x_in_stage else 0)
     flops = 32.0 * r * r * pairs  # 2 x (4r) x (4r) per V or X interaction
     achieved = flops / (ms_gemm * 1e-3) / 1e12
-    gemm16 = f"k_m2l_gemm_hu<{p},{32 if p <= 5 else 16}>" if p <= 6 else f"k_m2l_gemm<{p},float>"
+    gemm16 = f"k_m2l_gemm_h32<{p}>" if p <= 5 else f"k_m2l_gemm_hu<{p},16>" if p == 6 else f"k_m2l_gemm<{p},float>"
     kern, peak = {"mixed_f32": (f"k_m2l_gemm<{p},float> (FP32 MFMA)", PEAK_TFLOPS["f32"]),
                   "mixed_f16": (f"k_m_to_half + {gemm16} + k_v_reduce (FP16 MFMA)" if p <= 6 else
                                 f"{gemm16} (FP32 MFMA; the FP16 tiles stop at p = 6)",

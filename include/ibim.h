@@ -51,8 +51,9 @@ extern "C" {
 
 #define IBIM_PREC_FP64 0      /* everything in FP64 (the reference's arithmetic) */
 #define IBIM_PREC_MIXED 1     /* FP32 far/near field inside FP64 refinement */
-#define IBIM_PREC_MIXED_F16 2 /* as MIXED, V-list M2L on FP16 MFMA with power-of-two
-                                 scales per box and per operator row, FP32 accumulation */
+#define IBIM_PREC_MIXED_F16 2 /* as MIXED, V- and X-list M2L on FP16 MFMA with power-of-two
+                                 scales per box and per operator row, FP32 accumulation
+                                 (p <= 6; FP32 M2L above) */
 
 #define IBIM_KERNEL_PB 0      /* the 16 PB kernels of electric.cpp:67-291 */
 #define IBIM_KERNEL_POLY 1    /* polynomial surrogates (FMM-exact; for testing) */
