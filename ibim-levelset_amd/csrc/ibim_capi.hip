@@ -113,6 +113,8 @@ void ibim_plan_destroy(ibim_plan *pl) {
     }
     for (hipEvent_t e : pl->timer.pool) (void)hipEventDestroy(e);
     for (void *p : pl->allocs) (void)hipFree(p);
+    pl->pool.free_all();
+    pl->spool.free_all();
     if (pl->owns_stream && pl->stream) (void)hipStreamDestroy(pl->stream);
     delete pl;
 }

@@ -914,21 +914,14 @@ int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, P2PWork &work, std::vector<vo
         }
         return dalloc(pl, p, n);
     };
-    // build-time scratch, freed on return
-    std::vector<void *> scratch;
-    struct Free {
-        std::vector<void *> &v;
-        ~Free() {
-            for (void *p : v) (void)hipFree(p);
-        }
-    } free_scratch{scratch};
+    // build-time scratch, returned to the plan's pool on return
+    ScratchScope scope(pl);
     auto salloc = [&](auto **p, size_t n) -> int {
-        hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(**p));
+        hipError_t e = smalloc(pl, p, (n ? n : 1) * sizeof(**p));
         if (e != hipSuccess) {
             set_error("hipMalloc (near-field work scratch)");
             return IBIM_ERR_NOMEM;
         }
-        scratch.push_back((void *)*p);
         return IBIM_OK;
     };
     int rc;

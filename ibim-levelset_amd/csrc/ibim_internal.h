@@ -15,6 +15,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/ibim.h"
@@ -281,6 +282,54 @@ struct StageTimer {
     }
 };
 
+// Device memory pool of one plan: chunks from hipMalloc handed out in
+// 256-byte aligned pieces.  The persistent pool frees its chunks with the
+// plan; the scratch pool is reset to a mark at the end of each build step
+// (every use is ordered on the plan's stream, so reuse needs no wait).  A
+// plan then holds a few chunks instead of a few hundred allocations, whose
+// hipMalloc/hipFree calls cost about 10 ms per plan.
+struct DevPool {
+    struct Chunk {
+        char *p;
+        size_t size;
+    };
+    std::vector<Chunk> chunks;
+    size_t cur = 0, off = 0;  // next free byte: chunk cur, offset off
+    size_t min_chunk;
+    int64_t bytes = 0;        // device memory held
+    explicit DevPool(size_t mc) : min_chunk(mc) {}
+    hipError_t alloc(void **ptr, size_t n) {
+        n = n ? (n + 255) & ~(size_t)255 : 256;
+        for (; cur < chunks.size(); ++cur, off = 0)
+            if (off + n <= chunks[cur].size) {
+                *ptr = chunks[cur].p + off;
+                off += n;
+                return hipSuccess;
+            }
+        const size_t sz = n > min_chunk ? n : min_chunk;
+        char *p = nullptr;
+        hipError_t e = hipMalloc((void **)&p, sz);
+        if (e != hipSuccess) return e;
+        chunks.push_back({p, sz});
+        bytes += (int64_t)sz;
+        cur = chunks.size() - 1;
+        off = n;
+        *ptr = p;
+        return hipSuccess;
+    }
+    std::pair<size_t, size_t> mark() const { return {cur, off}; }
+    void release(std::pair<size_t, size_t> m) {
+        cur = m.first;
+        off = m.second;
+    }
+    void free_all() {
+        for (const Chunk &c : chunks) (void)hipFree(c.p);
+        chunks.clear();
+        cur = off = 0;
+        bytes = 0;
+    }
+};
+
 // V-list interactions grouped by M2L operator class (level, |offset|)
 struct VPairs {
     int npairs = 0;
@@ -404,26 +453,53 @@ struct ibim_plan {
     ibim::ChebTab tab{};
     ibim::PBConst kc{};
 
-    std::vector<void *> allocs;          // everything to free
+    std::vector<void *> allocs;          // large allocations to free
+    ibim::DevPool pool{(size_t)32 << 20};  // the plan's other buffers
+    ibim::DevPool spool{(size_t)32 << 20}; // build-time scratch
     int64_t footprint = 0;
     ibim_tree_stats stats{};
 };
 
 namespace ibim {
-// allocation helper: records for release, counts footprint
+// allocation helper for buffers that live as long as the plan: large ones
+// get their own allocation, the rest come from the plan's pool; counts the
+// device memory held
 template <typename T>
 int dalloc(ibim_plan *pl, T **ptr, size_t count) {
     *ptr = nullptr;
     if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void **)ptr, count * sizeof(T));
+    const size_t bytes = count * sizeof(T);
+    hipError_t e;
+    if (bytes >= ((size_t)16 << 20)) {
+        e = hipMalloc((void **)ptr, bytes);
+        if (e == hipSuccess) {
+            pl->allocs.push_back((void *)*ptr);
+            pl->footprint += (int64_t)bytes;
+        }
+    } else {
+        const int64_t held = pl->pool.bytes;
+        e = pl->pool.alloc((void **)ptr, bytes);
+        pl->footprint += pl->pool.bytes - held;
+    }
     if (e != hipSuccess) {
-        set_error(std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes: " + hipGetErrorString(e));
+        set_error(std::string("hipMalloc of ") + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
         return IBIM_ERR_NOMEM;
     }
-    pl->allocs.push_back((void *)*ptr);
-    pl->footprint += (int64_t)(count * sizeof(T));
     return IBIM_OK;
 }
+
+// build-time scratch of `bytes` bytes from the plan's scratch pool, valid
+// until the enclosing ScratchScope ends
+template <typename T>
+hipError_t smalloc(ibim_plan *pl, T **ptr, size_t bytes) {
+    return pl->spool.alloc((void **)ptr, bytes);
+}
+struct ScratchScope {
+    ibim_plan *pl;
+    std::pair<size_t, size_t> m;
+    explicit ScratchScope(ibim_plan *p) : pl(p), m(p->spool.mark()) {}
+    ~ScratchScope() { pl->spool.release(m); }
+};
 
 // stages implemented in the .hip files
 int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const double *d_w);

@@ -847,9 +847,10 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
     };
     vp = VPairs{};
     if (pl->n_cls == 0) return IBIM_OK;
+    ScratchScope scope(pl);  // temporaries below
     int *cnt = nullptr, *off = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&cnt, ((size_t)nn + 1) * sizeof(int)));
-    IBIM_HIP_CHECK(hipMalloc(&off, ((size_t)nn + 1) * sizeof(int)));
+    IBIM_HIP_CHECK(smalloc(pl, &cnt, ((size_t)nn + 1) * sizeof(int)));
+    IBIM_HIP_CHECK(smalloc(pl, &off, ((size_t)nn + 1) * sizeof(int)));
     IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, ((size_t)nn + 1) * sizeof(int), st));
     k_vpairs<0><<<nblk(nn), 256, 0, st>>>(nn, need, pl->loff[1], pl->litem[1], pl->flags, pl->level, pl->ctr,
                                           pl->d_lvl_rad, cnt, nullptr, nullptr, nullptr, nullptr, nullptr);
@@ -857,43 +858,40 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
     size_t bytes = 0;
     IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt, off, nn + 1, st));
     void *tmp = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
+    IBIM_HIP_CHECK(smalloc(pl, &tmp, bytes ? bytes : 1));
     IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, cnt, off, nn + 1, st));
     int np = 0;
     IBIM_HIP_CHECK(hipMemcpyAsync(&np, off + nn, sizeof(int), hipMemcpyDeviceToHost, st));
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
-    (void)hipFree(tmp);
     vp.npairs = np;
     int rc;
     int *a0 = nullptr, *b0 = nullptr, *key0 = nullptr, *key1 = nullptr, *iota = nullptr, *order = nullptr;
     unsigned char *m0 = nullptr;
     if (np > 0) {
-        IBIM_HIP_CHECK(hipMalloc(&a0, (size_t)np * sizeof(int)));
-        IBIM_HIP_CHECK(hipMalloc(&b0, (size_t)np * sizeof(int)));
-        IBIM_HIP_CHECK(hipMalloc(&m0, (size_t)np));
-        IBIM_HIP_CHECK(hipMalloc(&key0, (size_t)np * sizeof(int)));
-        IBIM_HIP_CHECK(hipMalloc(&key1, (size_t)np * sizeof(int)));
-        IBIM_HIP_CHECK(hipMalloc(&iota, (size_t)np * sizeof(int)));
-        IBIM_HIP_CHECK(hipMalloc(&order, (size_t)np * sizeof(int)));
+        IBIM_HIP_CHECK(smalloc(pl, &a0, (size_t)np * sizeof(int)));
+        IBIM_HIP_CHECK(smalloc(pl, &b0, (size_t)np * sizeof(int)));
+        IBIM_HIP_CHECK(smalloc(pl, &m0, (size_t)np));
+        IBIM_HIP_CHECK(smalloc(pl, &key0, (size_t)np * sizeof(int)));
+        IBIM_HIP_CHECK(smalloc(pl, &key1, (size_t)np * sizeof(int)));
+        IBIM_HIP_CHECK(smalloc(pl, &iota, (size_t)np * sizeof(int)));
+        IBIM_HIP_CHECK(smalloc(pl, &order, (size_t)np * sizeof(int)));
         k_vpairs<1><<<nblk(nn), 256, 0, st>>>(nn, need, pl->loff[1], pl->litem[1], pl->flags, pl->level, pl->ctr,
                                               pl->d_lvl_rad, nullptr, off, a0, b0, m0, key0);
         hipcub::CountingInputIterator<int> it(0);
         // iota via a copy of the counting iterator
         IBIM_HIP_CHECK(hipcub::DeviceSelect::If(nullptr, bytes, it, iota, cnt, np,
                                                 [] __device__(int) { return true; }, st));
-        IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
+        IBIM_HIP_CHECK(smalloc(pl, &tmp, bytes ? bytes : 1));
         IBIM_HIP_CHECK(hipcub::DeviceSelect::If(tmp, bytes, it, iota, cnt, np, [] __device__(int) { return true; },
                                                 st));
         IBIM_HIP_CHECK(hipStreamSynchronize(st));
-        (void)hipFree(tmp);
         int endbit = 1;
         while ((1 << endbit) <= 8 * pl->n_cls) ++endbit;
         bytes = 0;
         IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key0, key1, iota, order, np, 0, endbit, st));
-        IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
+        IBIM_HIP_CHECK(smalloc(pl, &tmp, bytes ? bytes : 1));
         IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, key0, key1, iota, order, np, 0, endbit, st));
         IBIM_HIP_CHECK(hipStreamSynchronize(st));
-        (void)hipFree(tmp);
         if ((rc = alloc(&vp.A, np)) || (rc = alloc(&vp.B, np)) || (rc = alloc(&vp.mask, np))) return rc;
         k_permute_pairs<<<nblk(np), 256, 0, st>>>(np, order, a0, b0, m0, vp.A, vp.B, vp.mask);
         IBIM_LAUNCH_CHECK();
@@ -940,9 +938,6 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
         }
     }
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
-    for (void *p : {(void *)cnt, (void *)off, (void *)a0, (void *)b0, (void *)m0, (void *)key0, (void *)key1,
-                    (void *)iota, (void *)order})
-        if (p) (void)hipFree(p);
     return IBIM_OK;
 }
 
@@ -1073,20 +1068,13 @@ int build_xpairs(ibim_plan *pl) {
     if (!pl->ops || !pl->gtab) return IBIM_OK;
     hipStream_t st = pl->stream;
     const int nn = pl->nnodes;
-    std::vector<void *> scratch;
-    struct Free {
-        std::vector<void *> &v;
-        ~Free() {
-            for (void *p : v) (void)hipFree(p);
-        }
-    } free_scratch{scratch};
+    ScratchScope scope(pl);  // temporaries below
     auto salloc = [&](auto **p, size_t n) -> int {
-        hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(**p));
+        hipError_t e = smalloc(pl, p, (n ? n : 1) * sizeof(**p));
         if (e != hipSuccess) {
             set_error("hipMalloc (X pairs)");
             return IBIM_ERR_NOMEM;
         }
-        scratch.push_back((void *)*p);
         return IBIM_OK;
     };
     int rc;
@@ -1242,20 +1230,13 @@ int build_m2l_operators_f16(ibim_plan *pl) {
     const VPairs &vp = pl->vp, &xp = pl->xp;
     const bool with_x = xp.npairs > 0 && pl->xops && xp.hnblocks > 0;
     const int npv = vp.npairs, npx = with_x ? xp.npairs : 0, npa = npv + npx;
-    std::vector<void *> scratch;
-    struct Free {
-        std::vector<void *> &v;
-        ~Free() {
-            for (void *p : v) (void)hipFree(p);
-        }
-    } free_scratch{scratch};
+    ScratchScope scope(pl);  // temporaries below
     auto salloc = [&](auto **p, size_t n) -> int {
-        hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(**p));
+        hipError_t e = smalloc(pl, p, (n ? n : 1) * sizeof(**p));
         if (e != hipSuccess) {
             set_error("hipMalloc (FP16 M2L set-up)");
             return IBIM_ERR_NOMEM;
         }
-        scratch.push_back((void *)*p);
         return IBIM_OK;
     };
     int rc;

@@ -350,14 +350,14 @@ inline int nblk(int64_t n, int b = kBlk) { return (int)((n + b - 1) / b); }
 
 // exclusive scan of n ints into out (n+1 entries, out[n] = total)
 static int scan_ints(ibim_plan *pl, const int *in, int *out, int n, int *total) {
+    ScratchScope scope(pl);
     size_t bytes = 0;
     IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n + 1, pl->stream));
     void *tmp = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
+    IBIM_HIP_CHECK(smalloc(pl, &tmp, bytes ? bytes : 1));
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n + 1, pl->stream);
     if (e == hipSuccess && total) e = hipMemcpyAsync(total, out + n, sizeof(int), hipMemcpyDeviceToHost, pl->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
-    (void)hipFree(tmp);
     if (e != hipSuccess) {
         set_error(std::string("scan: ") + hipGetErrorString(e));
         return IBIM_ERR_HIP;
@@ -366,20 +366,19 @@ static int scan_ints(ibim_plan *pl, const int *in, int *out, int n, int *total) 
 }
 
 static int select_flagged(ibim_plan *pl, const unsigned char *flags, int n, int **out, int *count) {
+    ScratchScope scope(pl);
     int rc = dalloc(pl, out, (size_t)n);
     if (rc) return rc;
     int *d_num = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&d_num, sizeof(int)));
+    IBIM_HIP_CHECK(smalloc(pl, &d_num, sizeof(int)));
     size_t bytes = 0;
     hipcub::CountingInputIterator<int> it(0);
     IBIM_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, bytes, it, flags, *out, d_num, n, pl->stream));
     void *tmp = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
+    IBIM_HIP_CHECK(smalloc(pl, &tmp, bytes ? bytes : 1));
     hipError_t e = hipcub::DeviceSelect::Flagged(tmp, bytes, it, flags, *out, d_num, n, pl->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(count, d_num, sizeof(int), hipMemcpyDeviceToHost, pl->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
-    (void)hipFree(tmp);
-    (void)hipFree(d_num);
     if (e != hipSuccess) {
         set_error(std::string("select: ") + hipGetErrorString(e));
         return IBIM_ERR_HIP;
@@ -388,6 +387,7 @@ static int select_flagged(ibim_plan *pl, const unsigned char *flags, int n, int 
 }
 
 int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const double *d_w) {
+    ScratchScope scope(pl);  // every temporary below
     const int n = pl->n;
     const int maxlev = pl->prm.max_level;
     hipStream_t st = pl->stream;
@@ -396,16 +396,14 @@ int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const do
     // ---- root box: bounding box of the sources (bbfmm.cpp:43-65)
     int g = std::min(1024, std::max(1, nblk(n)));
     double *part = nullptr, *bb = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&part, 6 * (size_t)g * sizeof(double)));
-    IBIM_HIP_CHECK(hipMalloc(&bb, 6 * sizeof(double)));
+    IBIM_HIP_CHECK(smalloc(pl, &part, 6 * (size_t)g * sizeof(double)));
+    IBIM_HIP_CHECK(smalloc(pl, &bb, 6 * sizeof(double)));
     k_bbox_partial<<<g, kBlk, 0, st>>>(d_xyz, n, part);
     k_bbox_final<<<1, 64, 0, st>>>(part, g, bb);
     IBIM_LAUNCH_CHECK();
     double hb[6];
     IBIM_HIP_CHECK(hipMemcpyAsync(hb, bb, sizeof(hb), hipMemcpyDeviceToHost, st));
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
-    (void)hipFree(part);
-    (void)hipFree(bb);
     for (int d = 0; d < 3; ++d) {
         if (!std::isfinite(hb[d]) || !std::isfinite(hb[3 + d])) {
             set_error("non-finite point coordinates");
@@ -424,9 +422,9 @@ int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const do
     // ---- keys and stable sort
     uint64_t *k_in = nullptr, *k_out = nullptr;
     int *i_in = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&k_in, (size_t)n * sizeof(uint64_t)));
-    IBIM_HIP_CHECK(hipMalloc(&k_out, (size_t)n * sizeof(uint64_t)));
-    IBIM_HIP_CHECK(hipMalloc(&i_in, (size_t)n * sizeof(int)));
+    IBIM_HIP_CHECK(smalloc(pl, &k_in, (size_t)n * sizeof(uint64_t)));
+    IBIM_HIP_CHECK(smalloc(pl, &k_out, (size_t)n * sizeof(uint64_t)));
+    IBIM_HIP_CHECK(smalloc(pl, &i_in, (size_t)n * sizeof(int)));
     if ((rc = dalloc(pl, &pl->perm, (size_t)n))) return rc;
     k_point_keys<<<nblk(n), kBlk, 0, st>>>(d_xyz, n, pl->root_c[0], pl->root_c[1], pl->root_c[2], pl->root_r[0],
                                            pl->root_r[1], pl->root_r[2], maxlev, k_in, i_in);
@@ -436,17 +434,14 @@ int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const do
         IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k_in, k_out, i_in, pl->perm, n, 0,
                                                           3 * maxlev, st));
         void *tmp = nullptr;
-        IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
+        IBIM_HIP_CHECK(smalloc(pl, &tmp, bytes ? bytes : 1));
         IBIM_HIP_CHECK(
             hipcub::DeviceRadixSort::SortPairs(tmp, bytes, k_in, k_out, i_in, pl->perm, n, 0, 3 * maxlev, st));
         IBIM_HIP_CHECK(hipStreamSynchronize(st));
-        (void)hipFree(tmp);
     } else {
         IBIM_HIP_CHECK(hipMemcpyAsync(k_out, k_in, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
         IBIM_HIP_CHECK(hipMemcpyAsync(pl->perm, i_in, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, st));
     }
-    (void)hipFree(k_in);
-    (void)hipFree(i_in);
 
     // ---- sorted geometry
     if ((rc = dalloc(pl, &pl->sx, n)) || (rc = dalloc(pl, &pl->sy, n)) || (rc = dalloc(pl, &pl->sz, n)) ||
@@ -465,20 +460,15 @@ int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const do
         double *ctr = nullptr;
     };
     std::vector<Level> lv;
-    std::vector<void *> tmp;
     auto talloc = [&](auto **p, size_t cnt) -> int {
-        hipError_t e = hipMalloc((void **)p, (cnt ? cnt : 1) * sizeof(**p));
+        hipError_t e = smalloc(pl, p, (cnt ? cnt : 1) * sizeof(**p));
         if (e != hipSuccess) {
             set_error(std::string("hipMalloc (tree build): ") + hipGetErrorString(e));
             return IBIM_ERR_NOMEM;
         }
-        tmp.push_back((void *)*p);
         return IBIM_OK;
     };
-    auto cleanup = [&]() {
-        for (void *p : tmp) (void)hipFree(p);
-        tmp.clear();
-    };
+    auto cleanup = [&]() {};  // scratch returns to the pool with the scope
     {
         Level r0;
         r0.nl = 1;
@@ -562,9 +552,9 @@ int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const do
     if (maxlev > 0) {
         uint64_t *tk = nullptr;
         int *iota = nullptr, *perm2 = nullptr;
-        IBIM_HIP_CHECK(hipMalloc(&tk, (size_t)n * sizeof(uint64_t)));
-        IBIM_HIP_CHECK(hipMalloc(&iota, (size_t)n * sizeof(int)));
-        IBIM_HIP_CHECK(hipMalloc(&perm2, (size_t)n * sizeof(int)));
+        IBIM_HIP_CHECK(smalloc(pl, &tk, (size_t)n * sizeof(uint64_t)));
+        IBIM_HIP_CHECK(smalloc(pl, &iota, (size_t)n * sizeof(int)));
+        IBIM_HIP_CHECK(smalloc(pl, &perm2, (size_t)n * sizeof(int)));
         k_leaf_keys<<<nblk(nn), kBlk, 0, st>>>(nn, pl->flags, pl->nkey, pl->level, pl->sbeg, pl->send, pl->perm,
                                                maxlev, tk);
         k_iota<<<nblk(n), kBlk, 0, st>>>(n, iota);
@@ -572,19 +562,14 @@ int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const do
         size_t bytes = 0;
         IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, tk, k_out, iota, perm2, n, 0, 3 * maxlev, st));
         void *tmp2 = nullptr;
-        IBIM_HIP_CHECK(hipMalloc(&tmp2, bytes ? bytes : 1));
+        IBIM_HIP_CHECK(smalloc(pl, &tmp2, bytes ? bytes : 1));
         IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp2, bytes, tk, k_out, iota, perm2, n, 0, 3 * maxlev, st));
         IBIM_HIP_CHECK(hipMemcpyAsync(pl->perm, perm2, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, st));
         k_gather_geometry<<<nblk(n), kBlk, 0, st>>>(pl->perm, n, d_xyz, d_nrm, d_w, pl->sx, pl->sy, pl->sz, pl->snx,
                                                     pl->sny, pl->snz, pl->sw);
         IBIM_LAUNCH_CHECK();
         IBIM_HIP_CHECK(hipStreamSynchronize(st));
-        (void)hipFree(tmp2);
-        (void)hipFree(tk);
-        (void)hipFree(iota);
-        (void)hipFree(perm2);
     }
-    (void)hipFree(k_out);
 
     // ---- interaction lists (count, scan, fill)
     TreeView tv{pl->ctr, pl->d_lvl_rad, pl->level, pl->parent, pl->first_child, pl->flags};
@@ -594,14 +579,14 @@ int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const do
         mx[d] = pl->root_c[d] + pl->root_r[d];
     }
     int *counts = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&counts, 4 * ((size_t)nn + 1) * sizeof(int)));
+    IBIM_HIP_CHECK(smalloc(pl, &counts, 4 * ((size_t)nn + 1) * sizeof(int)));
     IBIM_HIP_CHECK(hipMemsetAsync(counts, 0, 4 * ((size_t)nn + 1) * sizeof(int), st));
     k_lists<0><<<nblk(nn, 64), 64, 0, st>>>(tv, nn, mn[0], mn[1], mn[2], mx[0], mx[1], mx[2], counts, nullptr,
                                             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     IBIM_LAUNCH_CHECK();
     // counts laid out [4][nn]; scan each kind separately (needs n+1 input entries)
     int *cnt_pad = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&cnt_pad, ((size_t)nn + 1) * sizeof(int)));
+    IBIM_HIP_CHECK(smalloc(pl, &cnt_pad, ((size_t)nn + 1) * sizeof(int)));
     for (int k = 0; k < 4; ++k) {
         if ((rc = dalloc(pl, &pl->loff[k], (size_t)nn + 1))) return rc;
         IBIM_HIP_CHECK(hipMemsetAsync(cnt_pad, 0, ((size_t)nn + 1) * sizeof(int), st));
@@ -612,8 +597,6 @@ int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const do
         pl->ltot[k] = tot;
         if ((rc = dalloc(pl, &pl->litem[k], (size_t)tot))) return rc;
     }
-    (void)hipFree(cnt_pad);
-    (void)hipFree(counts);
     k_lists<1><<<nblk(nn, 64), 64, 0, st>>>(tv, nn, mn[0], mn[1], mn[2], mx[0], mx[1], mx[2], nullptr, pl->loff[0],
                                             pl->loff[1], pl->loff[2], pl->loff[3], pl->litem[0], pl->litem[1],
                                             pl->litem[2], pl->litem[3]);
@@ -621,7 +604,7 @@ int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const do
 
     // ---- work lists for the points as their own targets
     unsigned char *fl = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&fl, (size_t)nn));
+    IBIM_HIP_CHECK(smalloc(pl, &fl, (size_t)nn));
     if ((rc = dalloc(pl, &pl->need_local, (size_t)nn))) return rc;
     k_flag_nonempty_leaf<<<nblk(nn), kBlk, 0, st>>>(pl->flags, nn, fl);
     IBIM_LAUNCH_CHECK();
@@ -629,7 +612,6 @@ int build_tree(ibim_plan *pl, const double *d_xyz, const double *d_nrm, const do
     k_flag_nonempty_nonroot<<<nblk(nn), kBlk, 0, st>>>(pl->flags, nn, fl, pl->need_local);
     IBIM_LAUNCH_CHECK();
     if ((rc = select_flagged(pl, fl, nn, &pl->m2l_work, &pl->n_m2l_work))) return rc;
-    (void)hipFree(fl);
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
 
     // deepest non-empty leaf
