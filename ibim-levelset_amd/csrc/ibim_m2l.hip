@@ -387,40 +387,61 @@ __global__ __launch_bounds__(64) void k_m_to_half(const double *__restrict__ M, 
     for (int k = threadIdx.x; k < KP; k += 64) Mh[a * KP + k] = k < R4 ? (_Float16)ldexp(src[k], -e) : (_Float16)0;
 }
 
-// one block of 64 per row i of an unfolded FP16 operator: slot s holds the
-// key (class c, reflection mask m) and U[i][j] = so * si * K_c[s(i)][g(j)]
-// 2^-esig, where g and s are the mask's gather and scatter permutations
-// (reflections, their own inverses) and so, si the output / input signs,
-// so that L_b += U M_a needs neither permutation
-__global__ __launch_bounds__(64) void k_unfold_half(const int *__restrict__ slot_key, const double *__restrict__ ops,
-                                                    int MRP, int KRP, int R4, const int *__restrict__ gtab,
-                                                    const int *__restrict__ stab, _Float16 *__restrict__ ops16u,
-                                                    int *__restrict__ esig) {
-    const int i = blockIdx.x, sl = blockIdx.y;
-    const int key = slot_key[sl], c = key >> 3, m = key & 7;
-    _Float16 *dst = ops16u + ((size_t)sl * MRP + i) * KRP;
-    if (i >= R4) {
-        for (int j = threadIdx.x; j < KRP; j += 64) dst[j] = (_Float16)0;
-        if (threadIdx.x == 0) esig[(size_t)sl * MRP + i] = 0;
+// Unfolded FP16 operators: slot s holds the key (class c, reflection mask m)
+// and U[i][j] = so * si * K_c[s(i)][g(j)] 2^-esig[i], where g and s are the
+// mask's gather and scatter permutations (reflections, their own inverses)
+// and so, si the output / input signs, so that L_b += U M_a needs neither
+// permutation.  One block of 64 per (class, class row R): the row is read
+// once into LDS and written, permuted and signed, as row i = s(R) of every
+// slot of the class (the row's scale exponent does not depend on the mask).
+// cls_slots[8 q + m] is the slot of (cls_list[q], m), or -1.
+__global__ __launch_bounds__(64) void k_unfold_half(const int *__restrict__ cls_list,
+                                                    const int *__restrict__ cls_slots,
+                                                    const double *__restrict__ ops, int MRP, int KRP, int R4,
+                                                    const int *__restrict__ gtab, const int *__restrict__ stab,
+                                                    _Float16 *__restrict__ ops16u, int *__restrict__ esig) {
+    extern __shared__ double srow[];  // KRP doubles
+    const int R = blockIdx.x, q = blockIdx.y;
+    const int c = cls_list[q];
+    const int *sl8 = cls_slots + 8 * q;
+    if (R >= R4) {  // padding row: zero in every slot
+        for (int m = 0; m < 8; ++m) {
+            const int sl = sl8[m];
+            if (sl < 0) continue;
+            _Float16 *dst = ops16u + ((size_t)sl * MRP + R) * KRP;
+            for (int j = threadIdx.x; j < KRP; j += 64) dst[j] = (_Float16)0;
+            if (threadIdx.x == 0) esig[(size_t)sl * MRP + R] = 0;
+        }
         return;
     }
-    const int row = stab[m * R4 + i] & 0x7fffffff;
-    const bool nout = stab[m * R4 + row] < 0;
-    const double *src = ops + ((size_t)c * MRP + row) * KRP;
+    const double *src = ops + ((size_t)c * MRP + R) * KRP;
     double mx = 0.0;
-    for (int k = threadIdx.x; k < R4; k += 64) mx = fmax(mx, fabs(src[k]));
+    for (int k = threadIdx.x; k < R4; k += 64) {
+        const double v = src[k];
+        srow[k] = v;
+        mx = fmax(mx, fabs(v));
+    }
     for (int o = 32; o; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
     const int e = scale_exp(mx);
-    if (threadIdx.x == 0) esig[(size_t)sl * MRP + i] = e;
-    for (int j = threadIdx.x; j < KRP; j += 64) {
-        _Float16 v = (_Float16)0;
-        if (j < R4) {
-            const int k = gtab[m * R4 + j] & 0x7fffffff;
-            const bool nin = gtab[m * R4 + k] < 0;
-            const double x = ldexp(src[k], -e);
-            v = (_Float16)(nout != nin ? -x : x);
+    __syncthreads();
+    for (int m = 0; m < 8; ++m) {
+        const int sl = sl8[m];
+        if (sl < 0) continue;
+        const int sr = stab[m * R4 + R];
+        const int i = sr & 0x7fffffff;
+        const bool nout = sr < 0;
+        _Float16 *dst = ops16u + ((size_t)sl * MRP + i) * KRP;
+        if (threadIdx.x == 0) esig[(size_t)sl * MRP + i] = e;
+        for (int j = threadIdx.x; j < KRP; j += 64) {
+            _Float16 v = (_Float16)0;
+            if (j < R4) {
+                const int k = gtab[m * R4 + j] & 0x7fffffff;
+                const bool nin = gtab[m * R4 + k] < 0;
+                const double x = ldexp(srow[k], -e);
+                v = (_Float16)(nout != nin ? -x : x);
+            }
+            dst[j] = v;
         }
-        dst[j] = v;
     }
 }
 
@@ -1259,10 +1280,8 @@ int build_m2l_operators_f16(ibim_plan *pl) {
     if (with_x) slots(xp.cls_off, nsv, xkeyslot, xslot_key);
     const int nsx = (int)xslot_key.size();
     const int ns = nsv + nsx;
-    int *d_slot_key = nullptr, *d_xslot_key = nullptr;
     if ((rc = dalloc(pl, &pl->keyslot, std::max<size_t>(1, keyslot.size()))) ||
         (rc = dalloc(pl, &pl->xkeyslot, std::max<size_t>(1, xkeyslot.size()))) ||
-        (rc = salloc(&d_slot_key, (size_t)nsv)) || (rc = salloc(&d_xslot_key, (size_t)nsx)) ||
         (rc = dalloc(pl, &pl->ops16, std::max<size_t>(1, (size_t)ns * pl->MRP * pl->KRP))) ||
         (rc = dalloc(pl, &pl->esig, std::max<size_t>(1, (size_t)ns * pl->MRP))))
         return rc;
@@ -1272,19 +1291,31 @@ int build_m2l_operators_f16(ibim_plan *pl) {
     if (!xkeyslot.empty())
         IBIM_HIP_CHECK(hipMemcpyAsync(pl->xkeyslot, xkeyslot.data(), xkeyslot.size() * sizeof(int),
                                       hipMemcpyHostToDevice, st));
-    if (nsv > 0) {
-        IBIM_HIP_CHECK(hipMemcpyAsync(d_slot_key, slot_key.data(), nsv * sizeof(int), hipMemcpyHostToDevice, st));
-        k_unfold_half<<<dim3(pl->MRP, nsv), 64, 0, st>>>(d_slot_key, pl->ops, pl->MRP, pl->KRP, R4, pl->gtab,
-                                                         pl->stab, pl->ops16, pl->esig);
+    // per class with slots: its 8 (mask) slots, -1 where the key has no pairs
+    auto unfold = [&](const std::vector<int> &ks, const double *ops) -> int {
+        std::vector<int> cls, cslots;
+        for (size_t k = 0; k < ks.size(); ++k) {
+            if (ks[k] < 0) continue;
+            const int c = (int)(k >> 3);
+            if (cls.empty() || cls.back() != c) {
+                cls.push_back(c);
+                cslots.insert(cslots.end(), 8, -1);
+            }
+            cslots[8 * (cls.size() - 1) + (k & 7)] = ks[k];
+        }
+        if (cls.empty()) return IBIM_OK;
+        int *d_cls = nullptr, *d_cslots = nullptr;
+        IBIM_HIP_CHECK(smalloc(pl, &d_cls, cls.size() * sizeof(int)));
+        IBIM_HIP_CHECK(smalloc(pl, &d_cslots, cslots.size() * sizeof(int)));
+        IBIM_HIP_CHECK(hipMemcpyAsync(d_cls, cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        IBIM_HIP_CHECK(
+            hipMemcpyAsync(d_cslots, cslots.data(), cslots.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        k_unfold_half<<<dim3(pl->MRP, (unsigned)cls.size()), 64, pl->KRP * sizeof(double), st>>>(
+            d_cls, d_cslots, ops, pl->MRP, pl->KRP, R4, pl->gtab, pl->stab, pl->ops16, pl->esig);
         IBIM_LAUNCH_CHECK();
-    }
-    if (nsx > 0) {
-        IBIM_HIP_CHECK(hipMemcpyAsync(d_xslot_key, xslot_key.data(), nsx * sizeof(int), hipMemcpyHostToDevice, st));
-        k_unfold_half<<<dim3(pl->MRP, nsx), 64, 0, st>>>(d_xslot_key, pl->xops, pl->MRP, pl->KRP, R4, pl->gtab,
-                                                         pl->stab, pl->ops16 + (size_t)nsv * pl->MRP * pl->KRP,
-                                                         pl->esig + (size_t)nsv * pl->MRP);
-        IBIM_LAUNCH_CHECK();
-    }
+        return IBIM_OK;
+    };
+    if ((rc = unfold(keyslot, pl->ops)) || (with_x && (rc = unfold(xkeyslot, pl->xops)))) return rc;
     // rows of Yv per target node: a stable sort of the target boxes of the
     // V pairs followed by the X pairs
     const int nn = pl->nnodes;
