@@ -509,7 +509,8 @@ __global__ __launch_bounds__(64) void k_p2p_unit(
     const int2 *__restrict__ ranges, PBConst kc, double *__restrict__ F, int nt, double *__restrict__ part) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using T4 = typename Vec4<T>::type;
-    __shared__ T4 s0[64], s1[64];  // {x, y, z, q_psi}, {q_x, q_y, q_z, q_psi / (4 pi)}
+    __shared__ T4 sbuf[2][64];  // {x, y, z, q_psi}, {q_x, q_y, q_z, q_psi / (4 pi)} per source
+    T4 *s0 = sbuf[0], *s1 = sbuf[1];
     __shared__ double red[4][64];
     const int lane = threadIdx.x;
     const int4 a = ua[blockIdx.x], c = ub[blockIdx.x];
@@ -549,14 +550,18 @@ __global__ __launch_bounds__(64) void k_p2p_unit(
           if (active) {
             // FP32 PB kernels: F0 = f0a / (4 pi) - f0b
             float ga = 0, gb = 0, gx = 0, gy = 0, gz = 0, ha = 0, hb = 0, hx = 0, hy = 0, hz = 0;
-            int k = sl;
-            for (; k + nsl < cnt; k += 2 * nsl) {
-                const float4 u = s0[k], v = s1[k], u2 = s0[k + nsl], v2 = s1[k + nsl];
+            // sources k = sl, sl + nsl, ... two per step; u at s0[k], v at
+            // s1[k] = s0[k + 64]
+            const float4 *pu = (const float4 *)s0 + sl;
+            const float4 *pend = (const float4 *)s0 + max(cnt - nsl, 0);
+            for (; pu < pend; pu += 2 * nsl) {
+                const float4 u = pu[0], v = pu[64], u2 = pu[nsl], v2 = pu[nsl + 64];
                 pair_pb32(u.x - (float)px, u.y - (float)py, u.z - (float)pz, v.x, v.y, v.z, v.w, k32, ga, gb, gx, gy,
                           gz);
                 pair_pb32(u2.x - (float)px, u2.y - (float)py, u2.z - (float)pz, v2.x, v2.y, v2.z, v2.w, k32, ha, hb,
                           hx, hy, hz);
             }
+            const int k = (int)(pu - (const float4 *)s0);
             if (k < cnt) {
                 const float4 u = s0[k], v = s1[k];
                 pair_pb32(u.x - (float)px, u.y - (float)py, u.z - (float)pz, v.x, v.y, v.z, v.w, k32, ga, gb, gx, gy,

@@ -191,25 +191,24 @@ __device__ __forceinline__ void pair_accum(T dx, T dy, T dz, T qx, T qy, T qz, T
 // (1 - e) / r (4 pi k2reg below the vacant radius), a3 = (1 - e (t+1)) / r^3
 // and b3 = (e (t^2 + 3t + 3) - 3) / r^5 = (e (t+1)(t+2) + e - 3) / r^5.
 // The hardware rsq and exp2 are used without range fix-ups: r^2 = 0 and r
-// below the vacant radius are masked (1/r and r replaced by 0), and exp2 of
-// the non-positive argument -kappa r log2(e) can only underflow to 0.
+// below the vacant radius (r^2 < vr^2) are masked (1/r, hence r = r^2 / r,
+// replaced by 0), and exp2 of the non-positive argument -kappa r log2(e) can
+// only underflow to 0.
 struct PBConst32 {
-    float kappa, ck, eps, inv_eps, vr, k2reg4pi;  // ck = -kappa log2(e)
+    float kappa, ck, eps, inv_eps, vr2, k2reg4pi;  // ck = -kappa log2(e), vr2 = vacant radius squared
 };
 __host__ __device__ inline PBConst32 pb_const32(const PBConst &k) {
     const double pi4 = 1.0 / k.inv4pi;
     return PBConst32{(float)k.kappa, (float)(-k.kappa * 1.4426950408889634), (float)k.eps, (float)k.inv_eps,
-                     (float)k.vr, (float)(k.k2reg * pi4)};
+                     (float)(k.vr * k.vr), (float)(k.k2reg * pi4)};
 }
 __device__ __forceinline__ void pair_pb32(float dx, float dy, float dz, float qx, float qy, float qz, float qs,
                                           const PBConst32 &k, float &f0a, float &f0b, float &fx, float &fy,
                                           float &fz) {
     const float r2 = dx * dx + dy * dy + dz * dz;
-    const float ir0 = __builtin_amdgcn_rsqf(r2);
-    const float r0 = r2 * ir0;
-    const bool reg = !(r0 >= k.vr);  // also catches r2 == 0 (0 * inf)
-    const float ir = reg ? 0.0f : ir0;
-    const float r = reg ? 0.0f : r0;
+    const bool reg = !(r2 >= k.vr2);  // also r2 == 0
+    const float ir = reg ? 0.0f : __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * ir;
     const float ir2 = ir * ir, ir3 = ir2 * ir;
     const float t = k.kappa * r;
     const float e = __builtin_amdgcn_exp2f(k.ck * r);
@@ -224,9 +223,9 @@ __device__ __forceinline__ void pair_pb32(float dx, float dy, float dz, float qx
     f0a = fmaf(g1, dq, f0a);
     f0b = fmaf(k2, qs, f0b);
     const float s = fmaf(b3, dq, g4 * qs);
-    fx += a3 * qx + dx * s;
-    fy += a3 * qy + dy * s;
-    fz += a3 * qz + dz * s;
+    fx = fmaf(dx, s, fmaf(a3, qx, fx));
+    fy = fmaf(dy, s, fmaf(a3, qy, fy));
+    fz = fmaf(dz, s, fmaf(a3, qz, fz));
 }
 
 template <bool POLY, typename T>
