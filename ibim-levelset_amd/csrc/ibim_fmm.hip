@@ -572,6 +572,28 @@ __global__ __launch_bounds__(64) void k_p2p_unit(
             fy += (double)(gy + hy);
             fz += (double)(gz + hz);
           }
+        } else if constexpr (!MIXED && !POLY) {
+          if (active) {
+            // FP64 PB kernels: F0 = f0a / (4 pi) - f0b
+            const double k2r = kc.k2reg / kc.inv4pi;
+            double ga = 0, gb = 0, gx = 0, gy = 0, gz = 0, ha = 0, hb = 0, hx = 0, hy = 0, hz = 0;
+            const double4 *pu = (const double4 *)s0 + sl;
+            const double4 *pend = (const double4 *)s0 + max(cnt - nsl, 0);
+            for (; pu < pend; pu += 2 * nsl) {
+                const double4 u = pu[0], v = pu[64], u2 = pu[nsl], v2 = pu[nsl + 64];
+                pair_pb64(u.x - px, u.y - py, u.z - pz, v.x, v.y, v.z, v.w, kc, k2r, ga, gb, gx, gy, gz);
+                pair_pb64(u2.x - px, u2.y - py, u2.z - pz, v2.x, v2.y, v2.z, v2.w, kc, k2r, ha, hb, hx, hy, hz);
+            }
+            const int k = (int)(pu - (const double4 *)s0);
+            if (k < cnt) {
+                const double4 u = s0[k], v = s1[k];
+                pair_pb64(u.x - px, u.y - py, u.z - pz, v.x, v.y, v.z, v.w, kc, k2r, ga, gb, gx, gy, gz);
+            }
+            f0 += (ga + ha) * kc.inv4pi - (gb + hb);
+            fx += gx + hx;
+            fy += gy + hy;
+            fz += gz + hz;
+          }
         } else if (active) {
             T g0 = 0, gx = 0, gy = 0, gz = 0, h0 = 0, hx = 0, hy = 0, hz = 0;
             int k = sl;

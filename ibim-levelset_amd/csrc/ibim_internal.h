@@ -228,6 +228,36 @@ __device__ __forceinline__ void pair_pb32(float dx, float dy, float dz, float qx
     fz = fmaf(dz, s, fmaf(a3, qz, fz));
 }
 
+// FP64 near-field form of the same arithmetic (1/(4 pi) out of the sums,
+// shared 1/r^3 product, fused accumulation), for the near-field work units.
+// The regularisation test stays r < vacant radius on the square-rooted
+// distance, as in pair_accum_pb (electric.cpp:58-101).
+__device__ __forceinline__ void pair_pb64(double dx, double dy, double dz, double qx, double qy, double qz,
+                                          double qs, const PBConst &k, double k2reg4pi, double &f0a, double &f0b,
+                                          double &fx, double &fy, double &fz) {
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double r = sqrt(r2);
+    const bool reg = r < k.vr;
+    const double ir = reg ? 0.0 : recip(r);  // the discarded value may be inf or NaN (r = 0)
+    const double ir2 = ir * ir, ir3 = ir2 * ir;
+    const double t = k.kappa * r;
+    const double e = exp(-t);
+    const double et1 = fma(e, t, e);
+    const double m = et1 * ir3;
+    const double a3 = ir3 - m;
+    const double g1 = fma(k.eps, m, -ir3);
+    const double g4 = fma(k.inv_eps, m, -ir3);
+    const double k2 = reg ? k2reg4pi : (1.0 - e) * ir;
+    const double b3 = (fma(et1, t + 2.0, e) - 3.0) * ir3 * ir2;
+    const double dq = dx * qx + dy * qy + dz * qz;
+    f0a = fma(g1, dq, f0a);
+    f0b = fma(k2, qs, f0b);
+    const double s = fma(b3, dq, g4 * qs);
+    fx = fma(dx, s, fma(a3, qx, fx));
+    fy = fma(dy, s, fma(a3, qy, fy));
+    fz = fma(dz, s, fma(a3, qz, fz));
+}
+
 template <bool POLY, typename T>
 __device__ __forceinline__ void pair_accum_m(T dx, T dy, T dz, T qx, T qy, T qz, T qp, const PBConst &k, T &f0,
                                              T &fx, T &fy, T &fz) {
