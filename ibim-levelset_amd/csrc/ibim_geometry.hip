@@ -34,6 +34,7 @@ namespace {
 struct GridDims {
     int nx, ny, nz;
     double sx, sy, sz, h;
+    double rh;  // 1 / h
 };
 
 inline int nblk(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
@@ -54,9 +55,13 @@ __device__ __forceinline__ void weno(const double *__restrict__ g, const GridDim
         const int o = l - 3;
         w[l] = gget(g, d, i + (dir == 0 ? o : 0), j + (dir == 1 ? o : 0), k + (dir == 2 ? o : 0));
     }
+    // the divisions of the reference are multiplications by constant
+    // reciprocals and Newton-refined reciprocals here (each within an ulp;
+    // an FP64 division costs about four times as many instructions)
     const double EPS_LS = 1e-6;
+    const double r12 = 1.0 / 12.0, r3 = 1.0 / 3.0, r6 = 1.0 / 6.0, rh = d.rh;
     double a = w[1] - w[0], b = w[2] - w[1], c = w[3] - w[2], dd = w[4] - w[3], e = w[5] - w[4], f = w[6] - w[5];
-    const double df = (-b + 7.0 * (c + dd) - e) / 12.0;
+    const double df = (-b + 7.0 * (c + dd) - e) * r12;
     a = b - a;
     b = c - b;
     c = dd - c;
@@ -65,22 +70,22 @@ __device__ __forceinline__ void weno(const double *__restrict__ g, const GridDim
     double is0 = 13.0 * (e - dd) * (e - dd) + 3.0 * (e - 3.0 * dd) * (e - 3.0 * dd);
     double is1 = 13.0 * (dd - c) * (dd - c) + 3.0 * (dd + c) * (dd + c);
     double is2 = 13.0 * (c - b) * (c - b) + 3.0 * (3.0 * c - b) * (3.0 * c - b);
-    double a0 = 1.0 / ((EPS_LS + is0) * (EPS_LS + is0));
-    double a1 = 6.0 / ((EPS_LS + is1) * (EPS_LS + is1));
-    double a2 = 3.0 / ((EPS_LS + is2) * (EPS_LS + is2));
-    double w1 = 1.0 / (a0 + a1 + a2);
+    double a0 = recip((EPS_LS + is0) * (EPS_LS + is0));
+    double a1 = 6.0 * recip((EPS_LS + is1) * (EPS_LS + is1));
+    double a2 = 3.0 * recip((EPS_LS + is2) * (EPS_LS + is2));
+    double w1 = recip(a0 + a1 + a2);
     double w0 = a0 * w1, w2 = a2 * w1;
-    up = (df + w0 * (e - 2 * dd + c) / 3.0 + (w2 - 0.5) * (dd - 2.0 * c + b) / 6.0) / d.h;
+    up = (df + w0 * (e - 2 * dd + c) * r3 + (w2 - 0.5) * (dd - 2.0 * c + b) * r6) * rh;
     is0 = 13.0 * (a - b) * (a - b) + 3.0 * (a - 3.0 * b) * (a - 3.0 * b);
     is1 = 13.0 * (b - c) * (b - c) + 3.0 * (b + c) * (b + c);
     is2 = 13.0 * (c - dd) * (c - dd) + 3.0 * (3.0 * c - dd) * (3.0 * c - dd);
-    a0 = 1.0 / ((EPS_LS + is0) * (EPS_LS + is0));
-    a1 = 6.0 / ((EPS_LS + is1) * (EPS_LS + is1));
-    a2 = 3.0 / ((EPS_LS + is2) * (EPS_LS + is2));
-    w1 = 1.0 / (a0 + a1 + a2);
+    a0 = recip((EPS_LS + is0) * (EPS_LS + is0));
+    a1 = 6.0 * recip((EPS_LS + is1) * (EPS_LS + is1));
+    a2 = 3.0 * recip((EPS_LS + is2) * (EPS_LS + is2));
+    w1 = recip(a0 + a1 + a2);
     w0 = a0 * w1;
     w2 = a2 * w1;
-    un = (df - w0 * (a - 2 * b + c) / 3.0 - (w2 - 0.5) * (b - 2 * c + dd) / 6.0) / d.h;
+    un = (df - w0 * (a - 2 * b + c) * r3 - (w2 - 0.5) * (b - 2 * c + dd) * r6) * rh;
 }
 
 // Godunov norm getNorm(A, B) = sqrt(sum max(A+, B-)^2) (levelset.cpp:369-376)
@@ -179,8 +184,8 @@ __global__ __launch_bounds__(256) void k_ls_stage(const double *base, const doub
     const double b = base[I];
     double v;
     if (stage == 1) v = b - T;
-    else if (stage == 2) v = (3 * b + src[I] - T) / 4.0;
-    else v = (b + 2 * (src[I] - T)) / 3.0;
+    else if (stage == 2) v = (3 * b + src[I] - T) * 0.25;
+    else v = (b + 2 * (src[I] - T)) * (1.0 / 3.0);
     out[I] = v;
 }
 
@@ -189,21 +194,48 @@ __global__ void k_scale(double *__restrict__ g, int64_t n, double s) {
     if (I < n) g[I] *= s;
 }
 
-// countGradient (levelset.cpp:473-507): tube points, bad points, sum of errors
+// countGradient (levelset.cpp:473-507): tube points, bad points, sum of
+// errors.  Reduced per block (wave shuffles, then LDS), one atomic per block
+// and counter: the counts are exact integers, the error sum's order is free.
 __global__ __launch_bounds__(256) void k_count(const double *__restrict__ g, GridDims d, double tube, double thres,
                                                unsigned long long *__restrict__ counts, double *__restrict__ err) {
     const int64_t n = (int64_t)d.nx * d.ny * d.nz;
     const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= n || !(fabs(g[I]) < tube)) return;
-    int i, j, k;
-    index3(I, d, i, j, k);
-    double up[3], un[3];
+    unsigned int c0 = 0, c1 = 0;
+    double e = 0.0;
+    if (I < n && fabs(g[I]) < tube) {
+        int i, j, k;
+        index3(I, d, i, j, k);
+        double up[3], un[3];
 #pragma unroll
-    for (int dir = 0; dir < 3; ++dir) weno(g, d, i, j, k, dir, up[dir], un[dir]);
-    const double nr = godunov(un, up);
-    atomicAdd(&counts[0], 1ull);
-    if (fabs(nr - 1.0) > thres) atomicAdd(&counts[1], 1ull);
-    atomicAdd(err, fabs(nr - 1.0));
+        for (int dir = 0; dir < 3; ++dir) weno(g, d, i, j, k, dir, up[dir], un[dir]);
+        const double nr = godunov(un, up);
+        c0 = 1;
+        c1 = fabs(nr - 1.0) > thres ? 1 : 0;
+        e = fabs(nr - 1.0);
+    }
+    for (int o = 32; o; o >>= 1) {
+        c0 += __shfl_xor(c0, o);
+        c1 += __shfl_xor(c1, o);
+        e += __shfl_xor(e, o);
+    }
+    __shared__ unsigned int s0[4], s1[4];
+    __shared__ double se[4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s0[w] = c0;
+        s1[w] = c1;
+        se[w] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int t0 = s0[0] + s0[1] + s0[2] + s0[3], t1 = s1[0] + s1[1] + s1[2] + s1[3];
+        if (t0) {
+            atomicAdd(&counts[0], (unsigned long long)t0);
+            if (t1) atomicAdd(&counts[1], (unsigned long long)t1);
+            atomicAdd(err, (se[0] + se[1]) + (se[2] + se[3]));
+        }
+    }
 }
 
 // exterior flood fill (levelset.cpp:519-599): a point is reached when a
@@ -466,6 +498,7 @@ int ibim_surface_from_atoms(const ibim_geom_params *gp, int64_t m, const double 
     if (rc) return rc;
     LsRun L;
     L.d.h = gp->h;
+    L.d.rh = 1.0 / gp->h;
     if (gp->dims[0] > 0 && gp->dims[1] > 0 && gp->dims[2] > 0) {
         L.d.nx = gp->dims[0];
         L.d.ny = gp->dims[1];
@@ -558,7 +591,7 @@ int ibim_surface_from_grid(const ibim_geom_params *gp, const double *phi, ibim_s
     int rc = check_geom(gp);
     if (rc) return rc;
     LsRun L;
-    L.d = GridDims{gp->dims[0], gp->dims[1], gp->dims[2], gp->lo[0], gp->lo[1], gp->lo[2], gp->h};
+    L.d = GridDims{gp->dims[0], gp->dims[1], gp->dims[2], gp->lo[0], gp->lo[1], gp->lo[2], gp->h, 1.0 / gp->h};
     IBIM_HIP_CHECK(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
     if ((rc = alloc_grid(L, false))) {
         free_grid(L);
