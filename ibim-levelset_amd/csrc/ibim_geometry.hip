@@ -195,10 +195,11 @@ __global__ void k_scale(double *__restrict__ g, int64_t n, double s) {
 }
 
 // countGradient (levelset.cpp:473-507): tube points, bad points, sum of
-// errors.  Reduced per block (wave shuffles, then LDS), one atomic per block
-// and counter: the counts are exact integers, the error sum's order is free.
+// errors.  Each block writes its sums (wave shuffles, then LDS) to its own
+// slot; k_count_sum adds the slots in block order (exact counts, a
+// deterministic error sum).
 __global__ __launch_bounds__(256) void k_count(const double *__restrict__ g, GridDims d, double tube, double thres,
-                                               unsigned long long *__restrict__ counts, double *__restrict__ err) {
+                                               unsigned int *__restrict__ pc, double *__restrict__ pe) {
     const int64_t n = (int64_t)d.nx * d.ny * d.nz;
     const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned int c0 = 0, c1 = 0;
@@ -229,34 +230,102 @@ __global__ __launch_bounds__(256) void k_count(const double *__restrict__ g, Gri
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned int t0 = s0[0] + s0[1] + s0[2] + s0[3], t1 = s1[0] + s1[1] + s1[2] + s1[3];
-        if (t0) {
-            atomicAdd(&counts[0], (unsigned long long)t0);
-            if (t1) atomicAdd(&counts[1], (unsigned long long)t1);
-            atomicAdd(err, (se[0] + se[1]) + (se[2] + se[3]));
+        pc[2 * (size_t)blockIdx.x] = s0[0] + s0[1] + s0[2] + s0[3];
+        pc[2 * (size_t)blockIdx.x + 1] = s1[0] + s1[1] + s1[2] + s1[3];
+        pe[blockIdx.x] = (se[0] + se[1]) + (se[2] + se[3]);
+    }
+}
+
+// one block of 1024: the per-block slots of k_count summed in a fixed order
+__global__ __launch_bounds__(1024) void k_count_sum(int nb, const unsigned int *__restrict__ pc,
+                                                    const double *__restrict__ pe,
+                                                    unsigned long long *__restrict__ counts, double *__restrict__ err) {
+    unsigned long long c0 = 0, c1 = 0;
+    double e = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 1024) {
+        c0 += pc[2 * (size_t)b];
+        c1 += pc[2 * (size_t)b + 1];
+        e += pe[b];
+    }
+    for (int o = 32; o; o >>= 1) {
+        c0 += __shfl_xor(c0, o);
+        c1 += __shfl_xor(c1, o);
+        e += __shfl_xor(e, o);
+    }
+    __shared__ unsigned long long s0[16], s1[16];
+    __shared__ double se[16];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s0[w] = c0;
+        s1[w] = c1;
+        se[w] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t0 = 0, t1 = 0;
+        double te = 0.0;
+        for (int q = 0; q < 16; ++q) {
+            t0 += s0[q];
+            t1 += s1[q];
+            te += se[q];
         }
+        counts[0] = t0;
+        counts[1] = t1;
+        *err = te;
     }
 }
 
 // exterior flood fill (levelset.cpp:519-599): a point is reached when a
-// reached neighbour has phi > -tube
-__global__ void k_flood(const double *__restrict__ g, GridDims d, unsigned char *vis, double neg_tube,
-                        int *__restrict__ changed) {
-    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
-    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= n || vis[I]) return;
-    int i, j, k;
-    index3(I, d, i, j, k);
-    const int64_t sI = (int64_t)d.ny * d.nz, sJ = d.nz;
-    bool reach = false;
-    if (i > 0) reach |= vis[I - sI] && g[I - sI] > neg_tube;
-    if (i < d.nx - 1) reach |= vis[I + sI] && g[I + sI] > neg_tube;
-    if (j > 0) reach |= vis[I - sJ] && g[I - sJ] > neg_tube;
-    if (j < d.ny - 1) reach |= vis[I + sJ] && g[I + sJ] > neg_tube;
-    if (k > 0) reach |= vis[I - 1] && g[I - 1] > neg_tube;
-    if (k < d.nz - 1) reach |= vis[I + 1] && g[I + 1] > neg_tube;
-    if (reach) {
-        vis[I] = 1;
+// reached neighbour has phi > -tube.  Computed on 8 x 8 x 8 tiles: a block
+// loads its tile and a one-cell halo (reached flags and phi > -tube) into LDS
+// and propagates inside the tile until nothing changes, so one launch
+// carries the front across a whole tile; the host repeats launches until no
+// tile changes.  The reached set is the fixpoint of the rule, whatever the
+// order of updates.
+__global__ __launch_bounds__(512) void k_flood_tile(const double *__restrict__ g, GridDims d, unsigned char *vis,
+                                                    double neg_tube, int *__restrict__ changed) {
+    constexpr int T = 8, H = T + 2;
+    __shared__ unsigned char sv[H * H * H], sp[H * H * H];
+    __shared__ int chg;
+    const int ntx = (d.nx + T - 1) / T, nty = (d.ny + T - 1) / T, ntz = (d.nz + T - 1) / T;
+    const int b = blockIdx.x;
+    const int tk = b % ntz, tj = (b / ntz) % nty, ti = b / (ntz * nty);
+    if (ti >= ntx) return;
+    const int i0 = ti * T - 1, j0 = tj * T - 1, k0 = tk * T - 1;
+    for (int c = threadIdx.x; c < H * H * H; c += blockDim.x) {
+        const int ck = c % H, cj = (c / H) % H, ci = c / (H * H);
+        const int i = i0 + ci, j = j0 + cj, k = k0 + ck;
+        unsigned char v = 0, pass = 0;
+        if (i >= 0 && i < d.nx && j >= 0 && j < d.ny && k >= 0 && k < d.nz) {
+            const int64_t I = ((int64_t)i * d.ny + j) * d.nz + k;
+            v = vis[I];
+            pass = g[I] > neg_tube ? 1 : 0;
+        }
+        sv[c] = v;
+        sp[c] = pass;
+    }
+    // this thread's interior cell
+    const int kk = threadIdx.x % T, jj = (threadIdx.x / T) % T, ii = threadIdx.x / (T * T);
+    const int i = ti * T + ii, j = tj * T + jj, k = tk * T + kk;
+    const bool inside = i < d.nx && j < d.ny && k < d.nz;
+    const int c = ((ii + 1) * H + (jj + 1)) * H + (kk + 1);
+    bool mine = false;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) chg = 0;
+        __syncthreads();
+        if (inside && !sv[c] &&
+            ((sv[c - H * H] && sp[c - H * H]) || (sv[c + H * H] && sp[c + H * H]) || (sv[c - H] && sp[c - H]) ||
+             (sv[c + H] && sp[c + H]) || (sv[c - 1] && sp[c - 1]) || (sv[c + 1] && sp[c + 1]))) {
+            sv[c] = 1;
+            mine = true;
+            chg = 1;
+        }
+        __syncthreads();
+        if (!chg) break;
+    }
+    if (mine) {
+        vis[((int64_t)i * d.ny + j) * d.nz + k] = 1;
         *changed = 1;
     }
 }
@@ -323,13 +392,15 @@ static int ls_rk3(LsRun &L, int mode, double final_t, double vel, double cfl, do
     const double eps = L.d.h * L.d.h;
     const int nb = nblk(L.n);
     unsigned long long *cnt = nullptr;
-    double *err = nullptr;
+    double *err = nullptr, *pe = nullptr;
+    unsigned int *pc = nullptr;
     IBIM_HIP_CHECK(hipMalloc(&cnt, 2 * sizeof(unsigned long long)));
     IBIM_HIP_CHECK(hipMalloc(&err, sizeof(double)));
+    IBIM_HIP_CHECK(hipMalloc(&pc, 2 * (size_t)nb * sizeof(unsigned int)));
+    IBIM_HIP_CHECK(hipMalloc(&pe, (size_t)nb * sizeof(double)));
     auto count = [&](unsigned long long *h) -> int {
-        IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), L.st));
-        IBIM_HIP_CHECK(hipMemsetAsync(err, 0, sizeof(double), L.st));
-        k_count<<<nb, 256, 0, L.st>>>(L.g, L.d, tube, thres, cnt, err);
+        k_count<<<nb, 256, 0, L.st>>>(L.g, L.d, tube, thres, pc, pe);
+        k_count_sum<<<1, 1024, 0, L.st>>>(nb, pc, pe, cnt, err);
         IBIM_LAUNCH_CHECK();
         double e = 0;
         IBIM_HIP_CHECK(hipMemcpyAsync(h, cnt, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, L.st));
@@ -355,6 +426,8 @@ static int ls_rk3(LsRun &L, int mode, double final_t, double vel, double cfl, do
     if (mode == 0) IBIM_HIP_CHECK(hipStreamSynchronize(L.st));
     (void)hipFree(cnt);
     (void)hipFree(err);
+    (void)hipFree(pc);
+    (void)hipFree(pe);
     *steps_out = step;
     return IBIM_OK;
 }
@@ -370,9 +443,10 @@ static int ls_surface(LsRun &L, const ibim_geom_params &gp, ibim_surface *S) {
     IBIM_HIP_CHECK(hipMemsetAsync(vis, 0, (size_t)L.n, L.st));
     const unsigned char one = 1;
     IBIM_HIP_CHECK(hipMemcpyAsync(vis, &one, 1, hipMemcpyHostToDevice, L.st));
+    const int ntiles = ((L.d.nx + 7) / 8) * ((L.d.ny + 7) / 8) * ((L.d.nz + 7) / 8);
     for (int it = 0;; ++it) {
         IBIM_HIP_CHECK(hipMemsetAsync(changed, 0, sizeof(int), L.st));
-        for (int r = 0; r < 8; ++r) k_flood<<<nb, 256, 0, L.st>>>(L.g, L.d, vis, -tube, changed);
+        for (int r = 0; r < 2; ++r) k_flood_tile<<<ntiles, 512, 0, L.st>>>(L.g, L.d, vis, -tube, changed);
         IBIM_LAUNCH_CHECK();
         int hch = 0;
         IBIM_HIP_CHECK(hipMemcpyAsync(&hch, changed, sizeof(int), hipMemcpyDeviceToHost, L.st));
