@@ -70,21 +70,25 @@ __device__ __forceinline__ void weno(const double *__restrict__ g, const GridDim
     double is0 = 13.0 * (e - dd) * (e - dd) + 3.0 * (e - 3.0 * dd) * (e - 3.0 * dd);
     double is1 = 13.0 * (dd - c) * (dd - c) + 3.0 * (dd + c) * (dd + c);
     double is2 = 13.0 * (c - b) * (c - b) + 3.0 * (3.0 * c - b) * (3.0 * c - b);
-    double a0 = recip((EPS_LS + is0) * (EPS_LS + is0));
-    double a1 = 6.0 * recip((EPS_LS + is1) * (EPS_LS + is1));
-    double a2 = 3.0 * recip((EPS_LS + is2) * (EPS_LS + is2));
-    double w1 = recip(a0 + a1 + a2);
-    double w0 = a0 * w1, w2 = a2 * w1;
+    // weights a_i / (a0 + a1 + a2) with a_i = (1, 6, 3)_i / q_i, q_i = (eps +
+    // is_i)^2, formed as products of the q's over one reciprocal
+    double q0 = (EPS_LS + is0) * (EPS_LS + is0), q1 = (EPS_LS + is1) * (EPS_LS + is1),
+           q2 = (EPS_LS + is2) * (EPS_LS + is2);
+    double p0 = q1 * q2, p2 = 3.0 * q0 * q1;
+    double rs = recip(p0 + 6.0 * q0 * q2 + p2);
+    double w0 = p0 * rs, w2 = p2 * rs;
     up = (df + w0 * (e - 2 * dd + c) * r3 + (w2 - 0.5) * (dd - 2.0 * c + b) * r6) * rh;
     is0 = 13.0 * (a - b) * (a - b) + 3.0 * (a - 3.0 * b) * (a - 3.0 * b);
     is1 = 13.0 * (b - c) * (b - c) + 3.0 * (b + c) * (b + c);
     is2 = 13.0 * (c - dd) * (c - dd) + 3.0 * (3.0 * c - dd) * (3.0 * c - dd);
-    a0 = recip((EPS_LS + is0) * (EPS_LS + is0));
-    a1 = 6.0 * recip((EPS_LS + is1) * (EPS_LS + is1));
-    a2 = 3.0 * recip((EPS_LS + is2) * (EPS_LS + is2));
-    w1 = recip(a0 + a1 + a2);
-    w0 = a0 * w1;
-    w2 = a2 * w1;
+    q0 = (EPS_LS + is0) * (EPS_LS + is0);
+    q1 = (EPS_LS + is1) * (EPS_LS + is1);
+    q2 = (EPS_LS + is2) * (EPS_LS + is2);
+    p0 = q1 * q2;
+    p2 = 3.0 * q0 * q1;
+    rs = recip(p0 + 6.0 * q0 * q2 + p2);
+    w0 = p0 * rs;
+    w2 = p2 * rs;
     un = (df - w0 * (a - 2 * b + c) * r3 - (w2 - 0.5) * (b - 2 * c + dd) * r6) * rh;
 }
 
