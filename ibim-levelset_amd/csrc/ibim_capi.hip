@@ -2,6 +2,10 @@
 #include <cmath>
 #include <cstring>
 
+#include <map>
+#include <mutex>
+#include <unordered_map>
+
 #include "ibim_internal.h"
 
 namespace ibim {
@@ -99,6 +103,84 @@ int with_host_vectors(ibim_plan *pl, size_t n_in, const double *in, size_t n_out
 }
 }  // namespace
 
+namespace ibim {
+namespace {
+struct BlockCache {
+    std::mutex mu;
+    std::unordered_map<void *, std::pair<int, size_t>> live;  // block -> (device, size)
+    std::map<int, std::multimap<size_t, void *>> idle;         // per device: size -> block
+    std::map<int, size_t> idle_bytes;
+};
+BlockCache &block_cache() {
+    static BlockCache *c = new BlockCache();  // kept to process exit
+    return *c;
+}
+constexpr size_t kBlockRound = (size_t)2 << 20;
+constexpr size_t kIdleCap = (size_t)64 << 30;  // idle bytes kept per device
+}  // namespace
+
+hipError_t block_alloc(void **p, size_t bytes) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const size_t sz = (bytes + kBlockRound - 1) / kBlockRound * kBlockRound;
+    BlockCache &c = block_cache();
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        auto &m = c.idle[dev];
+        auto it = m.lower_bound(sz);
+        if (it != m.end() && it->first <= sz + sz / 4) {  // at most a quarter larger
+            *p = it->second;
+            c.live[*p] = {dev, it->first};
+            c.idle_bytes[dev] -= it->first;
+            m.erase(it);
+            return hipSuccess;
+        }
+    }
+    e = hipMalloc(p, sz);
+    if (e != hipSuccess) {  // return this device's idle blocks and retry
+        (void)hipGetLastError();
+        std::vector<void *> drop;
+        {
+            std::lock_guard<std::mutex> lk(c.mu);
+            for (auto &kv : c.idle[dev]) drop.push_back(kv.second);
+            c.idle[dev].clear();
+            c.idle_bytes[dev] = 0;
+        }
+        for (void *q : drop) (void)hipFree(q);
+        e = hipMalloc(p, sz);
+        if (e != hipSuccess) return e;
+    }
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.live[*p] = {dev, sz};
+    return hipSuccess;
+}
+
+void block_release(void *p) {
+    if (!p) return;
+    BlockCache &c = block_cache();
+    std::vector<void *> drop;
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        auto it = c.live.find(p);
+        if (it == c.live.end()) return;  // not a block of this cache
+        const int dev = it->second.first;
+        const size_t sz = it->second.second;
+        c.live.erase(it);
+        auto &m = c.idle[dev];
+        m.emplace(sz, p);
+        c.idle_bytes[dev] += sz;
+        while (c.idle_bytes[dev] > kIdleCap && !m.empty()) {  // drop the largest
+            auto last = std::prev(m.end());
+            drop.push_back(last->second);
+            c.idle_bytes[dev] -= last->first;
+            m.erase(last);
+        }
+    }
+    for (void *q : drop) (void)hipFree(q);
+}
+}  // namespace ibim
+
 extern "C" {
 
 const char *ibim_last_error(void) { return g_last_error.c_str(); }
@@ -112,7 +194,7 @@ void ibim_plan_destroy(ibim_plan *pl) {
         (void)hipEventDestroy(r.b);
     }
     for (hipEvent_t e : pl->timer.pool) (void)hipEventDestroy(e);
-    for (void *p : pl->allocs) (void)hipFree(p);
+    for (void *p : pl->allocs) block_release(p);
     pl->pool.free_all();
     pl->spool.free_all();
     if (pl->owns_stream && pl->stream) (void)hipStreamDestroy(pl->stream);
@@ -496,8 +578,12 @@ int ibim_tree_stats_get(ibim_plan *pl, ibim_tree_stats *st) {
         const int64_t nt = se[i] - sb[i];
         if (leaf && nt > 0) {
             int64_t ns = 0;
-            for (int k : {0, 2})
-                for (int q = off[k][i]; q < off[k][i + 1]; ++q) ns += se[it[k][q]] - sb[it[k][q]];
+            for (int k : {0, 2}) {
+                int64_t nk = 0;
+                for (int q = off[k][i]; q < off[k][i + 1]; ++q) nk += se[it[k][q]] - sb[it[k][q]];
+                ns += nk;
+                if (k == 0) s.p2p_pairs_u += nt * nk;
+            }
             s.p2p_pairs += nt * ns;
         }
         if (i != 0 && nt > 0)

@@ -311,6 +311,14 @@ struct StageTimer {
     }
 };
 
+// Device blocks for plan-owned memory: released blocks go to a process-wide
+// cache (per device) that serves later requests of similar size without
+// hipMalloc; freeing a GB-sized block costs about a millisecond, and every
+// solve of a request stream builds and releases a plan.  A released block
+// must be idle (its stream synchronised).
+hipError_t block_alloc(void **p, size_t bytes);
+void block_release(void *p);
+
 // Device memory pool of one plan: chunks from hipMalloc handed out in
 // 256-byte aligned pieces.  The persistent pool frees its chunks with the
 // plan; the scratch pool is reset to a mark at the end of each build step
@@ -337,7 +345,7 @@ struct DevPool {
             }
         const size_t sz = n > min_chunk ? n : min_chunk;
         char *p = nullptr;
-        hipError_t e = hipMalloc((void **)&p, sz);
+        hipError_t e = block_alloc((void **)&p, sz);
         if (e != hipSuccess) return e;
         chunks.push_back({p, sz});
         bytes += (int64_t)sz;
@@ -352,7 +360,7 @@ struct DevPool {
         off = m.second;
     }
     void free_all() {
-        for (const Chunk &c : chunks) (void)hipFree(c.p);
+        for (const Chunk &c : chunks) block_release(c.p);
         chunks.clear();
         cur = off = 0;
         bytes = 0;
@@ -500,7 +508,7 @@ int dalloc(ibim_plan *pl, T **ptr, size_t count) {
     const size_t bytes = count * sizeof(T);
     hipError_t e;
     if (bytes >= ((size_t)16 << 20)) {
-        e = hipMalloc((void **)ptr, bytes);
+        e = block_alloc((void **)ptr, bytes);
         if (e == hipSuccess) {
             pl->allocs.push_back((void *)*ptr);
             pl->footprint += (int64_t)bytes;
