@@ -50,7 +50,7 @@ class TreeStats(C.Structure):
                 ("n_empty_leaves", C.c_int64), ("depth", C.c_int),
                 ("n_u", C.c_int64), ("n_v", C.c_int64), ("n_w", C.c_int64), ("n_x", C.c_int64),
                 ("p2p_pairs", C.c_int64), ("m2l_pairs", C.c_int64), ("v_pairs", C.c_int64),
-                ("x_pairs", C.c_int64)]
+                ("x_pairs", C.c_int64), ("p2p_pairs_u", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -91,6 +91,7 @@ typedef struct ibim_tree_stats {
     int64_t m2l_pairs;    /* non-empty V + X interactions into nodes with targets */
     int64_t v_pairs;      /* the V part of m2l_pairs */
     int64_t x_pairs;      /* the X part of m2l_pairs */
+    int64_t p2p_pairs_u;  /* the U-list part of p2p_pairs (the rest is the W list) */
 } ibim_tree_stats;
 
 typedef struct ibim_solve_info {
