@@ -31,16 +31,19 @@ namespace {
 inline int nblk(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
 
 // ----------------------------------------------------------------- P2M ---
-template <int P>
-__global__ __launch_bounds__(256) void k_p2m(const int *__restrict__ leaves, const double *__restrict__ sx,
+// One block of BS threads per non-empty leaf, a thread per node (and field
+// set); the leaf's points pass in chunks of BS, whose Chebyshev factors are
+// formed one point per thread.  Each node sums its points in order.
+template <int P, int BS>
+__global__ __launch_bounds__(BS) void k_p2m(const int *__restrict__ leaves, const double *__restrict__ sx,
                                              const double *__restrict__ sy, const double *__restrict__ sz,
                                              const double *__restrict__ Q, int n, const int *__restrict__ sbeg,
                                              const int *__restrict__ send, const double *__restrict__ ctr,
                                              const int *__restrict__ level, const double *__restrict__ lrad,
                                              ChebTab tab, double *__restrict__ M) {
     constexpr int R = P * P * P;
-    constexpr int PER = (R + 255) / 256;
-    constexpr int CH = 64;
+    constexpr int PER = (R + BS - 1) / BS;
+    constexpr int CH = BS;
     __shared__ double sS[3][P][CH];
     __shared__ double sq[4][CH];
     const int b = leaves[blockIdx.x];
@@ -74,7 +77,7 @@ __global__ __launch_bounds__(256) void k_p2m(const int *__restrict__ leaves, con
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            const int m = tid + 256 * k;
+            const int m = tid + BS * k;
             if (m < R) {
                 const int i = m % P, j = (m / P) % P, l = m / (P * P);
                 for (int s = 0; s < cnt; ++s) {
@@ -91,7 +94,7 @@ __global__ __launch_bounds__(256) void k_p2m(const int *__restrict__ leaves, con
     double *Mb = M + (size_t)b * 4 * R;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const int m = tid + 256 * k;
+        const int m = tid + BS * k;
         if (m < R)
 #pragma unroll
             for (int f = 0; f < 4; ++f) Mb[f * R + m] = acc[k][f];
@@ -120,57 +123,86 @@ __device__ __forceinline__ void load_transfer(const ChebTab &tab, int c, double 
 
 // ----------------------------------------------------------------- M2M ---
 // out(f,l,m,n) += sum_i Tz(i,l) sum_j Ty(j,m) sum_k Tx(k,n) Mc(f,i,j,k)
-// with T(i,l) = S_p(child node i, parent node l), index i*P*P + j*P + k
-template <int P>
+// with T(i,l) = S_p(child node i, parent node l), index i*P*P + j*P + k.
+// One block per parent; the children are transformed G at a time (all 8 on
+// levels with few parents, where each block's chain of load latencies is
+// the time), so their loads and the two inner contractions share one pass
+// and three barriers; each output sums the children's terms in child order.
+template <int P, int G>
 __global__ __launch_bounds__(256) void k_m2m(int lo, const int *__restrict__ flags, const int *__restrict__ fc,
                                              ChebTab tab, double *__restrict__ M) {
     constexpr int R = P * P * P;
     constexpr int R4 = 4 * R;
     constexpr int PER = (R4 + 255) / 256;
-    __shared__ double sIn[R4];
-    __shared__ double sA[R4];
-    __shared__ double sT[3][P][P];
+    static_assert(8 % G == 0, "children per pass");
+    __shared__ double sIn[G * R4];
+    __shared__ double sA[G * R4];
+    __shared__ double sT[2][P][P];  // the transfer factors of the lower and the upper child half
     const int id = lo + blockIdx.x;
     if (flags[id] & kLeaf) return;
     const int tid = threadIdx.x;
+    for (int e = tid; e < 2 * P * P; e += 256) {
+        const int b = e / (P * P), i = (e / P) % P, l = e % P;
+        // kernel-argument arrays indexed at run time are read through a copy
+        double v = 0.0;
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int ii = 0; ii < P; ++ii)
+#pragma unroll
+                for (int ll = 0; ll < P; ++ll)
+                    if (bb == b && ii == i && ll == l) v = tab.tr[bb][ii][ll];
+        sT[b][i][l] = v;
+    }
     double acc[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) acc[k] = 0.0;
     const int c0 = fc[id];
-    for (int c = 0; c < 8; ++c) {
-        const int ch = c0 + c;
-        if (flags[ch] & kEmpty) continue;  // bbfmm.h:531
-        load_transfer<P>(tab, c, sT);
-        for (int e = tid; e < R4; e += 256) sIn[e] = M[(size_t)ch * R4 + e];
+    unsigned live = 0;  // bit c: child c is not empty (bbfmm.h:531 skips empty children)
+    for (int c = 0; c < 8; ++c)
+        if (!(flags[c0 + c] & kEmpty)) live |= 1u << c;
+    for (int g0 = 0; g0 < 8; g0 += G) {
+        if (!((live >> g0) & ((1u << G) - 1))) continue;  // (block-uniform)
+        for (int e = tid; e < G * R4; e += 256) {
+            const int c = e / R4;
+            sIn[e] = (live >> (g0 + c)) & 1 ? M[(size_t)(c0 + g0 + c) * R4 + e % R4] : 0.0;
+        }
         __syncthreads();
-        for (int e = tid; e < R4; e += 256) {  // A(f,i,j,n)
-            const int f = e / R, rem = e % R, i = rem / (P * P), j = (rem / P) % P, nn = rem % P;
-            const double *in = sIn + f * R + i * P * P + j * P;
+        for (int e = tid; e < G * R4; e += 256) {  // A(c,f,i,j,n)
+            const int c = e / R4, rem = e % R, i = rem / (P * P), j = (rem / P) % P, nn = rem % P;
+            const double(*T)[P] = sT[(g0 + c) & 1];
+            const double *in = sIn + (e - rem) + i * P * P + j * P;
             double s = 0.0;
 #pragma unroll
-            for (int k = 0; k < P; ++k) s += sT[0][k][nn] * in[k];
+            for (int k = 0; k < P; ++k) s += T[k][nn] * in[k];
             sA[e] = s;
         }
         __syncthreads();
-        for (int e = tid; e < R4; e += 256) {  // B(f,i,m,n) -> sIn
-            const int f = e / R, rem = e % R, i = rem / (P * P), m = (rem / P) % P, nn = rem % P;
-            const double *in = sA + f * R + i * P * P + nn;
+        for (int e = tid; e < G * R4; e += 256) {  // B(c,f,i,m,n) -> sIn
+            const int c = e / R4, rem = e % R, i = rem / (P * P), m = (rem / P) % P, nn = rem % P;
+            const double(*T)[P] = sT[((g0 + c) >> 1) & 1];
+            const double *in = sA + (e - rem) + i * P * P + nn;
             double s = 0.0;
 #pragma unroll
-            for (int j = 0; j < P; ++j) s += sT[1][j][m] * in[j * P];
+            for (int j = 0; j < P; ++j) s += T[j][m] * in[j * P];
             sIn[e] = s;
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {  // out(f,l,m,n)
+        for (int k = 0; k < PER; ++k) {  // out(f,l,m,n), children in order
             const int e = tid + 256 * k;
             if (e < R4) {
                 const int f = e / R, rem = e % R, l = rem / (P * P), m = (rem / P) % P, nn = rem % P;
-                const double *in = sIn + f * R + m * P + nn;
-                double s = 0.0;
 #pragma unroll
-                for (int i = 0; i < P; ++i) s += sT[2][i][l] * in[i * P * P];
-                acc[k] += s;
+                for (int c = 0; c < G; ++c) {
+                    if (!((live >> (g0 + c)) & 1)) continue;
+                    const double(*T)[P] = sT[((g0 + c) >> 2) & 1];
+                    const double *in = sIn + c * R4 + f * R + m * P + nn;
+                    double s = 0.0;
+#pragma unroll
+                    for (int i = 0; i < P; ++i) s += T[i][l] * in[i * P * P];
+                    acc[k] += s;
+                }
             }
         }
         __syncthreads();
@@ -838,6 +870,7 @@ __global__ void k_scatter_field(int n, const int *__restrict__ perm, const doubl
 
 template <int P>
 int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mixed) {
+    constexpr int kP2mBS = P * P * P <= 128 ? 128 : 256;
     constexpr int R = P * P * P;
     constexpr int BS = R <= 64 ? 64 : (R <= 128 ? 128 : 256);
     hipStream_t st = pl->stream;
@@ -845,7 +878,7 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     {
     StageScope sc(pl, kStP2M);
     if (pl->n_leaves_ne > 0)
-        k_p2m<P><<<pl->n_leaves_ne, 256, 0, st>>>(pl->leaves_ne, pl->sx, pl->sy, pl->sz, pl->Q, pl->n, pl->sbeg,
+        k_p2m<P, kP2mBS><<<pl->n_leaves_ne, kP2mBS, 0, st>>>(pl->leaves_ne, pl->sx, pl->sy, pl->sz, pl->Q, pl->n, pl->sbeg,
                                                    pl->send, pl->ctr, pl->level, pl->d_lvl_rad, pl->tab, pl->M);
     IBIM_LAUNCH_CHECK();
     }
@@ -853,7 +886,14 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     StageScope sc(pl, kStM2M);
     for (int l = pl->nlevels - 2; l >= 0; --l) {
         const int cnt = pl->lvl_off[l + 1] - pl->lvl_off[l];
-        k_m2m<P><<<cnt, 256, 0, st>>>(pl->lvl_off[l], pl->flags, pl->first_child, pl->tab, pl->M);
+        // all children per pass where parents are few (latency), two where
+        // they fill the chip (LDS per block sets the occupancy)
+        constexpr int R4 = 4 * P * P * P;
+        constexpr int GS = R4 <= 512 ? 8 : (R4 <= 1024 ? 4 : 2);  // LDS <= 64 KB
+        if (cnt < 1024)
+            k_m2m<P, GS><<<cnt, 256, 0, st>>>(pl->lvl_off[l], pl->flags, pl->first_child, pl->tab, pl->M);
+        else
+            k_m2m<P, 2><<<cnt, 256, 0, st>>>(pl->lvl_off[l], pl->flags, pl->first_child, pl->tab, pl->M);
     }
     IBIM_LAUNCH_CHECK();
     }

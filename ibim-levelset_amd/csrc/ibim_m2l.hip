@@ -616,8 +616,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // pair l & 31, the same k; accumulator j of lane l is row 8 (j >> 2) + 4 (l
 // >> 5) + (j & 3), pair l & 31.  For p <= 5 (the B-fragments of 32 pairs over
 // all k fit the register budget).
-template <int P>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_m2l_gemm_h32(
+// waves per block of the V- and X-list launches: a block reads its key's
+// operator once for 32 NW pairs, and a V key has about a thousand pairs
+// while an X key has a few dozen
+constexpr int kH32WavesV = 8, kH32WavesX = 4;
+
+template <int P, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) void k_m2l_gemm_h32(
     const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
     const int *__restrict__ pA, const int *__restrict__ keyslot, const _Float16 *__restrict__ ops16u,
     const int *__restrict__ esig, int MRP, const _Float16 *__restrict__ Mh, const int *__restrict__ ebox,
@@ -630,13 +635,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     constexpr int NKC = KP / KC;
     constexpr int SEGS = KC / 8;                 // 16-byte segments per chunk row
     constexpr int RPI = 64 / SEGS;               // chunk rows per LDS-load instruction (64 lanes x 16 B)
-    constexpr int DPW = SEGS / 4;                // LDS-load instructions per wave per chunk
+    constexpr int DPW = SEGS / NW;               // LDS-load instructions per wave per chunk
     constexpr int NST = 4;                       // chunks in the LDS ring
     constexpr int CH = 64 * KC;                  // halves per chunk
-    constexpr int NPB = 128;                     // pairs per block (32 per wave)
+    constexpr int NPB = 32 * NW;                 // pairs per block (32 per wave)
     constexpr int YS = (R4 + 31) / 32 * 32;      // row stride of Y
     constexpr int MR = (R4 + 63) / 64 * 64;      // operator rows (padded)
-    static_assert(KP % KC == 0 && DPW >= 1 && R4 % 4 == 0 && P <= 5, "tile shape");
+    static_assert(KP % KC == 0 && DPW >= 1 && DPW * NW == SEGS && R4 % 4 == 0 && P <= 5, "tile shape");
     __shared__ __attribute__((aligned(1024))) _Float16 sA[NST][CH];
     __shared__ int sEA[NPB];
     __shared__ __attribute__((aligned(16))) int sE[MR];  // the operator's row exponents
@@ -669,7 +674,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const int j = j0 + tid;
         sEA[tid] = j < jend ? ebox[pA[j]] : 0;
     }
-    for (int i = tid; i < MR; i += 256) sE[i] = esig[(size_t)sl * MRP + i];
+    for (int i = tid; i < MR; i += 64 * NW) sE[i] = esig[(size_t)sl * MRP + i];
     // B-fragments of this wave's 32 pairs, all k
     h8 xr[KS];
     {
@@ -679,6 +684,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) xr[ks] = ok ? *(const h8 *)(src + 16 * ks) : h8{};
     }
+    const bool live = j0 + 32 * w < jend;
     int c = 0;
     for (int rt = 0; rt < nrt; ++rt) {
         f16v acc[2];
@@ -704,6 +710,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             static_assert((KC / 16) % 2 == 0, "k-steps per chunk");
             // (sched_barrier: the scheduler would sink the reads next to
             // their use to save registers)
+            if (!live) continue;  // (wave-uniform: no valid pair in this wave's 32)
             h8 ka[2], kb[2];
             frag(0, ka);
 #pragma unroll
@@ -785,19 +792,25 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
             constexpr int YS = (R4 + 31) / 32 * 32;
             k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, KP, pl->Mh, pl->ebox);
             // V pairs, then (x16) the X pairs into the rows after them
-            auto gemm = [&](const VPairs &pp, const int *keyslot, unsigned *Y) {
+            auto gemm = [&](const VPairs &pp, const int *keyslot, unsigned *Y, bool vlist) {
                 const int pxcd = (pp.hnblocks + 7) / 8;
-                if constexpr (P <= 5)
-                    k_m2l_gemm_h32<P><<<8 * pxcd, 256, 0, st>>>(pp.hblk_cls, pp.hblk_j0, pp.cls_off_d, pp.A, keyslot,
-                                                                pl->ops16, pl->esig, pl->MRP, pl->Mh, pl->ebox, Y,
-                                                                pp.hnblocks, pxcd);
-                else
+                if constexpr (P <= 5) {
+                    if (vlist)
+                        k_m2l_gemm_h32<P, kH32WavesV><<<8 * pxcd, 64 * kH32WavesV, 0, st>>>(
+                            pp.hblk_cls, pp.hblk_j0, pp.cls_off_d, pp.A, keyslot, pl->ops16, pl->esig, pl->MRP,
+                            pl->Mh, pl->ebox, Y, pp.hnblocks, pxcd);
+                    else
+                        k_m2l_gemm_h32<P, kH32WavesX><<<8 * pxcd, 64 * kH32WavesX, 0, st>>>(
+                            pp.hblk_cls, pp.hblk_j0, pp.cls_off_d, pp.A, keyslot, pl->ops16, pl->esig, pl->MRP,
+                            pl->Mh, pl->ebox, Y, pp.hnblocks, pxcd);
+                } else
                     k_m2l_gemm_hu<P, 16><<<8 * pxcd, 256, 0, st>>>(pp.hblk_cls, pp.hblk_j0, pp.cls_off_d, pp.A,
                                                                    keyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
                                                                    pl->ebox, Y, pp.hnblocks, pxcd);
             };
-            if (vp.hnblocks > 0) gemm(vp, pl->keyslot, pl->Yv);
-            if (pl->x16 && pl->xp.hnblocks > 0) gemm(pl->xp, pl->xkeyslot, pl->Yv + (size_t)vp.npairs * YS / 4 * 3);
+            if (vp.hnblocks > 0) gemm(vp, pl->keyslot, pl->Yv, true);
+            if (pl->x16 && pl->xp.hnblocks > 0)
+                gemm(pl->xp, pl->xkeyslot, pl->Yv + (size_t)vp.npairs * YS / 4 * 3, false);
             k_v_reduce<R4><<<pl->n_m2l_work, R4 / 4 <= 128 ? 128 : 256, 0, st>>>(pl->m2l_work, pl->ytptr, pl->ytlist,
                                                                                 pl->Yv, pl->L);
             IBIM_LAUNCH_CHECK();
@@ -943,7 +956,7 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
     }
     // FP16 GEMM block table (blocks of 128 pairs for p <= 5, 64 for p = 6)
     if (pl->p <= 6) {
-        const int hnp = pl->p <= 5 ? 128 : 64;
+        const int hnp = pl->p <= 5 ? 32 * kH32WavesV : 64;
         std::vector<int> hc, hj;
         for (int c = 0; c < nkeys; ++c)
             for (int j = vp.cls_off[c]; j < vp.cls_off[c + 1]; j += hnp) {
@@ -1165,7 +1178,7 @@ int build_xpairs(ibim_plan *pl) {
     xp.nblocks = (int)bc.size();
     std::vector<int> hc, hj;  // FP16 GEMM blocks, as for the V pairs
     if (pl->p <= 6) {
-        const int hnp = pl->p <= 5 ? 128 : 64;
+        const int hnp = pl->p <= 5 ? 32 * kH32WavesX : 64;
         for (int key = 0; key < 8 * nc; ++key)
             for (int j = xp.cls_off[key]; j < xp.cls_off[key + 1]; j += hnp) {
                 hc.push_back(key);
