@@ -195,6 +195,7 @@ void ibim_plan_destroy(ibim_plan *pl) {
     }
     for (hipEvent_t e : pl->timer.pool) (void)hipEventDestroy(e);
     for (void *p : pl->allocs) block_release(p);
+    if (pl->gm_host) (void)hipHostFree(pl->gm_host);
     pl->pool.free_all();
     pl->spool.free_all();
     if (pl->owns_stream && pl->stream) (void)hipStreamDestroy(pl->stream);

@@ -456,6 +456,7 @@ struct ibim_plan {
     double *gm_part = nullptr;           // partial sums
     double *gm_rr = nullptr, *gm_d = nullptr;  // refinement residual and correction
     double *gm_small = nullptr;          // H, s, cs, sn, scalars on device
+    double *gm_host = nullptr;           // pinned host copies of those scalars (hipHostMalloc)
 
     ibim::StageTimer timer;
 

@@ -407,6 +407,8 @@ int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_
     return target_fmm_device(pl, -1, m, d_axyz, d_x, d_out);
 }
 
+constexpr int kHostScalars = 8;
+
 static int gmres_alloc(ibim_plan *pl, int restart) {
     if (pl->gm_V && pl->gm_restart >= restart) return IBIM_OK;
     const size_t n2 = 2 * (size_t)pl->n;
@@ -417,8 +419,16 @@ static int gmres_alloc(ibim_plan *pl, int restart) {
         (rc = dalloc(pl, &pl->gm_part, 4 * (size_t)kRG)) ||
         (rc = dalloc(pl, &pl->gm_small, (size_t)(restart + 1) * (restart + 1) + 4 * (size_t)(restart + 1) + 8)))
         return rc;
+    if (!pl->gm_host) IBIM_HIP_CHECK(hipHostMalloc((void **)&pl->gm_host, kHostScalars * sizeof(double)));
     pl->gm_restart = restart;
     return IBIM_OK;
+}
+
+// Device scalar -> slot k of the pinned host buffer (a direct copy: no
+// staging through a pageable bounce buffer); read the slot after the stream
+// synchronises.
+static inline hipError_t fetch_scalar(ibim_plan *pl, const double *d, int k) {
+    return hipMemcpyAsync(pl->gm_host + k, d, sizeof(double), hipMemcpyDeviceToHost, pl->stream);
 }
 
 // restarted GMRES, gmres.cpp:50-164, all vectors in HBM; `mixed` selects
@@ -447,16 +457,16 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
     k_dot_part<<<kRG, kRB, 0, st>>>(d_b, d_b, n2, partA);
     k_sqrt_sum<<<1, kRB, 0, st>>>(partA, scal + 2);
     IBIM_LAUNCH_CHECK();
-    double normb = 0.0;
-    IBIM_HIP_CHECK(hipMemcpyAsync(&normb, scal + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+    IBIM_HIP_CHECK(fetch_scalar(pl, scal + 2, 2));
     // r = b - A x; for x = 0 this is b exactly (the matvec of 0 is 0), so the
     // operator is not applied
-    double xnorm = 0.0;
     k_dot_part<<<kRG, kRB, 0, st>>>(d_x, d_x, n2, partB);
     k_sqrt_sum<<<1, kRB, 0, st>>>(partB, scal + 3);
     IBIM_LAUNCH_CHECK();
-    IBIM_HIP_CHECK(hipMemcpyAsync(&xnorm, scal + 3, sizeof(double), hipMemcpyDeviceToHost, st));
+    IBIM_HIP_CHECK(fetch_scalar(pl, scal + 3, 3));
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    double normb = pl->gm_host[2];
+    const double xnorm = pl->gm_host[3];
     if (xnorm != 0.0) {
         if ((rc = matvec_device(pl, d_x, p, mixed))) return rc;
         nmv++;
@@ -466,9 +476,9 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
     k_residual<<<kRG, kRB, 0, st>>>(d_b, p, r, n2, partA);
     k_sqrt_sum<<<1, kRB, 0, st>>>(partA, scal);
     IBIM_LAUNCH_CHECK();
-    double beta = 0.0;
-    IBIM_HIP_CHECK(hipMemcpyAsync(&beta, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+    IBIM_HIP_CHECK(fetch_scalar(pl, scal, 0));
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    double beta = pl->gm_host[0];
     if (normb == 0.0) normb = 1;
     double res = beta / normb;
     int ret = IBIM_NOT_CONVERGED;
@@ -500,8 +510,9 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
                     k_givens<<<1, 64, 0, st>>>(H, ld, s, cs, sn, i, normb, scal + 1);
                 }
                 IBIM_LAUNCH_CHECK();
-                IBIM_HIP_CHECK(hipMemcpyAsync(&res, scal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+                IBIM_HIP_CHECK(fetch_scalar(pl, scal + 1, 1));
                 IBIM_HIP_CHECK(hipStreamSynchronize(st));
+                res = pl->gm_host[1];
                 if (res < tol) {
                     k_update<<<kRG, kRB, 0, st>>>(d_x, V, ldv, n2, H, ld, s, i);
                     IBIM_LAUNCH_CHECK();
@@ -518,8 +529,9 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
             k_residual<<<kRG, kRB, 0, st>>>(d_b, p, r, n2, partA);
             k_sqrt_sum<<<1, kRB, 0, st>>>(partA, scal);
             IBIM_LAUNCH_CHECK();
-            IBIM_HIP_CHECK(hipMemcpyAsync(&beta, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+            IBIM_HIP_CHECK(fetch_scalar(pl, scal, 0));
             IBIM_HIP_CHECK(hipStreamSynchronize(st));
+            beta = pl->gm_host[0];
             if ((res = beta / normb) < tol) {
                 ret = IBIM_OK;
                 break;
@@ -567,8 +579,9 @@ static int gmres_refine(ibim_plan *pl, const double *d_b, double *d_x, int resta
         k_dot_part<<<kRG, kRB, 0, st>>>(v, v, n2, part);
         k_sqrt_sum<<<1, kRB, 0, st>>>(part, scal);
         IBIM_LAUNCH_CHECK();
-        IBIM_HIP_CHECK(hipMemcpyAsync(out, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+        IBIM_HIP_CHECK(fetch_scalar(pl, scal, 4));
         IBIM_HIP_CHECK(hipStreamSynchronize(st));
+        *out = pl->gm_host[4];
         return IBIM_OK;
     };
     double normb = 0.0, rn = 0.0;
@@ -590,8 +603,9 @@ static int gmres_refine(ibim_plan *pl, const double *d_b, double *d_x, int resta
         k_residual<<<kRG, kRB, 0, st>>>(d_b, pl->gm_x, pl->gm_rr, n2, part);
         k_sqrt_sum<<<1, kRB, 0, st>>>(part, scal);
         IBIM_LAUNCH_CHECK();
-        IBIM_HIP_CHECK(hipMemcpyAsync(&rn, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+        IBIM_HIP_CHECK(fetch_scalar(pl, scal, 5));
         IBIM_HIP_CHECK(hipStreamSynchronize(st));
+        rn = pl->gm_host[5];
         return IBIM_OK;
     };
     if ((rc = residual())) return rc;
