@@ -143,6 +143,7 @@ def lib():
     L.ibim_tree_lists.argtypes = [vp, C.c_int, i, i, i64]
     L.ibim_tree_points.argtypes = [vp, C.c_int, i, i64]
     L.ibim_plan_profile.argtypes = [vp, C.c_int]
+    L.ibim_plan_profile_stages.argtypes = [vp, C.c_uint]
     L.ibim_plan_stage_times.argtypes = [vp, d, C.POINTER(C.c_int64), C.c_int]
     L.ibim_surface_from_atoms.argtypes = [C.POINTER(GeomParams), i64, d, d, pp]
     L.ibim_surface_from_grid.argtypes = [C.POINTER(GeomParams), d, pp]
@@ -294,8 +295,16 @@ class Plan:
         _check(lib().ibim_polarize_device(self._h, m, d_axyz, d_x, d_phi), "ibim_polarize_device")
 
     # ------------------------------------------------------------ timing --
-    def profile(self, enable=True):
-        _check(lib().ibim_plan_profile(self._h, int(enable)), "ibim_plan_profile")
+    def profile(self, enable=True, stages=None):
+        """Stage timers on (every stage), off, or only for the named stages
+        (a list of stage names such as ["M2L_MIXED"])."""
+        if stages is None:
+            _check(lib().ibim_plan_profile(self._h, int(enable)), "ibim_plan_profile")
+            return
+        mask = 0
+        for name in stages:
+            mask |= 1 << STAGES.index(name)
+        _check(lib().ibim_plan_profile_stages(self._h, mask if enable else 0), "ibim_plan_profile_stages")
 
     def stage_times(self, reset=True):
         """{stage: (milliseconds, launches)} from HIP events on the plan's stream."""

@@ -321,6 +321,17 @@ int ibim_plan_profile(ibim_plan *pl, int enable) {
         return IBIM_ERR_ARG;
     }
     pl->timer.on = enable != 0;
+    pl->timer.mask = enable ? ~0u : 0u;
+    return IBIM_OK;
+}
+
+int ibim_plan_profile_stages(ibim_plan *pl, unsigned stage_mask) {
+    if (!pl) {
+        set_error("plan is NULL");
+        return IBIM_ERR_ARG;
+    }
+    pl->timer.mask = stage_mask & ((1u << IBIM_STAGE_COUNT) - 1u);
+    pl->timer.on = pl->timer.mask != 0;
     return IBIM_OK;
 }
 

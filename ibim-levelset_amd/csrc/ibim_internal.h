@@ -291,6 +291,7 @@ enum Stage {
 
 struct StageTimer {
     bool on = false;
+    unsigned mask = 0;  // bit s: stage s is recorded
     std::vector<hipEvent_t> pool;
     struct Rec {
         int stage;
@@ -475,6 +476,7 @@ struct ibim_plan {
     int *ebox = nullptr;                 // per box exponent
     unsigned *Yv = nullptr;              // V then X interaction results as FP24 (3 dwords per 4 rows),
                                          // pairs * (4r rounded up to 32) values
+    int y_rows = 0;                      // pair rows of Yv; one spare row follows them
     int *ytptr = nullptr, *ytlist = nullptr; // rows of Yv of target node b: ytlist[ytptr[b] .. ytptr[b+1])
     int *gtab = nullptr, *stab = nullptr; // gather / scatter tables of the 8 reflections
     ibim::VPairs vp;                     // V pairs for the points as their own targets
@@ -567,7 +569,7 @@ struct StageScope {
     int stage;
     hipEvent_t a = nullptr;
     StageScope(ibim_plan *p, int s) : pl(p), stage(s) {
-        if (pl->timer.on) {
+        if (pl->timer.on && ((pl->timer.mask >> stage) & 1u)) {
             a = pl->timer.take();
             if (a) (void)hipEventRecord(a, pl->stream);
         }

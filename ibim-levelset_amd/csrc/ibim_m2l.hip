@@ -626,7 +626,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) vo
     const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
     const int *__restrict__ pA, const int *__restrict__ keyslot, const _Float16 *__restrict__ ops16u,
     const int *__restrict__ esig, int MRP, const _Float16 *__restrict__ Mh, const int *__restrict__ ebox,
-    unsigned *__restrict__ Y, int nblocks, int per_xcd) {
+    unsigned *__restrict__ Y, int yspare, int nblocks, int per_xcd) {
     constexpr int R = P * P * P;
     constexpr int R4 = 4 * R;
     constexpr int KP = (R4 + 63) / 64 * 64;
@@ -636,12 +636,23 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) vo
     constexpr int SEGS = KC / 8;                 // 16-byte segments per chunk row
     constexpr int RPI = 64 / SEGS;               // chunk rows per LDS-load instruction (64 lanes x 16 B)
     constexpr int DPW = SEGS / NW;               // LDS-load instructions per wave per chunk
-    constexpr int NST = 4;                       // chunks in the LDS ring
+    constexpr int NST = 5;                       // chunks in the LDS ring
+    constexpr int DIST = NST - 2;                // chunks loading ahead of the one being read
+    constexpr int KSC = KC / 16;                 // MFMA k-steps per chunk
+    constexpr int PD = 3;                        // A-fragment prefetch depth (k-steps)
     constexpr int CH = 64 * KC;                  // halves per chunk
     constexpr int NPB = 32 * NW;                 // pairs per block (32 per wave)
     constexpr int YS = (R4 + 31) / 32 * 32;      // row stride of Y
     constexpr int MR = (R4 + 63) / 64 * 64;      // operator rows (padded)
     static_assert(KP % KC == 0 && DPW >= 1 && DPW * NW == SEGS && R4 % 4 == 0 && P <= 5, "tile shape");
+    static_assert(KSC > PD && (NKC * KSC) % (PD + 1) == 0, "pipeline shape");
+    // Every row tile's results leave as exactly NSTO stores per wave (the
+    // padded rows and the lanes without a pair write too, the latter to the
+    // spare row), so the loads of a chunk can be waited for exactly while
+    // the stores are in flight: vmcnt completes in issue order.  Otherwise
+    // the waits also drain the stores and the chunks loading behind them.
+    constexpr bool COUNTED = YS >= MR && NKC >= DIST;
+    constexpr int NSTO = 8;
     __shared__ __attribute__((aligned(1024))) _Float16 sA[NST][CH];
     __shared__ int sEA[NPB];
     __shared__ __attribute__((aligned(16))) int sE[MR];  // the operator's row exponents
@@ -668,7 +679,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) vo
         }
     };
 #pragma unroll
-    for (int c = 0; c < NST - 1; ++c)
+    for (int c = 0; c < DIST; ++c)
         if (c < NC) issue(c);
     if (tid < NPB) {
         const int j = j0 + tid;
@@ -684,61 +695,73 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) vo
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) xr[ks] = ok ? *(const h8 *)(src + 16 * ks) : h8{};
     }
-    const bool live = j0 + 32 * w < jend;
-    int c = 0;
+    // Chunk cc becomes readable at arrive(cc): each wave waits for its own
+    // part of the chunk, then the block barrier; the chunk DIST ahead is then
+    // issued into the slot of chunk cc - 2, which every wave has finished.
+    // arrive(cc + 1) runs PD k-steps before the end of chunk cc, so the
+    // A-fragments of the next chunk are read while this one multiplies: the
+    // fragments of k-step g + PD are requested at step g, across chunk and
+    // row-tile boundaries.  (Stores only add to vmcnt and make a wait
+    // stricter, never looser.)
+    auto arrive = [&](int cc, bool after_stores) {
+        if (cc + DIST - 1 >= NC) wait_vm<0>();
+        else if (COUNTED && after_stores) wait_vm<DPW * (DIST - 1) + NSTO>();
+        else wait_vm<DPW * (DIST - 1)>();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        if (cc + DIST < NC) issue(cc + DIST);
+    };
+    auto frag = [&](int cc, int ks, h8 *k) {
+        const _Float16 *sa = sA[cc % NST];
+        const int sg = (2 * ks + (l >> 5)) ^ ((l & 31) % SEGS);
+        k[0] = *(const h8 *)(sa + (l & 31) * KC + 8 * sg);
+        k[1] = *(const h8 *)(sa + (32 + (l & 31)) * KC + 8 * sg);
+    };
+    h8 F[PD + 1][2];  // fragments of k-steps g .. g + PD (g mod (PD + 1))
+    arrive(0, false);
+#pragma unroll
+    for (int q = 0; q < PD; ++q) frag(0, q, F[q]);
     for (int rt = 0; rt < nrt; ++rt) {
         f16v acc[2];
         acc[0] = f16v{};
         acc[1] = f16v{};
 #pragma unroll
-        for (int kc = 0; kc < NKC; ++kc, ++c) {
-            if (c + NST - 2 < NC) wait_vm<DPW * (NST - 2)>();
-            else wait_vm<0>();
-            asm volatile("" ::: "memory");
-            __builtin_amdgcn_s_barrier();
-            asm volatile("" ::: "memory");
-            __builtin_amdgcn_sched_barrier(0);
-            if (c + NST - 1 < NC) issue(c + NST - 1);
-            const _Float16 *sa = sA[c % NST];
-            // A-fragments double-buffered: step ks + 1 is read while step
-            // ks multiplies (the LDS latency would otherwise stall each step)
-            auto frag = [&](int ks, h8 *k) {
-                const int sg = (2 * ks + (l >> 5)) ^ ((l & 31) % SEGS);
-                k[0] = *(const h8 *)(sa + (l & 31) * KC + 8 * sg);
-                k[1] = *(const h8 *)(sa + (32 + (l & 31)) * KC + 8 * sg);
-            };
-            static_assert((KC / 16) % 2 == 0, "k-steps per chunk");
-            // (sched_barrier: the scheduler would sink the reads next to
-            // their use to save registers)
-            if (!live) continue;  // (wave-uniform: no valid pair in this wave's 32)
-            h8 ka[2], kb[2];
-            frag(0, ka);
+        for (int kc = 0; kc < NKC; ++kc) {
+            const int c = rt * NKC + kc;
 #pragma unroll
-            for (int ks = 0; ks < KC / 16; ks += 2) {
-                frag(ks + 1, kb);
+            for (int ks = 0; ks < KSC; ++ks) {
+                const int gi = kc * KSC + ks;  // step within the row tile (a multiple of PD + 1 long)
+                // the previous row tile's stores are younger than the loads
+                // of chunk c + 1 when kc < DIST
+                if (ks == KSC - PD && c + 1 < NC) arrive(c + 1, rt > 0 && kc < DIST);
+                // (past the last chunk the read repeats a resident slot and
+                // is not used: no branch, so the LDS wait counts stay exact)
+                const int c2 = min(c + (ks + PD) / KSC, NC - 1);
+                frag(c2, (ks + PD) % KSC, F[(gi + PD) % (PD + 1)]);
+                // (sched_barrier: the scheduler would sink the reads next to
+                // their use to save registers)
                 __builtin_amdgcn_sched_barrier(0);
-                const h8 b0 = xr[kc * (KC / 16) + ks], b1 = xr[kc * (KC / 16) + ks + 1];
-                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ka[0], b0, acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ka[1], b0, acc[1], 0, 0, 0);
-                if (ks + 2 < KC / 16) frag(ks + 2, ka);
-                __builtin_amdgcn_sched_barrier(0);
-                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(kb[0], b1, acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(kb[1], b1, acc[1], 0, 0, 0);
+                const h8 b = xr[gi];
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(F[gi % (PD + 1)][0], b, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(F[gi % (PD + 1)][1], b, acc[1], 0, 0, 0);
             }
         }
         // rows 8 q + 4 (l >> 5) .. + 3 of each 32-row half, pair l & 31; R4
         // is a multiple of 4, so a lane's four rows are all below R4 or all
         // above
         const int col = 32 * w + (l & 31);
-        if (j0 + col < jend) {
-            const int ea = sEA[col];
-            unsigned *yp = Y + (size_t)(j0 + col) * YS / 4 * 3;
+        const bool has_pair = j0 + col < jend;
+        if (COUNTED || has_pair) {
+            const int ea = sEA[col];  // 0 without a pair
+            unsigned *yp = Y + (size_t)(has_pair ? j0 + col : yspare) * YS / 4 * 3;
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int row = rt * 64 + 32 * h + 8 * q + 4 * (l >> 5);
-                    if (row >= R4) continue;
+                    if (!COUNTED && row >= R4) continue;  // (COUNTED: row < MR <= YS)
                     const int4 er = *(const int4 *)&sE[row];
                     const f16v &a = acc[h];
                     store4_fp24(yp + row / 4 * 3, ldexpf(a[4 * q], er.x + ea), ldexpf(a[4 * q + 1], er.y + ea),
@@ -792,25 +815,27 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
             constexpr int YS = (R4 + 31) / 32 * 32;
             k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, KP, pl->Mh, pl->ebox);
             // V pairs, then (x16) the X pairs into the rows after them
-            auto gemm = [&](const VPairs &pp, const int *keyslot, unsigned *Y, bool vlist) {
+            // Y: the launch's first pair row; yspare: the spare row after
+            // all pair rows, relative to Y
+            auto gemm = [&](const VPairs &pp, const int *keyslot, unsigned *Y, int yspare, bool vlist) {
                 const int pxcd = (pp.hnblocks + 7) / 8;
                 if constexpr (P <= 5) {
                     if (vlist)
                         k_m2l_gemm_h32<P, kH32WavesV><<<8 * pxcd, 64 * kH32WavesV, 0, st>>>(
                             pp.hblk_cls, pp.hblk_j0, pp.cls_off_d, pp.A, keyslot, pl->ops16, pl->esig, pl->MRP,
-                            pl->Mh, pl->ebox, Y, pp.hnblocks, pxcd);
+                            pl->Mh, pl->ebox, Y, yspare, pp.hnblocks, pxcd);
                     else
                         k_m2l_gemm_h32<P, kH32WavesX><<<8 * pxcd, 64 * kH32WavesX, 0, st>>>(
                             pp.hblk_cls, pp.hblk_j0, pp.cls_off_d, pp.A, keyslot, pl->ops16, pl->esig, pl->MRP,
-                            pl->Mh, pl->ebox, Y, pp.hnblocks, pxcd);
+                            pl->Mh, pl->ebox, Y, yspare, pp.hnblocks, pxcd);
                 } else
                     k_m2l_gemm_hu<P, 16><<<8 * pxcd, 256, 0, st>>>(pp.hblk_cls, pp.hblk_j0, pp.cls_off_d, pp.A,
                                                                    keyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
                                                                    pl->ebox, Y, pp.hnblocks, pxcd);
             };
-            if (vp.hnblocks > 0) gemm(vp, pl->keyslot, pl->Yv, true);
+            if (vp.hnblocks > 0) gemm(vp, pl->keyslot, pl->Yv, pl->y_rows, true);
             if (pl->x16 && pl->xp.hnblocks > 0)
-                gemm(pl->xp, pl->xkeyslot, pl->Yv + (size_t)vp.npairs * YS / 4 * 3, false);
+                gemm(pl->xp, pl->xkeyslot, pl->Yv + (size_t)vp.npairs * YS / 4 * 3, pl->y_rows - vp.npairs, false);
             k_v_reduce<R4><<<pl->n_m2l_work, R4 / 4 <= 128 ? 128 : 256, 0, st>>>(pl->m2l_work, pl->ytptr, pl->ytlist,
                                                                                 pl->Yv, pl->L);
             IBIM_LAUNCH_CHECK();
@@ -1275,8 +1300,9 @@ int build_m2l_operators_f16(ibim_plan *pl) {
     };
     int rc;
     if ((rc = dalloc(pl, &pl->Mh, (size_t)pl->nnodes * KP)) || (rc = dalloc(pl, &pl->ebox, (size_t)pl->nnodes)) ||
-        (rc = dalloc(pl, &pl->Yv, std::max<size_t>(1, (size_t)npa * YS / 4 * 3))))
+        (rc = dalloc(pl, &pl->Yv, (size_t)(npa + 1) * YS / 4 * 3)))
         return rc;
+    pl->y_rows = npa;
     // unfolded operators for the (class, mask) keys that have pairs: V keys
     // in slots [0, nsv), X keys in [nsv, nsv + nsx)
     auto slots = [](const std::vector<int> &off, int first, std::vector<int> &keyslot, std::vector<int> &slot_key) {

@@ -132,6 +132,10 @@ int64_t ibim_plan_footprint(const ibim_plan *plan);
 #define IBIM_STAGE_P2P_MIXED 12    /* near field + L2P of a mixed-precision matvec (not in P2P) */
 #define IBIM_STAGE_COUNT 13
 int ibim_plan_profile(ibim_plan *plan, int enable);
+/* record only the stages whose bit (1u << IBIM_STAGE_x) is set; 0 = off.
+   Each recorded stage costs two events on the plan's stream per launch.
+   No counterpart in the reference (which has no stage timers). */
+int ibim_plan_profile_stages(ibim_plan *plan, unsigned stage_mask);
 int ibim_plan_stage_times(ibim_plan *plan, double *ms, int64_t *launches, int reset);
 
 /* ---- the hot path -------------------------------------------------------- */
