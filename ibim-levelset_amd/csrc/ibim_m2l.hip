@@ -651,8 +651,14 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) vo
     // spare row), so the loads of a chunk can be waited for exactly while
     // the stores are in flight: vmcnt completes in issue order.  Otherwise
     // the waits also drain the stores and the chunks loading behind them.
+    // COUNTED tiles pass through a per-wave LDS stage so that each pair's
+    // 64 rows (48 dwords of FP24) leave as 16-byte stores of consecutive
+    // lanes instead of 12-byte pieces 32 pairs apart.
     constexpr bool COUNTED = YS >= MR && NKC >= DIST;
-    constexpr int NSTO = 8;
+    constexpr bool STAGED = COUNTED && NW == 8;   // (the LDS of 4-wave blocks stays at two blocks per CU)
+    constexpr int PS = 52;                        // staged dwords per pair (48 used; 16-byte rows)
+    constexpr int NSTO = STAGED ? 6 : 8;          // 32 pairs x 12 16-byte pieces / 64 lanes, or 8 x 12 bytes
+    __shared__ __attribute__((aligned(16))) unsigned sStg[STAGED ? NW * 32 * PS : 1];
     __shared__ __attribute__((aligned(1024))) _Float16 sA[NST][CH];
     __shared__ int sEA[NPB];
     __shared__ __attribute__((aligned(16))) int sE[MR];  // the operator's row exponents
@@ -753,15 +759,59 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) vo
         // above
         const int col = 32 * w + (l & 31);
         const bool has_pair = j0 + col < jend;
-        if (COUNTED || has_pair) {
+        if constexpr (STAGED) {
+            const int ea = sEA[col];  // 0 without a pair
+            unsigned *stg = sStg + w * 32 * PS;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r4 = 8 * h + 2 * q + (l >> 5);  // 4-row group within the tile
+                    const int4 er = *(const int4 *)&sE[rt * 64 + 4 * r4];
+                    const f16v &a = acc[h];
+                    store4_fp24(stg + (l & 31) * PS + 3 * r4, ldexpf(a[4 * q], er.x + ea),
+                                ldexpf(a[4 * q + 1], er.y + ea), ldexpf(a[4 * q + 2], er.z + ea),
+                                ldexpf(a[4 * q + 3], er.w + ea));
+                }
+            // (a wave's LDS operations execute in order: no barrier; the
+            // pieces go two at a time to bound the live registers)
+#pragma unroll
+            for (int i = 0; i < NSTO; i += 2) {
+                uint4 v[2];
+                int o[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int t = 64 * (i + u) + l, pp = t / 12, k = t - 12 * pp;
+                    v[u] = *(const uint4 *)(stg + pp * PS + 4 * k);
+                    const int jj = j0 + 32 * w + pp;
+                    o[u] = (jj < jend ? jj : yspare) * (YS / 4 * 3) + rt * 48 + 4 * k;
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) *(uint4 *)(Y + (size_t)(unsigned)o[u]) = v[u];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else if (COUNTED) {
             const int ea = sEA[col];  // 0 without a pair
             unsigned *yp = Y + (size_t)(has_pair ? j0 + col : yspare) * YS / 4 * 3;
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
+                    const int row = rt * 64 + 32 * h + 8 * q + 4 * (l >> 5);  // < MR <= YS
+                    const int4 er = *(const int4 *)&sE[row];
+                    const f16v &a = acc[h];
+                    store4_fp24(yp + row / 4 * 3, ldexpf(a[4 * q], er.x + ea), ldexpf(a[4 * q + 1], er.y + ea),
+                                ldexpf(a[4 * q + 2], er.z + ea), ldexpf(a[4 * q + 3], er.w + ea));
+                }
+        } else if (has_pair) {
+            const int ea = sEA[col];
+            unsigned *yp = Y + (size_t)(j0 + col) * YS / 4 * 3;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
                     const int row = rt * 64 + 32 * h + 8 * q + 4 * (l >> 5);
-                    if (!COUNTED && row >= R4) continue;  // (COUNTED: row < MR <= YS)
+                    if (row >= R4) continue;
                     const int4 er = *(const int4 *)&sE[row];
                     const f16v &a = acc[h];
                     store4_fp24(yp + row / 4 * 3, ldexpf(a[4 * q], er.x + ea), ldexpf(a[4 * q + 1], er.y + ea),
