@@ -622,7 +622,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 constexpr int kH32WavesV = 8, kH32WavesX = 4;
 
 template <int P, int NW>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) void k_m2l_gemm_h32(
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW >= 8 ? 2 : 1))) void k_m2l_gemm_h32(
     const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
     const int *__restrict__ pA, const int *__restrict__ keyslot, const _Float16 *__restrict__ ops16u,
     const int *__restrict__ esig, int MRP, const _Float16 *__restrict__ Mh, const int *__restrict__ ebox,
