@@ -334,7 +334,8 @@ def main():
     pairs = stats["v_pairs"] + (stats["x_pairs"] if x_in_stage else 0)
     flops = 32.0 * r * r * pairs  # 2 x (4r) x (4r) per V or X interaction
     achieved = flops / (ms_gemm * 1e-3) / 1e12
-    gemm16 = f"k_m2l_gemm_h32<{p}>" if p <= 5 else f"k_m2l_gemm_hu<{p},16>" if p == 6 else f"k_m2l_gemm<{p},float>"
+    gemm16 = (f"k_m2l_gemm_h32<{p},8> (V) + k_m2l_gemm_h32<{p},4> (X)" if p <= 5 else
+              f"k_m2l_gemm_hu<{p},16>" if p == 6 else f"k_m2l_gemm<{p},float>")
     kern, peak = {"mixed_f32": (f"k_m2l_gemm<{p},float> (FP32 MFMA)", PEAK_TFLOPS["f32"]),
                   "mixed_f16": (f"k_m_to_half + {gemm16} + k_v_reduce (FP16 MFMA)" if p <= 6 else
                                 f"{gemm16} (FP32 MFMA; the FP16 tiles stop at p = 6)",
