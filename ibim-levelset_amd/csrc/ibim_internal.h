@@ -379,6 +379,8 @@ struct VPairs {
     int *blk_cls = nullptr, *blk_j0 = nullptr;
     int hnblocks = 0;                   // FP16 GEMM blocks (128 pairs for p <= 5, 64 for p = 6)
     int *hblk_cls = nullptr, *hblk_j0 = nullptr;
+    int *hcls_off_d = nullptr;          // X list: pairs of class c in [hcls_off_d[c], hcls_off_d[c+1]); the FP16
+                                        // blocks of the X list span whole classes (all 8 masks)
 };
 
 // a set of targets sorted into the leaves of a plan's tree
@@ -477,6 +479,8 @@ struct ibim_plan {
     unsigned *Yv = nullptr;              // V then X interaction results as FP24 (3 dwords per 4 rows),
                                          // pairs * (4r rounded up to 32) values
     int y_rows = 0;                      // pair rows of Yv; one spare row follows them
+    _Float16 *Bx = nullptr;              // X pairs: reflected FP16 source multipoles, one row per pair
+    int *ebx = nullptr, *xiota = nullptr; // X pairs: the source box exponent; 0, 1, 2, ...
     int *ytptr = nullptr, *ytlist = nullptr; // rows of Yv of target node b: ytlist[ytptr[b] .. ytptr[b+1])
     int *gtab = nullptr, *stab = nullptr; // gather / scatter tables of the 8 reflections
     ibim::VPairs vp;                     // V pairs for the points as their own targets

@@ -821,28 +821,101 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW >= 8
     }
 }
 
+// X pair j: its source multipole reflected by the pair's mask (the gather
+// table's permutation and input signs), so that the class operator applies
+// unreflected; and the source box's exponent
+__global__ __launch_bounds__(64) void k_x_reflect_half(const int *__restrict__ A, const unsigned char *__restrict__ mask,
+                                                       const int *__restrict__ gtab, int R4, int KP,
+                                                       const _Float16 *__restrict__ Mh, const int *__restrict__ ebox,
+                                                       _Float16 *__restrict__ Bx, int *__restrict__ ebx) {
+    const int j = blockIdx.x;
+    const int a = A[j], m = mask[j];
+    const _Float16 *src = Mh + (size_t)a * KP;
+    _Float16 *dst = Bx + (size_t)j * KP;
+    constexpr int U = 8;  // elements per lane per pass, their loads in flight together
+    for (int k0 = 0; k0 < KP; k0 += 64 * U) {
+        int g[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + 64 * u + threadIdx.x;
+            g[u] = k < R4 ? gtab[m * R4 + k] : -1;
+        }
+        _Float16 x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = g[u] == -1 ? (_Float16)0 : src[g[u] & 0x7fffffff];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + 64 * u + threadIdx.x;
+            if (k < KP) dst[k] = g[u] != -1 && g[u] < 0 ? -x[u] : x[u];
+        }
+    }
+    if (threadIdx.x == 0) ebx[j] = ebox[a];
+}
+
 // L_b += sum over the V pairs (a, b) of the pair's result: one block per
-// target node, a thread per four rows, four pairs' rows in flight per step
-// (fixed order: the sum is deterministic)
+// target node, a thread per four rows (one pass: 4 blockDim >= R4), four
+// pairs' rows in flight per step.  The rows of X pairs (row index >= npv,
+// after the V rows of each target) hold the unreflected class operator's
+// result: element e of the target takes row element stab[m][e] of the
+// pair's mask m, with its sign; such a row is decoded into LDS first.
+// Fixed order: the sum is deterministic.
 template <int R4>
 __global__ __launch_bounds__(256) void k_v_reduce(const int *__restrict__ work, const int *__restrict__ tptr,
                                                   const int *__restrict__ tlist, const unsigned *__restrict__ Y,
-                                                  double *__restrict__ L) {
+                                                  int npv, const unsigned char *__restrict__ xmask,
+                                                  const int *__restrict__ stab, double *__restrict__ L) {
     constexpr int YS = (R4 + 31) / 32 * 32;
+    static_assert(R4 % 4 == 0 && R4 / 4 <= 256, "one pass");
+    __shared__ __attribute__((aligned(16))) float sx[R4];
     const int b = work[blockIdx.x];
     const int q0 = tptr[b], q1 = tptr[b + 1];
     if (q0 == q1) return;
-    for (int e = 4 * threadIdx.x; e < R4; e += 4 * blockDim.x) {
+    int qx = q1;  // the X rows close the list
+    while (qx > q0 && tlist[qx - 1] >= npv) --qx;
+    const int e = 4 * threadIdx.x;
+    const bool act = e < R4;
+    f4v ax = {0.f, 0.f, 0.f, 0.f};
+    // X rows, up to XB at a time: this thread's four elements of each are
+    // loaded together, then passed through LDS for the permutation
+    constexpr int XB = 4;
+    for (int qb = qx; qb < q1; qb += XB) {
+        f4v xv[XB];
+        int xm[XB];
+#pragma unroll
+        for (int i = 0; i < XB; ++i) {
+            const int q = min(qb + i, q1 - 1);
+            const int row = tlist[q];
+            xm[i] = xmask[row - npv];
+            xv[i] = act ? load4_fp24(Y + (size_t)row * YS / 4 * 3 + e / 4 * 3) : f4v{};
+        }
+#pragma unroll
+        for (int i = 0; i < XB; ++i) {
+            if (qb + i >= q1) break;  // (block-uniform)
+            __syncthreads();  // the previous row is read
+            if (act) *(f4v *)&sx[e] = xv[i];
+            __syncthreads();
+            if (act) {
+                const int *st = stab + xm[i] * R4 + e;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int sr = st[u];
+                    const float v = sx[sr & 0x7fffffff];
+                    ax[u] += sr < 0 ? -v : v;
+                }
+            }
+        }
+    }
+    if (act) {
         f4v a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0, a2 = a0, a3 = a0;
         int q = q0;
-        for (; q + 3 < q1; q += 4) {
+        for (; q + 3 < qx; q += 4) {
             a0 += load4_fp24(Y + ((size_t)tlist[q] * YS + e) / 4 * 3);
             a1 += load4_fp24(Y + ((size_t)tlist[q + 1] * YS + e) / 4 * 3);
             a2 += load4_fp24(Y + ((size_t)tlist[q + 2] * YS + e) / 4 * 3);
             a3 += load4_fp24(Y + ((size_t)tlist[q + 3] * YS + e) / 4 * 3);
         }
-        for (; q < q1; ++q) a0 += load4_fp24(Y + ((size_t)tlist[q] * YS + e) / 4 * 3);
-        const f4v t = (a0 + a1) + (a2 + a3);
+        for (; q < qx; ++q) a0 += load4_fp24(Y + ((size_t)tlist[q] * YS + e) / 4 * 3);
+        const f4v t = ((a0 + a1) + (a2 + a3)) + ax;
         double *dst = L + (size_t)b * R4 + e;
         dst[0] += (double)t[0];
         dst[1] += (double)t[1];
@@ -864,30 +937,41 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
             constexpr int KP = (R4 + 63) / 64 * 64;
             constexpr int YS = (R4 + 31) / 32 * 32;
             k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, KP, pl->Mh, pl->ebox);
-            // V pairs, then (x16) the X pairs into the rows after them
-            // Y: the launch's first pair row; yspare: the spare row after
-            // all pair rows, relative to Y
-            auto gemm = [&](const VPairs &pp, const int *keyslot, unsigned *Y, int yspare, bool vlist) {
+            // V pairs, then (x16) the X pairs into the rows after them.
+            // key_off: pair range of each block key; pA / Mh / ebox: the
+            // pairs' multipole rows and exponents; Y: the launch's first
+            // pair row; yspare: the spare row after all pair rows, relative
+            // to Y
+            auto gemm = [&](const VPairs &pp, const int *key_off, const int *pA, const _Float16 *Mh, const int *ebox,
+                            const int *keyslot, unsigned *Y, int yspare, bool vlist) {
                 const int pxcd = (pp.hnblocks + 7) / 8;
                 if constexpr (P <= 5) {
                     if (vlist)
                         k_m2l_gemm_h32<P, kH32WavesV><<<8 * pxcd, 64 * kH32WavesV, 0, st>>>(
-                            pp.hblk_cls, pp.hblk_j0, pp.cls_off_d, pp.A, keyslot, pl->ops16, pl->esig, pl->MRP,
-                            pl->Mh, pl->ebox, Y, yspare, pp.hnblocks, pxcd);
+                            pp.hblk_cls, pp.hblk_j0, key_off, pA, keyslot, pl->ops16, pl->esig, pl->MRP, Mh, ebox, Y,
+                            yspare, pp.hnblocks, pxcd);
                     else
                         k_m2l_gemm_h32<P, kH32WavesX><<<8 * pxcd, 64 * kH32WavesX, 0, st>>>(
-                            pp.hblk_cls, pp.hblk_j0, pp.cls_off_d, pp.A, keyslot, pl->ops16, pl->esig, pl->MRP,
-                            pl->Mh, pl->ebox, Y, yspare, pp.hnblocks, pxcd);
+                            pp.hblk_cls, pp.hblk_j0, key_off, pA, keyslot, pl->ops16, pl->esig, pl->MRP, Mh, ebox, Y,
+                            yspare, pp.hnblocks, pxcd);
                 } else
-                    k_m2l_gemm_hu<P, 16><<<8 * pxcd, 256, 0, st>>>(pp.hblk_cls, pp.hblk_j0, pp.cls_off_d, pp.A,
-                                                                   keyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
-                                                                   pl->ebox, Y, pp.hnblocks, pxcd);
+                    k_m2l_gemm_hu<P, 16><<<8 * pxcd, 256, 0, st>>>(pp.hblk_cls, pp.hblk_j0, key_off, pA, keyslot,
+                                                                   pl->ops16, pl->esig, pl->MRP, Mh, ebox, Y,
+                                                                   pp.hnblocks, pxcd);
             };
-            if (vp.hnblocks > 0) gemm(vp, pl->keyslot, pl->Yv, pl->y_rows, true);
-            if (pl->x16 && pl->xp.hnblocks > 0)
-                gemm(pl->xp, pl->xkeyslot, pl->Yv + (size_t)vp.npairs * YS / 4 * 3, pl->y_rows - vp.npairs, false);
-            k_v_reduce<R4><<<pl->n_m2l_work, R4 / 4 <= 128 ? 128 : 256, 0, st>>>(pl->m2l_work, pl->ytptr, pl->ytlist,
-                                                                                pl->Yv, pl->L);
+            if (vp.hnblocks > 0)
+                gemm(vp, vp.cls_off_d, vp.A, pl->Mh, pl->ebox, pl->keyslot, pl->Yv, pl->y_rows, true);
+            const VPairs &xp = pl->xp;
+            if (pl->x16 && xp.hnblocks > 0) {
+                // X list: blocks over whole classes, the reflection applied
+                // to the multipoles here and to the results in k_v_reduce
+                k_x_reflect_half<<<xp.npairs, 64, 0, st>>>(xp.A, xp.mask, pl->gtab, R4, KP, pl->Mh, pl->ebox, pl->Bx,
+                                                           pl->ebx);
+                gemm(xp, xp.hcls_off_d, pl->xiota, pl->Bx, pl->ebx, pl->xkeyslot,
+                     pl->Yv + (size_t)vp.npairs * YS / 4 * 3, pl->y_rows - vp.npairs, false);
+            }
+            k_v_reduce<R4><<<pl->n_m2l_work, R4 / 4 <= 128 ? 128 : 256, 0, st>>>(
+                pl->m2l_work, pl->ytptr, pl->ytlist, pl->Yv, vp.npairs, xp.mask, pl->stab, pl->L);
             IBIM_LAUNCH_CHECK();
             return IBIM_OK;
         }
@@ -1251,12 +1335,16 @@ int build_xpairs(ibim_plan *pl) {
             bj.push_back(j);
         }
     xp.nblocks = (int)bc.size();
-    std::vector<int> hc, hj;  // FP16 GEMM blocks, as for the V pairs
+    // FP16 GEMM blocks over whole classes: that form applies the reflection
+    // to the multipoles and to the results, so all 8 masks of a class share
+    // the class operator
+    std::vector<int> hc, hj, hco(nc + 1);
+    for (int c = 0; c <= nc; ++c) hco[c] = xp.cls_off[8 * c];
     if (pl->p <= 6) {
         const int hnp = pl->p <= 5 ? 32 * kH32WavesX : 64;
-        for (int key = 0; key < 8 * nc; ++key)
-            for (int j = xp.cls_off[key]; j < xp.cls_off[key + 1]; j += hnp) {
-                hc.push_back(key);
+        for (int c = 0; c < nc; ++c)
+            for (int j = hco[c]; j < hco[c + 1]; j += hnp) {
+                hc.push_back(c);
                 hj.push_back(j);
             }
     }
@@ -1265,13 +1353,15 @@ int build_xpairs(ibim_plan *pl) {
     if ((rc = dalloc(pl, &xp.A, np)) || (rc = dalloc(pl, &xp.B, np)) || (rc = dalloc(pl, &xp.mask, np)) ||
         (rc = dalloc(pl, &xp.cls_off_d, xp.cls_off.size())) || (rc = dalloc(pl, &xp.blk_cls, xp.nblocks)) ||
         (rc = dalloc(pl, &xp.blk_j0, xp.nblocks)) || (rc = dalloc(pl, &xp.hblk_cls, std::max(1, xp.hnblocks))) ||
-        (rc = dalloc(pl, &xp.hblk_j0, std::max(1, xp.hnblocks))) || (rc = salloc(&d_cls, cls.size())) ||
+        (rc = dalloc(pl, &xp.hblk_j0, std::max(1, xp.hnblocks))) || (rc = dalloc(pl, &xp.hcls_off_d, nc + 1)) ||
+        (rc = salloc(&d_cls, cls.size())) ||
         (rc = dalloc(pl, &pl->xops, (size_t)nc * pl->MRP * pl->KRP)))
         return rc;
     k_permute_pairs<<<nblk(np), 256, 0, st>>>(np, order, a0, b0, m0, xp.A, xp.B, xp.mask);
     IBIM_LAUNCH_CHECK();
     IBIM_HIP_CHECK(hipMemcpyAsync(xp.cls_off_d, xp.cls_off.data(), xp.cls_off.size() * sizeof(int),
                                   hipMemcpyHostToDevice, st));
+    IBIM_HIP_CHECK(hipMemcpyAsync(xp.hcls_off_d, hco.data(), hco.size() * sizeof(int), hipMemcpyHostToDevice, st));
     IBIM_HIP_CHECK(hipMemcpyAsync(xp.blk_cls, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice, st));
     IBIM_HIP_CHECK(hipMemcpyAsync(xp.blk_j0, bj.data(), bj.size() * sizeof(int), hipMemcpyHostToDevice, st));
     if (xp.hnblocks > 0) {
@@ -1363,11 +1453,19 @@ int build_m2l_operators_f16(ibim_plan *pl) {
                 slot_key.push_back((int)k);
             }
     };
-    std::vector<int> keyslot, slot_key, xkeyslot, xslot_key;
+    std::vector<int> keyslot, slot_key, xkeyslot, xunf;
     slots(vp.cls_off, 0, keyslot, slot_key);
     const int nsv = (int)slot_key.size();
-    if (with_x) slots(xp.cls_off, nsv, xkeyslot, xslot_key);
-    const int nsx = (int)xslot_key.size();
+    // X list: one slot per class (its unreflected operator, mask 0), indexed
+    // by class; xunf is the same in the key-indexed form the unfolding takes
+    if (with_x) {
+        const int nc = (int)(xp.cls_off.size() - 1) / 8;
+        xkeyslot.assign(nc, -1);
+        xunf.assign(8 * (size_t)nc, -1);
+        for (int c = 0; c < nc; ++c)
+            if (xp.cls_off[8 * c + 8] > xp.cls_off[8 * c]) xunf[8 * c] = xkeyslot[c] = nsv + c;
+    }
+    const int nsx = (int)xkeyslot.size();
     const int ns = nsv + nsx;
     if ((rc = dalloc(pl, &pl->keyslot, std::max<size_t>(1, keyslot.size()))) ||
         (rc = dalloc(pl, &pl->xkeyslot, std::max<size_t>(1, xkeyslot.size()))) ||
@@ -1404,7 +1502,14 @@ int build_m2l_operators_f16(ibim_plan *pl) {
         IBIM_LAUNCH_CHECK();
         return IBIM_OK;
     };
-    if ((rc = unfold(keyslot, pl->ops)) || (with_x && (rc = unfold(xkeyslot, pl->xops)))) return rc;
+    if ((rc = unfold(keyslot, pl->ops)) || (with_x && (rc = unfold(xunf, pl->xops)))) return rc;
+    if (with_x) {
+        if ((rc = dalloc(pl, &pl->Bx, (size_t)npx * KP)) || (rc = dalloc(pl, &pl->ebx, (size_t)npx)) ||
+            (rc = dalloc(pl, &pl->xiota, (size_t)npx)))
+            return rc;
+        k_iota_pairs<<<nblk(npx), 256, 0, st>>>(npx, pl->xiota);
+        IBIM_LAUNCH_CHECK();
+    }
     // rows of Yv per target node: a stable sort of the target boxes of the
     // V pairs followed by the X pairs
     const int nn = pl->nnodes;
