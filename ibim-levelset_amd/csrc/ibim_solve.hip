@@ -1,7 +1,9 @@
 // ibim_solve.hip -- the steps either side of the matvec, on the device:
 // right-hand side (electric.cpp:395-413), target sets for the reaction
 // potential at atom centres (polarizeMap, electric.cpp:442-485) and restarted
-// GMRES with modified Gram-Schmidt and Givens rotations (gmres.cpp:10-164).
+// GMRES with modified Gram-Schmidt and Givens rotations (gmres.cpp:10-164;
+// the inner solves of the mixed-precision refinement use an FP32 basis and
+// classical Gram-Schmidt).
 //
 // GMRES keeps the Krylov basis, the Hessenberg matrix, the rotations and the
 // residual estimate in HBM; the host only reads the residual estimate once
@@ -182,18 +184,20 @@ __global__ __launch_bounds__(kRB) void k_mgs(double *__restrict__ w, const doubl
     if (threadIdx.x == 0) part_out[blockIdx.x] = v;
 }
 
-// H(i+1,i) = |w|; v_{i+1} = w * (1/|w|)
-__global__ __launch_bounds__(kRB) void k_normalize(const double *__restrict__ w, double *__restrict__ vout, int n,
+// H(i+1,i) = |w|; v_{i+1} = w * (1/|w|) (VT: the basis type)
+template <typename VT>
+__global__ __launch_bounds__(kRB) void k_normalize(const double *__restrict__ w, VT *__restrict__ vout, int n,
                                                    const double *__restrict__ part_in, double *__restrict__ hslot) {
     __shared__ double sh[kRB];
     const double nr = sqrt(sum_partials(part_in, sh));
     if (blockIdx.x == 0 && threadIdx.x == 0) *hslot = nr;
     const double inv = 1.0 / nr;
-    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) vout[i] = w[i] * inv;
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) vout[i] = (VT)(w[i] * inv);
 }
 
 // v0 = r * (1/beta); s = 0; s(0) = beta
-__global__ __launch_bounds__(kRB) void k_start(const double *__restrict__ r, double *__restrict__ v0, int n,
+template <typename VT>
+__global__ __launch_bounds__(kRB) void k_start(const double *__restrict__ r, VT *__restrict__ v0, int n,
                                                const double *__restrict__ part_in, double *__restrict__ s, int m,
                                                double *__restrict__ beta_out) {
     __shared__ double sh[kRB];
@@ -204,7 +208,65 @@ __global__ __launch_bounds__(kRB) void k_start(const double *__restrict__ r, dou
         *beta_out = beta;
     }
     const double inv = 1.0 / beta;
-    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) v0[i] = r[i] * inv;
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) v0[i] = (VT)(r[i] * inv);
+}
+
+// ---- inner (mixed-precision) Arnoldi steps: FP32 basis, classical
+// Gram-Schmidt.  The inner solves of the refinement reduce the residual by
+// about 1e-5 (what the FP32 / FP16 operator supports), and the FP64 outer
+// residual decides convergence, so their basis is kept in FP32 and
+// orthogonalised with one pass over it for all projections and one for the
+// update, instead of two passes per basis vector (modified Gram-Schmidt).
+
+// FP32 basis vector -> the matvec's FP64 input
+__global__ void k_f2d(const float *__restrict__ v, double *__restrict__ d, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (double)v[i];
+}
+
+// partials of <w, v_k>, k < kn: part[k kRG + block]
+__global__ __launch_bounds__(kRB) void k_cgs_dots(const double *__restrict__ w, const float *__restrict__ V,
+                                                  size_t ldv, int n, int kn, double *__restrict__ part) {
+    __shared__ double sh[kRB];
+    for (int k = 0; k < kn; ++k) {
+        const float *vk = V + (size_t)k * ldv;
+        double v = 0.0;
+        for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) v += w[i] * (double)vk[i];
+        v = block_sum(v, sh);
+        if (threadIdx.x == 0) part[(size_t)k * kRG + blockIdx.x] = v;
+    }
+}
+
+// one block per k: coef[k] = <w, v_k>; H(k,i) = coef[k] (first pass) or
+// += coef[k] (reorthogonalisation)
+__global__ __launch_bounds__(kRB) void k_cgs_coef(const double *__restrict__ part, double *__restrict__ coef,
+                                                  double *__restrict__ hcol, int first) {
+    __shared__ double sh[kRB];
+    const int k = blockIdx.x;
+    const double c = sum_partials(part + (size_t)k * kRG, sh);
+    if (threadIdx.x == 0) {
+        coef[k] = c;
+        hcol[k] = first ? c : hcol[k] + c;
+    }
+}
+
+// w -= sum_k coef[k] v_k (k ascending); partial |w|^2
+__global__ __launch_bounds__(kRB) void k_cgs_update(double *__restrict__ w, const float *__restrict__ V, size_t ldv,
+                                                    int n, const double *__restrict__ coef, int kn,
+                                                    double *__restrict__ part) {
+    __shared__ double c[128];
+    __shared__ double sh[kRB];
+    for (int k = threadIdx.x; k < kn; k += kRB) c[k] = coef[k];
+    __syncthreads();
+    double v = 0.0;
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) {
+        double wi = w[i];
+        for (int k = 0; k < kn; ++k) wi -= c[k] * (double)V[(size_t)k * ldv + i];
+        w[i] = wi;
+        v += wi * wi;
+    }
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
 __device__ void gen_rot(double dx, double dy, double &cs, double &sn) {  // gmres.cpp:23-36
@@ -241,7 +303,8 @@ __global__ void k_givens(double *__restrict__ H, int ld, double *__restrict__ s,
 }
 
 // Update (gmres.cpp:10-21): back substitution, then x += sum_j y_j v_j
-__global__ __launch_bounds__(kRB) void k_update(double *__restrict__ x, const double *__restrict__ V, size_t ldv, int n,
+template <typename VT>
+__global__ __launch_bounds__(kRB) void k_update(double *__restrict__ x, const VT *__restrict__ V, size_t ldv, int n,
                                                 const double *__restrict__ H, int ld, const double *__restrict__ s,
                                                 int k) {
     __shared__ double y[128];
@@ -255,7 +318,7 @@ __global__ __launch_bounds__(kRB) void k_update(double *__restrict__ x, const do
     __syncthreads();
     for (int e = blockIdx.x * kRB + threadIdx.x; e < n; e += gridDim.x * kRB) {
         double acc = x[e];
-        for (int j = 0; j <= k; ++j) acc += y[j] * V[(size_t)j * ldv + e];
+        for (int j = 0; j <= k; ++j) acc += y[j] * (double)V[(size_t)j * ldv + e];
         x[e] = acc;
     }
 }
@@ -409,18 +472,28 @@ int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_
 
 constexpr int kHostScalars = 8;
 
-static int gmres_alloc(ibim_plan *pl, int restart) {
-    if (pl->gm_V && pl->gm_restart >= restart) return IBIM_OK;
+// the workspace for GMRES(restart): the FP64 basis, or (f32) the FP32 basis
+// of the inner solves, each allocated when first needed
+static int gmres_alloc(ibim_plan *pl, int restart, bool f32) {
     const size_t n2 = 2 * (size_t)pl->n;
     int rc;
     // earlier buffers stay on the plan's allocation list and are freed with it
-    if ((rc = dalloc(pl, &pl->gm_V, (size_t)(restart + 1) * n2)) || (rc = dalloc(pl, &pl->gm_w, n2)) ||
-        (rc = dalloc(pl, &pl->gm_r, n2)) || (rc = dalloc(pl, &pl->gm_x, n2)) ||
-        (rc = dalloc(pl, &pl->gm_part, 4 * (size_t)kRG)) ||
-        (rc = dalloc(pl, &pl->gm_small, (size_t)(restart + 1) * (restart + 1) + 4 * (size_t)(restart + 1) + 8)))
+    if (!pl->gm_w || pl->gm_restart < restart) {
+        if ((rc = dalloc(pl, &pl->gm_w, n2)) || (rc = dalloc(pl, &pl->gm_r, n2)) || (rc = dalloc(pl, &pl->gm_x, n2)) ||
+            (rc = dalloc(pl, &pl->gm_part, 4 * (size_t)kRG)) ||
+            (rc = dalloc(pl, &pl->gm_small, (size_t)(restart + 1) * (restart + 1) + 4 * (size_t)(restart + 1) + 8)))
+            return rc;
+        pl->gm_V = nullptr;
+        pl->gm_Vf = nullptr;
+        pl->gm_restart = restart;
+    }
+    const int rs = pl->gm_restart;
+    if (!f32 && !pl->gm_V && (rc = dalloc(pl, &pl->gm_V, (size_t)(rs + 1) * n2))) return rc;
+    if (f32 && !pl->gm_Vf &&
+        ((rc = dalloc(pl, &pl->gm_Vf, (size_t)(rs + 1) * n2)) || (rc = dalloc(pl, &pl->gm_vd, n2)) ||
+         (rc = dalloc(pl, &pl->gm_hpart, (size_t)(rs + 1) * kRG)) || (rc = dalloc(pl, &pl->gm_coef, (size_t)rs + 1))))
         return rc;
     if (!pl->gm_host) IBIM_HIP_CHECK(hipHostMalloc((void **)&pl->gm_host, kHostScalars * sizeof(double)));
-    pl->gm_restart = restart;
     return IBIM_OK;
 }
 
@@ -439,7 +512,10 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
         set_error("gmres: restart must be in 1..120 and maxit >= 0");
         return IBIM_ERR_ARG;
     }
-    int rc = gmres_alloc(pl, restart);
+    // the inner solves of the refinement keep an FP32 basis (classical
+    // Gram-Schmidt); the FP64 solve, the reference's modified Gram-Schmidt
+    const bool f32 = mixed;
+    int rc = gmres_alloc(pl, restart, f32);
     if (rc) return rc;
     hipStream_t st = pl->stream;
     const int m = restart;
@@ -447,6 +523,7 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
     const size_t ldv = (size_t)n2;
     const int ld = m + 1;
     double *V = pl->gm_V, *w = pl->gm_w, *r = pl->gm_r, *p = pl->gm_x;
+    float *Vf = pl->gm_Vf;
     double *partA = pl->gm_part, *partB = pl->gm_part + kRG;
     double *H = pl->gm_small;
     double *s = H + (size_t)ld * ld, *cs = s + ld, *sn = cs + ld, *scal = sn + ld;  // scal: beta, resid, normb
@@ -489,24 +566,33 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
         bool done = false;
         while (j <= maxit && !done) {
             // v0 = r / beta (partials of |r|^2 are in partA)
-            k_start<<<kRG, kRB, 0, st>>>(r, V, n2, partA, s, m, scal);
+            if (f32) k_start<float><<<kRG, kRB, 0, st>>>(r, Vf, n2, partA, s, m, scal);
+            else k_start<double><<<kRG, kRB, 0, st>>>(r, V, n2, partA, s, m, scal);
             IBIM_LAUNCH_CHECK();
             for (i = 0; i < m && j <= maxit; i++, j++) {
-                double *vi = V + (size_t)i * ldv;
+                if (f32) k_f2d<<<nblk(n2), 256, 0, st>>>(Vf + (size_t)i * ldv, pl->gm_vd, n2);
+                const double *vi = f32 ? pl->gm_vd : V + (size_t)i * ldv;
                 if ((rc = matvec_device(pl, vi, w, mixed))) return rc;
                 nmv++;
                 inner++;
                 double *hcol = H + (size_t)i * ld;
                 {
                     StageScope sc(pl, kStOrth);
-                    k_dot_part<<<kRG, kRB, 0, st>>>(w, V, n2, partA);  // <w, v_0>
-                    double *pin = partA, *pout = partB;
-                    for (int k = 0; k <= i; ++k) {
-                        const double *vnext = (k < i) ? V + (size_t)(k + 1) * ldv : nullptr;
-                        k_mgs<<<kRG, kRB, 0, st>>>(w, V + (size_t)k * ldv, vnext, n2, pin, pout, hcol + k);
-                        std::swap(pin, pout);
+                    if (f32) {
+                        k_cgs_dots<<<kRG, kRB, 0, st>>>(w, Vf, ldv, n2, i + 1, pl->gm_hpart);
+                        k_cgs_coef<<<i + 1, kRB, 0, st>>>(pl->gm_hpart, pl->gm_coef, hcol, 1);
+                        k_cgs_update<<<kRG, kRB, 0, st>>>(w, Vf, ldv, n2, pl->gm_coef, i + 1, partA);
+                        k_normalize<float><<<kRG, kRB, 0, st>>>(w, Vf + (size_t)(i + 1) * ldv, n2, partA, hcol + i + 1);
+                    } else {
+                        k_dot_part<<<kRG, kRB, 0, st>>>(w, V, n2, partA);  // <w, v_0>
+                        double *pin = partA, *pout = partB;
+                        for (int k = 0; k <= i; ++k) {
+                            const double *vnext = (k < i) ? V + (size_t)(k + 1) * ldv : nullptr;
+                            k_mgs<<<kRG, kRB, 0, st>>>(w, V + (size_t)k * ldv, vnext, n2, pin, pout, hcol + k);
+                            std::swap(pin, pout);
+                        }
+                        k_normalize<double><<<kRG, kRB, 0, st>>>(w, V + (size_t)(i + 1) * ldv, n2, pin, hcol + i + 1);
                     }
-                    k_normalize<<<kRG, kRB, 0, st>>>(w, V + (size_t)(i + 1) * ldv, n2, pin, hcol + i + 1);
                     k_givens<<<1, 64, 0, st>>>(H, ld, s, cs, sn, i, normb, scal + 1);
                 }
                 IBIM_LAUNCH_CHECK();
@@ -514,7 +600,8 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
                 IBIM_HIP_CHECK(hipStreamSynchronize(st));
                 res = pl->gm_host[1];
                 if (res < tol) {
-                    k_update<<<kRG, kRB, 0, st>>>(d_x, V, ldv, n2, H, ld, s, i);
+                    if (f32) k_update<float><<<kRG, kRB, 0, st>>>(d_x, Vf, ldv, n2, H, ld, s, i);
+                    else k_update<double><<<kRG, kRB, 0, st>>>(d_x, V, ldv, n2, H, ld, s, i);
                     IBIM_LAUNCH_CHECK();
                     ret = IBIM_OK;
                     done = true;
@@ -522,7 +609,8 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
                 }
             }
             if (done) break;
-            k_update<<<kRG, kRB, 0, st>>>(d_x, V, ldv, n2, H, ld, s, i - 1);
+            if (f32) k_update<float><<<kRG, kRB, 0, st>>>(d_x, Vf, ldv, n2, H, ld, s, i - 1);
+            else k_update<double><<<kRG, kRB, 0, st>>>(d_x, V, ldv, n2, H, ld, s, i - 1);
             IBIM_LAUNCH_CHECK();
             if ((rc = matvec_device(pl, d_x, p, mixed))) return rc;
             nmv++;
@@ -572,7 +660,7 @@ static int gmres_refine(ibim_plan *pl, const double *d_b, double *d_x, int resta
     int rc;
     if (!pl->gm_rr && ((rc = dalloc(pl, &pl->gm_rr, (size_t)n2)) || (rc = dalloc(pl, &pl->gm_d, (size_t)n2))))
         return rc;
-    if ((rc = gmres_alloc(pl, restart))) return rc;
+    if ((rc = gmres_alloc(pl, restart, true))) return rc;
     double *scal = pl->gm_small;  // borrowed scratch for the two norms below (before any inner solve)
     double *part = pl->gm_part + 3 * kRG;
     auto norm = [&](const double *v, double *out) -> int {
