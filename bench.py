@@ -337,7 +337,7 @@ def main():
     gemm16 = (f"k_m2l_gemm_h32<{p},8> (V) + k_m2l_gemm_h32<{p},4> (X)" if p <= 5 else
               f"k_m2l_gemm_hu<{p},16>" if p == 6 else f"k_m2l_gemm<{p},float>")
     kern, peak = {"mixed_f32": (f"k_m2l_gemm<{p},float> (FP32 MFMA)", PEAK_TFLOPS["f32"]),
-                  "mixed_f16": (f"k_m_to_half + {gemm16} + k_v_reduce (FP16 MFMA)" if p <= 6 else
+                  "mixed_f16": (f"k_m_to_half + {gemm16} (FP16 MFMA)" if p <= 6 else
                                 f"{gemm16} (FP32 MFMA; the FP16 tiles stop at p = 6)",
                                 PEAK_TFLOPS["f16"] if p <= 6 else PEAK_TFLOPS["f32"])}[best]
     traffic = None
@@ -353,8 +353,9 @@ def main():
                 "interactions_per_launch": pairs,
                 "note": "M2L stage of the inner (mixed) matvecs: 32 r^2 flops per interaction (r = p^3), V pairs "
                         + ("and X pairs " if x_in_stage else "") + "per launch; time from HIP events on the plan "
-                        "stream around the stage over the timed mixed solves (the only stage recorded there), all "
-                        "kernels of the stage included; dominant_stage from one extra untimed solve with every "
+                        "stream around the stage over the timed mixed solves (the only stage recorded there): the "
+                        "multipole conversion and the GEMMs (the per-target reduction of their results runs beside "
+                        "the near field and is not in it); dominant_stage from one extra untimed solve with every "
                         "stage recorded"}
     stage_ms = {k: v[0] for k, v in stage_full.items()}  # one solve
 

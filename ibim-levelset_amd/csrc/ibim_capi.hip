@@ -198,6 +198,12 @@ void ibim_plan_destroy(ibim_plan *pl) {
     if (pl->gm_host) (void)hipHostFree(pl->gm_host);
     pl->pool.free_all();
     pl->spool.free_all();
+    if (pl->stream2) {
+        (void)hipStreamSynchronize(pl->stream2);
+        (void)hipStreamDestroy(pl->stream2);
+    }
+    if (pl->ev_fork) (void)hipEventDestroy(pl->ev_fork);
+    if (pl->ev_join) (void)hipEventDestroy(pl->ev_join);
     if (pl->owns_stream && pl->stream) (void)hipStreamDestroy(pl->stream);
     delete pl;
 }

@@ -919,9 +919,44 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     }
     IBIM_LAUNCH_CHECK();
     }
+    // FP16 mixed form on the plan's own points: the near field (VALU bound)
+    // runs on a second stream beside the M2L reduction and L2L (memory and
+    // latency bound), forked after the M2L GEMMs, which take every CU's LDS
+    // and registers, and joined before L2P
+    const bool fork = mixed && P <= 6 && pl->prm.precision == IBIM_PREC_MIXED_F16 && ts.vpairs == &pl->vp &&
+                      pl->ops16 && pl->ytptr && ts.nleaves > 0 && ts.p2p;
+    if (fork && !pl->stream2) {
+        IBIM_HIP_CHECK(hipStreamCreateWithFlags(&pl->stream2, hipStreamNonBlocking));
+        IBIM_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming));
+        IBIM_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming));
+    }
     if (ts.vpairs) {
         StageScope sc(pl, mixed ? kStM2LMixed : kStM2L);
+        pl->defer_reduce = fork;
         int rc = m2l_gemm(pl, *ts.vpairs, mixed);
+        pl->defer_reduce = false;
+        if (rc) return rc;
+    }
+    auto near_field = [&](hipStream_t s) {
+        const P2PWork &w = *ts.p2p;
+        auto launch = [&](auto kern) {
+            kern<<<w.nunits, 64, 0, s>>>(w.ua, w.ub, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q, pl->n,
+                                         ts.ranges, pl->kc, F, nt, w.part);
+        };
+        const bool poly = pl->kc.mode == IBIM_KERNEL_POLY;
+        if (mixed) launch(poly ? k_p2p_unit<true, true> : k_p2p_unit<true, false>);
+        else launch(poly ? k_p2p_unit<false, true> : k_p2p_unit<false, false>);
+    };
+    if (fork) {
+        IBIM_HIP_CHECK(hipEventRecord(pl->ev_fork, st));
+        IBIM_HIP_CHECK(hipStreamWaitEvent(pl->stream2, pl->ev_fork, 0));
+        {
+        StageScope sc(pl, kStP2PMixed, pl->stream2);
+        near_field(pl->stream2);
+        IBIM_LAUNCH_CHECK();
+        }
+        IBIM_HIP_CHECK(hipEventRecord(pl->ev_join, pl->stream2));
+        int rc = m2l_reduce(pl);
         if (rc) return rc;
     }
     {
@@ -932,18 +967,15 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     }
     IBIM_LAUNCH_CHECK();
     }
+    if (fork) IBIM_HIP_CHECK(hipStreamWaitEvent(st, pl->ev_join, 0));
     {
-    StageScope sc(pl, mixed ? kStP2PMixed : kStP2P);
+    StageScope sc(pl, fork ? kStL2L : mixed ? kStP2PMixed : kStP2P);
     if (ts.nleaves > 0 && ts.p2p) {
         const P2PWork &w = *ts.p2p;
-        auto launch = [&](auto kern) {
-            kern<<<w.nunits, 64, 0, st>>>(w.ua, w.ub, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q, pl->n,
-                                          ts.ranges, pl->kc, F, nt, w.part);
-        };
-        const bool poly = pl->kc.mode == IBIM_KERNEL_POLY;
-        if (mixed) launch(poly ? k_p2p_unit<true, true> : k_p2p_unit<true, false>);
-        else launch(poly ? k_p2p_unit<false, true> : k_p2p_unit<false, false>);
-        IBIM_LAUNCH_CHECK();
+        if (!fork) {
+            near_field(st);
+            IBIM_LAUNCH_CHECK();
+        }
         k_l2p<P><<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->ctr, pl->level,
                                             pl->d_lvl_rad, pl->tab, pl->L, w.npart ? w.lsplit : nullptr, w.part,
                                             F, nt);

@@ -420,6 +420,11 @@ struct ibim_plan {
     int device = 0;
     hipStream_t stream = nullptr;
     bool owns_stream = false;
+    // FP16 mixed matvec: the near field runs on stream2 beside the M2L
+    // reduction and L2L (forked after the M2L GEMMs, joined before L2P)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool defer_reduce = false;           // the FP16 M2L stage leaves the reduction to the caller
 
     // sorted geometry
     double *sx = nullptr, *sy = nullptr, *sz = nullptr;
@@ -565,6 +570,7 @@ int compute_stats(ibim_plan *pl);
 int build_m2l_operators(ibim_plan *pl);
 int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vector<void *> *tmp_owner);
 int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed);
+int m2l_reduce(ibim_plan *pl);  // the FP16 stage's per-target reduction (after defer_reduce)
 int build_xpairs(ibim_plan *pl);
 int m2l_gemm_x(ibim_plan *pl, bool mixed);
 int build_m2l_operators_f32(ibim_plan *pl);
@@ -576,17 +582,18 @@ struct StageScope {
     ibim_plan *pl;
     int stage;
     hipEvent_t a = nullptr;
-    StageScope(ibim_plan *p, int s) : pl(p), stage(s) {
+    hipStream_t st;
+    StageScope(ibim_plan *p, int s, hipStream_t on = nullptr) : pl(p), stage(s), st(on ? on : p->stream) {
         if (pl->timer.on && ((pl->timer.mask >> stage) & 1u)) {
             a = pl->timer.take();
-            if (a) (void)hipEventRecord(a, pl->stream);
+            if (a) (void)hipEventRecord(a, st);
         }
     }
     ~StageScope() {
         if (a) {
             hipEvent_t b = pl->timer.take();
             if (b) {
-                (void)hipEventRecord(b, pl->stream);
+                (void)hipEventRecord(b, st);
                 pl->timer.pending.push_back({stage, a, b});
             } else {
                 pl->timer.pool.push_back(a);

@@ -970,8 +970,9 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
                 gemm(xp, xp.hcls_off_d, pl->xiota, pl->Bx, pl->ebx, pl->xkeyslot,
                      pl->Yv + (size_t)vp.npairs * YS / 4 * 3, pl->y_rows - vp.npairs, false);
             }
-            k_v_reduce<R4><<<pl->n_m2l_work, R4 / 4 <= 128 ? 128 : 256, 0, st>>>(
-                pl->m2l_work, pl->ytptr, pl->ytlist, pl->Yv, vp.npairs, xp.mask, pl->stab, pl->L);
+            if (!pl->defer_reduce)
+                k_v_reduce<R4><<<pl->n_m2l_work, R4 / 4 <= 128 ? 128 : 256, 0, st>>>(
+                    pl->m2l_work, pl->ytptr, pl->ytlist, pl->Yv, vp.npairs, xp.mask, pl->stab, pl->L);
             IBIM_LAUNCH_CHECK();
             return IBIM_OK;
         }
@@ -1398,6 +1399,28 @@ int m2l_gemm_x(ibim_plan *pl, bool mixed) {
     return IBIM_ERR_ARG;
 }
 
+template <int P>
+int reduce_p(ibim_plan *pl) {
+    constexpr int R4 = 4 * P * P * P;
+    if constexpr (P <= 6) {
+        k_v_reduce<R4><<<pl->n_m2l_work, R4 / 4 <= 128 ? 128 : 256, 0, pl->stream>>>(
+            pl->m2l_work, pl->ytptr, pl->ytlist, pl->Yv, pl->vp.npairs, pl->xp.mask, pl->stab, pl->L);
+        IBIM_LAUNCH_CHECK();
+    }
+    return IBIM_OK;
+}
+
+int m2l_reduce(ibim_plan *pl) {
+    switch (pl->p) {
+    case 2: return reduce_p<2>(pl);
+    case 3: return reduce_p<3>(pl);
+    case 4: return reduce_p<4>(pl);
+    case 5: return reduce_p<5>(pl);
+    case 6: return reduce_p<6>(pl);
+    }
+    return IBIM_OK;
+}
+
 int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed) {
     switch (pl->p) {
     case 2: return gemm_p<2>(pl, vp, mixed);
@@ -1414,9 +1437,9 @@ int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed) {
 // FP16 operators, their row exponents and the per-matvec FP16 multipole and
 // interaction-result buffers of the IBIM_PREC_MIXED_F16 matvec (the plan's V
 // and X pairs must exist).  The X list joins the V list in this stage: its
-// (class, mask) keys get unfolded operators after the V keys', its results
-// the rows of Yv after the V pairs', and one per-target list over both (V
-// pairs first, each kind in pair order) drives the reduction.
+// classes get (unreflected) operators after the V keys', its results the
+// rows of Yv after the V pairs', and one per-target list over both (V pairs
+// first, each kind in pair order) drives the reduction.
 static int build_xops_f32(ibim_plan *pl);
 
 int build_m2l_operators_f16(ibim_plan *pl) {
