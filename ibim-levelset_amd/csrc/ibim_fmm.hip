@@ -868,12 +868,47 @@ __global__ void k_scatter_field(int n, const int *__restrict__ perm, const doubl
     out[perm[i]] = sign * F[(size_t)field * n + i];
 }
 
+// share of the near-field work units (the costliest, they come first) that
+// runs beside the upward pass of the FP16 mixed matvec
+constexpr double kNearEarly = 0.15;
+
 template <int P>
 int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mixed) {
     constexpr int kP2mBS = P * P * P <= 128 ? 128 : 256;
     constexpr int R = P * P * P;
     constexpr int BS = R <= 64 ? 64 : (R <= 128 ? 128 : 256);
     hipStream_t st = pl->stream;
+    // FP16 mixed form on the plan's own points: the near field (VALU bound)
+    // runs on a second stream, its first (costliest) units beside the upward
+    // pass and the rest beside the M2L reduction and L2L (memory and latency
+    // bound): forked after the charges and after the M2L GEMMs, which take
+    // every CU's LDS and registers, and joined before L2P
+    const bool fork = mixed && P <= 6 && pl->prm.precision == IBIM_PREC_MIXED_F16 && ts.vpairs == &pl->vp &&
+                      pl->ops16 && pl->ytptr && ts.nleaves > 0 && ts.p2p;
+    if (fork && !pl->stream2) {
+        IBIM_HIP_CHECK(hipStreamCreateWithFlags(&pl->stream2, hipStreamNonBlocking));
+        IBIM_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming));
+        IBIM_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming));
+    }
+    const int ucut = fork ? (int)(ts.p2p->nunits * kNearEarly) : 0;
+    // work units [u0, u1) of the near field on stream s
+    auto near_field = [&](hipStream_t s, int u0, int u1) {
+        const P2PWork &w = *ts.p2p;
+        if (u1 <= u0) return;
+        auto launch = [&](auto kern) {
+            kern<<<u1 - u0, 64, 0, s>>>(w.ua + u0, w.ub + u0, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q, pl->n,
+                                        ts.ranges, pl->kc, F, nt, w.part);
+        };
+        const bool poly = pl->kc.mode == IBIM_KERNEL_POLY;
+        if (mixed) launch(poly ? k_p2p_unit<true, true> : k_p2p_unit<true, false>);
+        else launch(poly ? k_p2p_unit<false, true> : k_p2p_unit<false, false>);
+    };
+    if (fork && ucut > 0) {
+        IBIM_HIP_CHECK(hipEventRecord(pl->ev_fork, st));
+        IBIM_HIP_CHECK(hipStreamWaitEvent(pl->stream2, pl->ev_fork, 0));
+        near_field(pl->stream2, 0, ucut);
+        IBIM_LAUNCH_CHECK();
+    }
     // upward pass
     {
     StageScope sc(pl, kStP2M);
@@ -919,17 +954,6 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     }
     IBIM_LAUNCH_CHECK();
     }
-    // FP16 mixed form on the plan's own points: the near field (VALU bound)
-    // runs on a second stream beside the M2L reduction and L2L (memory and
-    // latency bound), forked after the M2L GEMMs, which take every CU's LDS
-    // and registers, and joined before L2P
-    const bool fork = mixed && P <= 6 && pl->prm.precision == IBIM_PREC_MIXED_F16 && ts.vpairs == &pl->vp &&
-                      pl->ops16 && pl->ytptr && ts.nleaves > 0 && ts.p2p;
-    if (fork && !pl->stream2) {
-        IBIM_HIP_CHECK(hipStreamCreateWithFlags(&pl->stream2, hipStreamNonBlocking));
-        IBIM_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming));
-        IBIM_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming));
-    }
     if (ts.vpairs) {
         StageScope sc(pl, mixed ? kStM2LMixed : kStM2L);
         pl->defer_reduce = fork;
@@ -937,22 +961,12 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
         pl->defer_reduce = false;
         if (rc) return rc;
     }
-    auto near_field = [&](hipStream_t s) {
-        const P2PWork &w = *ts.p2p;
-        auto launch = [&](auto kern) {
-            kern<<<w.nunits, 64, 0, s>>>(w.ua, w.ub, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q, pl->n,
-                                         ts.ranges, pl->kc, F, nt, w.part);
-        };
-        const bool poly = pl->kc.mode == IBIM_KERNEL_POLY;
-        if (mixed) launch(poly ? k_p2p_unit<true, true> : k_p2p_unit<true, false>);
-        else launch(poly ? k_p2p_unit<false, true> : k_p2p_unit<false, false>);
-    };
     if (fork) {
         IBIM_HIP_CHECK(hipEventRecord(pl->ev_fork, st));
         IBIM_HIP_CHECK(hipStreamWaitEvent(pl->stream2, pl->ev_fork, 0));
         {
         StageScope sc(pl, kStP2PMixed, pl->stream2);
-        near_field(pl->stream2);
+        near_field(pl->stream2, ucut, ts.p2p->nunits);
         IBIM_LAUNCH_CHECK();
         }
         IBIM_HIP_CHECK(hipEventRecord(pl->ev_join, pl->stream2));
@@ -973,7 +987,7 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     if (ts.nleaves > 0 && ts.p2p) {
         const P2PWork &w = *ts.p2p;
         if (!fork) {
-            near_field(st);
+            near_field(st, 0, w.nunits);
             IBIM_LAUNCH_CHECK();
         }
         k_l2p<P><<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->ctr, pl->level,
