@@ -585,13 +585,28 @@ __global__ __launch_bounds__(64) void k_p2p_unit(
             // sources k = sl, sl + nsl, ... two per step; u at s0[k], v at
             // s1[k] = s0[k + 64]
             const float4 *pu = (const float4 *)s0 + sl;
-            const float4 *pend = (const float4 *)s0 + max(cnt - nsl, 0);
-            for (; pu < pend; pu += 2 * nsl) {
-                const float4 u = pu[0], v = pu[64], u2 = pu[nsl], v2 = pu[nsl + 64];
-                pair_pb32(u.x - (float)px, u.y - (float)py, u.z - (float)pz, v.x, v.y, v.z, v.w, k32, ga, gb, gx, gy,
-                          gz);
-                pair_pb32(u2.x - (float)px, u2.y - (float)py, u2.z - (float)pz, v2.x, v2.y, v2.z, v2.w, k32, ha, hb,
-                          hx, hy, hz);
+            if (nsl == 1) {
+                // (one source slice, the common case: a block-uniform
+                // counter and constant LDS offsets, no per-lane loop control)
+                int q = 0;
+#pragma unroll 1
+                for (; q + 1 < cnt; q += 2) {
+                    const float4 u = s0[q], v = s1[q], u2 = s0[q + 1], v2 = s1[q + 1];
+                    pair_pb32(u.x - (float)px, u.y - (float)py, u.z - (float)pz, v.x, v.y, v.z, v.w, k32, ga, gb,
+                              gx, gy, gz);
+                    pair_pb32(u2.x - (float)px, u2.y - (float)py, u2.z - (float)pz, v2.x, v2.y, v2.z, v2.w, k32, ha,
+                              hb, hx, hy, hz);
+                }
+                pu += q;
+            } else {
+                const float4 *pend = (const float4 *)s0 + max(cnt - nsl, 0);
+                for (; pu < pend; pu += 2 * nsl) {
+                    const float4 u = pu[0], v = pu[64], u2 = pu[nsl], v2 = pu[nsl + 64];
+                    pair_pb32(u.x - (float)px, u.y - (float)py, u.z - (float)pz, v.x, v.y, v.z, v.w, k32, ga, gb,
+                              gx, gy, gz);
+                    pair_pb32(u2.x - (float)px, u2.y - (float)py, u2.z - (float)pz, v2.x, v2.y, v2.z, v2.w, k32, ha,
+                              hb, hx, hy, hz);
+                }
             }
             const int k = (int)(pu - (const float4 *)s0);
             if (k < cnt) {
@@ -610,11 +625,22 @@ __global__ __launch_bounds__(64) void k_p2p_unit(
             const double k2r = kc.k2reg / kc.inv4pi;
             double ga = 0, gb = 0, gx = 0, gy = 0, gz = 0, ha = 0, hb = 0, hx = 0, hy = 0, hz = 0;
             const double4 *pu = (const double4 *)s0 + sl;
-            const double4 *pend = (const double4 *)s0 + max(cnt - nsl, 0);
-            for (; pu < pend; pu += 2 * nsl) {
-                const double4 u = pu[0], v = pu[64], u2 = pu[nsl], v2 = pu[nsl + 64];
-                pair_pb64(u.x - px, u.y - py, u.z - pz, v.x, v.y, v.z, v.w, kc, k2r, ga, gb, gx, gy, gz);
-                pair_pb64(u2.x - px, u2.y - py, u2.z - pz, v2.x, v2.y, v2.z, v2.w, kc, k2r, ha, hb, hx, hy, hz);
+            if (nsl == 1) {  // (as in the FP32 form)
+                int q = 0;
+#pragma unroll 2
+                for (; q + 1 < cnt; q += 2) {
+                    const double4 u = s0[q], v = s1[q], u2 = s0[q + 1], v2 = s1[q + 1];
+                    pair_pb64(u.x - px, u.y - py, u.z - pz, v.x, v.y, v.z, v.w, kc, k2r, ga, gb, gx, gy, gz);
+                    pair_pb64(u2.x - px, u2.y - py, u2.z - pz, v2.x, v2.y, v2.z, v2.w, kc, k2r, ha, hb, hx, hy, hz);
+                }
+                pu += q;
+            } else {
+                const double4 *pend = (const double4 *)s0 + max(cnt - nsl, 0);
+                for (; pu < pend; pu += 2 * nsl) {
+                    const double4 u = pu[0], v = pu[64], u2 = pu[nsl], v2 = pu[nsl + 64];
+                    pair_pb64(u.x - px, u.y - py, u.z - pz, v.x, v.y, v.z, v.w, kc, k2r, ga, gb, gx, gy, gz);
+                    pair_pb64(u2.x - px, u2.y - py, u2.z - pz, v2.x, v2.y, v2.z, v2.w, kc, k2r, ha, hb, hx, hy, hz);
+                }
             }
             const int k = (int)(pu - (const double4 *)s0);
             if (k < cnt) {
