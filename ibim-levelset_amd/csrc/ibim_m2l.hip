@@ -465,9 +465,11 @@ __device__ __forceinline__ void wait_vm() {
 }
 
 // The FP16 V-list GEMM (v_mfma_f32_16x16x32_f16, FP32 accumulation) on
-// unfolded operators (one per (class, mask) key with pairs, k_unfold_half):
-// a block takes 4 PPW pairs of one key, wave w the pairs [PPW w, PPW w +
-// PPW), PPW = 32 for p <= 5 and 16 for p = 6 (register budget).  Each wave
+// unfolded operators (one per V-list (class, mask) key with pairs,
+// k_unfold_half; for the X list one per class, the reflection applied to
+// the multipoles and the results instead): a block takes 4 PPW pairs of one
+// key, wave w the pairs [PPW w, PPW w + PPW), PPW = 16 (register budget at
+// p = 6; p <= 5 runs k_m2l_gemm_h32).  Each wave
 // loads its pairs' MFMA B-fragments (8 consecutive k of one pair = 16 B of
 // Mh) for all k straight into registers once.  The operator streams through
 // an LDS ring of NST chunks of 64 rows x KC k, filled by direct global-to-LDS
@@ -1114,7 +1116,8 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
         IBIM_HIP_CHECK(hipMemcpyAsync(vp.blk_cls, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice, st));
         IBIM_HIP_CHECK(hipMemcpyAsync(vp.blk_j0, bj.data(), bj.size() * sizeof(int), hipMemcpyHostToDevice, st));
     }
-    // FP16 GEMM block table (blocks of 128 pairs for p <= 5, 64 for p = 6)
+    // FP16 GEMM block table (blocks of 32 kH32WavesV pairs for p <= 5, 64
+    // for p = 6)
     if (pl->p <= 6) {
         const int hnp = pl->p <= 5 ? 32 * kH32WavesV : 64;
         std::vector<int> hc, hj;

@@ -28,8 +28,8 @@ def sphere_case(ibim, gpu):
 MATVEC_BAR = {"mixed": 1e-5, "mixed16": 1e-3}
 
 
-# p = 5 runs the unfolded-operator FP16 GEMM, p = 6 the folded one, p = 8 the
-# FP32 GEMM (the FP16 tiles stop at p = 6), p = 3 the smallest tiles
+# p = 5 runs the 32x32 FP16 GEMM, p = 6 the 16x16 one, p = 8 the FP32 GEMM
+# (the FP16 tiles stop at p = 6), p = 3 the smallest tiles
 @pytest.mark.parametrize("p", [3, 5, 6, 8])
 @pytest.mark.parametrize("mode", ["mixed", "mixed16"])
 def test_mixed_matvec_close_to_fp64(ibim, sphere_case, mode, p):
@@ -74,3 +74,26 @@ def test_mixed_solve_energy_matches_fp64(ibim, sphere_case, kappa, mode):
     assert i32["outer_steps"] >= 1
     assert abs(e32 - e64) / abs(e64) < 1e-6
     assert rel(x32, x64) < 1e-6
+
+
+@pytest.mark.parametrize("p", [5, 6])
+def test_mixed16_matvec_repeatable(ibim, sphere_case, p):
+    """The FP16 form's matvec runs its near field on a second stream beside
+    the M2L reduction and reflects the X list's multipoles and results; two
+    matvecs in a row, and one after an FP64 matvec on the same plan, give the
+    same bits (every sum has a fixed order), and the X list is exercised."""
+    pts, nrm, w = sphere_case
+    x = np.random.default_rng(3).uniform(-1, 1, 2 * len(pts))
+    with ibim.Plan(pts, nrm, w, p=p, max_points=32, vacant_radius=0.125, precision=ibim.PREC_MIXED_F16,
+                   **PB) as plan:
+        a = plan.matvec(x)
+        b = plan.matvec(x)
+        plan.set_precision(ibim.PREC_FP64)
+        y64 = plan.matvec(x)
+        plan.set_precision(ibim.PREC_MIXED_F16)
+        c = plan.matvec(x)
+        st = plan.stats()
+    print(f"p={p}: {st['v_pairs']} V and {st['x_pairs']} X interactions, rel err vs FP64 {rel(a, y64):.2e}")
+    assert st["x_pairs"] > 0
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+    assert rel(a, y64) < MATVEC_BAR["mixed16"]
