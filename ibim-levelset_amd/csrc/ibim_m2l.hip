@@ -1016,6 +1016,12 @@ int build_m2l_operators(ibim_plan *pl) {
 
 // V pairs of the boxes flagged in `need`, grouped by operator class
 namespace {
+// first[k] = the first position of key k in the sorted keys (untouched
+// where k is absent)
+__global__ void k_key_starts(int np, const int *__restrict__ key, int *__restrict__ first) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < np && (j == 0 || key[j] != key[j - 1])) first[key[j]] = j;
+}
 __global__ void k_count_targets(int np, const int *__restrict__ B, int *__restrict__ cnt) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < np) atomicAdd(&cnt[B[j]], 1);
@@ -1092,14 +1098,24 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
         k_permute_pairs<<<nblk(np), 256, 0, st>>>(np, order, a0, b0, m0, vp.A, vp.B, vp.mask);
         IBIM_LAUNCH_CHECK();
     }
-    // class offsets and the GEMM block table, on the host
-    std::vector<int> hkey(np);
-    if (np > 0) IBIM_HIP_CHECK(hipMemcpyAsync(hkey.data(), key1, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, st));
-    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    // class offsets (the first pair of each key of the sorted keys, found on
+    // the device; a key without pairs takes the next key's) and the GEMM
+    // block table, on the host
     const int nkeys = 8 * pl->n_cls;  // (class, reflection mask)
-    vp.cls_off.assign(nkeys + 1, 0);
-    for (int j = 0; j < np; ++j) vp.cls_off[hkey[j] + 1]++;
-    for (int c = 0; c < nkeys; ++c) vp.cls_off[c + 1] += vp.cls_off[c];
+    vp.cls_off.assign(nkeys + 1, -1);
+    if (np > 0) {
+        int *kfirst = nullptr;
+        IBIM_HIP_CHECK(smalloc(pl, &kfirst, (size_t)nkeys * sizeof(int)));
+        IBIM_HIP_CHECK(hipMemsetAsync(kfirst, 0xff, (size_t)nkeys * sizeof(int), st));
+        k_key_starts<<<nblk(np), 256, 0, st>>>(np, key1, kfirst);
+        IBIM_LAUNCH_CHECK();
+        IBIM_HIP_CHECK(hipMemcpyAsync(vp.cls_off.data(), kfirst, (size_t)nkeys * sizeof(int), hipMemcpyDeviceToHost,
+                                      st));
+    }
+    IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    vp.cls_off[nkeys] = np;
+    for (int c = nkeys - 1; c >= 0; --c)
+        if (vp.cls_off[c] < 0) vp.cls_off[c] = vp.cls_off[c + 1];
     std::vector<int> bc, bj;
     for (int c = 0; c < nkeys; ++c)
         for (int j = vp.cls_off[c]; j < vp.cls_off[c + 1]; j += 64) {
