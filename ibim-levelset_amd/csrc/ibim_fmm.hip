@@ -44,7 +44,9 @@ __global__ __launch_bounds__(BS) void k_p2m(const int *__restrict__ leaves, cons
     constexpr int R = P * P * P;
     constexpr int PER = (R + BS - 1) / BS;
     constexpr int CH = BS;
-    __shared__ double sS[3][P][CH];
+    // rows padded by one double: lanes read P different rows at one point,
+    // which an unpadded stride of CH doubles puts in one LDS bank
+    __shared__ double sS[3][P][CH + 1];
     __shared__ double sq[4][CH];
     const int b = leaves[blockIdx.x];
     const int tid = threadIdx.x;
