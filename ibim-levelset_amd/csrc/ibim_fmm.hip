@@ -914,9 +914,22 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     const bool fork = mixed && P <= 6 && pl->prm.precision == IBIM_PREC_MIXED_F16 && ts.vpairs == &pl->vp &&
                       pl->ops16 && pl->ytptr && ts.nleaves > 0 && ts.p2p;
     if (fork && !pl->stream2) {
-        IBIM_HIP_CHECK(hipStreamCreateWithFlags(&pl->stream2, hipStreamNonBlocking));
-        IBIM_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming));
-        IBIM_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming));
+        // all three or none, so that a failed call leaves the plan as it was
+        hipStream_t s2 = nullptr;
+        hipEvent_t ef = nullptr, ej = nullptr;
+        hipError_t e = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ef, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ej, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            if (ej) (void)hipEventDestroy(ej);
+            if (ef) (void)hipEventDestroy(ef);
+            if (s2) (void)hipStreamDestroy(s2);
+            set_error(std::string("second stream: ") + hipGetErrorString(e));
+            return IBIM_ERR_HIP;
+        }
+        pl->stream2 = s2;
+        pl->ev_fork = ef;
+        pl->ev_join = ej;
     }
     const int ucut = fork ? (int)(ts.p2p->nunits * kNearEarly) : 0;
     // work units [u0, u1) of the near field on stream s
