@@ -97,3 +97,34 @@ def test_mixed16_matvec_repeatable(ibim, sphere_case, p):
     assert st["x_pairs"] > 0
     assert np.array_equal(a, b) and np.array_equal(a, c)
     assert rel(a, y64) < MATVEC_BAR["mixed16"]
+
+
+def test_stage_timers(ibim, sphere_case):
+    """Stage timers (HIP events on the plan's streams): with every stage
+    recorded, each FP16 matvec adds one launch to its matvec and M2L stages;
+    with only the M2L stage selected, no other stage records; switched off,
+    nothing records; and recording never changes the result's bits."""
+    pts, nrm, w = sphere_case
+    x = np.random.default_rng(4).uniform(-1, 1, 2 * len(pts))
+    with ibim.Plan(pts, nrm, w, p=5, max_points=32, vacant_radius=0.125, precision=ibim.PREC_MIXED_F16,
+                   **PB) as plan:
+        ref = plan.matvec(x)
+        plan.profile(True)
+        plan.stage_times(reset=True)
+        a = plan.matvec(x)
+        b = plan.matvec(x)
+        st_all = plan.stage_times(reset=True)
+        plan.profile(True, stages=["M2L_MIXED"])
+        c = plan.matvec(x)
+        st_m2l = plan.stage_times(reset=True)
+        plan.profile(False)
+        d = plan.matvec(x)
+        st_off = plan.stage_times(reset=True)
+    assert st_all["MATVEC_MIXED"][1] == 2 and st_all["M2L_MIXED"][1] == 2
+    assert st_all["MATVEC_MIXED"][0] > 0 and st_all["M2L_MIXED"][0] > 0
+    assert st_all["MATVEC_MIXED"][0] >= st_all["M2L_MIXED"][0]
+    assert st_m2l["M2L_MIXED"][1] == 1 and st_m2l["M2L_MIXED"][0] > 0
+    assert all(cnt == 0 for name, (ms, cnt) in st_m2l.items() if name != "M2L_MIXED")
+    assert all(cnt == 0 for ms, cnt in st_off.values())
+    for y in (a, b, c, d):
+        assert np.array_equal(y, ref)
