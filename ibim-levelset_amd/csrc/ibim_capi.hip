@@ -76,8 +76,8 @@ namespace {
 template <typename F>
 int with_host_vectors(ibim_plan *pl, size_t n_in, const double *in, size_t n_out, double *out, F fn) {
     double *d_in = nullptr, *d_out = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&d_in, (n_in ? n_in : 1) * sizeof(double)));
-    hipError_t e = hipMalloc(&d_out, (n_out ? n_out : 1) * sizeof(double));
+    IBIM_HIP_CHECK(dev_malloc(&d_in, (n_in ? n_in : 1) * sizeof(double)));
+    hipError_t e = dev_malloc(&d_out, (n_out ? n_out : 1) * sizeof(double));
     if (e != hipSuccess) {
         (void)hipFree(d_in);
         set_error("hipMalloc (host wrapper)");
@@ -117,7 +117,30 @@ BlockCache &block_cache() {
 }
 constexpr size_t kBlockRound = (size_t)2 << 20;
 constexpr size_t kIdleCap = (size_t)64 << 30;  // idle bytes kept per device
+
+// frees every idle cached block of device `dev`
+void flush_idle(int dev) {
+    BlockCache &c = block_cache();
+    std::vector<void *> drop;
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        for (auto &kv : c.idle[dev]) drop.push_back(kv.second);
+        c.idle[dev].clear();
+        c.idle_bytes[dev] = 0;
+    }
+    for (void *q : drop) (void)hipFree(q);
+}
 }  // namespace
+
+hipError_t dev_malloc_raw(void **p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) return e;
+    (void)hipGetLastError();
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return e;
+    flush_idle(dev);
+    return hipMalloc(p, bytes);
+}
 
 hipError_t block_alloc(void **p, size_t bytes) {
     int dev = 0;
@@ -137,20 +160,8 @@ hipError_t block_alloc(void **p, size_t bytes) {
             return hipSuccess;
         }
     }
-    e = hipMalloc(p, sz);
-    if (e != hipSuccess) {  // return this device's idle blocks and retry
-        (void)hipGetLastError();
-        std::vector<void *> drop;
-        {
-            std::lock_guard<std::mutex> lk(c.mu);
-            for (auto &kv : c.idle[dev]) drop.push_back(kv.second);
-            c.idle[dev].clear();
-            c.idle_bytes[dev] = 0;
-        }
-        for (void *q : drop) (void)hipFree(q);
-        e = hipMalloc(p, sz);
-        if (e != hipSuccess) return e;
-    }
+    e = dev_malloc_raw(p, sz);  // (returns this device's idle blocks and retries on failure)
+    if (e != hipSuccess) return e;
     std::lock_guard<std::mutex> lk(c.mu);
     c.live[*p] = {dev, sz};
     return hipSuccess;
@@ -188,7 +199,11 @@ const char *ibim_version(void) { return "ibim-levelset-amd 0.1 (gfx950)"; }
 
 void ibim_plan_destroy(ibim_plan *pl) {
     if (!pl) return;
+    // both streams idle before any block returns to the shared cache (a
+    // matvec that failed between its fork and join may still have near-field
+    // work in flight on the second stream)
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+    if (pl->stream2) (void)hipStreamSynchronize(pl->stream2);
     for (auto &r : pl->timer.pending) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
@@ -285,7 +300,7 @@ int ibim_plan_create(const ibim_params *prm, int64_t n, const double *xyz, const
         return IBIM_ERR_ARG;
     }
     double *d = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&d, 7 * (size_t)n * sizeof(double)));
+    IBIM_HIP_CHECK(dev_malloc(&d, 7 * (size_t)n * sizeof(double)));
     hipError_t e = hipMemcpy(d, xyz, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d + 3 * n, nrm, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d + 6 * n, w, (size_t)n * sizeof(double), hipMemcpyHostToDevice);
@@ -313,10 +328,13 @@ int ibim_plan_set_precision(ibim_plan *pl, int precision) {
         set_error("bad plan or precision");
         return IBIM_ERR_ARG;
     }
-    pl->prm.precision = precision;
-    if (precision == IBIM_PREC_MIXED) return build_m2l_operators_f32(pl);
-    if (precision == IBIM_PREC_MIXED_F16) return build_m2l_operators_f16(pl);
-    return IBIM_OK;
+    // the operators a precision needs are built before the plan switches to
+    // it: a failed build leaves the plan in its previous precision
+    int rc = IBIM_OK;
+    if (precision == IBIM_PREC_MIXED) rc = build_m2l_operators_f32(pl);
+    else if (precision == IBIM_PREC_MIXED_F16) rc = build_m2l_operators_f16(pl);
+    if (rc == IBIM_OK) pl->prm.precision = precision;
+    return rc;
 }
 
 int64_t ibim_plan_footprint(const ibim_plan *pl) { return pl ? pl->footprint : 0; }

@@ -912,7 +912,7 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     // bound): forked after the charges and after the M2L GEMMs, which take
     // every CU's LDS and registers, and joined before L2P
     const bool fork = mixed && P <= 6 && pl->prm.precision == IBIM_PREC_MIXED_F16 && ts.vpairs == &pl->vp &&
-                      pl->ops16 && pl->ytptr && ts.nleaves > 0 && ts.p2p;
+                      pl->f16_ready && ts.nleaves > 0 && ts.p2p;
     if (fork && !pl->stream2) {
         // all three or none, so that a failed call leaves the plan as it was
         hipStream_t s2 = nullptr;
@@ -1058,7 +1058,7 @@ int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, P2PWork &work, std::vector<vo
     const int nl = ts.nleaves;
     auto alloc = [&](auto **p, size_t n) -> int {
         if (tmp_owner) {
-            hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(**p));
+            hipError_t e = dev_malloc((void **)p, (n ? n : 1) * sizeof(**p));
             if (e != hipSuccess) {
                 set_error("hipMalloc (near-field work)");
                 return IBIM_ERR_NOMEM;

@@ -398,10 +398,10 @@ static int ls_rk3(LsRun &L, int mode, double final_t, double vel, double cfl, do
     unsigned long long *cnt = nullptr;
     double *err = nullptr, *pe = nullptr;
     unsigned int *pc = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&cnt, 2 * sizeof(unsigned long long)));
-    IBIM_HIP_CHECK(hipMalloc(&err, sizeof(double)));
-    IBIM_HIP_CHECK(hipMalloc(&pc, 2 * (size_t)nb * sizeof(unsigned int)));
-    IBIM_HIP_CHECK(hipMalloc(&pe, (size_t)nb * sizeof(double)));
+    IBIM_HIP_CHECK(dev_malloc(&cnt, 2 * sizeof(unsigned long long)));
+    IBIM_HIP_CHECK(dev_malloc(&err, sizeof(double)));
+    IBIM_HIP_CHECK(dev_malloc(&pc, 2 * (size_t)nb * sizeof(unsigned int)));
+    IBIM_HIP_CHECK(dev_malloc(&pe, (size_t)nb * sizeof(double)));
     auto count = [&](unsigned long long *h) -> int {
         k_count<<<nb, 256, 0, L.st>>>(L.g, L.d, tube, thres, pc, pe);
         k_count_sum<<<1, 1024, 0, L.st>>>(nb, pc, pe, cnt, err);
@@ -442,8 +442,8 @@ static int ls_surface(LsRun &L, const ibim_geom_params &gp, ibim_surface *S) {
     const int nb = nblk(L.n);
     unsigned char *vis = nullptr;
     int *changed = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&vis, (size_t)L.n));
-    IBIM_HIP_CHECK(hipMalloc(&changed, sizeof(int)));
+    IBIM_HIP_CHECK(dev_malloc(&vis, (size_t)L.n));
+    IBIM_HIP_CHECK(dev_malloc(&changed, sizeof(int)));
     IBIM_HIP_CHECK(hipMemsetAsync(vis, 0, (size_t)L.n, L.st));
     const unsigned char one = 1;
     IBIM_HIP_CHECK(hipMemcpyAsync(vis, &one, 1, hipMemcpyHostToDevice, L.st));
@@ -462,7 +462,7 @@ static int ls_surface(LsRun &L, const ibim_geom_params &gp, ibim_surface *S) {
         }
     }
     unsigned long long *cnt = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&cnt, sizeof(unsigned long long)));
+    IBIM_HIP_CHECK(dev_malloc(&cnt, sizeof(unsigned long long)));
     IBIM_HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), L.st));
     k_inclusion<<<nb, 256, 0, L.st>>>(L.g, L.n, vis, tube, cnt);
     IBIM_LAUNCH_CHECK();
@@ -473,15 +473,15 @@ static int ls_surface(LsRun &L, const ibim_geom_params &gp, ibim_surface *S) {
     (void)hipFree(cnt);
 
     int *flag = nullptr, *pos = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&flag, ((size_t)L.n + 1) * sizeof(int)));
-    IBIM_HIP_CHECK(hipMalloc(&pos, ((size_t)L.n + 1) * sizeof(int)));
+    IBIM_HIP_CHECK(dev_malloc(&flag, ((size_t)L.n + 1) * sizeof(int)));
+    IBIM_HIP_CHECK(dev_malloc(&pos, ((size_t)L.n + 1) * sizeof(int)));
     IBIM_HIP_CHECK(hipMemsetAsync(flag, 0, ((size_t)L.n + 1) * sizeof(int), L.st));
     k_tube_flag<<<nb, 256, 0, L.st>>>(L.g, L.n, tube, flag);
     IBIM_LAUNCH_CHECK();
     size_t bytes = 0;
     IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flag, pos, (int)(L.n + 1), L.st));
     void *tmp = nullptr;
-    IBIM_HIP_CHECK(hipMalloc(&tmp, bytes ? bytes : 1));
+    IBIM_HIP_CHECK(dev_malloc(&tmp, bytes ? bytes : 1));
     IBIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, flag, pos, (int)(L.n + 1), L.st));
     int total = 0;
     IBIM_HIP_CHECK(hipMemcpyAsync(&total, pos + L.n, sizeof(int), hipMemcpyDeviceToHost, L.st));
@@ -494,9 +494,9 @@ static int ls_surface(LsRun &L, const ibim_geom_params &gp, ibim_surface *S) {
         return IBIM_ERR_STATE;
     }
     S->n = total;
-    IBIM_HIP_CHECK(hipMalloc(&S->xyz, 3 * (size_t)total * sizeof(double)));
-    IBIM_HIP_CHECK(hipMalloc(&S->nrm, 3 * (size_t)total * sizeof(double)));
-    IBIM_HIP_CHECK(hipMalloc(&S->w, (size_t)total * sizeof(double)));
+    IBIM_HIP_CHECK(dev_malloc(&S->xyz, 3 * (size_t)total * sizeof(double)));
+    IBIM_HIP_CHECK(dev_malloc(&S->nrm, 3 * (size_t)total * sizeof(double)));
+    IBIM_HIP_CHECK(dev_malloc(&S->w, (size_t)total * sizeof(double)));
     const double s = gp.scale > 0 ? gp.scale : 1.0;
     const double hs = L.d.h / s;
     k_tube_emit<<<nb, 256, 0, L.st>>>(L.g, L.d, flag, pos, tube, 1.0 / s, s * hs * hs * hs, S->xyz, S->nrm, S->w);
@@ -537,11 +537,11 @@ static int alloc_grid(LsRun &L, bool full) {
         set_error("grid dimensions must be >= 7 and hold < 2^31 points");
         return IBIM_ERR_ARG;
     }
-    IBIM_HIP_CHECK(hipMalloc(&L.g, (size_t)L.n * sizeof(double)));
+    IBIM_HIP_CHECK(dev_malloc(&L.g, (size_t)L.n * sizeof(double)));
     if (full) {
-        IBIM_HIP_CHECK(hipMalloc(&L.phi0, (size_t)L.n * sizeof(double)));
-        IBIM_HIP_CHECK(hipMalloc(&L.u1, (size_t)L.n * sizeof(double)));
-        IBIM_HIP_CHECK(hipMalloc(&L.u2, (size_t)L.n * sizeof(double)));
+        IBIM_HIP_CHECK(dev_malloc(&L.phi0, (size_t)L.n * sizeof(double)));
+        IBIM_HIP_CHECK(dev_malloc(&L.u1, (size_t)L.n * sizeof(double)));
+        IBIM_HIP_CHECK(dev_malloc(&L.u2, (size_t)L.n * sizeof(double)));
     }
     return IBIM_OK;
 }
@@ -613,7 +613,7 @@ int ibim_surface_from_atoms(const ibim_geom_params *gp, int64_t m, const double 
         return code;
     };
     // expand (levelset.cpp:166-184) onto g = -2 * max(dims) * h
-    if (hipMalloc(&d_at, 4 * (size_t)m * sizeof(double)) != hipSuccess) {
+    if (dev_malloc(&d_at, 4 * (size_t)m * sizeof(double)) != hipSuccess) {
         set_error("hipMalloc (atoms)");
         return fail(IBIM_ERR_NOMEM);
     }

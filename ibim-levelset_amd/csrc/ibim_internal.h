@@ -319,6 +319,14 @@ struct StageTimer {
 // must be idle (its stream synchronised).
 hipError_t block_alloc(void **p, size_t bytes);
 void block_release(void *p);
+// hipMalloc for device buffers outside the block cache (level-set grids,
+// request vectors, target sets): when it fails, the current device's idle
+// cached blocks are freed and the allocation is tried once more
+hipError_t dev_malloc_raw(void **p, size_t bytes);
+template <typename T>
+hipError_t dev_malloc(T **p, size_t bytes) {
+    return dev_malloc_raw((void **)p, bytes);
+}
 
 // Device memory pool of one plan: chunks from hipMalloc handed out in
 // 256-byte aligned pieces.  The persistent pool frees its chunks with the
@@ -483,11 +491,13 @@ struct ibim_plan {
     int *esig = nullptr;                 // per (operator, row) exponent
     int *keyslot = nullptr, *xkeyslot = nullptr;
     bool x16 = false;                    // the X list runs in the FP16 stage
+    bool f16_ready = false;              // every FP16 stage buffer above is built
     _Float16 *Mh = nullptr;              // nnodes * (4r rounded up to 64), per box scaled
     int *ebox = nullptr;                 // per box exponent
     unsigned *Yv = nullptr;              // V then X interaction results as FP24 (3 dwords per 4 rows),
                                          // pairs * (4r rounded up to 32) values
     int y_rows = 0;                      // pair rows of Yv; one spare row follows them
+    size_t yv_dwords = 0;                // dwords allocated for Yv
     _Float16 *Bx = nullptr;              // X pairs: reflected FP16 source multipoles, one row per pair
     int *ebx = nullptr, *xiota = nullptr; // X pairs: the source box exponent; 0, 1, 2, ...
     int *ytptr = nullptr, *ytlist = nullptr; // rows of Yv of target node b: ytlist[ytptr[b] .. ytptr[b+1])

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 
+#include "ibim_fp24.h"
 #include "ibim_internal.h"
 
 namespace ibim {
@@ -603,7 +604,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 if (j0 + col >= jend) continue;
                 const int ea = sEA[col];
                 const f4v a = acc[pi][ri];
-                store4_fp24(Y + ((size_t)(j0 + col) * YS + row) / 4 * 3, ldexpf(a[0], e0 + ea), ldexpf(a[1], e1 + ea),
+                store4_fp24(Y + fp24_offset(j0 + col, YS, row), ldexpf(a[0], e0 + ea), ldexpf(a[1], e1 + ea),
                             ldexpf(a[2], e2 + ea), ldexpf(a[3], e3 + ea));
             }
         }
@@ -780,21 +781,21 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW >= 8
 #pragma unroll
             for (int i = 0; i < NSTO; i += 2) {
                 uint4 v[2];
-                int o[2];
+                size_t o[2];
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     const int t = 64 * (i + u) + l, pp = t / 12, k = t - 12 * pp;
                     v[u] = *(const uint4 *)(stg + pp * PS + 4 * k);
                     const int jj = j0 + 32 * w + pp;
-                    o[u] = (jj < jend ? jj : yspare) * (YS / 4 * 3) + rt * 48 + 4 * k;
+                    o[u] = fp24_offset(jj < jend ? jj : yspare, YS, 64 * rt) + 4 * k;
                 }
 #pragma unroll
-                for (int u = 0; u < 2; ++u) *(uint4 *)(Y + (size_t)(unsigned)o[u]) = v[u];
+                for (int u = 0; u < 2; ++u) *(uint4 *)(Y + o[u]) = v[u];
                 __builtin_amdgcn_sched_barrier(0);
             }
         } else if (COUNTED) {
             const int ea = sEA[col];  // 0 without a pair
-            unsigned *yp = Y + (size_t)(has_pair ? j0 + col : yspare) * YS / 4 * 3;
+            unsigned *yp = Y + fp24_offset(has_pair ? j0 + col : yspare, YS, 0);
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -807,7 +808,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW >= 8
                 }
         } else if (has_pair) {
             const int ea = sEA[col];
-            unsigned *yp = Y + (size_t)(j0 + col) * YS / 4 * 3;
+            unsigned *yp = Y + fp24_offset(j0 + col, YS, 0);
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -888,7 +889,7 @@ __global__ __launch_bounds__(256) void k_v_reduce(const int *__restrict__ work, 
             const int q = min(qb + i, q1 - 1);
             const int row = tlist[q];
             xm[i] = xmask[row - npv];
-            xv[i] = act ? load4_fp24(Y + (size_t)row * YS / 4 * 3 + e / 4 * 3) : f4v{};
+            xv[i] = act ? load4_fp24(Y + fp24_offset(row, YS, e)) : f4v{};
         }
 #pragma unroll
         for (int i = 0; i < XB; ++i) {
@@ -911,12 +912,12 @@ __global__ __launch_bounds__(256) void k_v_reduce(const int *__restrict__ work, 
         f4v a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0, a2 = a0, a3 = a0;
         int q = q0;
         for (; q + 3 < qx; q += 4) {
-            a0 += load4_fp24(Y + ((size_t)tlist[q] * YS + e) / 4 * 3);
-            a1 += load4_fp24(Y + ((size_t)tlist[q + 1] * YS + e) / 4 * 3);
-            a2 += load4_fp24(Y + ((size_t)tlist[q + 2] * YS + e) / 4 * 3);
-            a3 += load4_fp24(Y + ((size_t)tlist[q + 3] * YS + e) / 4 * 3);
+            a0 += load4_fp24(Y + fp24_offset(tlist[q], YS, e));
+            a1 += load4_fp24(Y + fp24_offset(tlist[q + 1], YS, e));
+            a2 += load4_fp24(Y + fp24_offset(tlist[q + 2], YS, e));
+            a3 += load4_fp24(Y + fp24_offset(tlist[q + 3], YS, e));
         }
-        for (; q < qx; ++q) a0 += load4_fp24(Y + ((size_t)tlist[q] * YS + e) / 4 * 3);
+        for (; q < qx; ++q) a0 += load4_fp24(Y + fp24_offset(tlist[q], YS, e));
         const f4v t = ((a0 + a1) + (a2 + a3)) + ax;
         double *dst = L + (size_t)b * R4 + e;
         dst[0] += (double)t[0];
@@ -935,9 +936,15 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
     const int grid = 8 * per_xcd;
     hipStream_t st = pl->stream;
     if constexpr (P <= 6) {
-        if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->ops16 && pl->ytptr && &vp == &pl->vp) {
+        if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->f16_ready && &vp == &pl->vp) {
             constexpr int KP = (R4 + 63) / 64 * 64;
             constexpr int YS = (R4 + 31) / 32 * 32;
+            // every store lands in a pair row or the spare row after them
+            // (rows y_rows), values below YS: check that against the buffer
+            if (fp24_offset(pl->y_rows, YS, YS - 4) + 3 > pl->yv_dwords) {
+                set_error("FP16 M2L: pair-result rows exceed their buffer");
+                return IBIM_ERR_STATE;
+            }
             k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, KP, pl->Mh, pl->ebox);
             // V pairs, then (x16) the X pairs into the rows after them.
             // key_off: pair range of each block key; pA / Mh / ebox: the
@@ -970,7 +977,7 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
                 k_x_reflect_half<<<xp.npairs, 64, 0, st>>>(xp.A, xp.mask, pl->gtab, R4, KP, pl->Mh, pl->ebox, pl->Bx,
                                                            pl->ebx);
                 gemm(xp, xp.hcls_off_d, pl->xiota, pl->Bx, pl->ebx, pl->xkeyslot,
-                     pl->Yv + (size_t)vp.npairs * YS / 4 * 3, pl->y_rows - vp.npairs, false);
+                     pl->Yv + fp24_offset(vp.npairs, YS, 0), pl->y_rows - vp.npairs, false);
             }
             if (!pl->defer_reduce)
                 k_v_reduce<R4><<<pl->n_m2l_work, R4 / 4 <= 128 ? 128 : 256, 0, st>>>(
@@ -1037,7 +1044,7 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
     const int nn = pl->nnodes;
     auto alloc = [&](auto **p, size_t cnt) -> int {
         if (tmp_owner) {
-            hipError_t e = hipMalloc((void **)p, (cnt ? cnt : 1) * sizeof(**p));
+            hipError_t e = dev_malloc((void **)p, (cnt ? cnt : 1) * sizeof(**p));
             if (e != hipSuccess) {
                 set_error("hipMalloc (V pairs)");
                 return IBIM_ERR_NOMEM;
@@ -1462,7 +1469,7 @@ int m2l_gemm(ibim_plan *pl, const VPairs &vp, bool mixed) {
 static int build_xops_f32(ibim_plan *pl);
 
 int build_m2l_operators_f16(ibim_plan *pl) {
-    if (pl->ops16 || !pl->ops || pl->n_cls == 0) return IBIM_OK;
+    if (pl->f16_ready || !pl->ops || pl->n_cls == 0) return IBIM_OK;
     if (pl->p > 6) return build_m2l_operators_f32(pl);  // the FP16 GEMM's LDS tiles stop at p = 6
     hipStream_t st = pl->stream;
     const int R4 = 4 * pl->r;
@@ -1481,9 +1488,14 @@ int build_m2l_operators_f16(ibim_plan *pl) {
         return IBIM_OK;
     };
     int rc;
+    if ((int64_t)vp.npairs + xp.npairs + 1 > (int64_t)0x7fffffff) {
+        set_error("FP16 M2L: more than 2^31 - 2 V and X interactions");
+        return IBIM_ERR_ARG;
+    }
     if ((rc = dalloc(pl, &pl->Mh, (size_t)pl->nnodes * KP)) || (rc = dalloc(pl, &pl->ebox, (size_t)pl->nnodes)) ||
-        (rc = dalloc(pl, &pl->Yv, (size_t)(npa + 1) * YS / 4 * 3)))
+        (rc = dalloc(pl, &pl->Yv, fp24_buffer_dwords((int64_t)npa + 1, YS))))
         return rc;
+    pl->yv_dwords = fp24_buffer_dwords((int64_t)npa + 1, YS);
     pl->y_rows = npa;
     // unfolded operators for the (class, mask) keys that have pairs: V keys
     // in slots [0, nsv), X keys in [nsv, nsv + nsx)
@@ -1587,7 +1599,10 @@ int build_m2l_operators_f16(ibim_plan *pl) {
     }
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
     pl->x16 = nsx > 0;
-    if (!pl->x16) return build_xops_f32(pl);  // X list in FP32 GEMMs or direct
+    if (!pl->x16 && (rc = build_xops_f32(pl))) return rc;  // X list in FP32 GEMMs or direct
+    // only a complete build enables the FP16 stage (a failed one leaves the
+    // plan on the FP32 / FP64 GEMMs, and a retry rebuilds from the start)
+    pl->f16_ready = true;
     return IBIM_OK;
 }
 

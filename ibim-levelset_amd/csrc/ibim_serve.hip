@@ -100,9 +100,9 @@ void run_job(ibim_server::Job &j) {
     if (rc == IBIM_OK) rc = ibim_plan_create_device(&j.req.fmm, inf.n_nodes, sx, sn, sw, &plan);
     const size_t n2 = 2 * (size_t)inf.n_nodes;
     if (rc == IBIM_OK) {
-        if (hipMalloc(&d_b, n2 * sizeof(double)) != hipSuccess || hipMalloc(&d_x, n2 * sizeof(double)) != hipSuccess ||
-            hipMalloc(&d_at, 4 * (size_t)m * sizeof(double)) != hipSuccess ||
-            hipMalloc(&d_phi, (size_t)m * sizeof(double)) != hipSuccess) {
+        if (dev_malloc(&d_b, n2 * sizeof(double)) != hipSuccess || dev_malloc(&d_x, n2 * sizeof(double)) != hipSuccess ||
+            dev_malloc(&d_at, 4 * (size_t)m * sizeof(double)) != hipSuccess ||
+            dev_malloc(&d_phi, (size_t)m * sizeof(double)) != hipSuccess) {
             set_error("hipMalloc (request vectors)");
             rc = IBIM_ERR_NOMEM;
         }

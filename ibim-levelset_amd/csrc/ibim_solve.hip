@@ -362,7 +362,7 @@ int make_target_set(ibim_plan *pl, int m, const double *d_txyz, TargetSet &ts, s
     hipStream_t st = pl->stream;
     const int nn = pl->nnodes, maxlev = pl->prm.max_level;
     auto talloc = [&](auto **p, size_t cnt) -> int {
-        hipError_t e = hipMalloc((void **)p, (cnt ? cnt : 1) * sizeof(**p));
+        hipError_t e = dev_malloc((void **)p, (cnt ? cnt : 1) * sizeof(**p));
         if (e != hipSuccess) {
             set_error(std::string("hipMalloc (targets): ") + hipGetErrorString(e));
             return IBIM_ERR_NOMEM;
@@ -433,7 +433,7 @@ int target_fmm_device(ibim_plan *pl, int kid, int m, const double *d_axyz, const
     double *F = nullptr;
     int rc = make_target_set(pl, m, d_axyz, ts, tmp, &tperm);
     if (rc == IBIM_OK) {
-        hipError_t e = hipMalloc((void **)&F, 4 * (size_t)m * sizeof(double));
+        hipError_t e = dev_malloc((void **)&F, 4 * (size_t)m * sizeof(double));
         if (e != hipSuccess) {
             set_error("hipMalloc (polarize fields)");
             rc = IBIM_ERR_NOMEM;
