@@ -1,56 +1,72 @@
 #!/usr/bin/env python3
-"""Benchmark: PB solves/s and latency on one MI355X, mixed precision vs FP64.
+"""Benchmark: served PB solves/s and p50/p99 latency on one MI355X, mixed
+precision vs FP64, on the serving stream of 256 synthetic globules.
 
-Workload (default): the synthetic 500-atom globule -- seeded self-avoiding
-walk of 3.8 A steps in a 20 A sphere, radii U[1.2,2.0] A, charges U[-0.5,0.5]
-shifted to net -2 e -- meshed at h = 0.25 A (tube width 2h, probe 1.4 A),
-eps_in = 2, eps_out = 80, kappa = 0.1257 /A, Chebyshev order p = 5, leaf
-capacity 192 (the fastest FP64 matvec of a 128..384 sweep), max level 10,
-tau = 1, GMRES(40) to 1e-8.
+Workload (default, `--config stream256`): 256 seeded synthetic globules whose
+atom counts are exp(U[ln 100, ln 20000]) (seed 4); molecule i is the
+self-avoiding walk of seed 10000 + i (3.8 A steps, 3.0 A exclusion) in a
+sphere of radius 2.6 m^(1/3) A, radii U[1.2, 2.0] A, charges U[-0.5, 0.5]
+shifted to net -2 e.  Each request is meshed at h = 0.35 A (tube width 2h,
+probe 1.4 A) and solved with eps_in = 2, eps_out = 80, kappa = 0.1257 /A,
+tau = 1, Chebyshev order p = 5, leaf capacity 192, max level 10, GMRES(40)
+to 1e-8, then its solvation energy is formed from the reaction potential at
+the atoms.  A request is the whole chain: level set, tube nodes, plan (tree,
+lists, operators), right-hand side, GMRES, reaction potential, energy.
 
-One step = one complete PB solve with every input already in HBM (surface
-nodes, normals, weights, atoms): octree + interaction lists + M2L operators,
-right-hand side, GMRES with the FMM matvec, reaction potential at the atoms
-and the solvation energy.  The level-set meshing runs once before the timed
-region.  The solve is timed in FP64 throughout and in two mixed forms
-inside FP64 iterative refinement (FP32 near field and X list, with the
-V-list M2L on FP32 MFMA or on FP16 MFMA with power-of-two scaling); `value`
-is the faster mixed form's solves/s among those whose energy is within 1e-6
-of the FP64 solve's (`energy_within_1e-6`).
+One step = one served request.  The requests go through the continuous
+batching server (`slots` requests at once, admission by device footprint):
 
-A served leg then pushes a burst of seeded globules of the same size through
-the continuous-batching server (geometry included per request) and reports
-its solves/s and p50/p99 latency.
+1. Saturated rates on a stated subsample (every `--fp64-every`-th request
+   of the stream): a burst of the subsample in the mixed form and in FP64.
+   Their ratio is the like-for-like served speed-up of the mixed form; the
+   energies of the two runs give the mixed form's energy error.
+2. The mixed stream (the headline): all 256 requests arrive as a Poisson
+   process at `--load` times the saturated mixed rate (scaled from the
+   subsample by mean atom count).  value = completed solves / wall time,
+   with p50/p99 latency (completion - arrival).  Stage timers in every
+   request's plan give the roofline of the dominant kernel (the near field)
+   and of the M2L stage over this timed region.
+3. The FP64 stream: the subsample as a Poisson process at `--load` times its
+   saturated FP64 rate: served FP64 solves/s and p50/p99.
 
-Prints one JSON line.  The roofline entry is computed from HIP events that
-the library records on its own stream around each FMM stage during the timed
-mixed solves (the V-list M2L stage is the dominant one); its traffic
-figure comes from the committed rocprofv3 counter summary when present.  The
-cpu_baseline entry times the CPU oracle (a restatement of the reference
-algorithm) on a bounded sample of the same surface.
+--steps does not change the stream: the workload is its 256 requests.
+--warmup W serves W small molecules in each precision before timing.
+
+The cpu_baseline entry times the CPU oracle (a restatement of the reference
+algorithm) on this machine's cores: one full matvec (the 16 kernel FMMs,
+each with its own tree, as the reference runs them) on a bounded sample of
+the stream's first molecule, scaled to the stream; and one complete Born
+ion solve (the 64^3 sphere, N = 12,790) end to end, with its energy against
+the GPU's.
+
+`--config globule500` / `globule5000` time single solves of one molecule
+instead (h = 0.25 A), FP64 and both mixed forms.  `--matvecs-only K` is the
+counter-collection mode used for the HBM traffic figures.
 """
 import argparse
 import json
 import os
 import platform
 import sys
+import threading
 import time
 
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
-PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "f16": 2500.0}   # MI355X dense MFMA
+# MI355X published peaks: vector FP32 / FP64 and dense MFMA, TFLOP/s; HBM TB/s
+PEAK = {"valu_f32": 157.3, "valu_f64": 78.6, "mfma_f16": 2500.0, "mfma_f32": 157.3, "mfma_f64": 78.6, "hbm": 8.0}
 METRIC = "PB solves/s + p50/p99 latency, mixed vs FP64 GPU at <=1e-6 rel energy error"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "m2l_gemm_globule500_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "p2p_mixed_stream256_traffic.json")
 
+FMM = dict(p=5, leaf=192, max_level=10, kappa=0.1257, eps_in=2.0, eps_out=80.0, tau=1.0, restart=40, tol=1e-8)
 CONFIGS = {
-    # name: molecule (atoms, seed, confinement), grid h, p, leaf, kappa, eps_in, eps_out
-    "globule500": dict(atoms=500, seed=500, confine=20.0, h=0.25, p=5, leaf=192, max_level=10, kappa=0.1257,
-                       eps_in=2.0, eps_out=80.0, tau=1.0, restart=40, tol=1e-8),
-    # the 5000-atom globule in a 45 A sphere at p = 6 (not the default workload)
-    "globule5000": dict(atoms=5000, seed=5000, confine=45.0, h=0.25, p=6, leaf=256, max_level=10, kappa=0.1257,
-                        eps_in=2.0, eps_out=80.0, tau=1.0, restart=40, tol=1e-8),
+    "stream256": dict(FMM, requests=256, h=0.35, atoms_lo=100, atoms_hi=20000, size_seed=4, mol_seed=10000,
+                      confine=2.6),
+    # single molecules (not the default workload)
+    "globule500": dict(FMM, atoms=500, seed=500, confine=20.0, h=0.25),
+    "globule5000": dict(FMM, atoms=5000, seed=5000, confine=45.0, h=0.25, p=6, leaf=256),
 }
 
 
@@ -75,141 +91,263 @@ def cpu_model():
     return platform.processor()
 
 
-def run_cpu_baseline(cfg, pts, nrm, w, matvecs_per_solve, threads):
-    """Time the CPU oracle on a bounded sample: one of the 16 kernel FMMs (K2,
-    tree build included, as the reference does per kernel) on the surface
-    points of one octant of the molecule, then scale to the full solve:
-    16 kernels per matvec, points/sample points (the FMM is O(N)), and the
-    FP64 solve's matvec count plus 4 FMMs for the energy."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    c = np.median(pts, axis=0)
-    sel = np.all(pts > c, axis=1)
-    sample = pts[sel]
-    q = np.random.default_rng(0).normal(size=len(sample))
-    t0 = time.perf_counter()
-    tree = O.Tree(sample, max_point=cfg["leaf"], max_level=cfg["max_level"])
-    tree.fmm(cfg["p"], "K2", q, kappa=cfg["kappa"], eps_in=cfg["eps_in"], eps_out=cfg["eps_out"],
-             vacant_radius=cfg["tau"] * cfg["h"])
-    t_fmm = time.perf_counter() - t0
-    tree.close()
-    scale = len(pts) / len(sample)
-    t_matvec = 16 * t_fmm * scale
-    t_solve = (matvecs_per_solve + 0.25) * t_matvec
-    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle K2 FMM (tree build + up/down pass, OpenMP) on the {len(sample)} surface points of "
-                       f"one octant ({t_fmm:.2f} s on {threads} threads, {cpu_model()}); scaled x16 kernels "
-                       f"x{scale:.2f} points x({matvecs_per_solve}+0.25) matvecs per FP64 solve"),
-            "matvec_s_est": t_matvec}
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def pct(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
 
 
-def serve_stream(ibim, n, slots, precision, cfg):
-    """A request stream like the serving configuration, scaled down: n
-    globules with atom counts exp(U[ln 100, ln 2000]) (seed 4), molecule i from
-    seed 10000 + i in a sphere of radius 2.6 m^(1/3) A, h = 0.35 A, the
-    workload's FMM settings.  A burst measures the saturated rate mu; the
-    same molecules then arrive as a Poisson process at 0.7 mu."""
-    import threading
-    rng = np.random.default_rng(4)
-    sizes = np.exp(rng.uniform(np.log(100), np.log(2000), n)).astype(int)
-    mols = [ibim.synthetic_globule(int(m), 10000 + i, 2.6 * m ** (1 / 3)) for i, m in enumerate(sizes)]
-    fmm = dict(p=cfg["p"], max_points=cfg["leaf"], max_level=cfg["max_level"], kappa=cfg["kappa"],
-               eps_in=cfg["eps_in"], eps_out=cfg["eps_out"], vacant_radius=cfg["tau"] * 0.35, precision=precision)
-    out = {"requests": n, "slots": slots, "atoms": [int(m) for m in sizes], "h": 0.35}
+def fmm_kw(cfg, h, precision):
+    return dict(p=cfg["p"], max_points=cfg["leaf"], max_level=cfg["max_level"], kappa=cfg["kappa"],
+                eps_in=cfg["eps_in"], eps_out=cfg["eps_out"], vacant_radius=cfg["tau"] * h, precision=precision)
+
+
+# ------------------------------------------------------------ the stream --
+def stream_molecules(ibim, cfg):
+    rng = np.random.default_rng(cfg["size_seed"])
+    sizes = np.exp(rng.uniform(np.log(cfg["atoms_lo"]), np.log(cfg["atoms_hi"]), cfg["requests"])).astype(int)
+    mols = [ibim.synthetic_globule(int(m), cfg["mol_seed"] + i, cfg["confine"] * m ** (1 / 3))
+            for i, m in enumerate(sizes)]
+    return sizes, mols
+
+
+def serve(ibim, mols, cfg, precision, slots, rate=None, profile=None, arrival_seed=5):
+    """Push molecules through a server: all at t = 0 (rate None) or as a
+    Poisson process of the given rate.  Returns the results (in molecule
+    order), the wall time from the first arrival to the last completion and
+    the server's stage report."""
+    h = cfg["h"]
+    kw = fmm_kw(cfg, h, precision)
     with ibim.Server(slots=slots) as srv:
-        t0 = time.perf_counter()
-        ids = [srv.submit(a, r, q, h=0.35, restart=cfg["restart"], tol=cfg["tol"], **fmm) for a, r, q in mols]
-        res = [srv.wait(i) for i in ids]
-        mu = n / (time.perf_counter() - t0)
-        out["burst"] = {"solves_per_s": mu, "all_ok": all(r["status"] == 0 for r in res),
-                        "tube_points": [int(r["n_points"]) for r in res]}
-        lam = 0.7 * mu
-        gaps = np.random.default_rng(5).exponential(1.0 / lam, n)
-        ids = []
+        if profile:
+            srv.profile(profile)
+        ids = [None] * len(mols)
+        gaps = (np.zeros(len(mols)) if rate is None else
+                np.random.default_rng(arrival_seed).exponential(1.0 / rate, len(mols)))
 
         def producer():
-            for (a, r, q), g in zip(mols, gaps):
-                time.sleep(g)
-                ids.append(srv.submit(a, r, q, h=0.35, restart=cfg["restart"], tol=cfg["tol"], **fmm))
+            t_next = time.perf_counter()
+            for i, (a, r, q) in enumerate(mols):
+                t_next += gaps[i]
+                dt = t_next - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids[i] = srv.submit(a, r, q, h=h, restart=cfg["restart"], tol=cfg["tol"], **kw)
 
-        t1 = time.perf_counter()
+        t0 = time.perf_counter()
         th = threading.Thread(target=producer)
         th.start()
         th.join()
         res = [srv.wait(i) for i in ids]
-        wall = time.perf_counter() - t1
+        wall = time.perf_counter() - t0
+        rep = srv.stage_report() if profile else None
+    return res, wall, rep
+
+
+def summarize(res, wall):
     lat = [1e3 * (r["t_end"] - r["t_submit"]) for r in res]
-    out["poisson"] = {"rate_per_s": lam, "solves_per_s": n / wall, "latency_ms": {"p50": pct(lat, 50),
-                                                                                  "p99": pct(lat, 99)},
-                      "all_ok": all(r["status"] == 0 and r["info"]["converged"] == 1 for r in res)}
+    return {"requests": len(res), "solves_per_s": len(res) / wall, "wall_s": wall,
+            "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": max(lat)},
+            "tube_points": int(sum(r["n_points"] for r in res)),
+            "all_converged": all(r["status"] == 0 and r["info"]["converged"] == 1 for r in res)}
+
+
+def roofline_entries(rep, p):
+    """Rooflines over the timed mixed stream from the stage report: the near
+    field (vector FP32), the FP16 M2L stage (matrix cores) and the whole
+    mixed matvec (vector + matrix + HBM terms of the matvec's roofline)."""
+    out = {}
+    nf = rep.get("P2P_MIXED")
+    if nf and nf["ms"] > 0:
+        ach = nf["valu_flops"] / (nf["ms"] * 1e-3) / 1e12
+        per = nf["valu_flops"] / nf["launches"]
+        out["near_field"] = {"bound": "valu", "achieved": ach, "peak": PEAK["valu_f32"], "unit": "TFLOP/s",
+                             "frac": ach / PEAK["valu_f32"], "kernel": "k_p2p_unit<true,false> (FP32 near field)",
+                             "launches": nf["launches"], "ms_per_launch": nf["ms"] / nf["launches"],
+                             "algorithmic_flops_per_launch": per, "pairs_per_launch": per / 60.0}
+    m2l = rep.get("M2L_MIXED")
+    if m2l and m2l["ms"] > 0:
+        f16 = p <= 6
+        peak = PEAK["mfma_f16"] if f16 else PEAK["mfma_f32"]
+        ach = m2l["mfma_flops"] / (m2l["ms"] * 1e-3) / 1e12
+        out["m2l"] = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                      "kernel": (f"k_m_to_half + k_m2l_gemm_h32<{p},8> (V) + k_m2l_gemm_h32<{p},4> (X), FP16 MFMA"
+                                 if p <= 5 else f"k_m2l_gemm_hu<{p},16>" if f16 else f"k_m2l_gemm<{p},float>"),
+                      "launches": m2l["launches"], "ms_per_launch": m2l["ms"] / m2l["launches"],
+                      "algorithmic_flops_per_launch": m2l["mfma_flops"] / m2l["launches"]}
+    mv = rep.get("MATVEC_MIXED")
+    if mv and mv["ms"] > 0:
+        t_roof = (mv["valu_flops"] / (PEAK["valu_f32"] * 1e12) +
+                  mv["mfma_flops"] / ((PEAK["mfma_f16"] if p <= 6 else PEAK["mfma_f32"]) * 1e12) +
+                  mv["bytes"] / (PEAK["hbm"] * 1e12))
+        out["matvec"] = {"frac": t_roof / (mv["ms"] * 1e-3), "launches": mv["launches"],
+                         "ms_per_launch": mv["ms"] / mv["launches"], "roofline_ms_per_launch":
+                         1e3 * t_roof / mv["launches"],
+                         "note": "sum of the near field at the FP32 vector peak, the V+X interactions at the FP16 "
+                                 "MFMA peak and the point and expansion bytes at 8 TB/s, over the measured time "
+                                 "of the mixed matvecs"}
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="globule500", choices=sorted(CONFIGS))
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--serve", type=int, default=8, help="requests in the served burst (0 = skip)")
-    ap.add_argument("--slots", type=int, default=2, help="server slots of the served burst")
-    ap.add_argument("--stream", type=int, default=0, metavar="N",
-                    help="also serve a stream of N seeded globules of 100..2000 atoms (h = 0.35 A): a burst for the "
-                         "saturated rate, then Poisson arrivals at 70%% of it")
-    ap.add_argument("--leaf", type=int, default=None, help="override the leaf capacity")
-    ap.add_argument("--matvecs-only", type=int, default=0, metavar="K",
-                    help="counter-collection mode: one 256 MiB device copy (calibration), then K matvecs in each "
-                         "precision (FP16-M2L mixed, FP32 mixed, FP64) on the workload's plan; prints a short JSON "
-                         "line, not the benchmark line")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (solves of different molecules "
-              f"are independent: N GPUs run N request streams with no exchange)", file=sys.stderr)
-        sys.exit(2)
-
-    import torch
-    if not torch.cuda.is_available():
-        print("bench.py needs a GPU", file=sys.stderr)
-        sys.exit(1)
-    torch.cuda.set_device(0)
-    ibim = load_ibim()
-    PRECS = {"mixed_f32": ibim.PREC_MIXED, "mixed_f16": ibim.PREC_MIXED_F16}
-    cfg = dict(CONFIGS[args.config])
-    if args.leaf:
-        cfg["leaf"] = args.leaf
-
-    # ---- setup (untimed): molecule, level set, tube nodes -- all in HBM
-    xyz, rad, qa = ibim.synthetic_globule(cfg["atoms"], cfg["seed"], cfg["confine"])
+def run_stream(ibim, args, cfg):
+    import torch  # noqa: F401  (device initialised for the library's streams)
+    slots = args.slots
     t0 = time.perf_counter()
+    sizes, mols = stream_molecules(ibim, cfg)
+    progress(f"stream: {len(mols)} molecules of {sizes.min()}..{sizes.max()} atoms (mean {sizes.mean():.0f}) "
+             f"generated in {time.perf_counter() - t0:.1f} s")
+    sub = list(range(0, len(mols), args.fp64_every))
+    sub_mols = [mols[i] for i in sub]
+    # warm-up: W small molecules per precision (code objects, block cache)
+    warm = [ibim.synthetic_globule(120 + 20 * i, 30000 + i, cfg["confine"] * (120 + 20 * i) ** (1 / 3))
+            for i in range(args.warmup)]
+    for prec in (ibim.PREC_MIXED_F16, ibim.PREC_FP64):
+        if warm:
+            serve(ibim, warm, cfg, prec, slots)
+    progress(f"warm-up: {args.warmup} requests per precision")
+
+    # 1. saturated rates on the subsample, both precisions
+    res_m_sub, wall_m_sub, _ = serve(ibim, sub_mols, cfg, ibim.PREC_MIXED_F16, slots)
+    mu_m_sub = len(sub) / wall_m_sub
+    progress(f"subsample burst mixed: {len(sub)} requests, {mu_m_sub:.3f} solves/s")
+    res_d_sub, wall_d_sub, _ = serve(ibim, sub_mols, cfg, ibim.PREC_FP64, slots)
+    mu_d_sub = len(sub) / wall_d_sub
+    progress(f"subsample burst fp64: {mu_d_sub:.3f} solves/s")
+    errs = [abs(a["energy"] - b["energy"]) / abs(b["energy"]) for a, b in zip(res_m_sub, res_d_sub)]
+
+    # 2. the mixed stream: all requests, Poisson arrivals
+    scale = float(np.mean([len(m[0]) for m in sub_mols]) / np.mean(sizes))
+    lam_m = args.load * mu_m_sub * scale
+    stages = ["P2P_MIXED", "M2L_MIXED", "MATVEC_MIXED", "P2P", "M2L", "MATVEC"]
+    res_m, wall_m, rep = serve(ibim, mols, cfg, ibim.PREC_MIXED_F16, slots, rate=lam_m, profile=stages)
+    mixed = summarize(res_m, wall_m)
+    mixed["arrival_rate_per_s"] = lam_m
+    progress(f"mixed stream: {mixed['solves_per_s']:.3f} solves/s, p50 {mixed['latency_ms']['p50']:.0f} ms, "
+             f"p99 {mixed['latency_ms']['p99']:.0f} ms")
+    # the subsample's requests inside the mixed stream against their FP64 energies
+    errs_in_stream = [abs(res_m[i]["energy"] - b["energy"]) / abs(b["energy"]) for i, b in zip(sub, res_d_sub)]
+
+    # 3. the FP64 stream on the subsample
+    lam_d = args.load * mu_d_sub
+    res_d, wall_d, _ = serve(ibim, sub_mols, cfg, ibim.PREC_FP64, slots, rate=lam_d, arrival_seed=6)
+    fp64 = summarize(res_d, wall_d)
+    fp64["arrival_rate_per_s"] = lam_d
+    progress(f"fp64 stream: {fp64['solves_per_s']:.3f} solves/s, p99 {fp64['latency_ms']['p99']:.0f} ms")
+
+    max_err = max(errs + errs_in_stream)
+    ok = max_err <= 1e-6 and mixed["all_converged"] and fp64["all_converged"]
+    roof = roofline_entries(rep, cfg["p"])
+    stage_tot = {k: {"ms": v["ms"], "launches": v["launches"]} for k, v in rep.items()}
+    return {
+        "sizes": sizes, "mols": mols, "res_m": res_m, "res_d_sub": res_d_sub, "sub": sub,
+        "mixed": mixed, "fp64": fp64,
+        "saturated": {"subsample": f"every {args.fp64_every}th request ({len(sub)} requests)",
+                      "mixed_solves_per_s": mu_m_sub, "fp64_solves_per_s": mu_d_sub,
+                      "mixed_speedup_vs_fp64": mu_m_sub / mu_d_sub},
+        "energy": {"max_rel_err_mixed_vs_fp64": max_err, "within_1e-6": bool(max_err <= 1e-6),
+                   "compared": f"{len(sub)} subsample requests, mixed burst and mixed stream vs FP64 burst"},
+        "ok": ok, "roof": roof, "stages": stage_tot,
+    }
+
+
+# ---------------------------------------------------------- CPU baseline --
+def born_sphere_tube(a=2.0, h=0.125, lo=-4.0, npts=64, thickness=2):
+    """Tube nodes of the sphere |x| = a on the grid lo + i h (analytic level
+    set; the node layout of levelset.cpp:437-586 and electric.cpp:33)."""
+    g = lo + h * np.arange(npts)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    r = np.linalg.norm(P, axis=1)
+    phi = r - a
+    tube = thickness * h
+    m = np.abs(phi) < tube
+    P, r, phi = P[m], r[m], phi[m]
+    n = P / r[:, None]
+    return P - n * phi[:, None], n, 0.5 * (1.0 + np.cos(np.pi * phi / tube)) / tube * h ** 3
+
+
+def cpu_baseline(ibim, cfg, stream, threads):
+    """The CPU oracle on this machine's cores (OpenMP, `threads` threads)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    out = {"unit": "solves/s", "cores": threads, "kind": "port", "cpu": cpu_model()}
+    # (a) one full matvec (16 kernel FMMs, 16 trees) on a bounded sample of
+    # the stream's first molecule: its tube points in a box around the centre
+    xyz, rad, _ = stream["mols"][0]
     surf = ibim.Surface.from_atoms(xyz, rad, h=cfg["h"], probe=1.4, thickness=2)
-    t_mesh = time.perf_counter() - t0
-    n = surf.n
+    pts, nrm, w = surf.arrays()
+    surf.close()
+    c = np.median(pts, axis=0)
+    d = np.max(np.abs(pts - c), axis=1)
+    n_sample = min(len(pts), 20000)
+    sel = np.argsort(d, kind="stable")[:n_sample]
+    x = np.random.default_rng(1).uniform(-1, 1, 2 * n_sample)
+    t0 = time.perf_counter()
+    O.dmapping(pts[sel], nrm[sel], w[sel], x, p=cfg["p"], max_point=cfg["leaf"], max_level=cfg["max_level"],
+               share_tree=False, kappa=cfg["kappa"], eps_in=cfg["eps_in"], eps_out=cfg["eps_out"],
+               vacant_radius=cfg["tau"] * cfg["h"])
+    t_mv = time.perf_counter() - t0
+    # scaled to the stream: O(N) per matvec, FP64 matvecs per solve from the
+    # GPU FP64 runs of the subsample (their mean where a request has none)
+    mv64 = {i: r["info"]["matvecs"] for i, r in zip(stream["sub"], stream["res_d_sub"])}
+    mean_mv = float(np.mean(list(mv64.values())))
+    t_stream = sum(mv64.get(i, mean_mv) * t_mv * r["n_points"] / n_sample for i, r in enumerate(stream["res_m"]))
+    out["value"] = len(stream["res_m"]) / t_stream
+    out["sample"] = (f"oracle dmapping (16 kernel FMMs, one tree each, OpenMP) on {n_sample} tube points of the "
+                     f"stream's first molecule ({t_mv:.2f} s on {threads} threads), scaled by tube points (O(N)) "
+                     f"and by FP64 matvecs per solve to the {len(stream['res_m'])} requests")
+    out["matvec_s"] = t_mv
+    out["matvec_points"] = n_sample
+    # (b) the Born ion solved end to end on the CPU: RHS, GMRES(20), reaction
+    # potential, energy; and on the GPU for comparison
+    pts, nrm, w = born_sphere_tube()
+    prm = dict(kappa=0.0, eps_in=1.0, eps_out=80.0, vacant_radius=0.125)
+    atom, q = np.zeros((1, 3)), np.ones(1)
+    t0 = time.perf_counter()
+    b = O.rhs(pts, nrm, atom, q, prm["eps_in"], prm["eps_out"])
+    xo, rc, it, res, nmv = O.solve(pts, nrm, w, b, p=4, max_point=64, max_level=10, restart=20, maxit=200,
+                                   tol=1e-8, share_tree=False, **prm)
+    e_cpu = 0.5 * float(O.polarize(pts, nrm, w, atom, xo, p=4, max_point=64, max_level=10, **prm) @ q)
+    t_c1 = time.perf_counter() - t0
+    with ibim.Plan(pts, nrm, w, p=4, max_points=64, max_level=10, **prm) as plan:
+        xg, _, info = plan.gmres(plan.rhs(atom, q), restart=20, maxit=200, tol=1e-8)
+        e_gpu = 0.5 * float(plan.polarize(atom, xg) @ q)
+    born = 1.0 / (8 * np.pi * 2.0) * (1 / 80.0 - 1.0)
+    out["born_ion_solve"] = {"points": len(pts), "cpu_s": t_c1, "cpu_matvecs": nmv, "cpu_energy": e_cpu,
+                             "gpu_energy": e_gpu, "gpu_matvecs": info["matvecs"],
+                             "rel_diff_gpu_vs_cpu": abs(e_gpu - e_cpu) / abs(e_cpu),
+                             "analytic_born": born, "rel_err_vs_born": abs(e_cpu - born) / abs(born)}
+    return out
+
+
+# ------------------------------------------------------- single molecule --
+def run_single(ibim, args, cfg):
+    """FP64 and both mixed forms of one molecule's solve, `steps` times each."""
+    import ctypes as C
+    import torch
     dev = torch.device("cuda:0")
+    xyz, rad, qa = ibim.synthetic_globule(cfg["atoms"], cfg["seed"], cfg["confine"])
+    surf = ibim.Surface.from_atoms(xyz, rad, h=cfg["h"], probe=1.4, thickness=2)
+    n, m = surf.n, cfg["atoms"]
     d_atoms = torch.from_numpy(np.ascontiguousarray(xyz)).to(dev)
     d_q = torch.from_numpy(np.ascontiguousarray(qa)).to(dev)
     d_b = torch.empty(2 * n, dtype=torch.float64, device=dev)
     d_x = torch.empty(2 * n, dtype=torch.float64, device=dev)
-    d_phi = torch.empty(cfg["atoms"], dtype=torch.float64, device=dev)
+    d_phi = torch.empty(m, dtype=torch.float64, device=dev)
     ptrs = surf.device_ptrs()
     torch.cuda.synchronize()
-    import ctypes as C
     P = lambda t: C.c_void_p(t.data_ptr())
-    m = cfg["atoms"]
-    stage_acc = {}   # timed mixed solves: the roofline stage only
-    stage_full = {}  # one untimed solve per mixed form: every stage
+
+    def plan_for(prec):
+        return ibim.Plan(device_ptrs=ptrs, **fmm_kw(cfg, cfg["h"], prec))
 
     if args.matvecs_only:
         cal = torch.ones(32 << 20, dtype=torch.float64, device=dev)  # 256 MiB
         cal2 = torch.empty_like(cal)
         cal2.copy_(cal)
-        plan = ibim.Plan(device_ptrs=ptrs, p=cfg["p"], max_points=cfg["leaf"], max_level=cfg["max_level"],
-                         kappa=cfg["kappa"], eps_in=cfg["eps_in"], eps_out=cfg["eps_out"],
-                         vacant_radius=cfg["tau"] * cfg["h"], precision=ibim.PREC_MIXED)
+        plan = plan_for(ibim.PREC_MIXED)
         d_x.uniform_(-1, 1)
         torch.cuda.synchronize()
         for prec in (ibim.PREC_MIXED_F16, ibim.PREC_MIXED, ibim.PREC_FP64):
@@ -221,16 +359,12 @@ def main():
                           "tree": plan.stats()}))
         plan.close()
         surf.close()
-        return
+        return None
 
-    def solve(precision, profile=None, query=True, acc=None):
-        plan = ibim.Plan(device_ptrs=ptrs, p=cfg["p"], max_points=cfg["leaf"], max_level=cfg["max_level"],
-                         kappa=cfg["kappa"], eps_in=cfg["eps_in"], eps_out=cfg["eps_out"],
-                         vacant_radius=cfg["tau"] * cfg["h"], precision=precision)
-        if profile == "all":
-            plan.profile(True)
-        elif profile == "m2l":  # only the stage the roofline is quoted on (two events per matvec)
-            plan.profile(True, stages=["M2L_MIXED"])
+    def solve(prec, rep_acc=None):
+        plan = plan_for(prec)
+        if rep_acc is not None:
+            plan.profile(True, stages=["P2P_MIXED", "M2L_MIXED", "MATVEC_MIXED"])
         plan.rhs_device(m, P(d_atoms), P(d_q), P(d_b))
         d_x.zero_()
         torch.cuda.synchronize()  # the library's stream does not wait for torch's
@@ -238,162 +372,146 @@ def main():
         plan.polarize_device(m, P(d_atoms), P(d_x), P(d_phi))
         plan.sync()
         energy = 0.5 * float(torch.dot(d_phi, d_q).item())
-        if profile:
-            for k, (ms, cnt) in plan.stage_times().items():
-                a = acc.setdefault(k, [0.0, 0])
-                a[0] += ms
-                a[1] += cnt
-        # tree statistics and footprint are the same for every solve of one
-        # geometry: the timed solves skip these queries (a few ms of small
-        # device-to-host copies) and report those of the warmup solve
-        stats = plan.stats() if query else None
-        foot = plan.footprint() if query else None
+        if rep_acc is not None:
+            for k, v in plan.stage_report().items():
+                a = rep_acc.setdefault(k, dict.fromkeys(v, 0))
+                for f in v:
+                    a[f] += v[f]
         plan.close()
-        return rc, info, energy, stats, foot
+        return rc, info, energy
 
-    def leg(precision, profile=False):
-        stats = foot = None
+    legs = {}
+    acc = {}
+    for name, prec in (("fp64", ibim.PREC_FP64), ("mixed_f32", ibim.PREC_MIXED), ("mixed_f16", ibim.PREC_MIXED_F16)):
         for _ in range(args.warmup):
-            _, _, _, stats, foot = solve(precision)
+            solve(prec)
         torch.cuda.synchronize()
         lat = []
-        t_start = time.perf_counter()
-        for i in range(args.steps):
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
             ts = time.perf_counter()
-            rc, info, energy, st_, ft_ = solve(precision, "m2l" if profile else None,
-                                               query=stats is None and i == args.steps - 1, acc=stage_acc)
+            rc, info, e = solve(prec, acc if name == "mixed_f16" else None)
             lat.append(1e3 * (time.perf_counter() - ts))
-            if st_ is not None:
-                stats, foot = st_, ft_
-        torch.cuda.synchronize()
-        total = time.perf_counter() - t_start
-        if profile:  # the per-stage breakdown: one more solve, untimed, with every stage's events
-            solve(precision, "all", query=False, acc=stage_full)
-        return {"solves_per_s": args.steps / total, "ms_per_solve": 1e3 * total / args.steps,
-                "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "all": lat}, "energy": energy,
-                "solve": dict(info, rc=rc), "plan_bytes": foot}, stats
+        wall = time.perf_counter() - t0
+        legs[name] = {"solves_per_s": args.steps / wall, "ms_per_solve": 1e3 * wall / args.steps,
+                      "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)}, "energy": e, "solve": info}
+        progress(f"{name}: {legs[name]['solves_per_s']:.3f} solves/s")
+    for k in ("mixed_f32", "mixed_f16"):
+        legs[k]["rel_energy_err_vs_fp64"] = abs(legs[k]["energy"] - legs["fp64"]["energy"]) / abs(
+            legs["fp64"]["energy"])
+    with plan_for(ibim.PREC_FP64) as pl:
+        stats = pl.stats()
+    surf.close()
+    return {"legs": legs, "tube_points": n, "atoms": m, "tree": stats, "roof": roofline_entries(acc, cfg["p"])}
 
-    def progress(msg):
-        print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
-    progress(f"{args.config}: {surf.n} tube points")
-    fp64, stats = leg(ibim.PREC_FP64)
-    progress(f"fp64: {fp64['solves_per_s']:.3f} solves/s")
-    forms = {}
-    for name, prec in (("mixed_f32", ibim.PREC_MIXED), ("mixed_f16", ibim.PREC_MIXED_F16)):
-        stage_acc.clear()
-        stage_full.clear()
-        res, _ = leg(prec, profile=True)
-        res["rel_energy_err_vs_fp64"] = abs(res["energy"] - fp64["energy"]) / abs(fp64["energy"])
-        res["energy_within_1e-6"] = bool(res["rel_energy_err_vs_fp64"] <= 1e-6 and res["solve"]["converged"] == 1
-                                         and fp64["solve"]["converged"] == 1)
-        res["stages"] = ({k: list(v) for k, v in stage_acc.items()}, {k: list(v) for k, v in stage_full.items()})
-        forms[name] = res
-        progress(f"{name}: {res['solves_per_s']:.3f} solves/s, energy error {res['rel_energy_err_vs_fp64']:.1e}")
-    ok = [k for k in forms if forms[k]["energy_within_1e-6"]]
-    best = max(ok or forms, key=lambda k: forms[k]["solves_per_s"])
-    mixed = forms[best]
-    rel_err = mixed["rel_energy_err_vs_fp64"]
-    energy_ok = mixed["energy_within_1e-6"]
-    timed_st, full_st = mixed.pop("stages")
-    stage_acc = {k: tuple(v) for k, v in timed_st.items()}
-    stage_full = {k: tuple(v) for k, v in full_st.items()}
-    for f in forms.values():
-        f.pop("stages", None)
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="single-molecule configs: timed solves per form")
+    ap.add_argument("--warmup", type=int, default=2, help="warm-up requests (stream) or solves (single molecule)")
+    ap.add_argument("--config", default="stream256", choices=sorted(CONFIGS))
+    ap.add_argument("--slots", type=int, default=2, help="server slots (concurrent requests)")
+    ap.add_argument("--load", type=float, default=0.7, help="Poisson arrival rate as a fraction of the saturated rate")
+    ap.add_argument("--fp64-every", type=int, default=16, help="FP64 legs run every k-th request of the stream")
+    ap.add_argument("--requests", type=int, default=None, help="stream length (default: the workload's 256)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--matvecs-only", type=int, default=0, metavar="K",
+                    help="single-molecule counter mode: a 256 MiB device copy (calibration), then K matvecs in "
+                         "each precision on the molecule's plan; prints a short JSON line")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the stream's solves are "
+              f"independent: N GPUs would serve N request streams with no exchange)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    ibim = load_ibim()
+    cfg = dict(CONFIGS[args.config])
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
-    # ---- served burst: seeded globules of the same size through the server
-    serving = None
-    if args.serve > 0:
-        mols = [ibim.synthetic_globule(cfg["atoms"], cfg["seed"] + 1 + i, cfg["confine"]) for i in range(args.serve)]
-        fmm = dict(p=cfg["p"], max_points=cfg["leaf"], max_level=cfg["max_level"], kappa=cfg["kappa"],
-                   eps_in=cfg["eps_in"], eps_out=cfg["eps_out"], vacant_radius=cfg["tau"] * cfg["h"],
-                   precision=PRECS[best])
-        with ibim.Server(slots=args.slots) as srv:
-            t0 = time.perf_counter()
-            ids = [srv.submit(a, r, q, h=cfg["h"], restart=cfg["restart"], tol=cfg["tol"], **fmm) for a, r, q in mols]
-            res = [srv.wait(i) for i in ids]
-            wall = time.perf_counter() - t0
-        lat = [1e3 * (r["t_end"] - r["t_submit"]) for r in res]
-        serving = {"requests": args.serve, "slots": args.slots, "precision": best,
-                   "solves_per_s": args.serve / wall, "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
-                   "all_ok": all(r["status"] == 0 and r["info"]["converged"] == 1 for r in res),
-                   "note": "burst submitted at t=0; latency = completion - submission, geometry (level set, tube) "
-                           "and plan build included per request"}
+    if args.config == "stream256" and args.matvecs_only:
+        # counter mode on the stream's median-size molecule
+        sizes, _ = stream_molecules(ibim, dict(cfg, requests=cfg["requests"]))
+        i = int(np.argsort(sizes)[len(sizes) // 2])
+        m = int(sizes[i])
+        cfg = dict(cfg, atoms=m, seed=cfg["mol_seed"] + i, confine=cfg["confine"] * m ** (1 / 3))
+        run_single(ibim, args, cfg)
+        return
+    if args.config != "stream256":
+        single = run_single(ibim, args, cfg)
+        if single is None:
+            return
+        legs = single["legs"]
+        best = legs["mixed_f16"]
+        out = {"metric": METRIC, "value": best["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+               "steps": args.steps, "warmup": args.warmup, "ms_per_step": best["ms_per_solve"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f16/f32/f64",
+               "data": "synthetic",
+               "config": {"workload": f"{args.config}: single solves of one synthetic {cfg['atoms']}-atom globule, "
+                                      f"h={cfg['h']} A, p={cfg['p']}, leaf {cfg['leaf']} (not the metric's "
+                                      f"serving workload)", "parallelism": "single GPU"},
+               "latency_ms": best["latency_ms"], "legs": legs,
+               "mixed_speedup_vs_fp64": best["solves_per_s"] / legs["fp64"]["solves_per_s"],
+               "tube_points": single["tube_points"], "tree": single["tree"],
+               "roofline": single["roof"].get("near_field"), "roofline_m2l": single["roof"].get("m2l"),
+               "roofline_matvec": single["roof"].get("matvec"), "cpu_baseline": None}
+        print(json.dumps(out))
+        return
 
-    stream = serve_stream(ibim, args.stream, args.slots, PRECS[best], cfg) if args.stream > 0 else None
-
-    # ---- roofline of the dominant kernel in the timed mixed solves
-    r = cfg["p"] ** 3
-    totals = {k: v[0] for k, v in stage_full.items() if v[1] > 0 and k not in ("MATVEC", "MATVEC_MIXED")}
-    # M2L_MIXED / P2P_MIXED: the inner (mixed) matvecs; M2L / P2P: the FP64 outer residuals
-    dom = max(totals, key=totals.get)  # the stage with the most time per solve
-    ms_gemm = stage_acc["M2L_MIXED"][0] / stage_acc["M2L_MIXED"][1]
-    p = cfg["p"]
-    # the FP16 stage also evaluates the X list (as GEMMs on the same kernel)
-    x_in_stage = best == "mixed_f16" and p <= 6 and stats["x_pairs"] > 0
-    pairs = stats["v_pairs"] + (stats["x_pairs"] if x_in_stage else 0)
-    flops = 32.0 * r * r * pairs  # 2 x (4r) x (4r) per V or X interaction
-    achieved = flops / (ms_gemm * 1e-3) / 1e12
-    gemm16 = (f"k_m2l_gemm_h32<{p},8> (V) + k_m2l_gemm_h32<{p},4> (X)" if p <= 5 else
-              f"k_m2l_gemm_hu<{p},16>" if p == 6 else f"k_m2l_gemm<{p},float>")
-    kern, peak = {"mixed_f32": (f"k_m2l_gemm<{p},float> (FP32 MFMA)", PEAK_TFLOPS["f32"]),
-                  "mixed_f16": (f"k_m_to_half + {gemm16} (FP16 MFMA)" if p <= 6 else
-                                f"{gemm16} (FP32 MFMA; the FP16 tiles stop at p = 6)",
-                                PEAK_TFLOPS["f16"] if p <= 6 else PEAK_TFLOPS["f32"])}[best]
-    traffic = None
-    if args.config == "globule500" and cfg["leaf"] == 192 and best == "mixed_f16" and os.path.exists(TRAFFIC_FILE):
-        try:
-            traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": traffic, "kernel": kern,
-                "dominant_stage": dom, "ms_per_launch": ms_gemm, "launches": stage_acc["M2L_MIXED"][1],
-                "algorithmic_flops_per_launch": flops,
-                "interactions_per_launch": pairs,
-                "note": "M2L stage of the inner (mixed) matvecs: 32 r^2 flops per interaction (r = p^3), V pairs "
-                        + ("and X pairs " if x_in_stage else "") + "per launch; time from HIP events on the plan "
-                        "stream around the stage over the timed mixed solves (the only stage recorded there): the "
-                        "multipole conversion and the GEMMs (the per-target reduction of their results runs beside "
-                        "the near field and is not in it); dominant_stage from one extra untimed solve with every "
-                        "stage recorded"}
-    stage_ms = {k: v[0] for k, v in stage_full.items()}  # one solve
-
+    if args.requests:
+        cfg["requests"] = args.requests
+    st = run_stream(ibim, args, cfg)
     cpu = None
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
-        pts, nrm, w = surf.arrays()
         try:
-            cpu = run_cpu_baseline(cfg, pts, nrm, w, fp64["solve"]["matvecs"], threads)
+            cpu = cpu_baseline(ibim, cfg, st, threads)
+            progress(f"cpu baseline: {cpu['value']:.3g} solves/s")
         except Exception as e:  # report, do not hide
             cpu = {"error": repr(e)}
-    surf.close()
-
-    value = mixed["solves_per_s"]
+    mixed, fp64 = st["mixed"], st["fp64"]
+    roof = st["roof"].get("near_field")
+    if roof is not None:
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            try:
+                traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_launch")
+            except (OSError, ValueError):
+                traffic = None
+        roof = dict(roof, traffic=traffic,
+                    note="the dominant kernel of the mixed solves: the FP32 near field (60 flops per pair), two "
+                         "launches per mixed matvec on the second stream; time from HIP events on that stream "
+                         "around each launch, summed over every mixed matvec of the timed stream; traffic from "
+                         "rocprofv3 FETCH_SIZE + WRITE_SIZE of the same kernel")
     out = {
         "metric": METRIC,
-        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": mixed["ms_per_solve"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32/f64", "data": "synthetic",
-        "config": {"workload": f"{args.config}: synthetic {cfg['atoms']}-atom globule, h={cfg['h']} A, p={cfg['p']}, "
-                               f"leaf {cfg['leaf']}, "
-                               f"kappa=0.1257, eps 2/80, GMRES(40) tol 1e-8; value = the faster mixed-precision "
-                               f"form meeting 1e-6 energy error ({best}: FP32 near field and X list, "
-                               f"{'FP16' if best == 'mixed_f16' else 'FP32'} V-list M2L, inside FP64 iterative "
-                               f"refinement), FP64 solve timed beside it",
-                   "mixed_form": best,
-                   "tube_points": n, "atoms": m, "parallelism": "single GPU"},
+        "value": mixed["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+        "steps": mixed["requests"], "warmup": args.warmup, "ms_per_step": 1e3 * mixed["wall_s"] / mixed["requests"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16/f32/f64 (FP16 M2L on MFMA with FP32 accumulation, FP32 near field, FP64 translations, "
+                 "Krylov and refinement)",
+        "data": "synthetic",
+        "config": {"workload": f"stream256: serving stream of {cfg['requests']} synthetic globules, "
+                               f"{cfg['atoms_lo']}..{cfg['atoms_hi']} atoms log-uniform (seed {cfg['size_seed']}), "
+                               f"h={cfg['h']} A, p={cfg['p']}, leaf {cfg['leaf']}, kappa={cfg['kappa']}, eps "
+                               f"{cfg['eps_in']:g}/{cfg['eps_out']:g}, GMRES({cfg['restart']}) tol {cfg['tol']}; "
+                               f"Poisson arrivals at {args.load:g} of the saturated rate; geometry, plan, solve and "
+                               f"energy per request; {args.slots} server slots",
+                   "steps": "one step = one served request", "requests": cfg["requests"],
+                   "slots": args.slots, "tube_points": mixed["tube_points"], "parallelism": "single GPU"},
         "latency_ms": {"p50": mixed["latency_ms"]["p50"], "p99": mixed["latency_ms"]["p99"]},
-        "mixed": mixed, "fp64": fp64, "mixed_forms": {k: {"solves_per_s": v["solves_per_s"],
-                                                          "rel_energy_err_vs_fp64": v["rel_energy_err_vs_fp64"],
-                                                          "energy_within_1e-6": v["energy_within_1e-6"],
-                                                          "solve": v["solve"]} for k, v in forms.items()},
-        "mixed_speedup_vs_fp64": mixed["solves_per_s"] / fp64["solves_per_s"],
-        "rel_energy_err_vs_fp64": rel_err, "energy_within_1e-6": energy_ok,
-        "fp64_matvecs_per_s": fp64["solve"]["matvecs"] * fp64["solves_per_s"],
-        "fp64_tube_points_per_s": n * fp64["solve"]["matvecs"] * fp64["solves_per_s"],
-        "serving": serving, "stream": stream, "stage_ms_per_mixed_solve": stage_ms, "tree": stats, "mesh_s": t_mesh,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "mixed": mixed, "fp64": dict(fp64, subsample=st["saturated"]["subsample"]),
+        "saturated": st["saturated"],
+        "mixed_speedup_vs_fp64": st["saturated"]["mixed_speedup_vs_fp64"],
+        "target_speedup": 3.0,
+        "energy": st["energy"],
+        "roofline": roof, "roofline_m2l": st["roof"].get("m2l"), "roofline_matvec": st["roof"].get("matvec"),
+        "stage_totals_mixed_stream": st["stages"],
+        "cpu_baseline": cpu,
+        "mixed_vs_cpu": (mixed["solves_per_s"] / cpu["value"]) if cpu and "value" in cpu else None,
     }
     print(json.dumps(out))
 

@@ -64,6 +64,27 @@ class SolveInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class StageReport(C.Structure):
+    _fields_ = [("ms", C.c_double * 16), ("launches", C.c_int64 * 16), ("valu_flops", C.c_double * 16),
+                ("mfma_flops", C.c_double * 16), ("bytes", C.c_double * 16)]
+
+    def as_dict(self):
+        """{stage: {ms, launches, valu_flops, mfma_flops, bytes}} for the stages that recorded."""
+        out = {}
+        for i, name in enumerate(STAGES):
+            if self.launches[i]:
+                out[name] = {"ms": self.ms[i], "launches": int(self.launches[i]), "valu_flops": self.valu_flops[i],
+                             "mfma_flops": self.mfma_flops[i], "bytes": self.bytes[i]}
+        return out
+
+
+def stage_mask(stages):
+    mask = 0
+    for name in stages:
+        mask |= 1 << STAGES.index(name)
+    return mask
+
+
 class GeomParams(C.Structure):
     _fields_ = [("h", C.c_double), ("lo", C.c_double * 3), ("dims", C.c_int * 3), ("pad", C.c_double),
                 ("probe", C.c_double), ("band", C.c_int), ("thickness", C.c_int), ("thres", C.c_double),
@@ -145,6 +166,9 @@ def lib():
     L.ibim_plan_profile.argtypes = [vp, C.c_int]
     L.ibim_plan_profile_stages.argtypes = [vp, C.c_uint]
     L.ibim_plan_stage_times.argtypes = [vp, d, C.POINTER(C.c_int64), C.c_int]
+    L.ibim_plan_stage_report.argtypes = [vp, C.POINTER(StageReport), C.c_int]
+    L.ibim_server_profile_stages.argtypes = [vp, C.c_uint]
+    L.ibim_server_stage_report.argtypes = [vp, C.POINTER(StageReport), C.c_int]
     L.ibim_surface_from_atoms.argtypes = [C.POINTER(GeomParams), i64, d, d, pp]
     L.ibim_surface_from_grid.argtypes = [C.POINTER(GeomParams), d, pp]
     L.ibim_surface_get_info.argtypes = [vp, C.POINTER(SurfaceInfo)]
@@ -301,9 +325,7 @@ class Plan:
         if stages is None:
             _check(lib().ibim_plan_profile(self._h, int(enable)), "ibim_plan_profile")
             return
-        mask = 0
-        for name in stages:
-            mask |= 1 << STAGES.index(name)
+        mask = stage_mask(stages)
         _check(lib().ibim_plan_profile_stages(self._h, mask if enable else 0), "ibim_plan_profile_stages")
 
     def stage_times(self, reset=True):
@@ -313,6 +335,12 @@ class Plan:
         _check(lib().ibim_plan_stage_times(self._h, _d(ms), cnt.ctypes.data_as(C.POINTER(C.c_int64)), int(reset)),
                "ibim_plan_stage_times")
         return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(STAGES)}
+
+    def stage_report(self, reset=True):
+        """Stage intervals with the algorithmic work done inside them."""
+        r = StageReport()
+        _check(lib().ibim_plan_stage_report(self._h, C.byref(r), int(reset)), "ibim_plan_stage_report")
+        return r.as_dict()
 
     def set_precision(self, precision):
         _check(lib().ibim_plan_set_precision(self._h, precision), "ibim_plan_set_precision")
@@ -458,6 +486,16 @@ class Server:
 
     def now(self):
         return float(lib().ibim_server_now(self._h))
+
+    def profile(self, stages):
+        """Record the named stages in the plan of every request started from now on."""
+        _check(lib().ibim_server_profile_stages(self._h, stage_mask(stages)), "ibim_server_profile_stages")
+
+    def stage_report(self, reset=True):
+        """Stage intervals and their algorithmic work, summed over finished requests."""
+        r = StageReport()
+        _check(lib().ibim_server_stage_report(self._h, C.byref(r), int(reset)), "ibim_server_stage_report")
+        return r.as_dict()
 
     def close(self):
         if self._h:

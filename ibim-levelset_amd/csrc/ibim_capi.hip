@@ -19,6 +19,8 @@ int target_fmm_device(ibim_plan *pl, int kid, int m, const double *d_axyz, const
 // one dmapping (electric.cpp:294-388): charges, fused FMM, combine
 int matvec_device(ibim_plan *pl, const double *d_x, double *d_y, bool mixed) {
     StageScope sc(pl, mixed ? kStMatvecMixed : kStMatvec);
+    const StageWork w = matvec_work(pl, mixed);
+    sc.add(w.valu, w.mfma, w.bytes);
     int rc = form_charges(pl, d_x);
     if (rc) return rc;
     TargetSet ts = source_targets(pl);
@@ -359,7 +361,7 @@ int ibim_plan_profile_stages(ibim_plan *pl, unsigned stage_mask) {
     return IBIM_OK;
 }
 
-int ibim_plan_stage_times(ibim_plan *pl, double *ms, int64_t *launches, int reset) {
+int ibim_plan_stage_report(ibim_plan *pl, ibim_stage_report *out, int reset) {
     if (!pl) {
         set_error("plan is NULL");
         return IBIM_ERR_ARG;
@@ -371,19 +373,40 @@ int ibim_plan_stage_times(ibim_plan *pl, double *ms, int64_t *launches, int rese
         IBIM_HIP_CHECK(hipEventElapsedTime(&el, r.a, r.b));
         t.ms[r.stage] += el;
         t.count[r.stage] += 1;
+        t.work[r.stage].valu += r.w.valu;
+        t.work[r.stage].mfma += r.w.mfma;
+        t.work[r.stage].bytes += r.w.bytes;
         t.pool.push_back(r.a);
         t.pool.push_back(r.b);
     }
     t.pending.clear();
-    for (int i = 0; i < IBIM_STAGE_COUNT; ++i) {
-        if (ms) ms[i] = t.ms[i];
-        if (launches) launches[i] = t.count[i];
+    if (out) {
+        *out = ibim_stage_report{};
+        for (int i = 0; i < IBIM_STAGE_COUNT; ++i) {
+            out->ms[i] = t.ms[i];
+            out->launches[i] = t.count[i];
+            out->valu_flops[i] = t.work[i].valu;
+            out->mfma_flops[i] = t.work[i].mfma;
+            out->bytes[i] = t.work[i].bytes;
+        }
     }
     if (reset)
         for (int i = 0; i < 16; ++i) {
             t.ms[i] = 0;
             t.count[i] = 0;
+            t.work[i] = StageWork{};
         }
+    return IBIM_OK;
+}
+
+int ibim_plan_stage_times(ibim_plan *pl, double *ms, int64_t *launches, int reset) {
+    ibim_stage_report r;
+    const int rc = ibim_plan_stage_report(pl, &r, reset);
+    if (rc) return rc;
+    for (int i = 0; i < IBIM_STAGE_COUNT; ++i) {
+        if (ms) ms[i] = r.ms[i];
+        if (launches) launches[i] = r.launches[i];
+    }
     return IBIM_OK;
 }
 

@@ -783,6 +783,13 @@ __global__ void k_p2p_fill(int nl, const int *__restrict__ tleaves, const int *_
     }
 }
 
+// target x source pairs of each work unit (for the roofline accounting)
+__global__ void k_unit_pairs(int n, const int4 *__restrict__ ua, const int4 *__restrict__ ub,
+                             long long *__restrict__ out) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < n) out[u] = (long long)ua[u].z * ub[u].z;
+}
+
 __global__ void k_p2p_order(int n, const int *__restrict__ idx, const int4 *__restrict__ ua,
                             const int4 *__restrict__ ub, int4 *__restrict__ oa, int4 *__restrict__ ob) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -944,11 +951,14 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
         if (mixed) launch(poly ? k_p2p_unit<true, true> : k_p2p_unit<true, false>);
         else launch(poly ? k_p2p_unit<false, true> : k_p2p_unit<false, false>);
     };
+    const bool own = ts.p2p == &pl->p2p;  // the plan's own points (pair counts known)
     if (fork && ucut > 0) {
         IBIM_HIP_CHECK(hipEventRecord(pl->ev_fork, st));
         IBIM_HIP_CHECK(hipStreamWaitEvent(pl->stream2, pl->ev_fork, 0));
+        StageScope sc(pl, kStP2PMixed, pl->stream2);
         near_field(pl->stream2, 0, ucut);
         IBIM_LAUNCH_CHECK();
+        sc.add(near_field_flops(pl, 0, ucut), 0, 0);
     }
     // upward pass
     {
@@ -983,6 +993,8 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
     if (ts.xgemm) {
         int rc = m2l_gemm_x(pl, mixed);
         if (rc) return rc;
+        if (!(mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->f16_ready && pl->x16))
+            sc.add(0, 32.0 * R * R * (double)pl->xp.npairs, 0);
     } else if (ts.n_m2l_work > 0) {
         auto launch = [&](auto kern) {
             kern<<<ts.n_m2l_work, BS, 0, st>>>(lists, ts.m2l_work, pl->loff[1], pl->litem[1], pl->loff[3],
@@ -1001,6 +1013,9 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
         int rc = m2l_gemm(pl, *ts.vpairs, mixed);
         pl->defer_reduce = false;
         if (rc) return rc;
+        // (the FP16 stage also evaluates the X list)
+        const bool x_here = mixed && P <= 6 && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->f16_ready && pl->x16;
+        sc.add(0, 32.0 * R * R * ((double)ts.vpairs->npairs + (x_here ? pl->xp.npairs : 0)), 0);
     }
     if (fork) {
         IBIM_HIP_CHECK(hipEventRecord(pl->ev_fork, st));
@@ -1009,6 +1024,7 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
         StageScope sc(pl, kStP2PMixed, pl->stream2);
         near_field(pl->stream2, ucut, ts.p2p->nunits);
         IBIM_LAUNCH_CHECK();
+        sc.add(near_field_flops(pl, ucut, ts.p2p->nunits), 0, 0);
         }
         IBIM_HIP_CHECK(hipEventRecord(pl->ev_join, pl->stream2));
         int rc = m2l_reduce(pl);
@@ -1030,6 +1046,7 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
         if (!fork) {
             near_field(st, 0, w.nunits);
             IBIM_LAUNCH_CHECK();
+            if (own) sc.add(near_field_flops(pl, 0, w.nunits), 0, 0);
         }
         k_l2p<P><<<ts.nleaves, 64, 0, st>>>(ts.leaves, ts.tbeg, ts.tend, ts.x, ts.y, ts.z, pl->ctr, pl->level,
                                             pl->d_lvl_rad, pl->tab, pl->L, w.npart ? w.lsplit : nullptr, w.part,
@@ -1135,6 +1152,20 @@ int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, P2PWork &work, std::vector<vo
     IBIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(tmp2, b2, cost, cost2, idx, idx2, nu, 0, 31, st));
     k_p2p_order<<<nblk(nu, 256), 256, 0, st>>>(nu, idx2, ua, ub, sa, sb);
     IBIM_LAUNCH_CHECK();
+    if (!tmp_owner && nu > 0) {  // the plan's own points: pair counts per unit prefix, on the host
+        long long *pu = nullptr, *ps = nullptr;
+        if ((rc = salloc(&pu, nu)) || (rc = salloc(&ps, nu))) return rc;
+        k_unit_pairs<<<nblk(nu, 256), 256, 0, st>>>(nu, sa, sb, pu);
+        IBIM_LAUNCH_CHECK();
+        size_t b3 = 0;
+        IBIM_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, b3, pu, ps, nu, st));
+        void *tmp3 = nullptr;
+        if ((rc = salloc((char **)&tmp3, b3))) return rc;
+        IBIM_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(tmp3, b3, pu, ps, nu, st));
+        work.pairs_prefix.assign((size_t)nu + 1, 0);
+        IBIM_HIP_CHECK(hipMemcpyAsync(work.pairs_prefix.data() + 1, ps, (size_t)nu * sizeof(long long),
+                                      hipMemcpyDeviceToHost, st));
+    }
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
     work.nunits = nu;
     work.ua = sa;
@@ -1144,6 +1175,23 @@ int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, P2PWork &work, std::vector<vo
     ts.ranges = ranges;
     ts.p2p = &work;
     return IBIM_OK;
+}
+
+double near_field_flops(const ibim_plan *pl, int u0, int u1) {
+    const std::vector<int64_t> &pp = pl->p2p.pairs_prefix;
+    if (pp.empty() || u1 <= u0) return 0.0;
+    return 60.0 * (double)(pp[u1] - pp[u0]);
+}
+
+StageWork matvec_work(const ibim_plan *pl, bool mixed) {
+    (void)mixed;
+    StageWork w;
+    w.valu = near_field_flops(pl, 0, pl->p2p.nunits);
+    w.mfma = 32.0 * (double)pl->r * pl->r * ((double)pl->vp.npairs + pl->xp.npairs);
+    // points in and out (xyz, n, w, phi, psi, two outputs, four charges) and
+    // four multipole and four local vectors per non-empty box
+    w.bytes = 8.0 * (14.0 * pl->n + 8.0 * pl->r * (pl->n_m2l_work + 1.0));
+    return w;
 }
 
 int build_chebyshev(ibim_plan *pl) {

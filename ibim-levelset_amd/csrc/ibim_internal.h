@@ -289,6 +289,14 @@ enum Stage {
     kStCount
 };
 
+// Algorithmic work of one recorded stage interval, counted as in the
+// roofline model of the matvec: 60 flops per near-field pair (vector ALU),
+// 32 r^2 flops per V or X interaction (matrix cores), and the point and
+// expansion bytes the translations move.
+struct StageWork {
+    double valu = 0, mfma = 0, bytes = 0;
+};
+
 struct StageTimer {
     bool on = false;
     unsigned mask = 0;  // bit s: stage s is recorded
@@ -296,10 +304,12 @@ struct StageTimer {
     struct Rec {
         int stage;
         hipEvent_t a, b;
+        StageWork w;
     };
     std::vector<Rec> pending;
     double ms[16] = {0};
     int64_t count[16] = {0};
+    StageWork work[16];
     hipEvent_t take() {
         if (pool.empty()) {
             hipEvent_t e;
@@ -417,6 +427,8 @@ struct P2PWork {
     int2 *lsplit = nullptr; // per target leaf: {first partial slot, source chunks}
     double *part = nullptr; // 256 doubles (4 fields x 64 targets) per partial slot
     size_t npart = 0;
+    std::vector<int64_t> pairs_prefix;  // host: target x source pairs of units [0, u), u = 0..nunits
+                                        // (the plan's own points only; empty for target sets)
 };
 
 }  // namespace ibim
@@ -593,22 +605,34 @@ struct StageScope {
     int stage;
     hipEvent_t a = nullptr;
     hipStream_t st;
+    StageWork w;
     StageScope(ibim_plan *p, int s, hipStream_t on = nullptr) : pl(p), stage(s), st(on ? on : p->stream) {
         if (pl->timer.on && ((pl->timer.mask >> stage) & 1u)) {
             a = pl->timer.take();
             if (a) (void)hipEventRecord(a, st);
         }
     }
+    // the algorithmic work done inside this interval
+    void add(double valu, double mfma, double bytes) {
+        w.valu += valu;
+        w.mfma += mfma;
+        w.bytes += bytes;
+    }
     ~StageScope() {
         if (a) {
             hipEvent_t b = pl->timer.take();
             if (b) {
                 (void)hipEventRecord(b, st);
-                pl->timer.pending.push_back({stage, a, b});
+                pl->timer.pending.push_back({stage, a, b, w});
             } else {
                 pl->timer.pool.push_back(a);
             }
         }
     }
 };
+
+// algorithmic work of near-field work units [u0, u1) of the plan's own
+// points (60 flops per pair), and of one whole matvec of the plan
+double near_field_flops(const ibim_plan *pl, int u0, int u1);
+StageWork matvec_work(const ibim_plan *pl, bool mixed);
 }  // namespace ibim

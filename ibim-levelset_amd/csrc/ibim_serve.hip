@@ -1,12 +1,14 @@
 // ibim_serve.hip -- continuous batching of PB solves on one GPU.
 //
-// Workers (one host thread per slot) pull requests from a FIFO queue.  A
-// request is admitted when a slot is free and its estimated device footprint
-// fits beside the running requests within the memory budget; otherwise it
-// waits (and runs alone if it could never fit).  Each request builds its own
-// level set, tube nodes and plan on the worker's streams, so requests of
-// different sizes overlap on the device; everything a solve needs stays in
-// HBM until it finishes.
+// Workers (one host thread per slot) take requests from a queue kept in
+// submission order.  A free worker starts the first queued request whose
+// estimated device footprint fits beside the running requests within the
+// memory budget, so a small request may pass a large one that does not fit
+// yet; a request passed over kMaxPassed times holds back everything behind
+// it until it runs (no starvation), and a request that could never fit runs
+// alone.  Each request builds its own level set, tube nodes and plan on the
+// worker's stream, so requests of different sizes overlap on the device;
+// everything a solve needs stays in HBM until it finishes.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -26,6 +28,7 @@ struct ibim_server {
         ibim_request req;
         std::vector<double> xyz, rad, q;
         int64_t est = 0;
+        int passed = 0;  // times a later request was started before this one
         ibim_result res{};
         bool done = false;
     };
@@ -39,6 +42,8 @@ struct ibim_server {
     int running = 0;
     bool stop = false;
     int device = 0;
+    unsigned stage_mask = 0;    // stages recorded in each request's plan
+    ibim_stage_report acc{};    // summed over finished requests
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -86,7 +91,10 @@ int64_t estimate_footprint(const ibim_request &r, const std::vector<double> &xyz
     return (int64_t)(std::max(grid * 8 * 4.2, npts * per_point + ops + yv) + npts * 56);
 }
 
-void run_job(ibim_server::Job &j) {
+// a request passed over this many times is started before any later one
+constexpr int kMaxPassed = 8;
+
+void run_job(ibim_server *s, ibim_server::Job &j, hipStream_t st) {
     ibim_result &res = j.res;
     ibim_surface *surf = nullptr;
     ibim_plan *plan = nullptr;
@@ -99,6 +107,12 @@ void run_job(ibim_server::Job &j) {
     if (rc == IBIM_OK) rc = ibim_surface_device(surf, &sx, &sn, &sw);
     if (rc == IBIM_OK) rc = ibim_plan_create_device(&j.req.fmm, inf.n_nodes, sx, sn, sw, &plan);
     const size_t n2 = 2 * (size_t)inf.n_nodes;
+    unsigned mask = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        mask = s->stage_mask;
+    }
+    if (rc == IBIM_OK && mask) rc = ibim_plan_profile_stages(plan, mask);
     if (rc == IBIM_OK) {
         if (dev_malloc(&d_b, n2 * sizeof(double)) != hipSuccess || dev_malloc(&d_x, n2 * sizeof(double)) != hipSuccess ||
             dev_malloc(&d_at, 4 * (size_t)m * sizeof(double)) != hipSuccess ||
@@ -107,8 +121,6 @@ void run_job(ibim_server::Job &j) {
             rc = IBIM_ERR_NOMEM;
         }
     }
-    hipStream_t st = nullptr;
-    if (rc == IBIM_OK && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) rc = IBIM_ERR_HIP;
     if (rc == IBIM_OK) {
         std::vector<double> packed(4 * (size_t)m);
         std::copy(j.xyz.begin(), j.xyz.end(), packed.begin());
@@ -137,38 +149,71 @@ void run_job(ibim_server::Job &j) {
     res.status = rc;
     res.n_points = inf.n_nodes;
     res.footprint = (plan ? ibim_plan_footprint(plan) : 0) + (int64_t)(7 * (size_t)inf.n_nodes * sizeof(double));
+    ibim_stage_report rep;
+    if (plan && mask && ibim_plan_stage_report(plan, &rep, 1) == IBIM_OK) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (int i = 0; i < IBIM_STAGE_SLOTS; ++i) {
+            s->acc.ms[i] += rep.ms[i];
+            s->acc.launches[i] += rep.launches[i];
+            s->acc.valu_flops[i] += rep.valu_flops[i];
+            s->acc.mfma_flops[i] += rep.mfma_flops[i];
+            s->acc.bytes[i] += rep.bytes[i];
+        }
+    }
+    (void)hipStreamSynchronize(st);
     for (double *p : {d_b, d_x, d_at, d_phi})
         if (p) (void)hipFree(p);
-    if (st) (void)hipStreamDestroy(st);
     ibim_plan_destroy(plan);
     ibim_surface_destroy(surf);
 }
 
+// the queued request to start next: the first one that fits, unless a
+// request before it has been passed over kMaxPassed times (then none until
+// that one fits); with nothing running, the first request always fits
+std::deque<int64_t>::iterator next_request(ibim_server *s) {
+    for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+        const ibim_server::Job &j = s->jobs[*it];
+        if (s->running == 0 || s->in_use + j.est <= s->budget) return it;
+        if (j.passed >= kMaxPassed) break;
+    }
+    return s->queue.end();
+}
+
 void worker(ibim_server *s) {
     (void)hipSetDevice(s->device);
+    hipStream_t st = nullptr;  // the worker's stream for its requests' copies
+    const bool have_stream = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
     for (;;) {
         int64_t id = -1;
         ibim_server::Job *job = nullptr;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
-                if (!s->queue.empty()) {
-                    ibim_server::Job &head = s->jobs[s->queue.front()];
-                    if (s->running == 0 || s->in_use + head.est <= s->budget) {
-                        id = s->queue.front();
-                        s->queue.pop_front();
-                        job = &head;
-                        s->in_use += head.est;
-                        s->running++;
-                        break;
-                    }
+                auto it = s->queue.end();
+                if (!s->queue.empty()) it = next_request(s);
+                if (it != s->queue.end()) {
+                    for (auto q = s->queue.begin(); q != it; ++q) s->jobs[*q].passed++;
+                    id = *it;
+                    s->queue.erase(it);
+                    job = &s->jobs[id];
+                    s->in_use += job->est;
+                    s->running++;
+                    break;
                 }
-                if (s->stop && s->queue.empty()) return;
+                if (s->stop && s->queue.empty()) {
+                    if (have_stream) (void)hipStreamDestroy(st);
+                    return;
+                }
                 s->cv_work.wait(lk);
             }
             job->res.t_start = s->now();
         }
-        run_job(*job);
+        if (have_stream) {
+            run_job(s, *job, st);
+        } else {
+            job->res.status = IBIM_ERR_HIP;
+            set_error("worker stream could not be created");
+        }
         {
             std::lock_guard<std::mutex> lk(s->mu);
             job->res.t_end = s->now();
@@ -249,6 +294,27 @@ int ibim_server_wait(ibim_server *s, int64_t id, ibim_result *out) {
 }
 
 double ibim_server_now(ibim_server *s) { return s ? s->now() : 0.0; }
+
+int ibim_server_profile_stages(ibim_server *s, unsigned stage_mask) {
+    if (!s) {
+        set_error("server is NULL");
+        return IBIM_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stage_mask = stage_mask & ((1u << IBIM_STAGE_COUNT) - 1u);
+    return IBIM_OK;
+}
+
+int ibim_server_stage_report(ibim_server *s, ibim_stage_report *out, int reset) {
+    if (!s || !out) {
+        set_error("bad argument to ibim_server_stage_report");
+        return IBIM_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->acc;
+    if (reset) s->acc = ibim_stage_report{};
+    return IBIM_OK;
+}
 
 void ibim_server_destroy(ibim_server *s) {
     if (!s) return;

@@ -129,7 +129,9 @@ int64_t ibim_plan_footprint(const ibim_plan *plan);
 #define IBIM_STAGE_M2L_X 9    /* X list, direct, when the V list runs as GEMM */
 #define IBIM_STAGE_MATVEC_MIXED 10 /* one mixed-precision matvec */
 #define IBIM_STAGE_M2L_MIXED 11    /* V list of a mixed-precision matvec (counted here, not in M2L) */
-#define IBIM_STAGE_P2P_MIXED 12    /* near field + L2P of a mixed-precision matvec (not in P2P) */
+#define IBIM_STAGE_P2P_MIXED 12    /* near field of a mixed-precision matvec (not in P2P): with FP16 M2L its
+                                      two launches on the second stream (costliest units beside the upward
+                                      pass, the rest beside the M2L reduction), else near field + L2P */
 #define IBIM_STAGE_COUNT 13
 int ibim_plan_profile(ibim_plan *plan, int enable);
 /* record only the stages whose bit (1u << IBIM_STAGE_x) is set; 0 = off.
@@ -137,6 +139,21 @@ int ibim_plan_profile(ibim_plan *plan, int enable);
    No counterpart in the reference (which has no stage timers). */
 int ibim_plan_profile_stages(ibim_plan *plan, unsigned stage_mask);
 int ibim_plan_stage_times(ibim_plan *plan, double *ms, int64_t *launches, int reset);
+/* the same, with the algorithmic work done inside the recorded intervals of
+ * each stage, counted as the roofline model of the matvec counts it: 60
+ * flops per near-field pair (vector ALU), 32 p^6 flops per V or X
+ * interaction (matrix cores), and 8 (14 n + 8 p^3 B) bytes of points and
+ * expansions per matvec (B non-empty boxes).  Only the matvec stages (whole
+ * matvec, M2L, near field) carry work.  No counterpart in the reference. */
+#define IBIM_STAGE_SLOTS 16
+typedef struct ibim_stage_report {
+    double ms[IBIM_STAGE_SLOTS];          /* summed interval lengths */
+    int64_t launches[IBIM_STAGE_SLOTS];   /* recorded intervals */
+    double valu_flops[IBIM_STAGE_SLOTS];
+    double mfma_flops[IBIM_STAGE_SLOTS];
+    double bytes[IBIM_STAGE_SLOTS];
+} ibim_stage_report;
+int ibim_plan_stage_report(ibim_plan *plan, ibim_stage_report *out, int reset);
 
 /* ---- the hot path -------------------------------------------------------- */
 /* y = A x, x = [phi (n); psi (n)], exactly the reference dmapping */
@@ -239,8 +256,10 @@ int ibim_synthetic_globule(int64_t m, uint64_t seed, double confine_radius, doub
  * HIP streams with its state resident in HBM; a request is admitted when a
  * slot is free and its estimated device footprint fits next to the running
  * ones within `mem_budget` bytes (0: 90% of the free device memory at
- * creation).  Requests start in submission order; a request that does not fit
- * waits, and runs alone if it would not fit even then.  This replaces the
+ * creation).  A free slot starts the first queued request (in submission
+ * order) that fits, so a small request may pass a large one that has to wait
+ * for memory; a request passed over 8 times holds back all later ones until
+ * it starts, and one that could never fit runs alone.  This replaces the
  * reference's one-molecule-per-process executable (main.cpp:14-79) for
  * streams of molecules. */
 typedef struct ibim_server ibim_server;
@@ -274,6 +293,11 @@ int ibim_server_wait(ibim_server *s, int64_t id, ibim_result *out);
 double ibim_server_now(ibim_server *s);
 /* finishes every queued request, then stops the workers */
 void ibim_server_destroy(ibim_server *s);
+/* stage timers in the plan of every request started after the call (see
+ * ibim_plan_profile_stages), and their reports summed over the requests that
+ * have finished.  No counterpart in the reference. */
+int ibim_server_profile_stages(ibim_server *s, unsigned stage_mask);
+int ibim_server_stage_report(ibim_server *s, ibim_stage_report *out, int reset);
 
 const char *ibim_last_error(void);
 const char *ibim_version(void);

@@ -128,3 +128,31 @@ def test_stage_timers(ibim, sphere_case):
     assert all(cnt == 0 for ms, cnt in st_off.values())
     for y in (a, b, c, d):
         assert np.array_equal(y, ref)
+
+
+def test_stage_report_counts_algorithmic_work(ibim, sphere_case):
+    """The stage report's work counts follow the matvec's roofline model: a
+    whole FP16 mixed matvec carries 60 flops per near-field pair and 32 p^6
+    flops per V or X interaction (the tree statistics count both
+    independently, on the host); the near field's two launches on the second
+    stream carry all of its pairs between them, and the FP16 M2L stage all the
+    interactions."""
+    pts, nrm, w = sphere_case
+    x = np.random.default_rng(5).uniform(-1, 1, 2 * len(pts))
+    p = 5
+    with ibim.Plan(pts, nrm, w, p=p, max_points=32, vacant_radius=0.125, precision=ibim.PREC_MIXED_F16,
+                   **PB) as plan:
+        st = plan.stats()
+        plan.profile(True, stages=["MATVEC_MIXED", "P2P_MIXED", "M2L_MIXED"])
+        plan.matvec(x)
+        plan.matvec(x)
+        rep = plan.stage_report()
+    r = p ** 3
+    nf, m2l = 60.0 * st["p2p_pairs"], 32.0 * r * r * (st["v_pairs"] + st["x_pairs"])
+    mv = rep["MATVEC_MIXED"]
+    assert mv["launches"] == 2
+    assert mv["valu_flops"] == 2 * nf and mv["mfma_flops"] == 2 * m2l
+    assert rep["P2P_MIXED"]["launches"] == 4 and rep["P2P_MIXED"]["valu_flops"] == 2 * nf
+    assert rep["M2L_MIXED"]["launches"] == 2 and rep["M2L_MIXED"]["mfma_flops"] == 2 * m2l
+    nb = st["n_nodes"] - st["n_empty_leaves"]  # boxes with sources (empty boxes are leaves)
+    assert mv["bytes"] == 2 * 8.0 * (14 * len(pts) + 8 * r * nb)

@@ -54,3 +54,27 @@ def test_admission_by_memory_budget(ibim, gpu):
     for k in range(len(spans) - 1):
         assert spans[k + 1][0] >= spans[k][1] - 1e-6
     assert all(r["status"] == 0 for r in res)
+
+
+def test_small_request_passes_a_blocked_large_one(ibim, gpu):
+    """A request that does not fit beside the running one waits, and a later
+    small request that fits starts before it (admission in submission order
+    among the requests that fit); every result is still correct."""
+    mols = molecules(ibim)
+    big, mid, small = mols[1], mols[3], mols[2]  # 150, 100 and 30 atoms
+    fmm = dict(FMM, precision=ibim.PREC_FP64)
+    with ibim.Server(slots=2) as srv:  # footprint estimates (budget = 90% of free memory)
+        est = [srv.wait(srv.submit(a, r, q, h=0.5, tol=1e-6, **fmm)) for a, r, q in (mid, big, small)]
+    e_mid, e_big, e_small = (r["footprint_est"] for r in est)
+    assert e_small < e_big
+    budget = e_mid + (e_small + e_big) // 2  # mid + small fit together, mid + big do not
+    assert e_mid + e_small <= budget < e_mid + e_big and e_big <= budget
+    with ibim.Server(slots=2, mem_budget=budget) as srv:
+        ids = [srv.submit(a, r, q, h=0.5, tol=1e-6, **fmm) for a, r, q in (mid, big, small)]
+        res = [srv.wait(i) for i in ids]
+    r_mid, r_big, r_small = res
+    assert all(r["status"] == 0 for r in res)
+    assert r_small["t_start"] < r_big["t_start"], "the small request should pass the blocked large one"
+    assert r_big["t_start"] >= min(r_mid["t_end"], r_small["t_end"]) - 1e-6
+    for a, b in zip(est, res):
+        assert abs(a["energy"] - b["energy"]) <= 1e-10 * abs(a["energy"])
