@@ -13,23 +13,29 @@ to 1e-8, then its solvation energy is formed from the reaction potential at
 the atoms.  A request is the whole chain: level set, tube nodes, plan (tree,
 lists, operators), right-hand side, GMRES, reaction potential, energy.
 
-One step = one served request.  The requests go through the continuous
-batching server (`slots` requests at once, admission by device footprint):
+One step = one served request.  The whole stream holds about 9.2e8 tube
+points (the walk-in-a-sphere globules are porous: ~800 tube points per atom
+at h = 0.35 A), about 9 minutes of mixed-precision GPU time and half an hour
+in FP64, so the timed runs serve stated subsamples of it, every k-th request
+in stream order, which keeps the log-uniform size mix.  The requests go
+through the continuous-batching server (`slots` requests at once, admission
+by device footprint):
 
-1. Saturated rates on a stated subsample (every `--fp64-every`-th request
-   of the stream): a burst of the subsample in the mixed form and in FP64.
-   Their ratio is the like-for-like served speed-up of the mixed form; the
-   energies of the two runs give the mixed form's energy error.
-2. The mixed stream (the headline): all 256 requests arrive as a Poisson
-   process at `--load` times the saturated mixed rate (scaled from the
-   subsample by mean atom count).  value = completed solves / wall time,
-   with p50/p99 latency (completion - arrival).  Stage timers in every
-   request's plan give the roofline of the dominant kernel (the near field)
-   and of the M2L stage over this timed region.
-3. The FP64 stream: the subsample as a Poisson process at `--load` times its
-   saturated FP64 rate: served FP64 solves/s and p50/p99.
+1. Like-for-like capacity on the FP64 subsample (every `--fp64-every`-th
+   request, 8 by default): a burst in the mixed form and one in FP64.  Their
+   ratio is the served speed-up of the mixed form; the energies of the two
+   runs give the mixed form's energy error.
+2. The mixed stream (the headline) on every `--every`-th request (32 by
+   default), arriving as a Poisson process at `--load` times the mixed
+   capacity (scaled from step 1 by mean atom count).  value = completed
+   solves / wall time from the first arrival to the last completion, with
+   p50/p99 latency (completion - arrival).  Stage timers in every request's
+   plan give the roofline of the dominant kernel (the near field), of the M2L
+   stage and of the whole matvec over this timed region.
+3. The FP64 stream: the FP64 subsample as a Poisson process at `--load` times
+   its FP64 capacity: served FP64 solves/s and p50/p99.
 
---steps does not change the stream: the workload is its 256 requests.
+--steps does not change the stream: its subsample fixes the timed requests.
 --warmup W serves W small molecules in each precision before timing.
 
 The cpu_baseline entry times the CPU oracle (a restatement of the reference
@@ -113,7 +119,7 @@ def stream_molecules(ibim, cfg):
     return sizes, mols
 
 
-def serve(ibim, mols, cfg, precision, slots, rate=None, profile=None, arrival_seed=5):
+def serve(ibim, mols, cfg, precision, slots, rate=None, profile=None, arrival_seed=5, label=None):
     """Push molecules through a server: all at t = 0 (rate None) or as a
     Poisson process of the given rate.  Returns the results (in molecule
     order), the wall time from the first arrival to the last completion and
@@ -139,8 +145,15 @@ def serve(ibim, mols, cfg, precision, slots, rate=None, profile=None, arrival_se
         t0 = time.perf_counter()
         th = threading.Thread(target=producer)
         th.start()
+        res = []
+        for k in range(len(mols)):
+            while ids[k] is None:  # not submitted yet
+                time.sleep(0.01)
+            res.append(srv.wait(ids[k]))
+            if label:
+                progress(f"{label}: request {k + 1}/{len(mols)} done ({res[-1]['n_points']} points, "
+                         f"{res[-1]['t_end'] - res[-1]['t_submit']:.2f} s)")
         th.join()
-        res = [srv.wait(i) for i in ids]
         wall = time.perf_counter() - t0
         rep = srv.stage_report() if profile else None
     return res, wall, rep
@@ -200,6 +213,8 @@ def run_stream(ibim, args, cfg):
              f"generated in {time.perf_counter() - t0:.1f} s")
     sub = list(range(0, len(mols), args.fp64_every))
     sub_mols = [mols[i] for i in sub]
+    head = list(range(0, len(mols), args.every))
+    head_mols = [mols[i] for i in head]
     # warm-up: W small molecules per precision (code objects, block cache)
     warm = [ibim.synthetic_globule(120 + 20 * i, 30000 + i, cfg["confine"] * (120 + 20 * i) ** (1 / 3))
             for i in range(args.warmup)]
@@ -208,32 +223,38 @@ def run_stream(ibim, args, cfg):
             serve(ibim, warm, cfg, prec, slots)
     progress(f"warm-up: {args.warmup} requests per precision")
 
-    # 1. saturated rates on the subsample, both precisions
-    res_m_sub, wall_m_sub, _ = serve(ibim, sub_mols, cfg, ibim.PREC_MIXED_F16, slots)
+    # 1. capacity on the FP64 subsample, both precisions (bursts)
+    res_m_sub, wall_m_sub, _ = serve(ibim, sub_mols, cfg, ibim.PREC_MIXED_F16, slots, label="burst mixed")
     mu_m_sub = len(sub) / wall_m_sub
-    progress(f"subsample burst mixed: {len(sub)} requests, {mu_m_sub:.3f} solves/s")
-    res_d_sub, wall_d_sub, _ = serve(ibim, sub_mols, cfg, ibim.PREC_FP64, slots)
+    progress(f"burst mixed: {len(sub)} requests, {mu_m_sub:.3f} solves/s")
+    res_d_sub, wall_d_sub, _ = serve(ibim, sub_mols, cfg, ibim.PREC_FP64, slots, label="burst fp64")
     mu_d_sub = len(sub) / wall_d_sub
-    progress(f"subsample burst fp64: {mu_d_sub:.3f} solves/s")
+    progress(f"burst fp64: {mu_d_sub:.3f} solves/s")
     errs = [abs(a["energy"] - b["energy"]) / abs(b["energy"]) for a, b in zip(res_m_sub, res_d_sub)]
 
-    # 2. the mixed stream: all requests, Poisson arrivals
-    scale = float(np.mean([len(m[0]) for m in sub_mols]) / np.mean(sizes))
+    # 2. the mixed stream (headline subsample), Poisson arrivals
+    scale = float(np.mean([len(m[0]) for m in sub_mols]) / np.mean([len(m[0]) for m in head_mols]))
     lam_m = args.load * mu_m_sub * scale
     stages = ["P2P_MIXED", "M2L_MIXED", "MATVEC_MIXED", "P2P", "M2L", "MATVEC"]
-    res_m, wall_m, rep = serve(ibim, mols, cfg, ibim.PREC_MIXED_F16, slots, rate=lam_m, profile=stages)
+    res_m, wall_m, rep = serve(ibim, head_mols, cfg, ibim.PREC_MIXED_F16, slots, rate=lam_m, profile=stages,
+                               label="mixed stream")
     mixed = summarize(res_m, wall_m)
     mixed["arrival_rate_per_s"] = lam_m
+    mixed["subsample"] = f"every {args.every}th request of the {len(mols)} ({len(head)} requests)"
     progress(f"mixed stream: {mixed['solves_per_s']:.3f} solves/s, p50 {mixed['latency_ms']['p50']:.0f} ms, "
              f"p99 {mixed['latency_ms']['p99']:.0f} ms")
-    # the subsample's requests inside the mixed stream against their FP64 energies
-    errs_in_stream = [abs(res_m[i]["energy"] - b["energy"]) / abs(b["energy"]) for i, b in zip(sub, res_d_sub)]
+    # requests of both subsamples: the mixed stream's energies against FP64
+    pos = {i: k for k, i in enumerate(head)}
+    errs_in_stream = [abs(res_m[pos[i]]["energy"] - b["energy"]) / abs(b["energy"])
+                      for i, b in zip(sub, res_d_sub) if i in pos]
 
-    # 3. the FP64 stream on the subsample
+    # 3. the FP64 stream on the FP64 subsample
     lam_d = args.load * mu_d_sub
-    res_d, wall_d, _ = serve(ibim, sub_mols, cfg, ibim.PREC_FP64, slots, rate=lam_d, arrival_seed=6)
+    res_d, wall_d, _ = serve(ibim, sub_mols, cfg, ibim.PREC_FP64, slots, rate=lam_d, arrival_seed=6,
+                             label="fp64 stream")
     fp64 = summarize(res_d, wall_d)
     fp64["arrival_rate_per_s"] = lam_d
+    fp64["subsample"] = f"every {args.fp64_every}th request of the {len(mols)} ({len(sub)} requests)"
     progress(f"fp64 stream: {fp64['solves_per_s']:.3f} solves/s, p99 {fp64['latency_ms']['p99']:.0f} ms")
 
     max_err = max(errs + errs_in_stream)
@@ -241,13 +262,15 @@ def run_stream(ibim, args, cfg):
     roof = roofline_entries(rep, cfg["p"])
     stage_tot = {k: {"ms": v["ms"], "launches": v["launches"]} for k, v in rep.items()}
     return {
-        "sizes": sizes, "mols": mols, "res_m": res_m, "res_d_sub": res_d_sub, "sub": sub,
+        "sizes": sizes, "mols": head_mols, "res_m": res_m, "res_d_sub": res_d_sub,
+        "sub_in_head": [pos[i] for i in sub if i in pos],
         "mixed": mixed, "fp64": fp64,
-        "saturated": {"subsample": f"every {args.fp64_every}th request ({len(sub)} requests)",
-                      "mixed_solves_per_s": mu_m_sub, "fp64_solves_per_s": mu_d_sub,
-                      "mixed_speedup_vs_fp64": mu_m_sub / mu_d_sub},
+        "capacity": {"subsample": f"every {args.fp64_every}th request ({len(sub)} requests), bursts",
+                     "mixed_solves_per_s": mu_m_sub, "fp64_solves_per_s": mu_d_sub,
+                     "mixed_speedup_vs_fp64": mu_m_sub / mu_d_sub},
         "energy": {"max_rel_err_mixed_vs_fp64": max_err, "within_1e-6": bool(max_err <= 1e-6),
-                   "compared": f"{len(sub)} subsample requests, mixed burst and mixed stream vs FP64 burst"},
+                   "compared": f"the {len(sub)} FP64-subsample requests: mixed burst and mixed stream against the "
+                               f"FP64 burst"},
         "ok": ok, "roof": roof, "stages": stage_tot,
     }
 
@@ -291,13 +314,14 @@ def cpu_baseline(ibim, cfg, stream, threads):
     t_mv = time.perf_counter() - t0
     # scaled to the stream: O(N) per matvec, FP64 matvecs per solve from the
     # GPU FP64 runs of the subsample (their mean where a request has none)
-    mv64 = {i: r["info"]["matvecs"] for i, r in zip(stream["sub"], stream["res_d_sub"])}
-    mean_mv = float(np.mean(list(mv64.values())))
-    t_stream = sum(mv64.get(i, mean_mv) * t_mv * r["n_points"] / n_sample for i, r in enumerate(stream["res_m"]))
+    mv64 = {k: r["info"]["matvecs"] for k, r in zip(stream["sub_in_head"], stream["res_d_sub"])}
+    mean_mv = float(np.mean([r["info"]["matvecs"] for r in stream["res_d_sub"]]))
+    t_stream = sum(mv64.get(k, mean_mv) * t_mv * r["n_points"] / n_sample for k, r in enumerate(stream["res_m"]))
     out["value"] = len(stream["res_m"]) / t_stream
     out["sample"] = (f"oracle dmapping (16 kernel FMMs, one tree each, OpenMP) on {n_sample} tube points of the "
                      f"stream's first molecule ({t_mv:.2f} s on {threads} threads), scaled by tube points (O(N)) "
-                     f"and by FP64 matvecs per solve to the {len(stream['res_m'])} requests")
+                     f"and by FP64 matvecs per solve to the {len(stream['res_m'])} requests of the timed mixed "
+                     f"stream")
     out["matvec_s"] = t_mv
     out["matvec_points"] = n_sample
     # (b) the Born ion solved end to end on the CPU: RHS, GMRES(20), reaction
@@ -413,7 +437,9 @@ def main():
     ap.add_argument("--config", default="stream256", choices=sorted(CONFIGS))
     ap.add_argument("--slots", type=int, default=2, help="server slots (concurrent requests)")
     ap.add_argument("--load", type=float, default=0.7, help="Poisson arrival rate as a fraction of the saturated rate")
-    ap.add_argument("--fp64-every", type=int, default=16, help="FP64 legs run every k-th request of the stream")
+    ap.add_argument("--every", type=int, default=8, help="the timed mixed stream serves every k-th request")
+    ap.add_argument("--fp64-every", type=int, default=32,
+                    help="the capacity bursts and the FP64 stream serve every k-th request")
     ap.add_argument("--requests", type=int, default=None, help="stream length (default: the workload's 256)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--matvecs-only", type=int, default=0, metavar="K",
@@ -499,13 +525,15 @@ def main():
                                f"h={cfg['h']} A, p={cfg['p']}, leaf {cfg['leaf']}, kappa={cfg['kappa']}, eps "
                                f"{cfg['eps_in']:g}/{cfg['eps_out']:g}, GMRES({cfg['restart']}) tol {cfg['tol']}; "
                                f"Poisson arrivals at {args.load:g} of the saturated rate; geometry, plan, solve and "
-                               f"energy per request; {args.slots} server slots",
-                   "steps": "one step = one served request", "requests": cfg["requests"],
+                               f"energy per request; {args.slots} server slots; timed on every {args.every}th "
+                               f"request (mixed) and every {args.fp64_every}th (FP64)",
+                   "steps": "one step = one served request of the timed mixed stream",
+                   "requests": cfg["requests"], "timed_requests": mixed["requests"],
                    "slots": args.slots, "tube_points": mixed["tube_points"], "parallelism": "single GPU"},
         "latency_ms": {"p50": mixed["latency_ms"]["p50"], "p99": mixed["latency_ms"]["p99"]},
-        "mixed": mixed, "fp64": dict(fp64, subsample=st["saturated"]["subsample"]),
-        "saturated": st["saturated"],
-        "mixed_speedup_vs_fp64": st["saturated"]["mixed_speedup_vs_fp64"],
+        "mixed": mixed, "fp64": fp64,
+        "capacity": st["capacity"],
+        "mixed_speedup_vs_fp64": st["capacity"]["mixed_speedup_vs_fp64"],
         "target_speedup": 3.0,
         "energy": st["energy"],
         "roofline": roof, "roofline_m2l": st["roof"].get("m2l"), "roofline_matvec": st["roof"].get("matvec"),
