@@ -24,8 +24,8 @@ def sphere_case(ibim, gpu):
 
 
 # matvec bars against FP64: FP32 everywhere 1e-5; FP16 V-list M2L (10-bit
-# mantissa after power-of-two scaling) 1e-3
-MATVEC_BAR = {"mixed": 1e-5, "mixed16": 1e-3}
+# mantissa after power-of-two scaling; observed 3-5e-6) 2e-5
+MATVEC_BAR = {"mixed": 1e-5, "mixed16": 2e-5}
 
 
 # p = 5 runs the 32x32 FP16 GEMM, p = 6 the 16x16 one, p = 8 the FP32 GEMM
