@@ -157,6 +157,7 @@ def lib():
     L.ibim_rhs_device.argtypes = [vp, i64, vp, vp, vp]
     L.ibim_polarize.argtypes = [vp, i64, d, d, d]
     L.ibim_polarize_device.argtypes = [vp, i64, vp, vp, vp]
+    L.ibim_polarize_device_noalloc.argtypes = [vp, i64, vp, vp, vp]
     L.ibim_gmres.argtypes = [vp, d, d, C.c_int, C.c_int, C.c_double, C.POINTER(SolveInfo)]
     L.ibim_gmres_device.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_double, C.POINTER(SolveInfo)]
     L.ibim_tree_stats_get.argtypes = [vp, C.POINTER(TreeStats)]
@@ -317,6 +318,9 @@ class Plan:
 
     def polarize_device(self, m, d_axyz, d_x, d_phi):
         _check(lib().ibim_polarize_device(self._h, m, d_axyz, d_x, d_phi), "ibim_polarize_device")
+
+    def polarize_device_noalloc(self, m, d_axyz, d_x, d_phi):
+        _check(lib().ibim_polarize_device_noalloc(self._h, m, d_axyz, d_x, d_phi), "ibim_polarize_device_noalloc")
 
     # ------------------------------------------------------------ timing --
     def profile(self, enable=True, stages=None):

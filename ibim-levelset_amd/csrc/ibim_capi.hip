@@ -14,6 +14,7 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 
 int kernel_fmm_device(ibim_plan *pl, int kid, const double *d_q, double *d_out);
 int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_x, double *d_out);
+int polarize_noalloc(ibim_plan *pl, int m, const double *d_axyz, const double *d_x, double *d_out);
 int target_fmm_device(ibim_plan *pl, int kid, int m, const double *d_axyz, const double *d_in, double *d_out);
 
 // one dmapping (electric.cpp:294-388): charges, fused FMM, combine
@@ -482,6 +483,14 @@ int ibim_polarize_device(ibim_plan *pl, int64_t m, const double *d_axyz, const d
         return IBIM_ERR_ARG;
     }
     return polarize_device(pl, (int)m, d_axyz, d_x, d_phi);
+}
+
+int ibim_polarize_device_noalloc(ibim_plan *pl, int64_t m, const double *d_axyz, const double *d_x, double *d_phi) {
+    if (!pl || m < 1 || m > (int64_t)0x7fffffff || !d_axyz || !d_x || !d_phi) {
+        set_error("bad argument to ibim_polarize_device_noalloc");
+        return IBIM_ERR_ARG;
+    }
+    return polarize_noalloc(pl, (int)m, d_axyz, d_x, d_phi);
 }
 
 int ibim_polarize(ibim_plan *pl, int64_t m, const double *axyz, const double *x, double *phi) {

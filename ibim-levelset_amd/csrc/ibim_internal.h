@@ -582,8 +582,22 @@ int build_chebyshev(ibim_plan *pl);
 int fmm_forward(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mixed = false);
 int form_charges(ibim_plan *pl, const double *d_x);
 int combine(ibim_plan *pl, const double *d_x, double *d_y);
-int make_target_set(ibim_plan *pl, int m, const double *d_txyz, TargetSet &ts, std::vector<void *> &tmp,
-                    int **perm_out);
+// Per-call device temporaries of a target-set evaluation: device memory from
+// dev_malloc (freed by release()), or consecutive 256-byte aligned pieces of
+// a caller-provided arena, in which case nothing is allocated and a request
+// past the arena's end fails
+struct TmpAlloc {
+    char *arena = nullptr;
+    size_t cap = 0, used = 0;
+    std::vector<void *> owned;
+    int alloc(void **p, size_t bytes);
+    template <typename T>
+    int get(T **p, size_t count) {
+        return alloc((void **)p, (count ? count : 1) * sizeof(T));
+    }
+    void release();
+};
+int make_target_set(ibim_plan *pl, int m, const double *d_txyz, TargetSet &ts, TmpAlloc &tmp, int **perm_out);
 TargetSet source_targets(ibim_plan *pl);
 int rhs_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_aq, double *d_b);
 int gmres_device(ibim_plan *pl, const double *d_b, double *d_x, int restart, int maxit, double tol,

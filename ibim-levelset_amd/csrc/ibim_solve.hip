@@ -11,6 +11,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
+#include <mutex>
 
 #include "ibim_internal.h"
 
@@ -357,19 +358,38 @@ int rhs_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_aq, d
     return IBIM_OK;
 }
 
-int make_target_set(ibim_plan *pl, int m, const double *d_txyz, TargetSet &ts, std::vector<void *> &tmp,
-                    int **perm_out) {
+int TmpAlloc::alloc(void **p, size_t bytes) {
+    *p = nullptr;
+    bytes = bytes ? (bytes + 255) & ~(size_t)255 : 256;
+    if (arena) {
+        if (bytes > cap - used) {
+            set_error("target set needs more than the " + std::to_string(cap >> 20) + " MiB scratch arena (" +
+                      std::to_string((used + bytes) >> 20) + " MiB asked so far)");
+            return IBIM_ERR_ARG;
+        }
+        *p = arena + used;
+        used += bytes;
+        return IBIM_OK;
+    }
+    hipError_t e = dev_malloc(p, bytes);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc (targets): ") + hipGetErrorString(e));
+        return IBIM_ERR_NOMEM;
+    }
+    owned.push_back(*p);
+    return IBIM_OK;
+}
+
+void TmpAlloc::release() {
+    for (void *q : owned) (void)hipFree(q);
+    owned.clear();
+    used = 0;
+}
+
+int make_target_set(ibim_plan *pl, int m, const double *d_txyz, TargetSet &ts, TmpAlloc &tmp, int **perm_out) {
     hipStream_t st = pl->stream;
     const int nn = pl->nnodes, maxlev = pl->prm.max_level;
-    auto talloc = [&](auto **p, size_t cnt) -> int {
-        hipError_t e = dev_malloc((void **)p, (cnt ? cnt : 1) * sizeof(**p));
-        if (e != hipSuccess) {
-            set_error(std::string("hipMalloc (targets): ") + hipGetErrorString(e));
-            return IBIM_ERR_NOMEM;
-        }
-        tmp.push_back((void *)*p);
-        return IBIM_OK;
-    };
+    auto talloc = [&](auto **p, size_t cnt) -> int { return tmp.get(p, cnt); };
     uint64_t *k_in, *k_out;
     int *i_in, *perm, *tbeg, *tend, *leaves, *work, *d_num;
     double *x, *y, *z;
@@ -424,30 +444,27 @@ int make_target_set(ibim_plan *pl, int m, const double *d_txyz, TargetSet &ts, s
 int single_charge_device(ibim_plan *pl, int kid, const double *d_q);
 int scatter_field_device(ibim_plan *pl, int kid, const int *perm, const double *F, int nt, double *d_out);
 
-// polarizeMap (kid < 0) or one kernel FMM (kid = 0..15) at m target points
-int target_fmm_device(ibim_plan *pl, int kid, int m, const double *d_axyz, const double *d_in, double *d_out) {
+// polarizeMap (kid < 0) or one kernel FMM (kid = 0..15) at m target points.
+// With an arena (`tmp.arena` set) every temporary comes from it and the far
+// field runs without the V-pair GEMM and its pair tables (V and X lists
+// evaluated directly) and the near field without work units (one block per
+// target leaf): nothing is allocated.
+int target_fmm_device(ibim_plan *pl, int kid, int m, const double *d_axyz, const double *d_in, double *d_out,
+                      TmpAlloc &tmp) {
     StageScope sc(pl, kStPolarize);
-    std::vector<void *> tmp;
+    const bool lean = tmp.arena != nullptr;
     TargetSet ts;
     int *tperm = nullptr;
     double *F = nullptr;
     int rc = make_target_set(pl, m, d_axyz, ts, tmp, &tperm);
-    if (rc == IBIM_OK) {
-        hipError_t e = dev_malloc((void **)&F, 4 * (size_t)m * sizeof(double));
-        if (e != hipSuccess) {
-            set_error("hipMalloc (polarize fields)");
-            rc = IBIM_ERR_NOMEM;
-        } else {
-            tmp.push_back(F);
-        }
-    }
+    if (rc == IBIM_OK) rc = tmp.get(&F, 4 * (size_t)m);
     VPairs vp;
-    if (rc == IBIM_OK && pl->ops) {
-        rc = build_vpairs(pl, ts.need_local, vp, &tmp);
+    if (rc == IBIM_OK && pl->ops && !lean) {
+        rc = build_vpairs(pl, ts.need_local, vp, &tmp.owned);
         ts.vpairs = &vp;
     }
     P2PWork work;
-    if (rc == IBIM_OK) rc = build_p2p_ranges(pl, ts, work, &tmp);
+    if (rc == IBIM_OK && !lean) rc = build_p2p_ranges(pl, ts, work, &tmp.owned);
     if (rc == IBIM_OK) rc = kid < 0 ? form_charges(pl, d_in) : single_charge_device(pl, kid, d_in);
     if (rc == IBIM_OK) rc = fmm_forward(pl, ts, F, m);
     if (rc == IBIM_OK) {
@@ -462,12 +479,71 @@ int target_fmm_device(ibim_plan *pl, int kid, int m, const double *d_axyz, const
     } else {
         (void)hipStreamSynchronize(pl->stream);
     }
-    for (void *p : tmp) (void)hipFree(p);
+    tmp.release();
     return rc;
+}
+
+int target_fmm_device(ibim_plan *pl, int kid, int m, const double *d_axyz, const double *d_in, double *d_out) {
+    TmpAlloc tmp;
+    return target_fmm_device(pl, kid, m, d_axyz, d_in, d_out, tmp);
 }
 
 int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_x, double *d_out) {
     return target_fmm_device(pl, -1, m, d_axyz, d_x, d_out);
+}
+
+// ---------------------------------------------- polarize without allocating --
+// The scratch of ibim_polarize_device_noalloc: one array per GPU (each device
+// holds its own copy of this file-scope array from the library's load to the
+// end of the process).  Host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device, kept per device in
+// g_scratch; the array's name on the host is not a device address.
+constexpr size_t kNoallocScratch = (size_t)64 << 20;  // bytes
+static __device__ __align__(256) unsigned char scratch[kNoallocScratch];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu[kMaxDevices];   // one call at a time per device (the call is synchronous)
+void *g_scratch[kMaxDevices] = {};      // device address of `scratch`, per device
+}  // namespace
+
+// the target set's per-call scratch in bytes, an upper bound from m targets
+// and the plan's nodes: keys, indices, coordinates and fields per target,
+// ranges, work lists and flags per node, and the radix sort's and
+// selection's temporaries
+static size_t noalloc_bytes(int m, int nn) {
+    return 256 * 24 + (size_t)m * (8 + 8 + 4 + 4 + 3 * 8 + 4 * 8) + (size_t)nn * (4 * 4 + 3) +
+           (size_t)m * 24 + (size_t)nn * 8 + ((size_t)1 << 20);
+}
+
+int polarize_noalloc(ibim_plan *pl, int m, const double *d_axyz, const double *d_x, double *d_out) {
+    int dev = 0;
+    IBIM_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices || dev != pl->device) {
+        set_error("polarize (no-alloc): the plan's device must be current (and below 64)");
+        return IBIM_ERR_ARG;
+    }
+    const size_t need = noalloc_bytes(m, pl->nnodes);
+    if (need > kNoallocScratch) {
+        set_error("polarize (no-alloc): " + std::to_string(m) + " targets on a plan of " +
+                  std::to_string(pl->nnodes) + " boxes need about " + std::to_string(need >> 20) +
+                  " MiB of scratch; the maximum is " + std::to_string(kNoallocScratch >> 20) + " MiB");
+        return IBIM_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu[dev]);
+    if (!g_scratch[dev]) {  // once per device, with that device current
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("hipGetSymbolAddress (scratch): ") + hipGetErrorString(e));
+            return IBIM_ERR_HIP;
+        }
+        g_scratch[dev] = p;
+    }
+    TmpAlloc tmp;
+    tmp.arena = (char *)g_scratch[dev];
+    tmp.cap = kNoallocScratch;
+    return target_fmm_device(pl, -1, m, d_axyz, d_x, d_out, tmp);
 }
 
 constexpr int kHostScalars = 8;

@@ -178,6 +178,20 @@ int ibim_rhs_device(ibim_plan *plan, int64_t m, const double *d_atom_xyz, const 
 int ibim_polarize(ibim_plan *plan, int64_t m, const double *atom_xyz, const double *x, double *phi_reac);
 int ibim_polarize_device(ibim_plan *plan, int64_t m, const double *d_atom_xyz, const double *d_x,
                          double *d_phi_reac);
+/* ibim_polarize_device without any device allocation, for callers that
+ * cannot allocate (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Its temporaries
+ * (the sorted targets, their ranges in the tree, the reaction fields) live in
+ * a 64 MiB array that each GPU holds from the library's load to the end of
+ * the process; a call whose targets and tree would need more (about
+ * 104 m + 27 boxes + 1 MiB bytes) is rejected with IBIM_ERR_ARG.  The far
+ * field evaluates the V and X lists directly and the near field per target
+ * leaf, so the result agrees with ibim_polarize_device to rounding (1e-10
+ * relative), not bit for bit.  Calls on one GPU share the array and are
+ * serialised by a lock inside the library (each returns when its result is
+ * in d_phi_reac); the array keeps the last call's contents. */
+int ibim_polarize_device_noalloc(ibim_plan *plan, int64_t m, const double *d_atom_xyz, const double *d_x,
+                                 double *d_phi_reac);
 
 /* restarted GMRES(restart) on A x = b with this plan's matvec; x holds the
  * initial guess on entry.  Returns IBIM_OK, IBIM_NOT_CONVERGED or an error. */
