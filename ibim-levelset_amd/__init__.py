@@ -147,6 +147,7 @@ def lib():
     L.ibim_plan_destroy.argtypes = [vp]
     L.ibim_plan_destroy.restype = None
     L.ibim_plan_sync.argtypes = [vp]
+    L.ibim_plan_set_stream.argtypes = [vp, vp]
     L.ibim_plan_set_precision.argtypes = [vp, C.c_int]
     L.ibim_plan_footprint.argtypes = [vp]
     L.ibim_plan_footprint.restype = i64
@@ -305,6 +306,11 @@ class Plan:
 
     def sync(self):
         _check(lib().ibim_plan_sync(self._h), "ibim_plan_sync")
+
+    def set_stream(self, stream):
+        """Order the plan's work on a caller's stream (a hipStream_t as int, e.g.
+        torch.cuda.current_stream().cuda_stream); None: the plan's own stream."""
+        _check(lib().ibim_plan_set_stream(self._h, C.c_void_p(stream) if stream else None), "ibim_plan_set_stream")
 
     # ---------------------------------------------- device-pointer variants --
     def rhs_device(self, m, d_axyz, d_aq, d_b):

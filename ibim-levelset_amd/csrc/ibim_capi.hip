@@ -222,7 +222,9 @@ void ibim_plan_destroy(ibim_plan *pl) {
     }
     if (pl->ev_fork) (void)hipEventDestroy(pl->ev_fork);
     if (pl->ev_join) (void)hipEventDestroy(pl->ev_join);
-    if (pl->owns_stream && pl->stream) (void)hipStreamDestroy(pl->stream);
+    if (pl->stream && pl->stream != pl->own_stream) (void)hipStreamSynchronize(pl->stream);
+    if (pl->ev_switch) (void)hipEventDestroy(pl->ev_switch);
+    if (pl->owns_stream && pl->own_stream) (void)hipStreamDestroy(pl->own_stream);
     delete pl;
 }
 
@@ -254,6 +256,7 @@ int ibim_plan_create_device(const ibim_params *prm, int64_t n, const double *d_x
         return IBIM_ERR_HIP;
     }
     pl->owns_stream = true;
+    pl->own_stream = pl->stream;
     if ((rc = build_chebyshev(pl)) || (rc = build_tree(pl, d_xyz, d_nrm, d_w)) || (rc = build_m2l_operators(pl)) ||
         (rc = build_vpairs(pl, pl->need_local, pl->vp, nullptr)) || (rc = build_xpairs(pl))) {
         ibim_plan_destroy(pl);
@@ -323,6 +326,21 @@ int ibim_plan_sync(ibim_plan *pl) {
         return IBIM_ERR_ARG;
     }
     IBIM_HIP_CHECK(hipStreamSynchronize(pl->stream));
+    return IBIM_OK;
+}
+
+int ibim_plan_set_stream(ibim_plan *pl, void *stream) {
+    if (!pl) {
+        set_error("plan is NULL");
+        return IBIM_ERR_ARG;
+    }
+    hipStream_t next = stream ? (hipStream_t)stream : pl->own_stream;
+    if (next == pl->stream) return IBIM_OK;
+    // what the plan has queued so far comes before anything on the new stream
+    if (!pl->ev_switch) IBIM_HIP_CHECK(hipEventCreateWithFlags(&pl->ev_switch, hipEventDisableTiming));
+    IBIM_HIP_CHECK(hipEventRecord(pl->ev_switch, pl->stream));
+    IBIM_HIP_CHECK(hipStreamWaitEvent(next, pl->ev_switch, 0));
+    pl->stream = next;
     return IBIM_OK;
 }
 

@@ -440,6 +440,8 @@ struct ibim_plan {
     int device = 0;
     hipStream_t stream = nullptr;
     bool owns_stream = false;
+    hipStream_t own_stream = nullptr;    // the plan's own stream (stream may be a caller's instead)
+    hipEvent_t ev_switch = nullptr;      // orders work across a change of stream
     // FP16 mixed matvec: the near field runs on stream2 beside the M2L
     // reduction and L2L (forked after the M2L GEMMs, joined before L2P)
     hipStream_t stream2 = nullptr;

@@ -8,7 +8,7 @@
 #include "bbfmm.h"
 #include "gmres.h"
 
-double electric(Grid &, levelset &ls, Surface &surf, Molecule &mol, double rescale, Config &cfg) {
+void electric(Grid &, levelset &ls, Surface &surf, Molecule &mol, double rescale, Config &cfg) {
     const int64_t N = surf.info.n_nodes;
     std::cout << std::setw(15) << "POINTS NUM" << " " << std::setw(8) << N << std::endl;
     std::cout << std::setw(15) << "AREA APPROX" << " " << std::setw(8) << surf.info.area << " A^2" << std::fixed
@@ -54,5 +54,4 @@ double electric(Grid &, levelset &ls, Surface &surf, Molecule &mol, double resca
     energy *= 0.5;
     std::cout << "polarized energy: " << std::setw(20) << std::scientific << energy << std::fixed << std::endl;
     ibim_plan_destroy(plan);
-    return energy;
 }

@@ -9,5 +9,4 @@
 // atoms (centres / rescale), solves with GMRES on the device, writes
 // potent_file (2N values) and energy_file (reaction potential per atom), and
 // prints the polarized energy (1/2) sum q phi_reac (electric.cpp:9-507).
-// Returns the energy.
-double electric(Grid &g, levelset &ls, Surface &surf, Molecule &mol, double rescale, Config &cfg);
+void electric(Grid &g, levelset &ls, Surface &surf, Molecule &mol, double rescale, Config &cfg);

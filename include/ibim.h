@@ -109,6 +109,14 @@ int  ibim_plan_create_device(const ibim_params *prm, int64_t n, const double *d_
                              const double *d_normals, const double *d_weights, ibim_plan **out);
 void ibim_plan_destroy(ibim_plan *plan);
 int  ibim_plan_sync(ibim_plan *plan);
+/* order the plan's work on a caller's HIP stream (hipStream_t passed as
+ * void*; NULL: back to the plan's own stream).  Every later call is then
+ * enqueued on that stream after whatever the caller queued there before it,
+ * so device inputs need no synchronisation and later work on the stream
+ * sees the results; the plan's earlier work is ordered before the switch.
+ * The stream must outlive its use by the plan.  No counterpart in the
+ * reference, which runs on the host. */
+int  ibim_plan_set_stream(ibim_plan *plan, void *stream);
 int  ibim_plan_set_precision(ibim_plan *plan, int precision);
 /* device memory held by the plan, bytes */
 int64_t ibim_plan_footprint(const ibim_plan *plan);

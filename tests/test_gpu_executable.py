@@ -67,3 +67,27 @@ def test_missing_pqr_fails_loudly(tmp_path, gpu):
     cfg.write_text("pqr_file = /nonexistent.pqr\ngrid_h = 0.25\n")
     out = subprocess.run([EXE, str(cfg)], capture_output=True, text=True, timeout=60)
     assert out.returncode != 0 and "pqr_file" in out.stderr
+
+
+def test_reference_scaled_grid_mode(tmp_path, gpu):
+    """The reference's own grid mode (grid_size, no grid_h: main.cpp:33-45):
+    the molecule is scaled so its half extent is 240 units of a [-300,300]^3
+    grid.  An ion of radius 6 A keeps its probe-expanded sphere (240 + 1.4 *
+    40 = 296 units) inside the grid; at grid_size 96 (h = 0.156 A) its
+    energy is the closed form's within the discretisation error."""
+    pqr = tmp_path / "ion6.pqr"
+    pqr.write_text("ATOM      1  NA  ION     1       0.000   0.000   0.000  1.0000 6.0000\n")
+    cfg = [f"pqr_file = {pqr}", f"energy_file = {tmp_path / 'out.energy'}", f"potent_file = {tmp_path / 'out.potent'}",
+           "grid_size = 96", "lvlset_thickness = 2", "lvlset_thres = 1e-4", "reinit_step = 60",
+           "gmres_restart = 20", "gmres_maxiter = 200", "gmres_tol = 1e-8", "fmm_np = 4", "fmm_maxpoint = 64",
+           "fmm_maxlevel = 10", "tau = 1.0", "solvent_kappa = 0.0", "solvent_dE = 80.0", "solvent_dI = 1.0"]
+    stdout = run(tmp_path, cfg)
+    print(stdout[-600:])
+    assert "CONVERGED" in stdout
+    h = float(re.search(r"h\s+(\S+)\s+Angstroms", stdout).group(1))
+    assert h == pytest.approx(600.0 / 96 / 40.0, rel=1e-6)
+    energy = float(re.search(r"polarized energy:\s+(\S+)", stdout).group(1))
+    born6 = 1.0 / (8 * np.pi * 6.0) * (1 / 80.0 - 1.0)
+    assert energy == pytest.approx(born6, rel=0.03), (energy, born6)
+    area = float(re.search(r"AREA APPROX\s+(\S+)", stdout).group(1))
+    assert area == pytest.approx(4 * np.pi * 36.0, rel=0.03)
