@@ -4,16 +4,16 @@
 //
 // Reference (levelset.cpp, main.cpp:47-62):
 //   expand        max over atoms of (r + probe) - |x - c| in a window of
-//                 (r + probe)/h + band cells (levelset.cpp:166-184)
+//                 (r + probe)/h + band cells (levelset.cpp:91-109)
 //   evolve        TVD-RK3 of u_t = -v |grad u| with WENO5 one-sided
 //                 derivatives and a Godunov norm, moving the surface in by
-//                 the probe radius (levelset.cpp:186-262, 365-463)
+//                 the probe radius (levelset.cpp:111-187, 294-396)
 //   reinitialize  TVD-RK3 of u_t = -sign(phi0)(|grad u| - 1) until every tube
-//                 point has ||grad u| - 1| <= thres (levelset.cpp:266-362, 473-507)
+//                 point has ||grad u| - 1| <= thres (levelset.cpp:191-287, 398-435)
 //   Surface       exterior flood fill from grid point 0, cavities pushed out
 //                 of the tube, then for each tube point the projection
 //                 P - grad(phi) phi with the averaged one-sided gradient (not
-//                 renormalised) and the cosine weight (levelset.cpp:512-661)
+//                 renormalised) and the cosine weight (levelset.cpp:437-586)
 // Each grid-wide stage is one kernel; the grid is HBM resident throughout.
 #include <hipcub/hipcub.hpp>
 
@@ -40,13 +40,13 @@ struct GridDims {
 inline int nblk(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
 
 __device__ __forceinline__ double gget(const double *__restrict__ g, const GridDims &d, int i, int j, int k) {
-    i = min(max(i, 0), d.nx - 1);  // clamped read (levelset.cpp:103-108)
+    i = min(max(i, 0), d.nx - 1);  // clamped read (levelset.cpp:28-33)
     j = min(max(j, 0), d.ny - 1);
     k = min(max(k, 0), d.nz - 1);
     return g[((size_t)i * d.ny + j) * d.nz + k];
 }
 
-// WENO5 forward (up) and backward (un) derivatives along dir (levelset.cpp:405-463)
+// WENO5 forward (up) and backward (un) derivatives along dir (levelset.cpp:330-396)
 __device__ __forceinline__ void weno(const double *__restrict__ g, const GridDims &d, int i, int j, int k, int dir,
                                      double &up, double &un) {
     double w[7];
@@ -92,7 +92,7 @@ __device__ __forceinline__ void weno(const double *__restrict__ g, const GridDim
     un = (df - w0 * (a - 2 * b + c) * r3 - (w2 - 0.5) * (b - 2 * c + dd) * r6) * rh;
 }
 
-// Godunov norm getNorm(A, B) = sqrt(sum max(A+, B-)^2) (levelset.cpp:369-376)
+// Godunov norm getNorm(A, B) = sqrt(sum max(A+, B-)^2) (levelset.cpp:294-301)
 __device__ __forceinline__ double godunov(const double *A, const double *B) {
     double v = 0.0;
 #pragma unroll
@@ -137,7 +137,7 @@ __global__ void k_from_ord(double *__restrict__ g, int64_t n) {
     if (I < n) g[I] = unord((unsigned long long)__double_as_longlong(g[I]));
 }
 
-// expand: one block row per atom, window cells over blockIdx.x (levelset.cpp:166-184)
+// expand: one block row per atom, window cells over blockIdx.x (levelset.cpp:91-109)
 __global__ void k_expand(unsigned long long *__restrict__ g, GridDims d, const double *__restrict__ axyz,
                          const double *__restrict__ rad, int m, double probe, int band, int wmax) {
     const int a = blockIdx.y + gridDim.y * blockIdx.z;
@@ -198,7 +198,7 @@ __global__ void k_scale(double *__restrict__ g, int64_t n, double s) {
     if (I < n) g[I] *= s;
 }
 
-// countGradient (levelset.cpp:473-507): tube points, bad points, sum of
+// countGradient (levelset.cpp:398-435): tube points, bad points, sum of
 // errors.  Each block writes its sums (wave shuffles, then LDS) to its own
 // slot; k_count_sum adds the slots in block order (exact counts, a
 // deterministic error sum).
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(1024) void k_count_sum(int nb, const unsigned int *
     }
 }
 
-// exterior flood fill (levelset.cpp:519-599): a point is reached when a
+// exterior flood fill (levelset.cpp:444-537): a point is reached when a
 // reached neighbour has phi > -tube.  Computed on 8 x 8 x 8 tiles: a block
 // loads its tile and a one-cell halo (reached flags and phi > -tube) into LDS
 // and propagates inside the tile until nothing changes, so one launch
@@ -349,7 +349,7 @@ __global__ void k_tube_flag(const double *__restrict__ g, int64_t n, double tube
     if (I < n) flag[I] = fabs(g[I]) < tube ? 1 : 0;
 }
 
-// tube nodes, normals and weights in grid order (levelset.cpp:615-651),
+// tube nodes, normals and weights in grid order (levelset.cpp:540-583),
 // converted as electric.cpp:17-41 (nodes / s, weights * s * (h/s)^3)
 __global__ __launch_bounds__(256) void k_tube_emit(const double *__restrict__ g, GridDims d,
                                                    const int *__restrict__ flag, const int *__restrict__ pos,
@@ -612,7 +612,7 @@ int ibim_surface_from_atoms(const ibim_geom_params *gp, int64_t m, const double 
         ibim_surface_destroy(S);
         return code;
     };
-    // expand (levelset.cpp:166-184) onto g = -2 * max(dims) * h
+    // expand (levelset.cpp:91-109) onto g = -2 * max(dims) * h
     if (dev_malloc(&d_at, 4 * (size_t)m * sizeof(double)) != hipSuccess) {
         set_error("hipMalloc (atoms)");
         return fail(IBIM_ERR_NOMEM);

@@ -481,9 +481,7 @@ struct ibim_plan {
 
     // GMRES workspace (allocated on first use)
     int gm_restart = 0;
-    double *gm_V = nullptr;              // (restart+1) * 2n: the FP64 solve's Krylov basis
-    float *gm_Vf = nullptr;              // (restart+1) * 2n: the inner (mixed) solves' basis, FP32
-    double *gm_vd = nullptr;             // 2n: a basis vector of gm_Vf as the matvec's FP64 input
+    double *gm_V = nullptr;              // (restart+1) * 2n: the Krylov basis (FP64, inner solves too)
     double *gm_hpart = nullptr;          // (restart+1) * partials of the Gram-Schmidt projections
     double *gm_coef = nullptr;           // restart+1 projections of one Gram-Schmidt pass
     double *gm_w = nullptr, *gm_r = nullptr, *gm_x = nullptr, *gm_b = nullptr;

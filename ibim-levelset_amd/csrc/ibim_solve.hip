@@ -2,8 +2,8 @@
 // right-hand side (electric.cpp:395-413), target sets for the reaction
 // potential at atom centres (polarizeMap, electric.cpp:442-485) and restarted
 // GMRES with modified Gram-Schmidt and Givens rotations (gmres.cpp:10-164;
-// the inner solves of the mixed-precision refinement use an FP32 basis and
-// classical Gram-Schmidt).
+// the inner solves of the mixed-precision refinement keep an FP64 basis too,
+// orthogonalised by classical Gram-Schmidt).
 //
 // GMRES keeps the Krylov basis, the Hessenberg matrix, the rotations and the
 // residual estimate in HBM; the host only reads the residual estimate once
@@ -212,27 +212,21 @@ __global__ __launch_bounds__(kRB) void k_start(const double *__restrict__ r, VT 
     for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) v0[i] = (VT)(r[i] * inv);
 }
 
-// ---- inner (mixed-precision) Arnoldi steps: FP32 basis, classical
+// ---- inner (mixed-precision) Arnoldi steps: FP64 basis, classical
 // Gram-Schmidt.  The inner solves of the refinement reduce the residual by
 // about 1e-5 (what the FP32 / FP16 operator supports), and the FP64 outer
-// residual decides convergence, so their basis is kept in FP32 and
-// orthogonalised with one pass over it for all projections and one for the
-// update, instead of two passes per basis vector (modified Gram-Schmidt).
-
-// FP32 basis vector -> the matvec's FP64 input
-__global__ void k_f2d(const float *__restrict__ v, double *__restrict__ d, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) d[i] = (double)v[i];
-}
+// residual decides convergence, so their FP64 basis is orthogonalised with
+// one pass over it for all projections and one for the update, instead of
+// two passes per basis vector (modified Gram-Schmidt).
 
 // partials of <w, v_k>, k < kn: part[k kRG + block]
-__global__ __launch_bounds__(kRB) void k_cgs_dots(const double *__restrict__ w, const float *__restrict__ V,
+__global__ __launch_bounds__(kRB) void k_cgs_dots(const double *__restrict__ w, const double *__restrict__ V,
                                                   size_t ldv, int n, int kn, double *__restrict__ part) {
     __shared__ double sh[kRB];
     for (int k = 0; k < kn; ++k) {
-        const float *vk = V + (size_t)k * ldv;
+        const double *vk = V + (size_t)k * ldv;
         double v = 0.0;
-        for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) v += w[i] * (double)vk[i];
+        for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) v += w[i] * vk[i];
         v = block_sum(v, sh);
         if (threadIdx.x == 0) part[(size_t)k * kRG + blockIdx.x] = v;
     }
@@ -252,8 +246,8 @@ __global__ __launch_bounds__(kRB) void k_cgs_coef(const double *__restrict__ par
 }
 
 // w -= sum_k coef[k] v_k (k ascending); partial |w|^2
-__global__ __launch_bounds__(kRB) void k_cgs_update(double *__restrict__ w, const float *__restrict__ V, size_t ldv,
-                                                    int n, const double *__restrict__ coef, int kn,
+__global__ __launch_bounds__(kRB) void k_cgs_update(double *__restrict__ w, const double *__restrict__ V,
+                                                    size_t ldv, int n, const double *__restrict__ coef, int kn,
                                                     double *__restrict__ part) {
     __shared__ double c[128];
     __shared__ double sh[kRB];
@@ -262,7 +256,7 @@ __global__ __launch_bounds__(kRB) void k_cgs_update(double *__restrict__ w, cons
     double v = 0.0;
     for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += kRG * kRB) {
         double wi = w[i];
-        for (int k = 0; k < kn; ++k) wi -= c[k] * (double)V[(size_t)k * ldv + i];
+        for (int k = 0; k < kn; ++k) wi -= c[k] * V[(size_t)k * ldv + i];
         w[i] = wi;
         v += wi * wi;
     }
@@ -548,9 +542,10 @@ int polarize_noalloc(ibim_plan *pl, int m, const double *d_axyz, const double *d
 
 constexpr int kHostScalars = 8;
 
-// the workspace for GMRES(restart): the FP64 basis, or (f32) the FP32 basis
-// of the inner solves, each allocated when first needed
-static int gmres_alloc(ibim_plan *pl, int restart, bool f32) {
+// the workspace for GMRES(restart): the FP64 basis, and (cgs) the
+// projection partials of classical Gram-Schmidt for the inner solves,
+// allocated when first needed
+static int gmres_alloc(ibim_plan *pl, int restart, bool cgs) {
     const size_t n2 = 2 * (size_t)pl->n;
     int rc;
     // earlier buffers stay on the plan's allocation list and are freed with it
@@ -560,14 +555,13 @@ static int gmres_alloc(ibim_plan *pl, int restart, bool f32) {
             (rc = dalloc(pl, &pl->gm_small, (size_t)(restart + 1) * (restart + 1) + 4 * (size_t)(restart + 1) + 8)))
             return rc;
         pl->gm_V = nullptr;
-        pl->gm_Vf = nullptr;
+        pl->gm_hpart = nullptr;
         pl->gm_restart = restart;
     }
     const int rs = pl->gm_restart;
-    if (!f32 && !pl->gm_V && (rc = dalloc(pl, &pl->gm_V, (size_t)(rs + 1) * n2))) return rc;
-    if (f32 && !pl->gm_Vf &&
-        ((rc = dalloc(pl, &pl->gm_Vf, (size_t)(rs + 1) * n2)) || (rc = dalloc(pl, &pl->gm_vd, n2)) ||
-         (rc = dalloc(pl, &pl->gm_hpart, (size_t)(rs + 1) * kRG)) || (rc = dalloc(pl, &pl->gm_coef, (size_t)rs + 1))))
+    if (!pl->gm_V && (rc = dalloc(pl, &pl->gm_V, (size_t)(rs + 1) * n2))) return rc;
+    if (cgs && !pl->gm_hpart &&
+        ((rc = dalloc(pl, &pl->gm_hpart, (size_t)(rs + 1) * kRG)) || (rc = dalloc(pl, &pl->gm_coef, (size_t)rs + 1))))
         return rc;
     if (!pl->gm_host) IBIM_HIP_CHECK(hipHostMalloc((void **)&pl->gm_host, kHostScalars * sizeof(double)));
     return IBIM_OK;
@@ -588,10 +582,10 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
         set_error("gmres: restart must be in 1..120 and maxit >= 0");
         return IBIM_ERR_ARG;
     }
-    // the inner solves of the refinement keep an FP32 basis (classical
-    // Gram-Schmidt); the FP64 solve, the reference's modified Gram-Schmidt
-    const bool f32 = mixed;
-    int rc = gmres_alloc(pl, restart, f32);
+    // the inner solves of the refinement orthogonalise by classical
+    // Gram-Schmidt; the FP64 solve by the reference's modified Gram-Schmidt
+    const bool cgs = mixed;
+    int rc = gmres_alloc(pl, restart, cgs);
     if (rc) return rc;
     hipStream_t st = pl->stream;
     const int m = restart;
@@ -599,7 +593,6 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
     const size_t ldv = (size_t)n2;
     const int ld = m + 1;
     double *V = pl->gm_V, *w = pl->gm_w, *r = pl->gm_r, *p = pl->gm_x;
-    float *Vf = pl->gm_Vf;
     double *partA = pl->gm_part, *partB = pl->gm_part + kRG;
     double *H = pl->gm_small;
     double *s = H + (size_t)ld * ld, *cs = s + ld, *sn = cs + ld, *scal = sn + ld;  // scal: beta, resid, normb
@@ -642,23 +635,21 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
         bool done = false;
         while (j <= maxit && !done) {
             // v0 = r / beta (partials of |r|^2 are in partA)
-            if (f32) k_start<float><<<kRG, kRB, 0, st>>>(r, Vf, n2, partA, s, m, scal);
-            else k_start<double><<<kRG, kRB, 0, st>>>(r, V, n2, partA, s, m, scal);
+            k_start<double><<<kRG, kRB, 0, st>>>(r, V, n2, partA, s, m, scal);
             IBIM_LAUNCH_CHECK();
             for (i = 0; i < m && j <= maxit; i++, j++) {
-                if (f32) k_f2d<<<nblk(n2), 256, 0, st>>>(Vf + (size_t)i * ldv, pl->gm_vd, n2);
-                const double *vi = f32 ? pl->gm_vd : V + (size_t)i * ldv;
+                const double *vi = V + (size_t)i * ldv;
                 if ((rc = matvec_device(pl, vi, w, mixed))) return rc;
                 nmv++;
                 inner++;
                 double *hcol = H + (size_t)i * ld;
                 {
                     StageScope sc(pl, kStOrth);
-                    if (f32) {
-                        k_cgs_dots<<<kRG, kRB, 0, st>>>(w, Vf, ldv, n2, i + 1, pl->gm_hpart);
+                    if (cgs) {
+                        k_cgs_dots<<<kRG, kRB, 0, st>>>(w, V, ldv, n2, i + 1, pl->gm_hpart);
                         k_cgs_coef<<<i + 1, kRB, 0, st>>>(pl->gm_hpart, pl->gm_coef, hcol, 1);
-                        k_cgs_update<<<kRG, kRB, 0, st>>>(w, Vf, ldv, n2, pl->gm_coef, i + 1, partA);
-                        k_normalize<float><<<kRG, kRB, 0, st>>>(w, Vf + (size_t)(i + 1) * ldv, n2, partA, hcol + i + 1);
+                        k_cgs_update<<<kRG, kRB, 0, st>>>(w, V, ldv, n2, pl->gm_coef, i + 1, partA);
+                        k_normalize<double><<<kRG, kRB, 0, st>>>(w, V + (size_t)(i + 1) * ldv, n2, partA, hcol + i + 1);
                     } else {
                         k_dot_part<<<kRG, kRB, 0, st>>>(w, V, n2, partA);  // <w, v_0>
                         double *pin = partA, *pout = partB;
@@ -676,8 +667,7 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
                 IBIM_HIP_CHECK(hipStreamSynchronize(st));
                 res = pl->gm_host[1];
                 if (res < tol) {
-                    if (f32) k_update<float><<<kRG, kRB, 0, st>>>(d_x, Vf, ldv, n2, H, ld, s, i);
-                    else k_update<double><<<kRG, kRB, 0, st>>>(d_x, V, ldv, n2, H, ld, s, i);
+                    k_update<double><<<kRG, kRB, 0, st>>>(d_x, V, ldv, n2, H, ld, s, i);
                     IBIM_LAUNCH_CHECK();
                     ret = IBIM_OK;
                     done = true;
@@ -685,8 +675,7 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
                 }
             }
             if (done) break;
-            if (f32) k_update<float><<<kRG, kRB, 0, st>>>(d_x, Vf, ldv, n2, H, ld, s, i - 1);
-            else k_update<double><<<kRG, kRB, 0, st>>>(d_x, V, ldv, n2, H, ld, s, i - 1);
+            k_update<double><<<kRG, kRB, 0, st>>>(d_x, V, ldv, n2, H, ld, s, i - 1);
             IBIM_LAUNCH_CHECK();
             if ((rc = matvec_device(pl, d_x, p, mixed))) return rc;
             nmv++;

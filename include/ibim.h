@@ -223,7 +223,7 @@ int ibim_tree_points(ibim_plan *plan, int id, int *out, int64_t cap);
 /* ---- geometry pre-step: level set and tube quadrature nodes ---------------
  * The solvent-excluded surface as a level set on a uniform grid and the IBIM
  * quadrature nodes in the tube |phi| < thickness*h around it.  Grid index
- * I = i*ny*nz + j*nz + k, coordinates lo + (i,j,k)*h (levelset.cpp:103-112). */
+ * I = i*ny*nz + j*nz + k, coordinates lo + (i,j,k)*h (levelset.cpp:28-37, 554-558). */
 typedef struct ibim_geom_params {
     double h;              /* grid spacing, in the atoms' length unit */
     double lo[3];          /* grid origin (sx, sy, sz) */
@@ -244,17 +244,17 @@ typedef struct ibim_surface_info {
     int dims[3];
     double lo[3];
     double h;
-    int64_t inclusions;    /* cavity grid points pushed out of the tube (levelset.cpp:601-612) */
+    int64_t inclusions;    /* cavity grid points pushed out of the tube (levelset.cpp:526-537) */
     int evolve_steps, reinit_steps[2];
-    double quality[2];     /* mean |grad phi| - 1 and bad fraction in the tube (levelset.cpp:500-504) */
+    double quality[2];     /* mean |grad phi| - 1 and bad fraction in the tube (levelset.cpp:398-435) */
     double area;           /* sum of weights */
 } ibim_surface_info;
 
 /* expand + evolve (probe) + flip + reinitialise twice + Surface, as the
- * reference pipeline main.cpp:47-62 (levelset.cpp:166-362, 512-661) */
+ * reference pipeline main.cpp:47-62 (levelset.cpp:91-287, 437-586) */
 int ibim_surface_from_atoms(const ibim_geom_params *gp, int64_t m, const double *atom_xyz, const double *radii,
                             ibim_surface **out);
-/* Surface (levelset.cpp:512-661) of a level set given on the grid (host array) */
+/* Surface (levelset.cpp:437-586) of a level set given on the grid (host array) */
 int ibim_surface_from_grid(const ibim_geom_params *gp, const double *phi, ibim_surface **out);
 int ibim_surface_get_info(const ibim_surface *s, ibim_surface_info *info);
 /* host copies (AoS xyz and normals, weights) */

@@ -40,7 +40,7 @@ def clusters(n, seed=0):
 
 def sphere_tube(a=2.0, h=0.125, lo=-4.0, npts=64, thickness=2):
     """Tube nodes of the sphere |x| = a on the grid x_i = lo + i h: projection
-    P - n phi, unit normal, cosine weight times h^3 (levelset.cpp:615-651,
+    P - n phi, unit normal, cosine weight times h^3 (levelset.cpp:552-576,
     electric.cpp:33)."""
     g = lo + h * np.arange(npts)
     X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
