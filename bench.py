@@ -143,6 +143,7 @@ def serve(ibim, mols, cfg, precision, slots, rate=None, profile=None, arrival_se
                 ids[i] = srv.submit(a, r, q, h=h, restart=cfg["restart"], tol=cfg["tol"], **kw)
 
         t0 = time.perf_counter()
+        win = [time.clock_gettime_ns(time.CLOCK_MONOTONIC), time.clock_gettime_ns(time.CLOCK_BOOTTIME)]
         th = threading.Thread(target=producer)
         th.start()
         res = []
@@ -155,7 +156,10 @@ def serve(ibim, mols, cfg, precision, slots, rate=None, profile=None, arrival_se
                          f"{res[-1]['t_end'] - res[-1]['t_submit']:.2f} s)")
         th.join()
         wall = time.perf_counter() - t0
+        win += [time.clock_gettime_ns(time.CLOCK_MONOTONIC), time.clock_gettime_ns(time.CLOCK_BOOTTIME)]
         rep = srv.stage_report() if profile else None
+    if profile is not None:
+        rep = dict(rep or {}, _window_ns={"monotonic": [win[0], win[2]], "boottime": [win[1], win[3]]})
     return res, wall, rep
 
 
@@ -259,6 +263,7 @@ def run_stream(ibim, args, cfg):
 
     max_err = max(errs + errs_in_stream)
     ok = max_err <= 1e-6 and mixed["all_converged"] and fp64["all_converged"]
+    window = rep.pop("_window_ns")
     roof = roofline_entries(rep, cfg["p"])
     stage_tot = {k: {"ms": v["ms"], "launches": v["launches"]} for k, v in rep.items()}
     return {
@@ -271,7 +276,7 @@ def run_stream(ibim, args, cfg):
         "energy": {"max_rel_err_mixed_vs_fp64": max_err, "within_1e-6": bool(max_err <= 1e-6),
                    "compared": f"the {len(sub)} FP64-subsample requests: mixed burst and mixed stream against the "
                                f"FP64 burst"},
-        "ok": ok, "roof": roof, "stages": stage_tot,
+        "ok": ok, "roof": roof, "stages": stage_tot, "window": window,
     }
 
 
@@ -406,7 +411,10 @@ def run_single(ibim, args, cfg):
 
     legs = {}
     acc = {}
-    for name, prec in (("fp64", ibim.PREC_FP64), ("mixed_f32", ibim.PREC_MIXED), ("mixed_f16", ibim.PREC_MIXED_F16)):
+    forms = (("fp64", ibim.PREC_FP64), ("mixed_f32", ibim.PREC_MIXED), ("mixed_f16", ibim.PREC_MIXED_F16))
+    for name, prec in forms:
+        if args.forms and name not in args.forms.split(","):
+            continue
         for _ in range(args.warmup):
             solve(prec)
         torch.cuda.synchronize()
@@ -421,8 +429,9 @@ def run_single(ibim, args, cfg):
                       "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)}, "energy": e, "solve": info}
         progress(f"{name}: {legs[name]['solves_per_s']:.3f} solves/s")
     for k in ("mixed_f32", "mixed_f16"):
-        legs[k]["rel_energy_err_vs_fp64"] = abs(legs[k]["energy"] - legs["fp64"]["energy"]) / abs(
-            legs["fp64"]["energy"])
+        if k in legs and "fp64" in legs:
+            legs[k]["rel_energy_err_vs_fp64"] = abs(legs[k]["energy"] - legs["fp64"]["energy"]) / abs(
+                legs["fp64"]["energy"])
     with plan_for(ibim.PREC_FP64) as pl:
         stats = pl.stats()
     surf.close()
@@ -442,6 +451,11 @@ def main():
                     help="the capacity bursts and the FP64 stream serve every k-th request")
     ap.add_argument("--requests", type=int, default=None, help="stream length (default: the workload's 256)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--leaf", type=int, default=None, help="override the leaf capacity")
+    ap.add_argument("--molecule", type=int, default=None, metavar="I",
+                    help="stream256: time single solves of the stream's molecule I instead of serving")
+    ap.add_argument("--forms", default=None, help="single-solve forms to time (comma list of fp64, mixed_f32, "
+                                                   "mixed_f16; default all)")
     ap.add_argument("--matvecs-only", type=int, default=0, metavar="K",
                     help="single-molecule counter mode: a 256 MiB device copy (calibration), then K matvecs in "
                          "each precision on the molecule's plan; prints a short JSON line")
@@ -459,20 +473,23 @@ def main():
     cfg = dict(CONFIGS[args.config])
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
-    if args.config == "stream256" and args.matvecs_only:
-        # counter mode on the stream's median-size molecule
+    if args.leaf:
+        cfg["leaf"] = args.leaf
+    if args.config == "stream256" and (args.matvecs_only or args.molecule is not None):
+        # one molecule of the stream (counter mode: the median-size one)
         sizes, _ = stream_molecules(ibim, dict(cfg, requests=cfg["requests"]))
-        i = int(np.argsort(sizes)[len(sizes) // 2])
+        i = args.molecule if args.molecule is not None else int(np.argsort(sizes)[len(sizes) // 2])
         m = int(sizes[i])
         cfg = dict(cfg, atoms=m, seed=cfg["mol_seed"] + i, confine=cfg["confine"] * m ** (1 / 3))
-        run_single(ibim, args, cfg)
-        return
-    if args.config != "stream256":
+        if args.matvecs_only:
+            run_single(ibim, args, cfg)
+            return
+    if args.config != "stream256" or args.molecule is not None:
         single = run_single(ibim, args, cfg)
         if single is None:
             return
         legs = single["legs"]
-        best = legs["mixed_f16"]
+        best = legs.get("mixed_f16") or next(iter(legs.values()))
         out = {"metric": METRIC, "value": best["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
                "steps": args.steps, "warmup": args.warmup, "ms_per_step": best["ms_per_solve"],
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f16/f32/f64",
@@ -481,7 +498,8 @@ def main():
                                       f"h={cfg['h']} A, p={cfg['p']}, leaf {cfg['leaf']} (not the metric's "
                                       f"serving workload)", "parallelism": "single GPU"},
                "latency_ms": best["latency_ms"], "legs": legs,
-               "mixed_speedup_vs_fp64": best["solves_per_s"] / legs["fp64"]["solves_per_s"],
+               "mixed_speedup_vs_fp64": (best["solves_per_s"] / legs["fp64"]["solves_per_s"]) if "fp64" in legs
+               else None,
                "tube_points": single["tube_points"], "tree": single["tree"],
                "roofline": single["roof"].get("near_field"), "roofline_m2l": single["roof"].get("m2l"),
                "roofline_matvec": single["roof"].get("matvec"), "cpu_baseline": None}
@@ -538,6 +556,7 @@ def main():
         "energy": st["energy"],
         "roofline": roof, "roofline_m2l": st["roof"].get("m2l"), "roofline_matvec": st["roof"].get("matvec"),
         "stage_totals_mixed_stream": st["stages"],
+        "timed_window_ns": st["window"],
         "cpu_baseline": cpu,
         "mixed_vs_cpu": (mixed["solves_per_s"] / cpu["value"]) if cpu and "value" in cpu else None,
     }
