@@ -520,16 +520,18 @@ def main():
     roof = st["roof"].get("near_field")
     if roof is not None:
         traffic = None
-        if os.path.exists(TRAFFIC_FILE):
+        if os.path.exists(TRAFFIC_FILE):  # HBM bytes per near-field pair from the counter runs
             try:
-                traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_launch")
-            except (OSError, ValueError):
+                traffic = json.load(open(TRAFFIC_FILE))["bytes_per_pair"] * roof["pairs_per_launch"]
+            except (OSError, ValueError, KeyError):
                 traffic = None
         roof = dict(roof, traffic=traffic,
                     note="the dominant kernel of the mixed solves: the FP32 near field (60 flops per pair), two "
                          "launches per mixed matvec on the second stream; time from HIP events on that stream "
-                         "around each launch, summed over every mixed matvec of the timed stream; traffic from "
-                         "rocprofv3 FETCH_SIZE + WRITE_SIZE of the same kernel")
+                         "around each launch, summed over every mixed matvec of the timed stream; traffic: HBM "
+                         "bytes per pair of the same kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE on the stream's "
+                         "median molecule (profiles/p2p_mixed_stream256_traffic.json) times this run's pairs per "
+                         "launch")
     out = {
         "metric": METRIC,
         "value": mixed["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
