@@ -88,13 +88,20 @@ def load_ibim():
 
 
 def cpu_model():
+    """The host CPU's model name, with its family/model/stepping numbers (some
+    hosts report only a short model name)."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("model name", "cpu family", "model", "stepping", "vendor_id") and k not in info:
+                info[k] = v.strip()
     except OSError:
-        pass
-    return platform.processor()
+        return platform.processor()
+    return (f"{info.get('model name', platform.processor())} ({info.get('vendor_id', '?')} family "
+            f"{info.get('cpu family', '?')} model {info.get('model', '?')} stepping {info.get('stepping', '?')}, "
+            f"{os.cpu_count()} logical CPUs visible)")
 
 
 def progress(msg):
@@ -165,9 +172,12 @@ def serve(ibim, mols, cfg, precision, slots, rate=None, profile=None, arrival_se
 
 def summarize(res, wall):
     lat = [1e3 * (r["t_end"] - r["t_submit"]) for r in res]
+    mv = sum(r["info"]["matvecs"] for r in res)
     return {"requests": len(res), "solves_per_s": len(res) / wall, "wall_s": wall,
             "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": max(lat)},
             "tube_points": int(sum(r["n_points"] for r in res)),
+            "matvecs_per_s": mv / wall,
+            "tube_point_matvecs_per_s": sum(r["n_points"] * r["info"]["matvecs"] for r in res) / wall,
             "all_converged": all(r["status"] == 0 and r["info"]["converged"] == 1 for r in res)}
 
 
@@ -557,7 +567,9 @@ def main():
         "target_speedup": 3.0,
         "energy": st["energy"],
         "roofline": roof, "roofline_m2l": st["roof"].get("m2l"), "roofline_matvec": st["roof"].get("matvec"),
-        "stage_totals_mixed_stream": st["stages"],
+        "stage_totals_mixed_stream": dict(st["stages"], note="HIP-event spans summed over the stream's requests; "
+                                          "spans on the two streams of a matvec, and of two concurrent requests, "
+                                          "overlap in time, so they do not add up to the wall time"),
         "timed_window_ns": st["window"],
         "cpu_baseline": cpu,
         "mixed_vs_cpu": (mixed["solves_per_s"] / cpu["value"]) if cpu and "value" in cpu else None,

@@ -175,19 +175,26 @@ def test_polarize_matches_oracle(ibim, oracle, gpu):
     assert rel(g, o) < TOL, rel(g, o)
 
 
-def test_gmres_matches_oracle(ibim, oracle, gpu):
+@pytest.mark.parametrize("tol", [1e-10, 1e-13])
+def test_gmres_matches_oracle(ibim, oracle, gpu, tol):
+    """GMRES(8) with restarts on the plan's operator against the oracle's
+    GMRES on its own: the same iteration count (the two operators differ by
+    ~1e-15, so a residual estimate may cross tol one step apart: +-1), and
+    solutions within the stopping tolerance's reach; at tol 1e-13 they agree
+    to the FP64 bar, 1e-10."""
     pts, nrm, w = G.shell(700, seed=12)
     rng = np.random.default_rng(13)
     atoms = rng.uniform(-0.8, 0.8, size=(5, 3))
     q = rng.uniform(-0.5, 0.5, size=5)
     b = oracle.rhs(pts, nrm, atoms, q, PB["eps_in"], PB["eps_out"])
     with ibim.Plan(pts, nrm, w, p=4, max_points=32, max_level=10, **PB) as plan:
-        xg, rc, info = plan.gmres(b, restart=8, maxit=60, tol=1e-10)
+        xg, rc, info = plan.gmres(b, restart=8, maxit=200, tol=tol)
     xo, orc, oit, ores, onmv = oracle.solve(pts, nrm, w, b, p=4, max_point=32, max_level=10, restart=8,
-                                            maxit=60, tol=1e-10, **PB)
+                                            maxit=200, tol=tol, **PB)
+    print(f"tol {tol:g}: iterations {info['iterations']} / {oit}, x rel {rel(xg, xo):.1e}")
     assert rc == 0 and orc == 0
     assert abs(info["iterations"] - oit) <= 1, (info, oit)
-    assert rel(xg, xo) < 1e-8
+    assert rel(xg, xo) < (1e-8 if tol > 1e-12 else 1e-10)
 
 
 def test_small_and_degenerate_trees(ibim, oracle, gpu):
