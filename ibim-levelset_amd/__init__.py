@@ -88,7 +88,7 @@ def stage_mask(stages):
 class GeomParams(C.Structure):
     _fields_ = [("h", C.c_double), ("lo", C.c_double * 3), ("dims", C.c_int * 3), ("pad", C.c_double),
                 ("probe", C.c_double), ("band", C.c_int), ("thickness", C.c_int), ("thres", C.c_double),
-                ("reinit_time", C.c_double), ("scale", C.c_double)]
+                ("reinit_time", C.c_double), ("scale", C.c_double), ("init", C.c_double)]
 
 
 class SurfaceInfo(C.Structure):
@@ -396,16 +396,17 @@ class Plan:
 
 # ------------------------------------------------------------ geometry --
 def geom_params(h, lo=(0.0, 0.0, 0.0), dims=(0, 0, 0), pad=None, probe=1.4, band=8, thickness=2, thres=1e-4,
-                reinit_time=None, scale=1.0):
+                reinit_time=None, scale=1.0, init=0.0):
     """Grid and level-set options.  reinit_time defaults to 26 h, the final time
     the reference's input.cfg (reinit_step = 60) gives at its usual grid of
-    256 cells over 600 scaled units (main.cpp:38-40)."""
+    256 cells over 600 scaled units (main.cpp:38-40).  init is the grid's value
+    before expand (main.cpp:43 uses -2 * size); 0 picks -2 max(dims) h."""
     if pad is None:
         pad = (band + 4) * h
     if reinit_time is None:
         reinit_time = 26.0 * h
     return GeomParams(h, (C.c_double * 3)(*lo), (C.c_int * 3)(*dims), pad, probe, band, thickness, thres,
-                      reinit_time, scale)
+                      reinit_time, scale, init)
 
 
 class Surface:

@@ -524,8 +524,8 @@ static int ls_surface(LsRun &L, const ibim_geom_params &gp, ibim_surface *S) {
 
 static int check_geom(const ibim_geom_params *gp) {
     if (!gp || !(gp->h > 0) || gp->thickness < 1 || gp->band < 0 || !(gp->probe >= 0) || !(gp->reinit_time >= 0) ||
-        !(gp->scale >= 0)) {
-        set_error("bad geometry parameters (h > 0, thickness >= 1, band >= 0, probe >= 0 required)");
+        !(gp->scale >= 0) || !(gp->init <= 0)) {
+        set_error("bad geometry parameters (h > 0, thickness >= 1, band >= 0, probe >= 0, init <= 0 required)");
         return IBIM_ERR_ARG;
     }
     return IBIM_OK;
@@ -612,7 +612,7 @@ int ibim_surface_from_atoms(const ibim_geom_params *gp, int64_t m, const double 
         ibim_surface_destroy(S);
         return code;
     };
-    // expand (levelset.cpp:91-109) onto g = -2 * max(dims) * h
+    // expand (levelset.cpp:91-109) onto the initial value (main.cpp:43: -2 * size)
     if (dev_malloc(&d_at, 4 * (size_t)m * sizeof(double)) != hipSuccess) {
         set_error("hipMalloc (atoms)");
         return fail(IBIM_ERR_NOMEM);
@@ -624,7 +624,7 @@ int ibim_surface_from_atoms(const ibim_geom_params *gp, int64_t m, const double 
         set_error("hipMemcpy (atoms)");
         return fail(IBIM_ERR_HIP);
     }
-    const double init = -2.0 * std::max(L.d.nx, std::max(L.d.ny, L.d.nz)) * L.d.h;
+    const double init = gp->init < 0 ? gp->init : -2.0 * std::max(L.d.nx, std::max(L.d.ny, L.d.nz)) * L.d.h;
     k_fill<<<nb, 256, 0, L.st>>>(L.g, L.n, init);
     k_to_ord<<<nb, 256, 0, L.st>>>(L.g, L.n);
     {

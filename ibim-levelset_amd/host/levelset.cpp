@@ -1,5 +1,6 @@
 #include "levelset.h"
 
+#include <algorithm>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -23,6 +24,8 @@ static ibim_geom_params geom_of(const levelset &ls, double scale) {
     gp.thickness = ls.thickness;
     gp.thres = ls.thres;
     gp.scale = scale;
+    // the grid starts at -2 * size before expand (main.cpp:43)
+    gp.init = -2.0 * std::max(ls.Nx, std::max(ls.Ny, ls.Nz));
     return gp;
 }
 

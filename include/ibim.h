@@ -235,6 +235,8 @@ typedef struct ibim_geom_params {
     double thres;          /* reinitialisation gradient tolerance, lvlset_thres */
     double reinit_time;    /* final time of each reinitialisation, reinit_step */
     double scale;          /* s: nodes/s, weights*s*(h/s)^3 (electric.cpp:17-41); 1 for Angstrom grids */
+    double init;           /* the grid's value before expand, <= 0 (main.cpp:43 starts from -2*size);
+                              0: -2*max(dims)*h */
 } ibim_geom_params;
 
 typedef struct ibim_surface ibim_surface;

@@ -12,6 +12,8 @@
  *   RHS load ........................ electric.cpp:395-413
  *   polarizeMap + energy ............ electric.cpp:432-505
  *   restarted GMRES ................. gmres.cpp:10-164
+ *   level set + tube nodes .......... levelset.cpp:91-586, electric.cpp:17-41
+ *                                     (ibim_levelset_oracle.c)
  *
  * Pinning (how each function is checked without running the reference,
  * which needs an external CBLAS library this image lacks):
@@ -28,7 +30,11 @@
  *   matvec -- FMM vs dense O(N^2) operator at p=6..8, A = I when
  *     eps_in = eps_out and kappa = 0;
  *   GMRES -- small dense systems against LAPACK (numpy.linalg.solve);
- *   whole solve -- the Born sphere energy.
+ *   whole solve -- the Born sphere energy;
+ *   level set -- WENO5 derivatives exact on quadratics, Godunov norm of
+ *     plane and sphere distance functions, evolve of a sphere moves it by
+ *     the final time, reinitialisation keeps a distance function, Surface
+ *     of a plane and a sphere against closed forms (tests/test_oracle_levelset.py).
  */
 #ifndef IBIM_ORACLE_H
 #define IBIM_ORACLE_H
@@ -154,6 +160,45 @@ int orc_gmres_dense(int n, const double *Amat, const double *b, double *x, int m
 int orc_solve(int n, const double *xyz, const double *nrm, const double *w,
               const orc_fmm_params *fp, const double *b, double *x, int m, int maxit,
               double tol, int share_tree, int *iters, double *resid, int *nmatvec);
+
+/* ---- geometry pre-step (ibim_levelset_oracle.c) ----------------------------
+ * The level set of the solvent-excluded surface and the tube nodes:
+ * expand, evolve, reinitialize, Surface (levelset.cpp:91-586, main.cpp:43-62).
+ * Grid index I = i*ny*nz + j*nz + k, coordinates lo + (i,j,k)*h. */
+typedef struct {
+    int dims[3];
+    double lo[3];
+    double h;
+    double probe;        /* probe radius; also the evolve's final time at speed 1 */
+    int band;            /* expand window margin in cells */
+    int thickness;       /* tube half width in cells */
+    double thres;        /* reinitialisation gradient tolerance */
+    double reinit_time;  /* final time of each reinitialisation */
+    double scale;        /* nodes / scale, weights * scale * (h / scale)^3 */
+} orc_lsparams;
+
+typedef struct {
+    int evolve_steps, reinit_steps[2];
+    double quality[2];   /* mean ||grad phi| - 1| and bad fraction in the tube, after the last pass */
+    int64_t inclusions, n_nodes;
+} orc_lsinfo;
+
+/* the pieces, for pinning: WENO5 one-sided derivatives at (i, j, k)
+ * (setGradient), the Godunov norm getNorm(A, B), and one evolve (mode 0) or
+ * reinitialize (mode 1) of g in place, returning the steps taken */
+void orc_ls_gradient(const orc_lsparams *p, const double *g, int i, int j, int k, double Dup[3], double Dun[3]);
+double orc_ls_norm(const double A[3], const double B[3]);
+int orc_ls_run(const orc_lsparams *p, int mode, double *g, const double *phi0, double final_t, double vel, double cfl,
+               double quality[2]);
+/* Surface of phi (not modified); xyz/nrm (3 per node) and w are filled when
+ * xyz is non-null and hold cap nodes.  Returns the node count, -1 if cap is short. */
+int64_t orc_surface_from_grid(const orc_lsparams *p, const double *phi, double *xyz, double *nrm, double *w,
+                              int64_t cap, int64_t *inclusions);
+/* the whole pre-step from atoms on the grid p->dims/p->lo, the grid starting
+ * at `init`; g_out (optional) receives the level set before Surface */
+int64_t orc_surface_from_atoms(const orc_lsparams *p, int m, const double *axyz, const double *radii,
+                               double init, double *xyz, double *nrm, double *w, int64_t cap, orc_lsinfo *info,
+                               double *g_out);
 
 #ifdef __cplusplus
 }

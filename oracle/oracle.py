@@ -1,4 +1,4 @@
-"""ctypes binding of the CPU oracle (oracle/ibim_oracle.c).
+"""ctypes binding of the CPU oracle (oracle/ibim_oracle.c, oracle/ibim_levelset_oracle.c).
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py -- never by the product.  The functions restate
@@ -28,6 +28,17 @@ class KParams(C.Structure):
 
 class FmmParams(C.Structure):
     _fields_ = [("p", C.c_int), ("max_point", C.c_int), ("max_level", C.c_int), ("k", KParams)]
+
+
+class LsParams(C.Structure):
+    _fields_ = [("dims", C.c_int * 3), ("lo", C.c_double * 3), ("h", C.c_double), ("probe", C.c_double),
+                ("band", C.c_int), ("thickness", C.c_int), ("thres", C.c_double), ("reinit_time", C.c_double),
+                ("scale", C.c_double)]
+
+
+class LsInfo(C.Structure):
+    _fields_ = [("evolve_steps", C.c_int), ("reinit_steps", C.c_int * 2), ("quality", C.c_double * 2),
+                ("inclusions", C.c_int64), ("n_nodes", C.c_int64)]
 
 
 def build():
@@ -64,6 +75,18 @@ def lib():
         _lib.orc_gmres_dense.argtypes = [C.c_int, d, d, d, C.c_int, C.c_int, C.c_double, i, d]
         _lib.orc_solve.argtypes = [C.c_int, d, d, d, C.POINTER(FmmParams), d, d, C.c_int, C.c_int,
                                    C.c_double, C.c_int, i, d, i]
+        i64 = C.POINTER(C.c_int64)
+        lp = C.POINTER(LsParams)
+        _lib.orc_ls_gradient.argtypes = [lp, d, C.c_int, C.c_int, C.c_int, d, d]
+        _lib.orc_ls_gradient.restype = None
+        _lib.orc_ls_norm.argtypes = [d, d]
+        _lib.orc_ls_norm.restype = C.c_double
+        _lib.orc_ls_run.argtypes = [lp, C.c_int, d, d, C.c_double, C.c_double, C.c_double, d]
+        _lib.orc_surface_from_grid.argtypes = [lp, d, d, d, d, C.c_int64, i64]
+        _lib.orc_surface_from_grid.restype = C.c_int64
+        _lib.orc_surface_from_atoms.argtypes = [lp, C.c_int, d, d, C.c_double, d, d, d, C.c_int64,
+                                                C.POINTER(LsInfo), d]
+        _lib.orc_surface_from_atoms.restype = C.c_int64
     return _lib
 
 
@@ -269,3 +292,74 @@ def solve(xyz, nrm, w, b, p=4, max_point=64, max_level=10, restart=20, maxit=100
     rc = lib().orc_solve(n, _d(xyz), _d(_f64(nrm, (-1, 3))), _d(_f64(w)), C.byref(fp), _d(_f64(b)), _d(x),
                          restart, maxit, tol, int(share_tree), _i(it), _d(res), _i(nmv))
     return x, int(rc), int(it[0]), float(res[0]), int(nmv[0])
+
+
+# ------------------------------------------------------------ level set --
+def ls_params(dims, lo, h, probe=1.4, band=8, thickness=4, thres=0.05, reinit_time=2.0, scale=1.0):
+    """Grid and pipeline parameters of the level-set pre-step (levelset.cpp)."""
+    return LsParams((C.c_int * 3)(*[int(v) for v in dims]), (C.c_double * 3)(*[float(v) for v in lo]), float(h),
+                    float(probe), int(band), int(thickness), float(thres), float(reinit_time), float(scale))
+
+
+def grid_points(lp):
+    """Coordinates (nx, ny, nz, 3) of the grid points lo + (i, j, k) h."""
+    ax = [lp.lo[q] + lp.h * np.arange(lp.dims[q]) for q in range(3)]
+    return np.stack(np.meshgrid(*ax, indexing="ij"), axis=-1)
+
+
+def ls_gradient(lp, g, i, j, k):
+    """WENO5 one-sided derivatives (Dup, Dun) at grid point (i, j, k) (levelset.cpp:330-396)."""
+    g = _f64(g).ravel()
+    up, un = np.zeros(3), np.zeros(3)
+    lib().orc_ls_gradient(C.byref(lp), _d(g), i, j, k, _d(up), _d(un))
+    return up, un
+
+
+def ls_norm(a, b):
+    """getNorm(a, b) (levelset.cpp:294-301)."""
+    return float(lib().orc_ls_norm(_d(_f64(a)), _d(_f64(b))))
+
+
+def ls_evolve(lp, g, final_t, vel=1.0, cfl=0.2):
+    """evolve (levelset.cpp:111-187) of a copy of g; returns (g, steps)."""
+    g = _f64(g).copy()
+    steps = lib().orc_ls_run(C.byref(lp), 0, _d(g.ravel()), None, final_t, vel, cfl, None)
+    return g, int(steps)
+
+
+def ls_reinitialize(lp, g, final_t, vel=1.0, cfl=0.5, phi0=None):
+    """reinitialize (levelset.cpp:191-287) of a copy of g against phi0 (default g);
+    returns (g, steps, (mean error, bad fraction))."""
+    g = _f64(g).copy()
+    p0 = g.copy() if phi0 is None else _f64(phi0).copy()
+    q = np.zeros(2)
+    steps = lib().orc_ls_run(C.byref(lp), 1, _d(g.ravel()), _d(p0.ravel()), final_t, vel, cfl, _d(q))
+    return g, int(steps), (float(q[0]), float(q[1]))
+
+
+def surface_from_grid(lp, phi):
+    """Surface (levelset.cpp:437-586, electric.cpp:17-41) of the level set phi:
+    (xyz, normals, weights, inclusions), nodes in grid order."""
+    phi = _f64(phi).ravel()
+    inc = np.zeros(1, np.int64)
+    n = lib().orc_surface_from_grid(C.byref(lp), _d(phi), None, None, None, 0, inc.ctypes.data_as(C.POINTER(C.c_int64)))
+    xyz, nrm, w = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+    m = lib().orc_surface_from_grid(C.byref(lp), _d(phi), _d(xyz), _d(nrm), _d(w), n,
+                                    inc.ctypes.data_as(C.POINTER(C.c_int64)))
+    assert m == n
+    return xyz, nrm, w, int(inc[0])
+
+
+def surface_from_atoms(lp, atom_xyz, radii, init):
+    """The whole pre-step (main.cpp:43-62) on the grid of lp with the grid
+    starting at init: (xyz, normals, weights, info, level set before Surface)."""
+    axyz = _f64(atom_xyz, (-1, 3))
+    rad = _f64(radii)
+    g = np.zeros(tuple(lp.dims))
+    info = LsInfo()
+    n = lib().orc_surface_from_atoms(C.byref(lp), len(axyz), _d(axyz), _d(rad), init, None, None, None, 0,
+                                     C.byref(info), _d(g.ravel()))
+    # Surface only reads g, so the node pass can start from the returned level set
+    xyz, nrm, w, inc = surface_from_grid(lp, g)
+    assert len(w) == n and inc == info.inclusions
+    return xyz, nrm, w, info, g

@@ -1,6 +1,11 @@
 """The C-ABI library loads and exports every function include/ibim.h declares
 (no compute: these run without a GPU)."""
 import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
 
 
 def test_library_exports_every_declared_symbol(ibim):
@@ -17,10 +22,24 @@ def test_version_and_error_strings(ibim):
     assert isinstance(L.ibim_last_error(), bytes)
 
 
-def test_struct_layouts_match_header(ibim):
-    # ibim_params: 3 ints, 4 doubles, 2 ints (C layout with padding)
-    assert ctypes.sizeof(ibim.Params) == 56
-    assert ctypes.sizeof(ibim.SolveInfo) == 24
+def test_struct_layouts_match_header(ibim, tmp_path):
+    """Every ctypes mirror has the C struct's size and field offsets (gcc on
+    include/ibim.h), so a field added to the header but not to the binding,
+    or in another place, fails here."""
+    exe = tmp_path / "abi_layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(HERE, "native", "abi_layout.c"), "-o", str(exe)], check=True)
+    lines = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    c = dict(zip(lines[0::2], (int(v) for v in lines[1::2])))
+    mirrors = {"ibim_params": ibim.Params, "ibim_tree_stats": ibim.TreeStats, "ibim_solve_info": ibim.SolveInfo,
+               "ibim_stage_report": ibim.StageReport, "ibim_geom_params": ibim.GeomParams,
+               "ibim_surface_info": ibim.SurfaceInfo, "ibim_request": ibim.Request, "ibim_result": ibim.Result}
+    for name, cls in mirrors.items():
+        assert ctypes.sizeof(cls) == c[name], name
+        fields = [f for f, _ in cls._fields_]
+        assert fields == [k.split(".", 1)[1] for k in c if k.startswith(name + ".")], name
+        for f in fields:
+            assert getattr(cls, f).offset == c[f"{name}.{f}"], f"{name}.{f}"
 
 
 def test_plan_create_rejects_bad_arguments_before_any_device_call(ibim):
