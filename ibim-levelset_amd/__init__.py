@@ -166,6 +166,7 @@ def lib():
     L.ibim_tree_lists.argtypes = [vp, C.c_int, i, i, i64]
     L.ibim_tree_points.argtypes = [vp, C.c_int, i, i64]
     L.ibim_plan_profile.argtypes = [vp, C.c_int]
+    L.ibim_plan_set_trace.argtypes = [vp, C.c_int]
     L.ibim_plan_profile_stages.argtypes = [vp, C.c_uint]
     L.ibim_plan_stage_times.argtypes = [vp, d, C.POINTER(C.c_int64), C.c_int]
     L.ibim_plan_stage_report.argtypes = [vp, C.POINTER(StageReport), C.c_int]
@@ -306,6 +307,10 @@ class Plan:
 
     def sync(self):
         _check(lib().ibim_plan_sync(self._h), "ibim_plan_sync")
+
+    def set_trace(self, on=True):
+        """Print GMRES's iteration table (iteration, relative residual, seconds) to stdout."""
+        _check(lib().ibim_plan_set_trace(self._h, int(on)), "ibim_plan_set_trace")
 
     def set_stream(self, stream):
         """Order the plan's work on a caller's stream (a hipStream_t as int, e.g.

@@ -370,6 +370,15 @@ int ibim_plan_profile(ibim_plan *pl, int enable) {
     return IBIM_OK;
 }
 
+int ibim_plan_set_trace(ibim_plan *pl, int on) {
+    if (!pl) {
+        set_error("plan is NULL");
+        return IBIM_ERR_ARG;
+    }
+    pl->trace = on != 0;
+    return IBIM_OK;
+}
+
 int ibim_plan_profile_stages(ibim_plan *pl, unsigned stage_mask) {
     if (!pl) {
         set_error("plan is NULL");

@@ -447,6 +447,7 @@ struct ibim_plan {
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool defer_reduce = false;           // the FP16 M2L stage leaves the reduction to the caller
+    bool trace = false;                  // GMRES prints its iteration table to stdout
 
     // sorted geometry
     double *sx = nullptr, *sy = nullptr, *sz = nullptr;

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                   const int *__restrict__ pB, const int *__restrict__ gtab,
                                                   const int *__restrict__ stab, const T *__restrict__ ops, int MRP,
                                                   int KRP, const double *__restrict__ M, double *__restrict__ L,
-                                                  int ntiles, int per_xcd) {
+                                                  int first, int count, int per_xcd) {
     constexpr int R = P * P * P;
     constexpr int R4 = 4 * R;
     constexpr int KC = 32;
@@ -226,11 +226,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __shared__ int sS[64];
     // Workgroups go to the 8 XCDs round robin; renumber so that each XCD
     // gets one contiguous range of tiles.  Tiles are ordered (class, mask,
-    // pair tile, row tile): the 8 row tiles of a pair tile, and the pair
-    // tiles of one operator class, then run on one XCD and share its L2.
-    const int logical = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    // pair tile, row tile): the row tiles of a pair tile, and the pair tiles
+    // of one operator class, then run on one XCD and share its L2.  A launch
+    // covers tiles first .. first + count - 1.
+    const int local = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (local >= count) return;
+    const int logical = first + local;
     const int nrt = MRP / 64;
-    if (logical >= ntiles * nrt) return;
     const int tile = logical / nrt;
     const int tid = threadIdx.x;
     const int key = blk_key[tile];
@@ -927,13 +929,33 @@ __global__ __launch_bounds__(256) void k_v_reduce(const int *__restrict__ work, 
     }
 }
 
+// The blocked FP64/FP32 GEMM over pair tiles x row tiles.  A dispatch's
+// grid is a 32-bit count of work-items, so one launch covers at most
+// kGemmChunk tiles of 256 threads; at p = 8 a 58M-point tube has 31M tiles.
+constexpr int64_t kGemmChunk = int64_t(1) << 23;
+
+template <int P, typename T>
+int launch_gemm(const VPairs &vp, const ibim_plan *pl, const T *ops, hipStream_t st) {
+    const int64_t total = (int64_t)vp.nblocks * (pl->MRP / 64);
+    if (total >= (int64_t(1) << 31)) {
+        set_error("M2L: too many pair tiles for one plan");
+        return IBIM_ERR_STATE;
+    }
+    for (int64_t first = 0; first < total; first += kGemmChunk) {
+        const int count = (int)std::min(kGemmChunk, total - first);
+        const int per_xcd = (count + 7) / 8;
+        k_m2l_gemm<P, T><<<8 * per_xcd, 256, 0, st>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, pl->gtab,
+                                                      pl->stab, ops, pl->MRP, pl->KRP, pl->M, pl->L, (int)first, count,
+                                                      per_xcd);
+    }
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
 template <int P>
 int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
     if (vp.nblocks == 0) return IBIM_OK;
     constexpr int R4 = 4 * P * P * P;
-    const int total = vp.nblocks * (pl->MRP / 64);
-    const int per_xcd = (total + 7) / 8;
-    const int grid = 8 * per_xcd;
     hipStream_t st = pl->stream;
     if constexpr (P <= 6) {
         if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->f16_ready && &vp == &pl->vp) {
@@ -986,16 +1008,8 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
             return IBIM_OK;
         }
     }
-    if (mixed && pl->ops32)
-        k_m2l_gemm<P, float><<<grid, 256, 0, st>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, pl->gtab,
-                                                   pl->stab, pl->ops32, pl->MRP, pl->KRP, pl->M, pl->L, vp.nblocks,
-                                                   per_xcd);
-    else
-        k_m2l_gemm<P, double><<<grid, 256, 0, st>>>(vp.blk_cls, vp.blk_j0, vp.cls_off_d, vp.A, vp.B, pl->gtab,
-                                                    pl->stab, pl->ops, pl->MRP, pl->KRP, pl->M, pl->L, vp.nblocks,
-                                                    per_xcd);
-    IBIM_LAUNCH_CHECK();
-    return IBIM_OK;
+    if (mixed && pl->ops32) return launch_gemm<P, float>(vp, pl, pl->ops32, st);
+    return launch_gemm<P, double>(vp, pl, pl->ops, st);
 }
 
 }  // namespace
@@ -1265,18 +1279,8 @@ int gemm_x_p(ibim_plan *pl, bool mixed) {
     const VPairs &xp = pl->xp;
     if (xp.nblocks == 0) return IBIM_OK;
     if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->x16) return IBIM_OK;  // in the FP16 V stage
-    const int total = xp.nblocks * (pl->MRP / 64);
-    const int per_xcd = (total + 7) / 8;
-    if (mixed && pl->xops32)
-        k_m2l_gemm<P, float><<<8 * per_xcd, 256, 0, pl->stream>>>(xp.blk_cls, xp.blk_j0, xp.cls_off_d, xp.A, xp.B,
-                                                                  pl->gtab, pl->stab, pl->xops32, pl->MRP, pl->KRP,
-                                                                  pl->M, pl->L, xp.nblocks, per_xcd);
-    else
-        k_m2l_gemm<P, double><<<8 * per_xcd, 256, 0, pl->stream>>>(xp.blk_cls, xp.blk_j0, xp.cls_off_d, xp.A, xp.B,
-                                                                   pl->gtab, pl->stab, pl->xops, pl->MRP, pl->KRP,
-                                                                   pl->M, pl->L, xp.nblocks, per_xcd);
-    IBIM_LAUNCH_CHECK();
-    return IBIM_OK;
+    if (mixed && pl->xops32) return launch_gemm<P, float>(xp, pl, pl->xops32, pl->stream);
+    return launch_gemm<P, double>(xp, pl, pl->xops, pl->stream);
 }
 }  // namespace
 

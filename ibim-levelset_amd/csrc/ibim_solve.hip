@@ -10,7 +10,9 @@
 // per Arnoldi step to decide when to stop, as the reference does.
 #include <hipcub/hipcub.hpp>
 
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <mutex>
 
 #include "ibim_internal.h"
@@ -628,6 +630,21 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
     if (normb == 0.0) normb = 1;
     double res = beta / normb;
     int ret = IBIM_NOT_CONVERGED;
+    // the reference's iteration table (gmres.cpp:79-106): iteration, the
+    // relative residual, the seconds the iteration took
+    auto clock_now = [] { return std::chrono::steady_clock::now(); };
+    auto t_last = clock_now();
+    if (pl->trace) {
+        std::printf("=============== GMRES =================\n    iter    |  rel error   |   time   \n");
+        std::fflush(stdout);
+    }
+    auto trace_line = [&](int it, double r) {
+        if (!pl->trace) return;
+        const auto t = clock_now();
+        std::printf("%6d%20.6e%12.6f\n", it, r, std::chrono::duration<double>(t - t_last).count());
+        std::fflush(stdout);
+        t_last = t;
+    };
     if (res <= tol) {
         ret = IBIM_OK;
     } else {
@@ -666,6 +683,7 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
                 IBIM_HIP_CHECK(fetch_scalar(pl, scal + 1, 1));
                 IBIM_HIP_CHECK(hipStreamSynchronize(st));
                 res = pl->gm_host[1];
+                trace_line(j, res);
                 if (res < tol) {
                     k_update<double><<<kRG, kRB, 0, st>>>(d_x, V, ldv, n2, H, ld, s, i);
                     IBIM_LAUNCH_CHECK();
@@ -692,6 +710,11 @@ static int gmres_core(ibim_plan *pl, const double *d_b, double *d_x, int restart
         }
     }
     IBIM_HIP_CHECK(hipStreamSynchronize(st));
+    if (pl->trace) {
+        std::printf(ret == IBIM_OK ? "=============== CONVERGED =============\n"
+                                   : "============= NOT CONVERGED ===========\n");
+        std::fflush(stdout);
+    }
     if (info) {
         info->iterations = inner;
         info->matvecs = nmv;
@@ -778,6 +801,11 @@ static int gmres_refine(ibim_plan *pl, const double *d_b, double *d_x, int resta
         ++outer;
         const double before = rn;
         if ((rc = residual())) return rc;
+        if (pl->trace) {
+            std::printf("refinement step %d: %d inner iterations, FP64 relative residual %.6e\n", outer,
+                        ii.iterations, rn / normb);
+            std::fflush(stdout);
+        }
         if (!(rn < before)) break;  // no progress: stop instead of looping
     }
     if (info) {

@@ -207,6 +207,11 @@ int ibim_gmres(ibim_plan *plan, const double *b, double *x, int restart, int max
                ibim_solve_info *info);
 int ibim_gmres_device(ibim_plan *plan, const double *d_b, double *d_x, int restart, int maxit,
                       double tol, ibim_solve_info *info);
+/* on: ibim_gmres[_device] print the reference GMRES's table to stdout, one
+ * line per iteration (iteration, relative residual, seconds), and the
+ * CONVERGED / NOT CONVERGED banner (gmres.cpp:79-106, 135-162); the mixed
+ * forms add one line per FP64 refinement step.  Off by default. */
+int ibim_plan_set_trace(ibim_plan *plan, int on);
 
 /* ---- inspection ----------------------------------------------------------- */
 int ibim_tree_stats_get(ibim_plan *plan, ibim_tree_stats *out);

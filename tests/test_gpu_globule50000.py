@@ -1,0 +1,110 @@
+"""The largest single solve at full size: a 50,000-atom synthetic globule in a
+100 A sphere, h = 0.3 A (58.5M tube points), p = 8, leaf 512,
+kappa = 0.1257 / A, eps 2 / 80.
+
+* The whole FMM at full size: with the polynomial surrogate kernels the
+  Chebyshev FMM is exact, and their exact sums over all 58.5M sources come
+  from source moments (tests/poly_moments.py), so every tree box, list entry,
+  M2L tile and near-field unit is checked without a direct sum.
+* A = I exactly at eps_in = eps_out, kappa = 0.
+* The FP64 solve and the FP16-mixed solve (FP16 V-list M2L with per-box
+  scaling, FP32 near field, FP64 refinement) both reach an FP64 relative
+  residual below 1e-8, and their solvation energies agree within 1e-6; the
+  mixed matvec repeats bit for bit.
+
+About 10 minutes of GPU time, so it runs only when IBIM_GLOBULE50000=1; GMRES
+prints its iteration table (run pytest with -s to see it).
+"""
+import ctypes as C
+import os
+import time
+
+import numpy as np
+import pytest
+
+from poly_moments import poly_kernel_sum
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(os.environ.get("IBIM_GLOBULE50000") != "1",
+                                 reason="about 10 minutes of GPU time: set IBIM_GLOBULE50000=1 to run")]
+
+H, P, LEAF = 0.3, 8, 512
+PB = dict(p=P, max_points=LEAF, max_level=10, kappa=0.1257, eps_in=2.0, eps_out=80.0, vacant_radius=H)
+
+
+def say(msg):
+    print(f"[{time.strftime('%H:%M:%S')}] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def globule(ibim, gpu):
+    xyz, rad, q = ibim.synthetic_globule(50000, 50000, 100.0)
+    t = time.time()
+    s = ibim.Surface.from_atoms(xyz, rad, h=H, probe=1.4, thickness=2)
+    say(f"surface: {s.n} tube points, grid {s.info['dims']}, {time.time() - t:.1f} s")
+    yield xyz, q, s
+    s.close()
+
+
+def test_fmm_exact_for_polynomial_kernels(ibim, globule):
+    _, _, s = globule
+    pts, _, _ = s.arrays()
+    q = np.random.default_rng(50000).normal(size=s.n)
+    with ibim.Plan(device_ptrs=s.device_ptrs(), p=P, max_points=LEAF, max_level=10,
+                   kernel_mode=ibim.KERNEL_POLY) as plan:
+        st = plan.stats()
+        say(f"tree: {st['n_nodes']} boxes, {st['v_pairs']} V and {st['x_pairs']} X interactions")
+        for k in ("K2", "K3xy", "K1z"):
+            t = time.time()
+            g = plan.kernel_fmm(k, q)
+            ref = poly_kernel_sum(k, pts, q)
+            err = np.abs(g - ref).max() / np.abs(ref).max()
+            say(f"{k}: FMM {time.time() - t:.1f} s, max error {err:.2e} of max |sum|")
+            assert err < 1e-9, k
+
+
+def test_identity_when_media_match(ibim, globule):
+    _, _, s = globule
+    x = np.random.default_rng(1).normal(size=2 * s.n)
+    with ibim.Plan(device_ptrs=s.device_ptrs(), **dict(PB, kappa=0.0, eps_in=1.0, eps_out=1.0)) as plan:
+        y = plan.matvec(x)
+    assert np.array_equal(y, x)
+
+
+def test_fp64_and_mixed_solves_agree(ibim, globule, gpu):
+    import torch
+    xyz, qa, s = globule
+    m = len(qa)
+    d_atoms = torch.from_numpy(np.ascontiguousarray(xyz)).to(gpu)
+    d_q = torch.from_numpy(np.ascontiguousarray(qa)).to(gpu)
+    d_b = torch.empty(2 * s.n, dtype=torch.float64, device=gpu)
+    d_x = torch.empty_like(d_b)
+    d_phi = torch.empty(m, dtype=torch.float64, device=gpu)
+    torch.cuda.synchronize()
+    Pt = lambda v: C.c_void_p(v.data_ptr())
+    out = {}
+    for name, prec in (("fp64", ibim.PREC_FP64), ("mixed16", ibim.PREC_MIXED_F16)):
+        with ibim.Plan(device_ptrs=s.device_ptrs(), precision=prec, **PB) as plan:
+            plan.set_trace(True)
+            plan.rhs_device(m, Pt(d_atoms), Pt(d_q), Pt(d_b))
+            d_x.zero_()
+            torch.cuda.synchronize()
+            t = time.time()
+            rc, info = plan.gmres_device(Pt(d_b), Pt(d_x), restart=40, maxit=2000, tol=1e-8)
+            plan.polarize_device(m, Pt(d_atoms), Pt(d_x), Pt(d_phi))
+            plan.sync()
+            e = 0.5 * float(torch.dot(d_phi, d_q).item())
+            say(f"{name}: rc {rc}, {info}, energy {e:.12e}, {time.time() - t:.1f} s, "
+                f"footprint {plan.footprint() / 2**30:.1f} GiB")
+            assert rc == 0 and info["rel_residual"] < 1e-8
+            out[name] = e
+            if name == "mixed16":
+                y1 = torch.empty_like(d_b)
+                y2 = torch.empty_like(d_b)
+                plan.matvec_device(Pt(d_x), Pt(y1))
+                plan.matvec_device(Pt(d_x), Pt(y2))
+                plan.sync()
+                assert torch.equal(y1, y2)
+    rel = abs(out["mixed16"] - out["fp64"]) / abs(out["fp64"])
+    say(f"energy: FP64 {out['fp64']:.12e}, mixed {out['mixed16']:.12e}, relative difference {rel:.2e}")
+    assert rel < 1e-6
