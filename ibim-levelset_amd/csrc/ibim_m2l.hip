@@ -826,6 +826,144 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW >= 8
     }
 }
 
+// The FP16 V-list GEMM for p >= 7, where a pair's results (4 p^3 >= 1372
+// rows) are too many to keep per pair as the p <= 6 stage does (FP24 rows of
+// 61.5M pairs at p = 8 would take 378 GB).  Same operands: the unfolded
+// operators of the (class, mask) keys with their row exponents, and the
+// per-box scaled FP16 multipoles.  A block computes kHtRows operator rows x
+// kHtPairs pairs of one key over all k; both operands stream through an LDS
+// ring of NST chunks of KC k, filled by direct global-to-LDS loads NST - 1
+// chunks ahead (the pairs' multipole rows gathered by source box), segments
+// XOR-swizzled by row as in k_m2l_gemm_hu.  Eight waves of 64 rows x 64
+// pairs.  The MFMA takes the pairs as its first operand and the operator
+// rows as its second, so result register j of lane l holds operator row
+// l & 31 of pair 8 (j >> 2) + 4 (l >> 5) + (j & 3): the results, scaled by
+// 2^(esig + ebox), are added into L in FP64 by atomics, each half-wave
+// covering 32 consecutive elements of one box.  Tiles are ordered (key,
+// pair tile, row tile), so the row tiles of a pair tile run side by side and
+// share its multipole rows in L2.
+constexpr int kHtRows = 128, kHtPairs = 256;
+
+template <int P>
+__global__ __launch_bounds__(512) void k_m2l_gemm_ht(
+    const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
+    const int *__restrict__ pA, const int *__restrict__ pB, const int *__restrict__ keyslot,
+    const _Float16 *__restrict__ ops16u, const int *__restrict__ esig, int MRP, const _Float16 *__restrict__ Mh,
+    const int *__restrict__ ebox, double *__restrict__ L, int first, int count, int per_xcd) {
+    constexpr int R = P * P * P;
+    constexpr int R4 = 4 * R;
+    constexpr int KP = (R4 + 63) / 64 * 64;   // operator columns and multipole row length
+    constexpr int KC = 64;                    // k per chunk
+    constexpr int NKC = KP / KC;
+    constexpr int SEGS = KC / 8;              // 16-byte segments per chunk row
+    constexpr int RPI = 64 / SEGS;            // chunk rows per LDS-load instruction
+    constexpr int DA = kHtRows / RPI / 8;     // operator LDS-load instructions per wave per chunk
+    constexpr int DB = kHtPairs / RPI / 8;    // multipole LDS-load instructions per wave per chunk
+    constexpr int NST = 3;                    // chunks in the ring
+    static_assert(KP % KC == 0 && DA >= 1 && DB >= 1 && R4 % 4 == 0 && NKC >= NST, "tile shape");
+    __shared__ __attribute__((aligned(1024))) _Float16 sA[NST][kHtRows * KC];
+    __shared__ __attribute__((aligned(1024))) _Float16 sB[NST][kHtPairs * KC];
+    __shared__ int sSrc[kHtPairs], sDst[kHtPairs], sEb[kHtPairs];
+    __shared__ int sE[kHtRows];
+    const int local = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (local >= count) return;
+    const int logical = first + local;
+    const int nrt = MRP / kHtRows;
+    const int tile = logical / nrt, rt = logical - tile * nrt;
+    const int key = blk_key[tile];
+    const int j0 = blk_j0[tile];
+    const int jend = key_off[key + 1];
+    const int sl = keyslot[key];
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    if (tid < kHtPairs) {
+        const int j = j0 + tid;
+        const bool ok = j < jend;
+        const int a = ok ? pA[j] : 0;  // (a pair-less lane reads box 0's row; its results are dropped)
+        sSrc[tid] = a;
+        sDst[tid] = ok ? pB[j] : -1;
+        sEb[tid] = ok ? ebox[a] : 0;
+    } else if (tid < kHtPairs + kHtRows) {
+        sE[tid - kHtPairs] = esig[(size_t)sl * MRP + rt * kHtRows + (tid - kHtPairs)];
+    }
+    __syncthreads();
+    const _Float16 *Aop = ops16u + ((size_t)sl * MRP + (size_t)rt * kHtRows) * KP;
+    // chunk c into ring slot c % NST: instruction i of wave w fills rows
+    // RPI (D w + i) .. + RPI - 1 of the slot; lane l's 16 B land at row
+    // r = RPI (D w + i) + l / SEGS, position l % SEGS, and hold segment
+    // (l % SEGS) ^ (r % SEGS) of that row
+    auto issue = [&](int c) {
+        _Float16 *da = sA[c % NST], *db = sB[c % NST];
+#pragma unroll
+        for (int i = 0; i < DA; ++i) {
+            const int r = RPI * (DA * w + i) + l / SEGS;
+            const int sg = (l % SEGS) ^ (r % SEGS);
+            lds_load16(Aop + (size_t)r * KP + c * KC + 8 * sg, (lds_void *)(da + (DA * w + i) * 512));
+        }
+#pragma unroll
+        for (int i = 0; i < DB; ++i) {
+            const int q = RPI * (DB * w + i) + l / SEGS;
+            const int sg = (l % SEGS) ^ (q % SEGS);
+            lds_load16(Mh + (size_t)sSrc[q] * KP + c * KC + 8 * sg, (lds_void *)(db + (DB * w + i) * 512));
+        }
+    };
+#pragma unroll
+    for (int c = 0; c < NST - 1; ++c) issue(c);
+    const int wr = (w & 1) * 64;   // this wave's operator rows in the tile
+    const int wp = (w >> 1) * 64;  // this wave's pairs in the tile
+    f16v acc[2][2];                // [pair group][row group]
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = f16v{};
+    for (int c = 0; c < NKC; ++c) {
+        // this wave's loads of chunk c are done once at most those of chunks
+        // c + 1 .. c + NST - 2 are outstanding; the barrier then publishes
+        // every wave's part and retires the reads of chunk c - 1, whose slot
+        // the next issue refills
+        if (c + NST - 2 < NKC) wait_vm<(DA + DB) * (NST - 2)>();
+        else wait_vm<0>();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        if (c + NST - 1 < NKC) issue(c + NST - 1);
+        const _Float16 *sa = sA[c % NST], *sb = sB[c % NST];
+#pragma unroll
+        for (int ks = 0; ks < KC / 16; ++ks) {
+            const int seg = 2 * ks + (l >> 5);
+            h8 fp[2], fr[2];
+#pragma unroll
+            for (int g = 0; g < 2; ++g) {
+                const int q = wp + 32 * g + (l & 31);
+                fp[g] = *(const h8 *)(sb + q * KC + 8 * (seg ^ (q % SEGS)));
+                const int r = wr + 32 * g + (l & 31);
+                fr[g] = *(const h8 *)(sa + r * KC + 8 * (seg ^ (r % SEGS)));
+            }
+#pragma unroll
+            for (int pg = 0; pg < 2; ++pg)
+#pragma unroll
+                for (int rg = 0; rg < 2; ++rg)
+                    acc[pg][rg] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fp[pg], fr[rg], acc[pg][rg], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int rg = 0; rg < 2; ++rg) {
+        const int rl = wr + 32 * rg + (l & 31);  // row in the tile
+        const int row = rt * kHtRows + rl;
+        if (row >= R4) continue;                 // padding rows
+        const int er = sE[rl];
+#pragma unroll
+        for (int pg = 0; pg < 2; ++pg)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int q = wp + 32 * pg + 8 * (j >> 2) + 4 * (l >> 5) + (j & 3);
+                const int b = sDst[q];
+                if (b < 0) continue;
+                unsafeAtomicAdd(&L[(size_t)b * R4 + row], ldexp((double)acc[pg][rg][j], er + sEb[q]));
+            }
+    }
+}
+
 // X pair j: its source multipole reflected by the pair's mask (the gather
 // table's permutation and input signs), so that the class operator applies
 // unreflected; and the source box's exponent
@@ -1008,6 +1146,28 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
             return IBIM_OK;
         }
     }
+    if constexpr (P >= 7) {
+        if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->f16_ready && &vp == &pl->vp) {
+            constexpr int KP = (R4 + 63) / 64 * 64;
+            k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, KP, pl->Mh, pl->ebox);
+            // tiles of 512 threads, at most 2^22 per launch (32-bit work-item count)
+            const int64_t total = (int64_t)vp.hnblocks * (pl->MRP / kHtRows);
+            if (total >= (int64_t(1) << 31)) {
+                set_error("FP16 M2L: too many pair tiles for one plan");
+                return IBIM_ERR_STATE;
+            }
+            constexpr int64_t chunk = int64_t(1) << 22;
+            for (int64_t first = 0; first < total; first += chunk) {
+                const int count = (int)std::min(chunk, total - first);
+                const int per_xcd = (count + 7) / 8;
+                k_m2l_gemm_ht<P><<<8 * per_xcd, 512, 0, st>>>(vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A, vp.B,
+                                                              pl->keyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
+                                                              pl->ebox, pl->L, (int)first, count, per_xcd);
+            }
+            IBIM_LAUNCH_CHECK();
+            return IBIM_OK;
+        }
+    }
     if (mixed && pl->ops32) return launch_gemm<P, float>(vp, pl, pl->ops32, st);
     return launch_gemm<P, double>(vp, pl, pl->ops, st);
 }
@@ -1154,9 +1314,9 @@ int build_vpairs(ibim_plan *pl, const unsigned char *need, VPairs &vp, std::vect
         IBIM_HIP_CHECK(hipMemcpyAsync(vp.blk_j0, bj.data(), bj.size() * sizeof(int), hipMemcpyHostToDevice, st));
     }
     // FP16 GEMM block table (blocks of 32 kH32WavesV pairs for p <= 5, 64
-    // for p = 6)
-    if (pl->p <= 6) {
-        const int hnp = pl->p <= 5 ? 32 * kH32WavesV : 64;
+    // for p = 6, kHtPairs above)
+    {
+        const int hnp = pl->p <= 5 ? 32 * kH32WavesV : (pl->p == 6 ? 64 : kHtPairs);
         std::vector<int> hc, hj;
         for (int c = 0; c < nkeys; ++c)
             for (int j = vp.cls_off[c]; j < vp.cls_off[c + 1]; j += hnp) {
@@ -1474,13 +1634,19 @@ static int build_xops_f32(ibim_plan *pl);
 
 int build_m2l_operators_f16(ibim_plan *pl) {
     if (pl->f16_ready || !pl->ops || pl->n_cls == 0) return IBIM_OK;
-    if (pl->p > 6) return build_m2l_operators_f32(pl);  // the FP16 GEMM's LDS tiles stop at p = 6
     hipStream_t st = pl->stream;
     const int R4 = 4 * pl->r;
     const int KP = (R4 + 63) / 64 * 64;
     const int YS = (R4 + 31) / 32 * 32;
+    // p >= 7: the V list on k_m2l_gemm_ht, whose results go into L directly
+    // (no pair rows, no per-target lists); the X list stays on the FP32 GEMM
+    const bool large = pl->p > 6;
+    if (large && (pl->MRP % kHtRows != 0 || KP != pl->KRP)) {
+        set_error("FP16 M2L: operator shape does not fit the p >= 7 tiles");
+        return IBIM_ERR_STATE;
+    }
     const VPairs &vp = pl->vp, &xp = pl->xp;
-    const bool with_x = xp.npairs > 0 && pl->xops && xp.hnblocks > 0;
+    const bool with_x = !large && xp.npairs > 0 && pl->xops && xp.hnblocks > 0;
     const int npv = vp.npairs, npx = with_x ? xp.npairs : 0, npa = npv + npx;
     ScratchScope scope(pl);  // temporaries below
     auto salloc = [&](auto **p, size_t n) -> int {
@@ -1496,11 +1662,13 @@ int build_m2l_operators_f16(ibim_plan *pl) {
         set_error("FP16 M2L: more than 2^31 - 2 V and X interactions");
         return IBIM_ERR_ARG;
     }
-    if ((rc = dalloc(pl, &pl->Mh, (size_t)pl->nnodes * KP)) || (rc = dalloc(pl, &pl->ebox, (size_t)pl->nnodes)) ||
-        (rc = dalloc(pl, &pl->Yv, fp24_buffer_dwords((int64_t)npa + 1, YS))))
+    if ((rc = dalloc(pl, &pl->Mh, (size_t)pl->nnodes * KP)) || (rc = dalloc(pl, &pl->ebox, (size_t)pl->nnodes)))
         return rc;
-    pl->yv_dwords = fp24_buffer_dwords((int64_t)npa + 1, YS);
-    pl->y_rows = npa;
+    if (!large) {
+        if ((rc = dalloc(pl, &pl->Yv, fp24_buffer_dwords((int64_t)npa + 1, YS)))) return rc;
+        pl->yv_dwords = fp24_buffer_dwords((int64_t)npa + 1, YS);
+        pl->y_rows = npa;
+    }
     // unfolded operators for the (class, mask) keys that have pairs: V keys
     // in slots [0, nsv), X keys in [nsv, nsv + nsx)
     auto slots = [](const std::vector<int> &off, int first, std::vector<int> &keyslot, std::vector<int> &slot_key) {
@@ -1567,6 +1735,13 @@ int build_m2l_operators_f16(ibim_plan *pl) {
             return rc;
         k_iota_pairs<<<nblk(npx), 256, 0, st>>>(npx, pl->xiota);
         IBIM_LAUNCH_CHECK();
+    }
+    if (large) {
+        IBIM_HIP_CHECK(hipStreamSynchronize(st));
+        pl->x16 = false;
+        if ((rc = build_xops_f32(pl))) return rc;
+        pl->f16_ready = true;
+        return IBIM_OK;
     }
     // rows of Yv per target node: a stable sort of the target boxes of the
     // V pairs followed by the X pairs

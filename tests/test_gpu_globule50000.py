@@ -9,8 +9,9 @@ kappa = 0.1257 / A, eps 2 / 80.
 * A = I exactly at eps_in = eps_out, kappa = 0.
 * The FP64 solve and the FP16-mixed solve (FP16 V-list M2L with per-box
   scaling, FP32 near field, FP64 refinement) both reach an FP64 relative
-  residual below 1e-8, and their solvation energies agree within 1e-6; the
-  mixed matvec repeats bit for bit.
+  residual below 1e-8, and their solvation energies agree within 1e-6.  (At
+  p = 8 the FP16 stage adds its results into L by atomics, so two mixed
+  matvecs agree to rounding, not bit for bit.)
 
 About 10 minutes of GPU time, so it runs only when IBIM_GLOBULE50000=1; GMRES
 prints its iteration table (run pytest with -s to see it).
@@ -104,7 +105,9 @@ def test_fp64_and_mixed_solves_agree(ibim, globule, gpu):
                 plan.matvec_device(Pt(d_x), Pt(y1))
                 plan.matvec_device(Pt(d_x), Pt(y2))
                 plan.sync()
-                assert torch.equal(y1, y2)
+                diff = float(torch.linalg.norm(y1 - y2) / torch.linalg.norm(y1))
+                say(f"two mixed matvecs differ by {diff:.1e} (atomic summation order)")
+                assert diff < 1e-12
     rel = abs(out["mixed16"] - out["fp64"]) / abs(out["fp64"])
     say(f"energy: FP64 {out['fp64']:.12e}, mixed {out['mixed16']:.12e}, relative difference {rel:.2e}")
     assert rel < 1e-6

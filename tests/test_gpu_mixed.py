@@ -28,9 +28,9 @@ def sphere_case(ibim, gpu):
 MATVEC_BAR = {"mixed": 1e-5, "mixed16": 2e-5}
 
 
-# p = 5 runs the 32x32 FP16 GEMM, p = 6 the 16x16 one, p = 8 the FP32 GEMM
-# (the FP16 tiles stop at p = 6), p = 3 the smallest tiles
-@pytest.mark.parametrize("p", [3, 5, 6, 8])
+# p = 5 runs the 32x32 FP16 GEMM, p = 6 the 16x16 one, p = 7 and 8 the
+# tiled FP16 GEMM that adds its results into L, p = 3 the smallest tiles
+@pytest.mark.parametrize("p", [3, 5, 6, 7, 8])
 @pytest.mark.parametrize("mode", ["mixed", "mixed16"])
 def test_mixed_matvec_close_to_fp64(ibim, sphere_case, mode, p):
     pts, nrm, w = sphere_case
@@ -48,16 +48,16 @@ def test_mixed_matvec_close_to_fp64(ibim, sphere_case, mode, p):
     assert err > 0  # the mixed path really ran
 
 
-@pytest.mark.parametrize("mode", ["mixed", "mixed16"])
+@pytest.mark.parametrize("mode,p", [("mixed", 5), ("mixed16", 5), ("mixed16", 8)])
 @pytest.mark.parametrize("kappa", [0.0, 0.1257])
-def test_mixed_solve_energy_matches_fp64(ibim, sphere_case, kappa, mode):
+def test_mixed_solve_energy_matches_fp64(ibim, sphere_case, kappa, mode, p):
     pts, nrm, w = sphere_case
     atoms = np.array([[0.0, 0.0, 0.0], [0.4, -0.3, 0.2]])
     qa = np.array([1.0, -0.5])
     low = {"mixed": ibim.PREC_MIXED, "mixed16": ibim.PREC_MIXED_F16}[mode]
     out = {}
     for prec in (ibim.PREC_FP64, low):
-        with ibim.Plan(pts, nrm, w, p=5, max_points=64, vacant_radius=0.125, precision=prec, kappa=kappa,
+        with ibim.Plan(pts, nrm, w, p=p, max_points=64, vacant_radius=0.125, precision=prec, kappa=kappa,
                        eps_in=1.0, eps_out=80.0) as plan:
             b = plan.rhs(atoms, qa)
             x, rc, info = plan.gmres(b, restart=20, maxit=300, tol=1e-8)
@@ -68,7 +68,7 @@ def test_mixed_solve_energy_matches_fp64(ibim, sphere_case, kappa, mode):
             phi = plan.polarize(atoms, x)
         out[prec] = (x, 0.5 * float(phi @ qa), info, res)
     (x64, e64, i64, r64), (x32, e32, i32, r32) = out[ibim.PREC_FP64], out[low]
-    print(f"{mode} kappa={kappa}: E64={e64:.10e} Emix={e32:.10e} its {i64['iterations']} / {i32['iterations']} "
+    print(f"{mode} p={p} kappa={kappa}: E64={e64:.10e} Emix={e32:.10e} its {i64['iterations']} / {i32['iterations']} "
           f"outer {i32['outer_steps']} res {r64:.1e} / {r32:.1e}")
     assert r64 < 1e-8 and r32 < 1e-8
     assert i32["outer_steps"] >= 1
