@@ -1014,7 +1014,7 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
         pl->defer_reduce = false;
         if (rc) return rc;
         // (the FP16 stage also evaluates the X list)
-        const bool x_here = mixed && P <= 6 && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->f16_ready && pl->x16;
+        const bool x_here = mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->f16_ready && pl->x16;
         sc.add(0, 32.0 * R * R * ((double)ts.vpairs->npairs + (x_here ? pl->xp.npairs : 0)), 0);
     }
     if (fork) {

@@ -396,7 +396,7 @@ struct VPairs {
     int nblocks = 0;                    // GEMM blocks of 64 pairs
     int *blk_cls = nullptr, *blk_j0 = nullptr;
     int hnblocks = 0;                   // FP16 GEMM blocks (256 V / 128 X pairs for p <= 5, 64 for p = 6,
-                                        // 256 V pairs for p >= 7)
+                                        // 256 for p >= 7)
     int *hblk_cls = nullptr, *hblk_j0 = nullptr;
     int *hcls_off_d = nullptr;          // X list: pairs of class c in [hcls_off_d[c], hcls_off_d[c+1]); the FP16
                                         // blocks of the X list span whole classes (all 8 masks)

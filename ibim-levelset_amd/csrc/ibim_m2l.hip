@@ -841,15 +841,20 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW >= 8
 // 2^(esig + ebox), are added into L in FP64 by atomics, each half-wave
 // covering 32 consecutive elements of one box.  Tiles are ordered (key,
 // pair tile, row tile), so the row tiles of a pair tile run side by side and
-// share its multipole rows in L2.
+// share its multipole rows in L2.  REFLECT (the X list): keys are classes,
+// the operators unreflected, the multipole rows those of k_x_reflect_half
+// (one per pair), and result row r goes to element s(r) of the target with
+// the sign of stab[mask * R4 + r] (the reflection is its own inverse and
+// keeps each field's block).
 constexpr int kHtRows = 128, kHtPairs = 256;
 
-template <int P>
+template <int P, bool REFLECT>
 __global__ __launch_bounds__(512) void k_m2l_gemm_ht(
     const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
     const int *__restrict__ pA, const int *__restrict__ pB, const int *__restrict__ keyslot,
     const _Float16 *__restrict__ ops16u, const int *__restrict__ esig, int MRP, const _Float16 *__restrict__ Mh,
-    const int *__restrict__ ebox, double *__restrict__ L, int first, int count, int per_xcd) {
+    const int *__restrict__ ebox, const unsigned char *__restrict__ pmask, const int *__restrict__ stab,
+    double *__restrict__ L, int first, int count, int per_xcd) {
     constexpr int R = P * P * P;
     constexpr int R4 = 4 * R;
     constexpr int KP = (R4 + 63) / 64 * 64;   // operator columns and multipole row length
@@ -864,6 +869,7 @@ __global__ __launch_bounds__(512) void k_m2l_gemm_ht(
     __shared__ __attribute__((aligned(1024))) _Float16 sA[NST][kHtRows * KC];
     __shared__ __attribute__((aligned(1024))) _Float16 sB[NST][kHtPairs * KC];
     __shared__ int sSrc[kHtPairs], sDst[kHtPairs], sEb[kHtPairs];
+    __shared__ int sMask[REFLECT ? kHtPairs : 1];
     __shared__ int sE[kHtRows];
     const int local = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (local >= count) return;
@@ -882,6 +888,7 @@ __global__ __launch_bounds__(512) void k_m2l_gemm_ht(
         sSrc[tid] = a;
         sDst[tid] = ok ? pB[j] : -1;
         sEb[tid] = ok ? ebox[a] : 0;
+        if constexpr (REFLECT) sMask[tid] = ok ? pmask[j] : 0;
     } else if (tid < kHtPairs + kHtRows) {
         sE[tid - kHtPairs] = esig[(size_t)sl * MRP + rt * kHtRows + (tid - kHtPairs)];
     }
@@ -959,7 +966,13 @@ __global__ __launch_bounds__(512) void k_m2l_gemm_ht(
                 const int q = wp + 32 * pg + 8 * (j >> 2) + 4 * (l >> 5) + (j & 3);
                 const int b = sDst[q];
                 if (b < 0) continue;
-                unsafeAtomicAdd(&L[(size_t)b * R4 + row], ldexp((double)acc[pg][rg][j], er + sEb[q]));
+                const double v = ldexp((double)acc[pg][rg][j], er + sEb[q]);
+                if constexpr (REFLECT) {
+                    const int sr = stab[sMask[q] * R4 + row];
+                    unsafeAtomicAdd(&L[(size_t)b * R4 + (sr & 0x7fffffff)], sr < 0 ? -v : v);
+                } else {
+                    unsafeAtomicAdd(&L[(size_t)b * R4 + row], v);
+                }
             }
     }
 }
@@ -1150,20 +1163,33 @@ int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
         if (mixed && pl->prm.precision == IBIM_PREC_MIXED_F16 && pl->f16_ready && &vp == &pl->vp) {
             constexpr int KP = (R4 + 63) / 64 * 64;
             k_m_to_half<<<pl->nnodes, 64, 0, st>>>(pl->M, R4, KP, pl->Mh, pl->ebox);
-            // tiles of 512 threads, at most 2^22 per launch (32-bit work-item count)
-            const int64_t total = (int64_t)vp.hnblocks * (pl->MRP / kHtRows);
-            if (total >= (int64_t(1) << 31)) {
-                set_error("FP16 M2L: too many pair tiles for one plan");
-                return IBIM_ERR_STATE;
+            // V pairs, then (x16) the X pairs; tiles of 512 threads, at most
+            // 2^22 per launch (32-bit work-item count)
+            auto launch = [&](auto kern, const VPairs &pp, const int *key_off, const int *pA, const int *keyslot,
+                              const _Float16 *B, const int *eb) {
+                const int64_t total = (int64_t)pp.hnblocks * (pl->MRP / kHtRows);
+                if (total >= (int64_t(1) << 31)) {
+                    set_error("FP16 M2L: too many pair tiles for one plan");
+                    return IBIM_ERR_STATE;
+                }
+                constexpr int64_t chunk = int64_t(1) << 22;
+                for (int64_t first = 0; first < total; first += chunk) {
+                    const int count = (int)std::min(chunk, total - first);
+                    const int per_xcd = (count + 7) / 8;
+                    kern<<<8 * per_xcd, 512, 0, st>>>(pp.hblk_cls, pp.hblk_j0, key_off, pA, pp.B, keyslot, pl->ops16,
+                                                      pl->esig, pl->MRP, B, eb, pp.mask, pl->stab, pl->L, (int)first,
+                                                      count, per_xcd);
+                }
+                return IBIM_OK;
+            };
+            int rc = launch(k_m2l_gemm_ht<P, false>, vp, vp.cls_off_d, vp.A, pl->keyslot, pl->Mh, pl->ebox);
+            const VPairs &xp = pl->xp;
+            if (!rc && pl->x16 && xp.hnblocks > 0) {
+                k_x_reflect_half<<<xp.npairs, 64, 0, st>>>(xp.A, xp.mask, pl->gtab, R4, KP, pl->Mh, pl->ebox, pl->Bx,
+                                                           pl->ebx);
+                rc = launch(k_m2l_gemm_ht<P, true>, xp, xp.hcls_off_d, pl->xiota, pl->xkeyslot, pl->Bx, pl->ebx);
             }
-            constexpr int64_t chunk = int64_t(1) << 22;
-            for (int64_t first = 0; first < total; first += chunk) {
-                const int count = (int)std::min(chunk, total - first);
-                const int per_xcd = (count + 7) / 8;
-                k_m2l_gemm_ht<P><<<8 * per_xcd, 512, 0, st>>>(vp.hblk_cls, vp.hblk_j0, vp.cls_off_d, vp.A, vp.B,
-                                                              pl->keyslot, pl->ops16, pl->esig, pl->MRP, pl->Mh,
-                                                              pl->ebox, pl->L, (int)first, count, per_xcd);
-            }
+            if (rc) return rc;
             IBIM_LAUNCH_CHECK();
             return IBIM_OK;
         }
@@ -1531,8 +1557,8 @@ int build_xpairs(ibim_plan *pl) {
     // the class operator
     std::vector<int> hc, hj, hco(nc + 1);
     for (int c = 0; c <= nc; ++c) hco[c] = xp.cls_off[8 * c];
-    if (pl->p <= 6) {
-        const int hnp = pl->p <= 5 ? 32 * kH32WavesX : 64;
+    {
+        const int hnp = pl->p <= 5 ? 32 * kH32WavesX : (pl->p == 6 ? 64 : kHtPairs);
         for (int c = 0; c < nc; ++c)
             for (int j = hco[c]; j < hco[c + 1]; j += hnp) {
                 hc.push_back(c);
@@ -1638,15 +1664,15 @@ int build_m2l_operators_f16(ibim_plan *pl) {
     const int R4 = 4 * pl->r;
     const int KP = (R4 + 63) / 64 * 64;
     const int YS = (R4 + 31) / 32 * 32;
-    // p >= 7: the V list on k_m2l_gemm_ht, whose results go into L directly
-    // (no pair rows, no per-target lists); the X list stays on the FP32 GEMM
+    // p >= 7: both lists on k_m2l_gemm_ht, whose results go into L directly
+    // (no pair rows, no per-target lists)
     const bool large = pl->p > 6;
     if (large && (pl->MRP % kHtRows != 0 || KP != pl->KRP)) {
         set_error("FP16 M2L: operator shape does not fit the p >= 7 tiles");
         return IBIM_ERR_STATE;
     }
     const VPairs &vp = pl->vp, &xp = pl->xp;
-    const bool with_x = !large && xp.npairs > 0 && pl->xops && xp.hnblocks > 0;
+    const bool with_x = xp.npairs > 0 && pl->xops && xp.hnblocks > 0;
     const int npv = vp.npairs, npx = with_x ? xp.npairs : 0, npa = npv + npx;
     ScratchScope scope(pl);  // temporaries below
     auto salloc = [&](auto **p, size_t n) -> int {
@@ -1738,8 +1764,8 @@ int build_m2l_operators_f16(ibim_plan *pl) {
     }
     if (large) {
         IBIM_HIP_CHECK(hipStreamSynchronize(st));
-        pl->x16 = false;
-        if ((rc = build_xops_f32(pl))) return rc;
+        pl->x16 = nsx > 0;
+        if (!pl->x16 && (rc = build_xops_f32(pl))) return rc;
         pl->f16_ready = true;
         return IBIM_OK;
     }

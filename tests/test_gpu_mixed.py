@@ -76,6 +76,27 @@ def test_mixed_solve_energy_matches_fp64(ibim, sphere_case, kappa, mode, p):
     assert rel(x32, x64) < 1e-6
 
 
+@pytest.mark.parametrize("p", [7, 8])
+def test_mixed16_large_p_with_x_list(ibim, sphere_case, p):
+    """p >= 7: the FP16 stage adds both lists into L, the X list through
+    reflected multipoles and reflected result rows; a wrong reflection or
+    sign would put the X list's share of the field (percent level) in the
+    error."""
+    pts, nrm, w = sphere_case
+    x = np.random.default_rng(4).uniform(-1, 1, 2 * len(pts))
+    with ibim.Plan(pts, nrm, w, p=p, max_points=32, vacant_radius=0.125, **PB) as plan:
+        y64 = plan.matvec(x)
+        plan.set_precision(ibim.PREC_MIXED_F16)
+        a = plan.matvec(x)
+        b = plan.matvec(x)
+        st = plan.stats()
+    print(f"p={p}: {st['v_pairs']} V and {st['x_pairs']} X interactions, rel err vs FP64 {rel(a, y64):.2e}, "
+          f"repeat {rel(a, b):.1e}")
+    assert st["x_pairs"] > 0
+    assert rel(a, y64) < MATVEC_BAR["mixed16"]
+    assert rel(a, b) < 1e-14
+
+
 @pytest.mark.parametrize("p", [5, 6])
 def test_mixed16_matvec_repeatable(ibim, sphere_case, p):
     """The FP16 form's matvec runs its near field on a second stream beside
