@@ -45,9 +45,11 @@ the stream's first molecule, scaled to the stream; and one complete Born
 ion solve (the 64^3 sphere, N = 12,790) end to end, with its energy against
 the GPU's.
 
-`--config globule500` / `globule5000` time single solves of one molecule
-instead (h = 0.25 A), FP64 and both mixed forms.  `--matvecs-only K` is the
-counter-collection mode used for the HBM traffic figures.
+`--config globule500` / `globule5000` / `globule50000` time single solves
+of one molecule instead (h = 0.25, 0.25 and 0.3 A; p = 5, 6 and 8), FP64 and
+both mixed forms.  `--matvecs-only K` runs K matvecs in each form (the
+counter-collection mode used for the HBM traffic figures, and the kernel
+profile of the 50,000-atom globule).
 """
 import argparse
 import json
@@ -73,6 +75,7 @@ CONFIGS = {
     # single molecules (not the default workload)
     "globule500": dict(FMM, atoms=500, seed=500, confine=20.0, h=0.25),
     "globule5000": dict(FMM, atoms=5000, seed=5000, confine=45.0, h=0.25, p=6, leaf=256),
+    "globule50000": dict(FMM, atoms=50000, seed=50000, confine=100.0, h=0.3, p=8, leaf=512),
 }
 
 
