@@ -188,6 +188,13 @@ def lib():
     L.ibim_server_destroy.argtypes = [vp]
     L.ibim_server_destroy.restype = None
     L.ibim_kernel_fmm_targets.argtypes = [vp, C.c_int, i64, d, d, d]
+    L.ibim_group_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, pp]
+    L.ibim_group_matvec_device.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
+    L.ibim_group_stage_report.argtypes = [vp, C.POINTER(StageReport), C.c_int]
+    L.ibim_group_destroy.argtypes = [vp]
+    L.ibim_group_destroy.restype = None
+    L.ibim_server_set_fusion.argtypes = [vp, i64, C.c_int]
+    L.ibim_server_fusion_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.ibim_last_error.restype = C.c_char_p
     L.ibim_version.restype = C.c_char_p
     _lib = L
@@ -475,14 +482,71 @@ def synthetic_globule(m, seed, confine_radius):
 
 
 # ------------------------------------------------------------- serving --
+class Group:
+    """A fused matvec over several plans (ibim_group_*): one launch per FMM
+    stage for all members.  mixed: each member's mixed-precision matvec,
+    else FP64."""
+
+    def __init__(self, plans, mixed=False):
+        self._plans = list(plans)  # (kept alive: the group must go first)
+        hs = (C.c_void_p * len(self._plans))(*[pl.handle for pl in self._plans])
+        self._h = C.c_void_p()
+        _check(lib().ibim_group_create(hs, len(self._plans), int(bool(mixed)), C.byref(self._h)),
+               "ibim_group_create")
+
+    def matvec_device(self, d_x, d_y):
+        """d_x, d_y: one device pointer (int or c_void_p) per member."""
+        k = len(self._plans)
+        assert len(d_x) == k and len(d_y) == k
+        xs = (C.c_void_p * k)(*[C.c_void_p(int(getattr(v, "value", v))) for v in d_x])
+        ys = (C.c_void_p * k)(*[C.c_void_p(int(getattr(v, "value", v))) for v in d_y])
+        _check(lib().ibim_group_matvec_device(self._h, xs, ys), "ibim_group_matvec_device")
+
+    def stage_report(self, reset=True):
+        r = StageReport()
+        _check(lib().ibim_group_stage_report(self._h, C.byref(r), int(reset)), "ibim_group_stage_report")
+        return r.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().ibim_group_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 class Server:
     """Continuous batching of PB solves: `slots` concurrent requests, admission
-    by estimated device footprint within `mem_budget` bytes (0 = 90% of free)."""
+    by estimated device footprint within `mem_budget` bytes (0 = 90% of free).
+    fuse_points > 0: solves of at most that many tube points share fused
+    matvec launches (ibim_server_set_fusion), waiting up to fuse_wait_us for
+    partners."""
 
-    def __init__(self, slots=4, mem_budget=0):
+    def __init__(self, slots=4, mem_budget=0, fuse_points=0, fuse_wait_us=2000):
         self._h = C.c_void_p()
         _check(lib().ibim_server_create(slots, mem_budget, C.byref(self._h)), "ibim_server_create")
         self._keep = {}
+        if fuse_points:
+            self.set_fusion(fuse_points, fuse_wait_us)
+
+    def set_fusion(self, max_points, wait_us=2000):
+        _check(lib().ibim_server_set_fusion(self._h, int(max_points), int(wait_us)), "ibim_server_set_fusion")
+
+    def fusion_stats(self):
+        """(fused group matvecs, member matvecs they ran) so far."""
+        a, b = C.c_int64(), C.c_int64()
+        _check(lib().ibim_server_fusion_stats(self._h, C.byref(a), C.byref(b)), "ibim_server_fusion_stats")
+        return int(a.value), int(b.value)
 
     def submit(self, atom_xyz, radii, charges, h=0.35, thickness=2, probe=1.4, restart=40, maxit=2000, tol=1e-8,
                **fmm):

@@ -17,8 +17,51 @@ int polarize_device(ibim_plan *pl, int m, const double *d_axyz, const double *d_
 int polarize_noalloc(ibim_plan *pl, int m, const double *d_axyz, const double *d_x, double *d_out);
 int target_fmm_device(ibim_plan *pl, int kid, int m, const double *d_axyz, const double *d_in, double *d_out);
 
+int timer_collect(StageTimer &t) {
+    for (auto &r : t.pending) {
+        IBIM_HIP_CHECK(hipEventSynchronize(r.b));
+        float el = 0.f;
+        IBIM_HIP_CHECK(hipEventElapsedTime(&el, r.a, r.b));
+        t.ms[r.stage] += el;
+        t.count[r.stage] += 1;
+        t.work[r.stage].valu += r.w.valu;
+        t.work[r.stage].mfma += r.w.mfma;
+        t.work[r.stage].bytes += r.w.bytes;
+        t.pool.push_back(r.a);
+        t.pool.push_back(r.b);
+    }
+    t.pending.clear();
+    return IBIM_OK;
+}
+
+int timer_report(StageTimer &t, ibim_stage_report *out, int reset) {
+    if (int rc = timer_collect(t)) return rc;
+    if (out) {
+        *out = ibim_stage_report{};
+        for (int i = 0; i < IBIM_STAGE_COUNT; ++i) {
+            out->ms[i] = t.ms[i];
+            out->launches[i] = t.count[i];
+            out->valu_flops[i] = t.work[i].valu;
+            out->mfma_flops[i] = t.work[i].mfma;
+            out->bytes[i] = t.work[i].bytes;
+        }
+    }
+    if (reset)
+        for (int i = 0; i < 16; ++i) {
+            t.ms[i] = 0;
+            t.count[i] = 0;
+            t.work[i] = StageWork{};
+        }
+    return IBIM_OK;
+}
+
 // one dmapping (electric.cpp:294-388): charges, fused FMM, combine
 int matvec_device(ibim_plan *pl, const double *d_x, double *d_y, bool mixed) {
+    if (pl->batcher) return batch_matvec(pl->batcher, pl, d_x, d_y, mixed);
+    return matvec_single(pl, d_x, d_y, mixed);
+}
+
+int matvec_single(ibim_plan *pl, const double *d_x, double *d_y, bool mixed) {
     StageScope sc(pl, mixed ? kStMatvecMixed : kStMatvec);
     const StageWork w = matvec_work(pl, mixed);
     sc.add(w.valu, w.mfma, w.bytes);
@@ -394,37 +437,7 @@ int ibim_plan_stage_report(ibim_plan *pl, ibim_stage_report *out, int reset) {
         set_error("plan is NULL");
         return IBIM_ERR_ARG;
     }
-    StageTimer &t = pl->timer;
-    for (auto &r : t.pending) {
-        IBIM_HIP_CHECK(hipEventSynchronize(r.b));
-        float el = 0.f;
-        IBIM_HIP_CHECK(hipEventElapsedTime(&el, r.a, r.b));
-        t.ms[r.stage] += el;
-        t.count[r.stage] += 1;
-        t.work[r.stage].valu += r.w.valu;
-        t.work[r.stage].mfma += r.w.mfma;
-        t.work[r.stage].bytes += r.w.bytes;
-        t.pool.push_back(r.a);
-        t.pool.push_back(r.b);
-    }
-    t.pending.clear();
-    if (out) {
-        *out = ibim_stage_report{};
-        for (int i = 0; i < IBIM_STAGE_COUNT; ++i) {
-            out->ms[i] = t.ms[i];
-            out->launches[i] = t.count[i];
-            out->valu_flops[i] = t.work[i].valu;
-            out->mfma_flops[i] = t.work[i].mfma;
-            out->bytes[i] = t.work[i].bytes;
-        }
-    }
-    if (reset)
-        for (int i = 0; i < 16; ++i) {
-            t.ms[i] = 0;
-            t.count[i] = 0;
-            t.work[i] = StageWork{};
-        }
-    return IBIM_OK;
+    return timer_report(pl->timer, out, reset);
 }
 
 int ibim_plan_stage_times(ibim_plan *pl, double *ms, int64_t *launches, int reset) {
@@ -455,6 +468,50 @@ int ibim_matvec(ibim_plan *pl, const double *x, double *y) {
     return with_host_vectors(pl, n2, x, n2, y, [&](const double *dx, double *dy) {
         return matvec_device(pl, dx, dy, pl->prm.precision != IBIM_PREC_FP64);
     });
+}
+
+struct ibim_group {
+    ibim::MatvecGroup *g = nullptr;
+};
+
+int ibim_group_create(ibim_plan *const *plans, int k, int mixed, ibim_group **out) {
+    if (!out || !plans) {
+        set_error("bad argument to ibim_group_create");
+        return IBIM_ERR_ARG;
+    }
+    *out = nullptr;
+    MatvecGroup *g = nullptr;
+    if (int rc = group_create(plans, k, mixed != 0, &g)) return rc;
+    *out = new ibim_group{g};
+    return IBIM_OK;
+}
+
+int ibim_group_matvec_device(ibim_group *g, const double *const *d_x, double *const *d_y) {
+    if (!g || !d_x || !d_y) {
+        set_error("bad argument to ibim_group_matvec_device");
+        return IBIM_ERR_ARG;
+    }
+    for (int m = 0; m < g->g->k; ++m)
+        if (!d_x[m] || !d_y[m]) {
+            set_error("ibim_group_matvec_device: NULL vector");
+            return IBIM_ERR_ARG;
+        }
+    IBIM_HIP_CHECK(hipSetDevice(g->g->device));
+    return group_matvec_ordered(g->g, d_x, d_y);
+}
+
+int ibim_group_stage_report(ibim_group *g, ibim_stage_report *out, int reset) {
+    if (!g || !out) {
+        set_error("bad argument to ibim_group_stage_report");
+        return IBIM_ERR_ARG;
+    }
+    return timer_report(g->g->timer, out, reset);
+}
+
+void ibim_group_destroy(ibim_group *g) {
+    if (!g) return;
+    group_destroy(g->g);
+    delete g;
 }
 
 int ibim_kernel_fmm(ibim_plan *pl, int kernel_id, const double *charge, double *potential) {

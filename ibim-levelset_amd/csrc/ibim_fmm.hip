@@ -34,13 +34,16 @@ inline int nblk(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
 // One block of BS threads per non-empty leaf, a thread per node (and field
 // set); the leaf's points pass in chunks of BS, whose Chebyshev factors are
 // formed one point per thread.  Each node sums its points in order.
-template <int P, int BS>
+// GROUP: one launch over the leaves of every member of a fused group (the
+// group forms of the kernels below read their member's buffers from its
+// view and take the same steps)
+template <int P, int BS, bool GROUP = false>
 __global__ __launch_bounds__(BS) void k_p2m(const int *__restrict__ leaves, const double *__restrict__ sx,
                                              const double *__restrict__ sy, const double *__restrict__ sz,
                                              const double *__restrict__ Q, int n, const int *__restrict__ sbeg,
                                              const int *__restrict__ send, const double *__restrict__ ctr,
                                              const int *__restrict__ level, const double *__restrict__ lrad,
-                                             ChebTab tab, double *__restrict__ M) {
+                                             ChebTab tab, double *__restrict__ M, GroupLaunch g = {}) {
     constexpr int R = P * P * P;
     constexpr int PER = (R + BS - 1) / BS;
     constexpr int CH = BS;
@@ -48,17 +51,33 @@ __global__ __launch_bounds__(BS) void k_p2m(const int *__restrict__ leaves, cons
     // which an unpadded stride of CH doubles puts in one LDS bank
     __shared__ double sS[3][P][CH + 1];
     __shared__ double sq[4][CH];
-    const int b = leaves[blockIdx.x];
+    int bid = blockIdx.x;
+    if constexpr (GROUP) {
+        const MemberView &v = g.mv[group_member(g, blockIdx.x, bid)];
+        leaves = uni(v.leaves_ne);
+        sx = uni(v.sx);
+        sy = uni(v.sy);
+        sz = uni(v.sz);
+        Q = uni(v.Q);
+        n = uni(v.n);
+        sbeg = uni(v.sbeg);
+        send = uni(v.send);
+        ctr = uni(v.ctr);
+        level = uni(v.level);
+        lrad = uni(v.lrad);
+        M = uni(v.M);
+    }
+    const int b = gu<GROUP>(leaves[bid]);
     const int tid = threadIdx.x;
-    const double cx = ctr[3 * b], cy = ctr[3 * b + 1], cz = ctr[3 * b + 2];
-    const double *rd = lrad + 3 * level[b];
-    const double rx = rd[0], ry = rd[1], rz = rd[2];
+    const double cx = gu<GROUP>(ctr[3 * b]), cy = gu<GROUP>(ctr[3 * b + 1]), cz = gu<GROUP>(ctr[3 * b + 2]);
+    const double *rd = lrad + 3 * gu<GROUP>(level[b]);
+    const double rx = gu<GROUP>(rd[0]), ry = gu<GROUP>(rd[1]), rz = gu<GROUP>(rd[2]);
     double acc[PER][4];
 #pragma unroll
     for (int k = 0; k < PER; ++k)
 #pragma unroll
         for (int f = 0; f < 4; ++f) acc[k][f] = 0.0;
-    const int beg = sbeg[b], end = send[b];
+    const int beg = gu<GROUP>(sbeg[b]), end = gu<GROUP>(send[b]);
     for (int base = beg; base < end; base += CH) {
         const int cnt = min(CH, end - base);
         if (tid < cnt) {
@@ -130,9 +149,9 @@ __device__ __forceinline__ void load_transfer(const ChebTab &tab, int c, double 
 // levels with few parents, where each block's chain of load latencies is
 // the time), so their loads and the two inner contractions share one pass
 // and three barriers; each output sums the children's terms in child order.
-template <int P, int G>
+template <int P, int G, bool GROUP = false>
 __global__ __launch_bounds__(256) void k_m2m(int lo, const int *__restrict__ flags, const int *__restrict__ fc,
-                                             ChebTab tab, double *__restrict__ M) {
+                                             ChebTab tab, double *__restrict__ M, GroupLaunch g = {}) {
     constexpr int R = P * P * P;
     constexpr int R4 = 4 * R;
     constexpr int PER = (R4 + 255) / 256;
@@ -140,8 +159,17 @@ __global__ __launch_bounds__(256) void k_m2m(int lo, const int *__restrict__ fla
     __shared__ double sIn[G * R4];
     __shared__ double sA[G * R4];
     __shared__ double sT[2][P][P];  // the transfer factors of the lower and the upper child half
-    const int id = lo + blockIdx.x;
-    if (flags[id] & kLeaf) return;
+    int id = lo + blockIdx.x;
+    if constexpr (GROUP) {  // (aux: the member's first node of the level)
+        int bid;
+        const int m = group_member(g, blockIdx.x, bid);
+        const MemberView &v = g.mv[m];
+        id = uni(g.aux[m]) + bid;
+        flags = uni(v.flags);
+        fc = uni(v.first_child);
+        M = uni(v.M);
+    }
+    if (gu<GROUP>(flags[id]) & kLeaf) return;
     const int tid = threadIdx.x;
     for (int e = tid; e < 2 * P * P; e += 256) {
         const int b = e / (P * P), i = (e / P) % P, l = e % P;
@@ -159,10 +187,10 @@ __global__ __launch_bounds__(256) void k_m2m(int lo, const int *__restrict__ fla
     double acc[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) acc[k] = 0.0;
-    const int c0 = fc[id];
+    const int c0 = gu<GROUP>(fc[id]);
     unsigned live = 0;  // bit c: child c is not empty (bbfmm.h:531 skips empty children)
     for (int c = 0; c < 8; ++c)
-        if (!(flags[c0 + c] & kEmpty)) live |= 1u << c;
+        if (!(gu<GROUP>(flags[c0 + c]) & kEmpty)) live |= 1u << c;
     for (int g0 = 0; g0 < 8; g0 += G) {
         if (!((live >> g0) & ((1u << G) - 1))) continue;  // (block-uniform)
         for (int e = tid; e < G * R4; e += 256) {
@@ -218,20 +246,30 @@ __global__ __launch_bounds__(256) void k_m2m(int lo, const int *__restrict__ fla
 
 // ----------------------------------------------------------------- L2L ---
 // Lc(f,i,j,k) += sum_l Tz(i,l) sum_m Ty(j,m) sum_n Tx(k,n) Lp(f,l,m,n)
-template <int P>
+template <int P, bool GROUP = false>
 __global__ __launch_bounds__(256) void k_l2l(int lo, const unsigned char *__restrict__ need,
                                              const int *__restrict__ parent, const int *__restrict__ slot,
-                                             ChebTab tab, double *__restrict__ L) {
+                                             ChebTab tab, double *__restrict__ L, GroupLaunch g = {}) {
     constexpr int R = P * P * P;
     constexpr int R4 = 4 * R;
     __shared__ double sIn[R4];
     __shared__ double sA[R4];
     __shared__ double sT[3][P][P];
-    const int id = lo + blockIdx.x;
-    if (!need[id]) return;
+    int id = lo + blockIdx.x;
+    if constexpr (GROUP) {  // (aux: the member's first node of the level)
+        int bid;
+        const int m = group_member(g, blockIdx.x, bid);
+        const MemberView &v = g.mv[m];
+        id = uni(g.aux[m]) + bid;
+        need = uni(v.need_local);
+        parent = uni(v.parent);
+        slot = uni(v.slot);
+        L = uni(v.L);
+    }
+    if (!gu<GROUP>((int)need[id])) return;
     const int tid = threadIdx.x;
-    const int pid = parent[id];
-    load_transfer<P>(tab, slot[id], sT);
+    const int pid = gu<GROUP>(parent[id]);
+    load_transfer<P>(tab, gu<GROUP>(slot[id]), sT);
     for (int e = tid; e < R4; e += 256) sIn[e] = L[(size_t)pid * R4 + e];
     __syncthreads();
     for (int e = tid; e < R4; e += 256) {  // A(f,l,m,k) = sum_n Tx(k,n) Lp(f,l,m,n)
@@ -535,25 +573,47 @@ template <> struct Vec4<double> { using type = double4; };
 // to the unit's first target are formed in FP64 and rounded to FP32, the
 // kernels are evaluated in FP32 (PB kernels by pair_pb32), each lane's FP32
 // sums over one LDS batch (at most 64 terms) are accumulated in FP64.
-template <bool MIXED, bool POLY>
+// (GROUP: the units [aux[m], aux[m] + blocks) of each member m, whose own
+// points are the targets and the sources)
+template <bool MIXED, bool POLY, bool GROUP = false>
 __global__ __launch_bounds__(64) void k_p2p_unit(
     const int4 *__restrict__ ua, const int4 *__restrict__ ub, const double *__restrict__ tx,
     const double *__restrict__ ty, const double *__restrict__ tz, const double *__restrict__ sx,
     const double *__restrict__ sy, const double *__restrict__ sz, const double *__restrict__ Q, int ns,
-    const int2 *__restrict__ ranges, PBConst kc, double *__restrict__ F, int nt, double *__restrict__ part) {
+    const int2 *__restrict__ ranges, PBConst kc, double *__restrict__ F, int nt, double *__restrict__ part,
+    GroupLaunch g = {}) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using T4 = typename Vec4<T>::type;
     __shared__ T4 sbuf[2][64];  // {x, y, z, q_psi}, {q_x, q_y, q_z, q_psi / (4 pi)} per source
     T4 *s0 = sbuf[0], *s1 = sbuf[1];
     __shared__ double red[4][64];
     const int lane = threadIdx.x;
-    const int4 a = ua[blockIdx.x], c = ub[blockIdx.x];
+    int u = blockIdx.x;
+    if constexpr (GROUP) {
+        int bid;
+        const int m = group_member(g, blockIdx.x, bid);
+        const MemberView &v = g.mv[m];
+        u = uni(g.aux[m]) + bid;
+        ua = uni(v.ua);
+        ub = uni(v.ub);
+        tx = sx = uni(v.sx);
+        ty = sy = uni(v.sy);
+        tz = sz = uni(v.sz);
+        Q = uni(v.Q);
+        ns = nt = uni(v.n);
+        ranges = uni(v.ranges);
+        kc = v.kc;
+        F = uni(v.F);
+        part = uni(v.part);
+    }
+    const int4 a = gu<GROUP>(ua[u]), c = gu<GROUP>(ub[u]);
     const int t0 = a.y, ntc = a.z, slot = a.w, total = c.z;
     const int nsl = 64 / ntc;
     const int mt = lane % ntc, sl = lane / ntc;
     const bool active = sl < nsl;
     const int t = t0 + mt;
-    const double ox = MIXED ? tx[t0] : 0.0, oy = MIXED ? ty[t0] : 0.0, oz = MIXED ? tz[t0] : 0.0;
+    const double ox = MIXED ? gu<GROUP>(tx[t0]) : 0.0, oy = MIXED ? gu<GROUP>(ty[t0]) : 0.0,
+                 oz = MIXED ? gu<GROUP>(tz[t0]) : 0.0;
     const T px = (T)(tx[t] - ox), py = (T)(ty[t] - oy), pz = (T)(tz[t] - oz);
     const PBConst32 k32 = pb_const32(kc);
     double f0 = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
@@ -802,27 +862,81 @@ __global__ void k_p2p_order(int n, const int *__restrict__ idx, const int4 *__re
 // target leaf.  For a leaf whose near
 // field was split over several work units (lsplit[leaf].y > 1) the near
 // field is first summed from the units' partial fields, in unit order.
-template <int P>
+// (GROUP: the non-empty leaves of every member; the leaf's expansion is
+// staged in LDS, since through a pointer read from memory its uniform reads
+// would be vector loads)
+template <int P, bool GROUP = false>
 __global__ __launch_bounds__(64) void k_l2p(const int *__restrict__ tleaves, const int *__restrict__ tbeg,
                                             const int *__restrict__ tend, const double *__restrict__ tx,
                                             const double *__restrict__ ty, const double *__restrict__ tz,
                                             const double *__restrict__ ctr, const int *__restrict__ level,
                                             const double *__restrict__ lrad, ChebTab tab,
                                             const double *__restrict__ L, const int2 *__restrict__ lsplit,
-                                            const double *__restrict__ part, double *__restrict__ F, int nt) {
+                                            const double *__restrict__ part, double *__restrict__ F, int nt,
+                                            GroupLaunch g = {}) {
     constexpr int R = P * P * P;
-    const int b = tleaves[blockIdx.x];
-    const int2 sp = lsplit ? lsplit[blockIdx.x] : make_int2(-1, 1);
+    __shared__ double sLg[GROUP ? 4 * R : 1];
+    int bid = blockIdx.x;
+    if constexpr (GROUP) {
+        const MemberView &v = g.mv[group_member(g, blockIdx.x, bid)];
+        tleaves = uni(v.leaves_ne);
+        tbeg = uni(v.sbeg);
+        tend = uni(v.send);
+        tx = uni(v.sx);
+        ty = uni(v.sy);
+        tz = uni(v.sz);
+        ctr = uni(v.ctr);
+        level = uni(v.level);
+        lrad = uni(v.lrad);
+        L = uni(v.L);
+        lsplit = uni(v.lsplit);
+        part = uni(v.part);
+        F = uni(v.F);
+        nt = uni(v.n);
+    }
+    const int b = gu<GROUP>(tleaves[bid]);
+    const int2 sp = lsplit ? gu<GROUP>(lsplit[bid]) : make_int2(-1, 1);
     // the leaf's expansion is read at wave-uniform addresses: scalar loads
     const double *sL = L + (size_t)b * 4 * R;
-    const double cx = ctr[3 * b], cy = ctr[3 * b + 1], cz = ctr[3 * b + 2];
-    const double *rd = lrad + 3 * level[b];
-    for (int t = tbeg[b] + threadIdx.x; t < tend[b]; t += 64) {
+    if constexpr (GROUP) {
+        for (int e = threadIdx.x; e < 4 * R; e += 64) sLg[e] = sL[e];
+        __syncthreads();
+        sL = sLg;
+    }
+    const double cx = gu<GROUP>(ctr[3 * b]), cy = gu<GROUP>(ctr[3 * b + 1]), cz = gu<GROUP>(ctr[3 * b + 2]);
+    const double *rd = lrad + 3 * gu<GROUP>(level[b]);
+    const int t_end = gu<GROUP>(tend[b]);
+    for (int t = gu<GROUP>(tbeg[b]) + threadIdx.x; t < t_end; t += 64) {
         double Sx[P], Sy[P], Sz[P];
         sp_values<P>((tx[t] - cx) / rd[0], tab, Sx);
         sp_values<P>((ty[t] - cy) / rd[1], tab, Sy);
         sp_values<P>((tz[t] - cz) / rd[2], tab, Sz);
         double g0 = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+        if constexpr (GROUP) {
+            // (one z-layer of the expansion per pass, rolled: unrolled, the
+            // compiler reads all 4 r values ahead and spills)
+#pragma unroll 1
+            for (int l = 0; l < P; ++l) {
+                double szl = Sz[0];
+#pragma unroll
+                for (int q = 1; q < P; ++q)
+                    if (q == l) szl = Sz[q];
+                const double *sl = sL + l * P * P;
+#pragma unroll
+                for (int jj = 0; jj < P; ++jj) {
+                    const double wyz = Sy[jj] * szl;
+#pragma unroll
+                    for (int i = 0; i < P; ++i) {
+                        const int m = jj * P + i;
+                        const double w = Sx[i] * wyz;
+                        g0 += w * sl[m];
+                        gx += w * sl[R + m];
+                        gy += w * sl[2 * R + m];
+                        gz += w * sl[3 * R + m];
+                    }
+                }
+            }
+        } else {
         for (int l = 0; l < P; ++l)
 #pragma unroll
             for (int jj = 0; jj < P; ++jj) {
@@ -837,6 +951,7 @@ __global__ __launch_bounds__(64) void k_l2p(const int *__restrict__ tleaves, con
                     gz += w * sL[3 * R + m];
                 }
             }
+        }
         double n0, nx, ny, nz;  // near field
         if (sp.y > 1) {
             const int i = t - tbeg[b];
@@ -884,6 +999,56 @@ __global__ void k_combine(int n, const int *__restrict__ perm, const double *__r
     const double f0 = F[i], fx = F[(size_t)n + i], fy = F[2 * (size_t)n + i], fz = F[3 * (size_t)n + i];
     y[o] = x[o] + f0 / c1;  // electric.cpp:382-383
     y[(size_t)n + o] = x[(size_t)n + o] + (snx[i] * fx + sny[i] * fy + snz[i] * fz) / c2;
+}
+
+// the group forms: 256 points of one member per block; x and y of member m
+// are picked from the kernel argument with an unrolled select (m is
+// wave-uniform), which keeps the argument array out of private memory
+template <typename T>
+__device__ __forceinline__ T group_pick(T const (&a)[kMaxGroup], int m) {
+    T r = a[0];
+#pragma unroll
+    for (int i = 1; i < kMaxGroup; ++i)
+        if (i == m) r = a[i];
+    return r;
+}
+__global__ __launch_bounds__(256) void k_charges_g(GroupLaunch g, GroupXY xy) {
+    int bid;
+    const int m = group_member(g, blockIdx.x, bid);
+    const MemberView &v = g.mv[m];
+    const int n = uni(v.n);
+    const int i = bid * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double *x = group_pick(xy.x, m);
+    const int o = v.perm[i];
+    const double phi = x[o], psi = x[(size_t)n + o], w = v.sw[i];
+    double *Q = v.Q;
+    Q[i] = phi * v.snx[i] * w;
+    Q[(size_t)n + i] = phi * v.sny[i] * w;
+    Q[2 * (size_t)n + i] = phi * v.snz[i] * w;
+    Q[3 * (size_t)n + i] = psi * w;
+}
+__global__ __launch_bounds__(256) void k_combine_g(GroupLaunch g, GroupXY xy) {
+    int bid;
+    const int m = group_member(g, blockIdx.x, bid);
+    const MemberView &v = g.mv[m];
+    const int n = uni(v.n);
+    const int i = bid * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double *x = group_pick(xy.x, m);
+    double *y = group_pick(xy.y, m);
+    const double *F = v.F;
+    const int o = v.perm[i];
+    const double f0 = F[i], fx = F[(size_t)n + i], fy = F[2 * (size_t)n + i], fz = F[3 * (size_t)n + i];
+    y[o] = x[o] + f0 / v.c1;
+    y[(size_t)n + o] = x[(size_t)n + o] + (v.snx[i] * fx + v.sny[i] * fy + v.snz[i] * fz) / v.c2;
+}
+// L = 0 for every member (256 doubles per block)
+__global__ __launch_bounds__(256) void k_zero_l_g(GroupLaunch g, int r4) {
+    int bid;
+    const MemberView &v = g.mv[group_member(g, blockIdx.x, bid)];
+    const size_t i = (size_t)bid * 256 + threadIdx.x;
+    if (i < (size_t)uni(v.nnodes) * r4) v.L[i] = 0.0;
 }
 
 // one input field from a charge vector, the others zero
@@ -945,7 +1110,7 @@ int fmm_forward_p(ibim_plan *pl, const TargetSet &ts, double *F, int nt, bool mi
         if (u1 <= u0) return;
         auto launch = [&](auto kern) {
             kern<<<u1 - u0, 64, 0, s>>>(w.ua + u0, w.ub + u0, ts.x, ts.y, ts.z, pl->sx, pl->sy, pl->sz, pl->Q, pl->n,
-                                        ts.ranges, pl->kc, F, nt, w.part);
+                                        ts.ranges, pl->kc, F, nt, w.part, GroupLaunch{});
         };
         const bool poly = pl->kc.mode == IBIM_KERNEL_POLY;
         if (mixed) launch(poly ? k_p2p_unit<true, true> : k_p2p_unit<true, false>);
@@ -1314,6 +1479,391 @@ int kernel_fmm_device(ibim_plan *pl, int kid, const double *d_q, double *d_out) 
     if (rc) return rc;
     k_scatter_field<<<nblk(pl->n), 256, 0, st>>>(pl->n, pl->perm, pl->F, fout[kid], sgn[kid], d_out);
     IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
+// ------------------------------------------------------- fused groups ---
+namespace {
+
+// the device view of a plan's matvec buffers; vops / xops are the class
+// operators of the form the group runs (FP64 or FP32)
+MemberView member_view(const ibim_plan *pl, int form) {
+    MemberView v{};
+    v.sx = pl->sx;
+    v.sy = pl->sy;
+    v.sz = pl->sz;
+    v.snx = pl->snx;
+    v.sny = pl->sny;
+    v.snz = pl->snz;
+    v.sw = pl->sw;
+    v.perm = pl->perm;
+    v.Q = pl->Q;
+    v.F = pl->F;
+    v.n = pl->n;
+    v.nnodes = pl->nnodes;
+    v.n_leaves_ne = pl->n_leaves_ne;
+    v.n_m2l_work = pl->n_m2l_work;
+    v.leaves_ne = pl->leaves_ne;
+    v.sbeg = pl->sbeg;
+    v.send = pl->send;
+    v.level = pl->level;
+    v.flags = pl->flags;
+    v.first_child = pl->first_child;
+    v.parent = pl->parent;
+    v.slot = pl->slot;
+    v.m2l_work = pl->m2l_work;
+    v.ctr = pl->ctr;
+    v.lrad = pl->d_lvl_rad;
+    v.need_local = pl->need_local;
+    v.M = pl->M;
+    v.L = pl->L;
+    v.kc = pl->kc;
+    const double dE = pl->prm.eps_out, dI = pl->prm.eps_in;
+    v.c1 = 0.5 * (1 + dE / dI);  // as combine()
+    v.c2 = 0.5 * (1 + dI / dE);
+    v.ua = pl->p2p.ua;
+    v.ub = pl->p2p.ub;
+    v.ranges = pl->p2p_ranges;
+    v.lsplit = pl->p2p.npart ? pl->p2p.lsplit : nullptr;
+    v.part = pl->p2p.part;
+    const VPairs &vp = pl->vp, &xp = pl->xp;
+    v.vblk = vp.blk_cls;
+    v.vblk_j0 = vp.blk_j0;
+    v.vkey_off = vp.cls_off_d;
+    v.vA = vp.A;
+    v.vB = vp.B;
+    v.xblk = xp.blk_cls;
+    v.xblk_j0 = xp.blk_j0;
+    v.xkey_off = xp.cls_off_d;
+    v.xA = xp.A;
+    v.xB = xp.B;
+    v.vops = form == IBIM_PREC_MIXED ? (const void *)pl->ops32 : (const void *)pl->ops;
+    v.xops = form == IBIM_PREC_MIXED ? (const void *)pl->xops32 : (const void *)pl->xops;
+    v.gtab = pl->gtab;
+    v.stab = pl->stab;
+    v.MRP = pl->MRP;
+    v.KRP = pl->KRP;
+    v.hvblk = vp.hblk_cls;
+    v.hvblk_j0 = vp.hblk_j0;
+    v.keyslot = pl->keyslot;
+    v.hxblk = xp.hblk_cls;
+    v.hxblk_j0 = xp.hblk_j0;
+    v.hxkey_off = xp.hcls_off_d;
+    v.xiota = pl->xiota;
+    v.xkeyslot = pl->xkeyslot;
+    v.xmask = xp.mask;
+    v.ops16 = pl->ops16;
+    v.esig = pl->esig;
+    v.Mh = pl->Mh;
+    v.Bx = pl->Bx;
+    v.ebox = pl->ebox;
+    v.ebx = pl->ebx;
+    v.Yv = pl->Yv;
+    v.y_rows = pl->y_rows;
+    v.vnpairs = vp.npairs;
+    v.xnpairs = xp.npairs;
+    v.ytptr = pl->ytptr;
+    v.ytlist = pl->ytlist;
+    return v;
+}
+
+// why a plan cannot run in a group of the given form ("" if it can)
+std::string group_misfit(const ibim_plan *pl, const ibim_plan *p0, int form) {
+    if (!pl) return "plan is NULL";
+    if (pl->device != p0->device || pl->p != p0->p || pl->kc.mode != p0->kc.mode)
+        return "plans of one group share the device, the Chebyshev order and the kernel family";
+    if (pl->n < 1 || !pl->p2p.ua || pl->n_leaves_ne < 1) return "plan without near-field work units";
+    if (!pl->ops || (pl->xp.npairs > 0 && !pl->xops)) return "plan without class operators for its V and X lists";
+    if (pl->p > 6) return "groups run p <= 6";
+    if (form == IBIM_PREC_MIXED_F16) {
+        if (pl->prm.precision != IBIM_PREC_MIXED_F16 || !pl->f16_ready || pl->p > 5 ||
+            (pl->xp.npairs > 0 && !pl->x16))
+            return "FP16-mixed groups need plans with their FP16 M2L stage built, p <= 5";
+    } else if (form == IBIM_PREC_MIXED) {
+        if (pl->prm.precision != IBIM_PREC_MIXED || !pl->ops32 || (pl->xp.npairs > 0 && !pl->xops32))
+            return "FP32-mixed groups need plans with FP32 operators";
+    }
+    return "";
+}
+
+// The fused matvec, in the stage order of fmm_forward_p
+template <int P>
+int group_matvec_p(MatvecGroup *g, const double *const *x, double *const *y) {
+    constexpr int R4 = 4 * P * P * P;
+    constexpr int kP2mBS = P * P * P <= 128 ? 128 : 256;
+    hipStream_t st = g->st;
+    const ChebTab tab = g->pl[0]->tab;
+    const bool f16 = g->form == IBIM_PREC_MIXED_F16;
+    const bool mixed = g->form != IBIM_PREC_FP64;
+    auto gl = [&](const MatvecGroup::Launch &L) {
+        return GroupLaunch{g->d_mv, g->d_tab + L.off, L.aux >= 0 ? g->d_tab + L.aux : nullptr, g->k};
+    };
+    GroupXY xy{};
+    for (int m = 0; m < g->k; ++m) {
+        xy.x[m] = x[m];
+        xy.y[m] = y[m];
+    }
+    auto near = [&](const MatvecGroup::Launch &L, hipStream_t s) {
+        if (L.blocks == 0) return;
+        auto nf = [&](auto kern) {
+            kern<<<L.blocks, 64, 0, s>>>(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                         nullptr, 0, nullptr, PBConst{}, nullptr, 0, nullptr, gl(L));
+        };
+        if (mixed) {
+            if (g->poly) nf(k_p2p_unit<true, true, true>);
+            else nf(k_p2p_unit<true, false, true>);
+        } else {
+            if (g->poly) nf(k_p2p_unit<false, true, true>);
+            else nf(k_p2p_unit<false, false, true>);
+        }
+    };
+    StageScope sm(g->timer, mixed ? kStMatvecMixed : kStMatvec, st);
+    sm.add(g->work.valu, g->work.mfma, g->work.bytes);
+    if (g->charges.blocks) k_charges_g<<<g->charges.blocks, 256, 0, st>>>(gl(g->charges), xy);
+    if (f16 && g->near_early.blocks) {
+        IBIM_HIP_CHECK(hipEventRecord(g->ev_fork, st));
+        IBIM_HIP_CHECK(hipStreamWaitEvent(g->st2, g->ev_fork, 0));
+        StageScope sc(g->timer, kStP2PMixed, g->st2);
+        near(g->near_early, g->st2);
+        sc.add(g->near_early_flops, 0, 0);
+    }
+    IBIM_LAUNCH_CHECK();
+    if (g->p2m.blocks)
+        k_p2m<P, kP2mBS, true><<<g->p2m.blocks, kP2mBS, 0, st>>>(nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                                                                 nullptr, nullptr, nullptr, nullptr, nullptr, tab,
+                                                                 nullptr, gl(g->p2m));
+    constexpr int GS = R4 <= 512 ? 8 : (R4 <= 1024 ? 4 : 2);
+    for (const MatvecGroup::Launch &L : g->m2m) {
+        if (!L.blocks) continue;
+        if (L.blocks < 1024) k_m2m<P, GS, true><<<L.blocks, 256, 0, st>>>(0, nullptr, nullptr, tab, nullptr, gl(L));
+        else k_m2m<P, 2, true><<<L.blocks, 256, 0, st>>>(0, nullptr, nullptr, tab, nullptr, gl(L));
+    }
+    if (g->zero_l.blocks) k_zero_l_g<<<g->zero_l.blocks, 256, 0, st>>>(gl(g->zero_l), R4);
+    IBIM_LAUNCH_CHECK();
+    {
+        StageScope sc(g->timer, mixed ? kStM2LMixed : kStM2L, st);
+        sc.add(0, g->m2l_flops, 0);
+        if (int rc = group_m2l(g)) return rc;
+    }
+    if (f16) {
+        IBIM_HIP_CHECK(hipEventRecord(g->ev_fork, st));
+        IBIM_HIP_CHECK(hipStreamWaitEvent(g->st2, g->ev_fork, 0));
+        {
+            StageScope sc(g->timer, kStP2PMixed, g->st2);
+            near(g->near_rest, g->st2);
+            sc.add(g->near_rest_flops, 0, 0);
+        }
+        IBIM_LAUNCH_CHECK();
+        IBIM_HIP_CHECK(hipEventRecord(g->ev_join, g->st2));
+        if (int rc = group_m2l_reduce(g)) return rc;
+    }
+    for (const MatvecGroup::Launch &L : g->l2l)
+        if (L.blocks) k_l2l<P, true><<<L.blocks, 256, 0, st>>>(0, nullptr, nullptr, nullptr, tab, nullptr, gl(L));
+    IBIM_LAUNCH_CHECK();
+    if (f16) {
+        IBIM_HIP_CHECK(hipStreamWaitEvent(st, g->ev_join, 0));
+    } else {
+        StageScope sc(g->timer, mixed ? kStP2PMixed : kStP2P, st);
+        near(g->near_all, st);
+        sc.add(g->near_early_flops + g->near_rest_flops, 0, 0);
+        IBIM_LAUNCH_CHECK();
+    }
+    if (g->l2p.blocks)
+        k_l2p<P, true><<<g->l2p.blocks, 64, 0, st>>>(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                     nullptr, nullptr, tab, nullptr, nullptr, nullptr, nullptr, 0,
+                                                     gl(g->l2p));
+    if (g->combine.blocks) k_combine_g<<<g->combine.blocks, 256, 0, st>>>(gl(g->combine), xy);
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
+}  // namespace
+
+int group_create(ibim_plan *const *plans, int k, bool mixed, MatvecGroup **out) {
+    if (!out || !plans || k < 1 || k > kMaxGroup) {
+        set_error("bad argument to a matvec group (1..16 plans)");
+        return IBIM_ERR_ARG;
+    }
+    *out = nullptr;
+    const ibim_plan *p0 = plans[0];
+    if (!p0) {
+        set_error("plan is NULL");
+        return IBIM_ERR_ARG;
+    }
+    const int form = mixed ? p0->prm.precision : IBIM_PREC_FP64;
+    for (int m = 0; m < k; ++m) {
+        const std::string why = group_misfit(plans[m], p0, form);
+        if (!why.empty()) {
+            set_error("matvec group: " + why);
+            return IBIM_ERR_ARG;
+        }
+        for (int q = 0; q < m; ++q)
+            if (plans[q] == plans[m]) {
+                set_error("matvec group: a plan appears twice");
+                return IBIM_ERR_ARG;
+            }
+    }
+    MatvecGroup *g = new MatvecGroup();
+    g->k = k;
+    g->p = p0->p;
+    g->device = p0->device;
+    g->form = form;
+    g->poly = p0->kc.mode == IBIM_KERNEL_POLY;
+    g->pl.assign(plans, plans + k);
+    const int R4 = 4 * g->p * g->p * g->p;
+    const bool f16 = form == IBIM_PREC_MIXED_F16;
+    std::vector<int> &tab = g->tab;
+    // one launch: blocks of member m, and an optional per-member value
+    auto add = [&](MatvecGroup::Launch &L, auto blocks, auto aux, bool with_aux) {
+        L.off = (int)tab.size();
+        int64_t acc = 0;
+        tab.push_back(0);
+        for (int m = 0; m < k; ++m) {
+            acc += blocks(g->pl[m]);
+            tab.push_back((int)std::min<int64_t>(acc, INT32_MAX));
+        }
+        L.blocks = (int)std::min<int64_t>(acc, INT32_MAX);
+        if (with_aux) {
+            L.aux = (int)tab.size();
+            for (int m = 0; m < k; ++m) tab.push_back(aux(g->pl[m]));
+        }
+        return acc;
+    };
+    auto none = [](const ibim_plan *) { return 0; };
+    auto ucut = [&](const ibim_plan *pl) { return f16 ? (int)(pl->p2p.nunits * kNearEarly) : 0; };
+    int64_t big = 0;
+    big = std::max(big, add(g->charges, [](const ibim_plan *pl) { return nblk(pl->n); }, none, false));
+    g->combine = g->charges;
+    add(g->p2m, [](const ibim_plan *pl) { return pl->n_leaves_ne; }, none, false);
+    big = std::max(big, add(g->zero_l, [&](const ibim_plan *pl) { return nblk((int64_t)pl->nnodes * R4); }, none,
+                            false));
+    if (f16) {
+        add(g->near_early, ucut, [](const ibim_plan *) { return 0; }, true);
+        big = std::max(big, add(g->near_rest, [&](const ibim_plan *pl) { return pl->p2p.nunits - ucut(pl); }, ucut,
+                                true));
+        add(g->m_to_half, [](const ibim_plan *pl) { return pl->nnodes; }, none, false);
+        add(g->x_reflect, [](const ibim_plan *pl) { return pl->x16 ? pl->xp.npairs : 0; }, none, false);
+        big = std::max(big, add(g->v_gemm, [](const ibim_plan *pl) { return pl->vp.hnblocks; }, none, false));
+        add(g->x_gemm, [](const ibim_plan *pl) { return pl->x16 ? pl->xp.hnblocks : 0; }, none, false);
+        add(g->v_reduce, [](const ibim_plan *pl) { return pl->n_m2l_work; }, none, false);
+    } else {
+        big = std::max(big, add(g->near_all, [](const ibim_plan *pl) { return pl->p2p.nunits; },
+                                [](const ibim_plan *) { return 0; }, true));
+        const int nrt = p0->MRP / 64;
+        big = std::max(big, add(g->v_gemm, [&](const ibim_plan *pl) { return (int64_t)pl->vp.nblocks * nrt; }, none,
+                                false));
+        big = std::max(big, add(g->x_gemm, [&](const ibim_plan *pl) {
+            return pl->xp.npairs > 0 ? (int64_t)pl->xp.nblocks * nrt : (int64_t)0; }, none, false));
+    }
+    add(g->l2p, [](const ibim_plan *pl) { return pl->n_leaves_ne; }, none, false);
+    int max_lev = 0;
+    for (const ibim_plan *pl : g->pl) max_lev = std::max(max_lev, pl->nlevels);
+    // M2M: step s is level nlevels - 2 - s of each member (finest first)
+    for (int s = 0; s + 2 <= max_lev; ++s) {
+        g->m2m.emplace_back();
+        auto lev = [s](const ibim_plan *pl) { return pl->nlevels - 2 - s; };
+        add(g->m2m.back(), [&](const ibim_plan *pl) {
+            const int l = lev(pl);
+            return l >= 0 ? pl->lvl_off[l + 1] - pl->lvl_off[l] : 0; }, [&](const ibim_plan *pl) {
+            const int l = lev(pl);
+            return l >= 0 ? pl->lvl_off[l] : 0; }, true);
+    }
+    // L2L: step l is level l of each member (coarsest first)
+    for (int l = 1; l < max_lev; ++l) {
+        g->l2l.emplace_back();
+        add(g->l2l.back(), [l](const ibim_plan *pl) { return l < pl->nlevels ? pl->lvl_off[l + 1] - pl->lvl_off[l] : 0; },
+            [l](const ibim_plan *pl) { return l < pl->nlevels ? pl->lvl_off[l] : 0; }, true);
+    }
+    if (big >= (int64_t(1) << 23)) {
+        delete g;
+        set_error("matvec group: too many blocks for one fused launch (groups are for small solves)");
+        return IBIM_ERR_ARG;
+    }
+    for (const ibim_plan *pl : g->pl) {
+        const StageWork w = matvec_work(pl, mixed);
+        g->work.valu += w.valu;
+        g->work.mfma += w.mfma;
+        g->work.bytes += w.bytes;
+        g->near_early_flops += near_field_flops(pl, 0, ucut(pl));
+        g->near_rest_flops += near_field_flops(pl, ucut(pl), pl->p2p.nunits);
+        g->m2l_flops += 32.0 * (double)pl->r * pl->r * ((double)pl->vp.npairs + pl->xp.npairs);
+    }
+    // device tables, on the group's own streams
+    std::vector<MemberView> mv(k);
+    for (int m = 0; m < k; ++m) mv[m] = member_view(g->pl[m], form);
+    hipError_t e = hipSetDevice(g->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->st2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->ev_done, hipEventDisableTiming);
+    for (int m = 0; m < k && e == hipSuccess; ++m) {
+        hipEvent_t ev = nullptr;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) g->ev_in.push_back(ev);
+    }
+    // (from the block cache: a server makes and drops groups as its solves
+    // come and go, and hipFree would wait for the whole device)
+    if (e == hipSuccess) e = block_alloc((void **)&g->d_mv, k * sizeof(MemberView));
+    if (e == hipSuccess) e = block_alloc((void **)&g->d_tab, tab.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpyAsync(g->d_mv, mv.data(), k * sizeof(MemberView), hipMemcpyHostToDevice, g->st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(g->d_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, g->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->st);
+    if (e != hipSuccess) {
+        set_error(std::string("matvec group set-up: ") + hipGetErrorString(e));
+        group_destroy(g);
+        return IBIM_ERR_HIP;
+    }
+    // the members' plans: the stage timers of a fused launch follow the
+    // first member's selection
+    g->timer.on = p0->timer.on;
+    g->timer.mask = p0->timer.mask;
+    *out = g;
+    return IBIM_OK;
+}
+
+void group_destroy(MatvecGroup *g) {
+    if (!g) return;
+    if (g->st) (void)hipStreamSynchronize(g->st);
+    if (g->st2) (void)hipStreamSynchronize(g->st2);
+    for (auto &r : g->timer.pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t ev : g->timer.pool) (void)hipEventDestroy(ev);
+    if (g->d_mv) block_release(g->d_mv);
+    if (g->d_tab) block_release(g->d_tab);
+    for (hipEvent_t ev : {g->ev_fork, g->ev_join, g->ev_done})
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : g->ev_in) (void)hipEventDestroy(ev);
+    if (g->st2) (void)hipStreamDestroy(g->st2);
+    if (g->st) (void)hipStreamDestroy(g->st);
+    delete g;
+}
+
+int group_matvec_ordered(MatvecGroup *g, const double *const *x, double *const *y) {
+    for (int m = 0; m < g->k; ++m) {
+        IBIM_HIP_CHECK(hipEventRecord(g->ev_in[m], g->pl[m]->stream));
+        IBIM_HIP_CHECK(hipStreamWaitEvent(g->st, g->ev_in[m], 0));
+    }
+    if (int rc = group_matvec(g, x, y)) return rc;
+    for (int m = 0; m < g->k; ++m) IBIM_HIP_CHECK(hipStreamWaitEvent(g->pl[m]->stream, g->ev_done, 0));
+    return IBIM_OK;
+}
+
+int group_matvec(MatvecGroup *g, const double *const *x, double *const *y) {
+    int rc = IBIM_ERR_ARG;
+    switch (g->p) {
+    case 2: rc = group_matvec_p<2>(g, x, y); break;
+    case 3: rc = group_matvec_p<3>(g, x, y); break;
+    case 4: rc = group_matvec_p<4>(g, x, y); break;
+    case 5: rc = group_matvec_p<5>(g, x, y); break;
+    case 6: rc = group_matvec_p<6>(g, x, y); break;
+    default: set_error("matvec group: unsupported Chebyshev order");
+    }
+    if (rc) return rc;
+    IBIM_HIP_CHECK(hipEventRecord(g->ev_done, g->st));
     return IBIM_OK;
 }
 

@@ -432,7 +432,107 @@ struct P2PWork {
                                         // (the plan's own points only; empty for target sets)
 };
 
+// ---------------------------------------------------------------------------
+// Fused matvecs over a group of plans (concurrent solves of a server).  One
+// launch per stage covers every member: its blocks are the members' blocks
+// one after another, and each block finds its member in the launch's
+// offset table, then reads that member's buffers from its MemberView (in
+// device memory).  The members' own data are used in place; nothing is
+// copied.  The kernels are the single-plan kernels' bodies, so a member's
+// results do not depend on the group it runs in.
+constexpr int kMaxGroup = 16;
+
+struct MemberView {
+    // sorted points, charges and fields
+    const double *sx, *sy, *sz, *snx, *sny, *snz, *sw;
+    const int *perm;
+    double *Q, *F;
+    int n;
+    // tree
+    int nnodes, n_leaves_ne, n_m2l_work;
+    const int *leaves_ne, *sbeg, *send, *level, *flags, *first_child, *parent, *slot, *m2l_work;
+    const double *ctr, *lrad;
+    const unsigned char *need_local;
+    double *M, *L;
+    PBConst kc;
+    double c1, c2;  // the combination's divisors (electric.cpp:382-383)
+    // near field
+    const int4 *ua, *ub;
+    const int2 *ranges, *lsplit;  // lsplit null when no leaf is split
+    double *part;
+    // V and X lists, FP64 / FP32 class GEMMs
+    const int *vblk, *vblk_j0, *vkey_off, *vA, *vB;
+    const int *xblk, *xblk_j0, *xkey_off, *xA, *xB;
+    const void *vops, *xops;      // FP64 or FP32 class operators
+    const int *gtab, *stab;
+    int MRP, KRP;
+    // FP16 stage (p <= 5)
+    const int *hvblk, *hvblk_j0, *keyslot, *hxblk, *hxblk_j0, *hxkey_off, *xiota, *xkeyslot;
+    const unsigned char *xmask;
+    const _Float16 *ops16;
+    const int *esig;
+    _Float16 *Mh, *Bx;
+    int *ebox, *ebx;
+    unsigned *Yv;
+    int y_rows, vnpairs, xnpairs;
+    const int *ytptr, *ytlist;
+};
+
+// this call's input and output vector of each member (kernel argument)
+struct GroupXY {
+    const double *x[kMaxGroup];
+    double *y[kMaxGroup];
+};
+
+// one fused launch: members' block ranges [off[m], off[m+1]) and an
+// optional per-member integer (a level's first node)
+struct GroupLaunch {
+    const MemberView *mv;
+    const int *off;
+    const int *aux;
+    int k;
+};
+
+// Wave-uniform copies (scalar registers) of values a group kernel reads at
+// block-uniform addresses of its member's tables.  Through pointers read
+// from memory (the member's view) such loads land in vector registers, and
+// loops bounded by them compile as divergent; readfirstlane makes them
+// uniform again.  gu<G>(x) applies it in the group form of a kernel only, so
+// the single-plan form compiles as before.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int2 uni(int2 x) { return make_int2(uni(x.x), uni(x.y)); }
+__device__ __forceinline__ int4 uni(int4 x) { return make_int4(uni(x.x), uni(x.y), uni(x.z), uni(x.w)); }
+__device__ __forceinline__ double uni(double x) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)uni((int)b), hi = (unsigned)uni((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <typename T>
+__device__ __forceinline__ T *uni(T *p) {
+    const unsigned long long u = (unsigned long long)p;
+    const unsigned lo = (unsigned)uni((int)u), hi = (unsigned)uni((int)(u >> 32));
+    return (T *)(((unsigned long long)hi << 32) | lo);
+}
+template <bool G, typename T>
+__device__ __forceinline__ T gu(T x) {
+    if constexpr (G) return uni(x);
+    else return x;
+}
+
+// the member of block b (off is nondecreasing, off[0] = 0), and b's index
+// among that member's blocks (both wave-uniform)
+__device__ __forceinline__ int group_member(const GroupLaunch &g, int b, int &local) {
+    int m = 0;
+    while (m + 1 < g.k && uni(g.off[m + 1]) <= b) ++m;
+    local = uni(b - g.off[m]);
+    return uni(m);
+}
+
 }  // namespace ibim
+
+namespace ibim {
+struct Batcher;
+}
 
 struct ibim_plan {
     ibim_params prm{};
@@ -449,6 +549,7 @@ struct ibim_plan {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool defer_reduce = false;           // the FP16 M2L stage leaves the reduction to the caller
     bool trace = false;                  // GMRES prints its iteration table to stdout
+    ibim::Batcher *batcher = nullptr;    // a server's matvec batcher, while the plan's solve uses it
 
     // sorted geometry
     double *sx = nullptr, *sy = nullptr, *sz = nullptr;
@@ -615,16 +716,18 @@ int build_m2l_operators_f32(ibim_plan *pl);
 int build_m2l_operators_f16(ibim_plan *pl);
 int build_p2p_ranges(ibim_plan *pl, TargetSet &ts, P2PWork &work, std::vector<void *> *tmp_owner);
 
-// scoped stage timing on the plan's stream (no-op when profiling is off)
+// scoped stage timing on the plan's stream (no-op when profiling is off),
+// or on a fused group's timer and stream
 struct StageScope {
-    ibim_plan *pl;
+    StageTimer *tm;
     int stage;
     hipEvent_t a = nullptr;
     hipStream_t st;
     StageWork w;
-    StageScope(ibim_plan *p, int s, hipStream_t on = nullptr) : pl(p), stage(s), st(on ? on : p->stream) {
-        if (pl->timer.on && ((pl->timer.mask >> stage) & 1u)) {
-            a = pl->timer.take();
+    StageScope(ibim_plan *p, int s, hipStream_t on = nullptr) : StageScope(p->timer, s, on ? on : p->stream) {}
+    StageScope(StageTimer &t, int s, hipStream_t on) : tm(&t), stage(s), st(on) {
+        if (tm->on && ((tm->mask >> stage) & 1u)) {
+            a = tm->take();
             if (a) (void)hipEventRecord(a, st);
         }
     }
@@ -636,19 +739,72 @@ struct StageScope {
     }
     ~StageScope() {
         if (a) {
-            hipEvent_t b = pl->timer.take();
+            hipEvent_t b = tm->take();
             if (b) {
                 (void)hipEventRecord(b, st);
-                pl->timer.pending.push_back({stage, a, b, w});
+                tm->pending.push_back({stage, a, b, w});
             } else {
-                pl->timer.pool.push_back(a);
+                tm->pool.push_back(a);
             }
         }
     }
 };
+// resolve a timer's recorded intervals into its per-stage sums (waits for
+// their end events)
+int timer_collect(StageTimer &t);
+// collect, then copy the per-stage sums into *out (if out), and zero them (if
+// reset)
+int timer_report(StageTimer &t, ibim_stage_report *out, int reset);
 
 // algorithmic work of near-field work units [u0, u1) of the plan's own
 // points (60 flops per pair), and of one whole matvec of the plan
 double near_field_flops(const ibim_plan *pl, int u0, int u1);
 StageWork matvec_work(const ibim_plan *pl, bool mixed);
+
+// A fused matvec over k plans of one device, Chebyshev order and kernel
+// family, all in the same matvec form (FP64, FP32-mixed, or FP16-mixed with
+// p <= 5).  Its launch tables (block offsets of every member in every fused
+// launch) are built once, for the plans as they are; the group must be
+// destroyed before any member is.
+struct MatvecGroup {
+    struct Launch {
+        int off = 0, aux = -1;  // entry of the member offsets (k + 1) and per-member values (k) in tab
+        int blocks = 0;         // blocks (tiles for the GEMMs) over all members
+    };
+    int k = 0, p = 0, device = 0, form = IBIM_PREC_FP64;
+    bool poly = false;
+    std::vector<ibim_plan *> pl;
+    std::vector<int> tab;                  // host copy of the launch tables
+    MemberView *d_mv = nullptr;            // device: the members' views
+    int *d_tab = nullptr;                  // device: tab
+    Launch charges, p2m, zero_l, near_early, near_rest, near_all, m_to_half, x_reflect, v_gemm, x_gemm, v_reduce,
+        l2p, combine;
+    std::vector<Launch> m2m, l2l;          // one per level step (M2M from each member's finest level up)
+    StageWork work;                        // algorithmic work of one fused matvec (sum over members)
+    double near_early_flops = 0, near_rest_flops = 0, m2l_flops = 0;
+    hipStream_t st = nullptr, st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_done = nullptr;
+    std::vector<hipEvent_t> ev_in;         // per member: its input is ready on its stream
+    StageTimer timer;                      // stage timers of the fused launches
+};
+// the members' streams -> group -> the members' streams: a fused matvec
+// ordered after what each member's stream holds, and before what it gets next
+int group_matvec_ordered(MatvecGroup *g, const double *const *x, double *const *y);
+
+// one plan's matvec on its own stream, and the matvec a solve asks for: the
+// batcher's (fused with other solves' matvecs) while the plan has one
+int matvec_single(ibim_plan *pl, const double *d_x, double *d_y, bool mixed);
+int batch_matvec(Batcher *b, ibim_plan *pl, const double *d_x, double *d_y, bool mixed);
+// checks the plans and builds the group on its own streams; a plan that
+// cannot join (form, order, device, missing tables) gives IBIM_ERR_ARG
+int group_create(ibim_plan *const *plans, int k, bool mixed, MatvecGroup **out);
+void group_destroy(MatvecGroup *g);
+// y_m = A_m x_m for every member, on the group's streams: the caller orders
+// the members' inputs before it (group stream waits) and their later work
+// after g->ev_done
+int group_matvec(MatvecGroup *g, const double *const *x, double *const *y);
+// the fused M2L stage (V and X lists) of a group, on g->st; FP16 form: the
+// per-target reduction is left to group_m2l_reduce
+int group_m2l(MatvecGroup *g);
+int group_m2l_reduce(MatvecGroup *g);
 }  // namespace ibim

@@ -207,7 +207,9 @@ __global__ void k_gather_table(int *__restrict__ gtab, int *__restrict__ stab) {
 // instruction hit 16 neighbouring elements of one box.  The next K-chunk of
 // the operator and of the gathered multipoles is loaded into registers while
 // the current one is multiplied from LDS.
-template <int P, typename T>
+// GROUP (XL: the X list): the tiles of every member of a fused group, count
+// = their total, the XCD renumbering over the whole launch.
+template <int P, typename T, bool GROUP = false, bool XL = false>
 // four waves per SIMD: the register budget (at most 128) keeps the chunk
 // prefetch without spilling
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_m2l_gemm(const int *__restrict__ blk_key, const int *__restrict__ blk_j0,
@@ -215,7 +217,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                   const int *__restrict__ pB, const int *__restrict__ gtab,
                                                   const int *__restrict__ stab, const T *__restrict__ ops, int MRP,
                                                   int KRP, const double *__restrict__ M, double *__restrict__ L,
-                                                  int first, int count, int per_xcd) {
+                                                  int first, int count, int per_xcd, GroupLaunch g = {}) {
     constexpr int R = P * P * P;
     constexpr int R4 = 4 * R;
     constexpr int KC = 32;
@@ -231,14 +233,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // covers tiles first .. first + count - 1.
     const int local = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (local >= count) return;
-    const int logical = first + local;
+    int logical = first + local;
+    if constexpr (GROUP) {
+        const MemberView &v = g.mv[group_member(g, local, logical)];
+        blk_key = uni(XL ? v.xblk : v.vblk);
+        blk_j0 = uni(XL ? v.xblk_j0 : v.vblk_j0);
+        key_off = uni(XL ? v.xkey_off : v.vkey_off);
+        pA = uni(XL ? v.xA : v.vA);
+        pB = uni(XL ? v.xB : v.vB);
+        gtab = uni(v.gtab);
+        stab = uni(v.stab);
+        ops = uni((const T *)(XL ? v.xops : v.vops));
+        MRP = uni(v.MRP);
+        KRP = uni(v.KRP);
+        M = uni(v.M);
+        L = uni(v.L);
+    }
     const int nrt = MRP / 64;
     const int tile = logical / nrt;
     const int tid = threadIdx.x;
-    const int key = blk_key[tile];
+    const int key = gu<GROUP>(blk_key[tile]);
     const int cls = key >> 3, m = key & 7;
-    const int j0 = blk_j0[tile];
-    const int jend = key_off[key + 1];
+    const int j0 = gu<GROUP>(blk_j0[tile]);
+    const int jend = gu<GROUP>(key_off[key + 1]);
     const int row0 = (logical % nrt) * 64;
     if (tid < 64) {
         const int j = j0 + tid;
@@ -378,9 +395,19 @@ __device__ __forceinline__ int scale_exp(double vmax) {
 
 // per box: FP16 multipoles with one power-of-two scale (every matvec)
 // rows of Mh are KP = R4 rounded up to 64 long, zero beyond R4
+template <bool GROUP = false>
 __global__ __launch_bounds__(64) void k_m_to_half(const double *__restrict__ M, int R4, int KP,
-                                                  _Float16 *__restrict__ Mh, int *__restrict__ ebox) {
-    const size_t a = blockIdx.x;
+                                                  _Float16 *__restrict__ Mh, int *__restrict__ ebox,
+                                                  GroupLaunch g = {}) {
+    size_t a = blockIdx.x;
+    if constexpr (GROUP) {  // (every box of every member)
+        int bid;
+        const MemberView &v = g.mv[group_member(g, blockIdx.x, bid)];
+        a = bid;
+        M = uni(v.M);
+        Mh = uni(v.Mh);
+        ebox = uni(v.ebox);
+    }
     const double *src = M + a * R4;
     double mx = 0.0;
     for (int k = threadIdx.x; k < R4; k += 64) mx = fmax(mx, fabs(src[k]));
@@ -626,12 +653,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // while an X key has a few dozen
 constexpr int kH32WavesV = 8, kH32WavesX = 4;
 
-template <int P, int NW>
+// (GROUP: the pair blocks of every member of a fused group, nblocks their
+// total, the XCD renumbering over the whole launch; 8-wave blocks run the V
+// list, 4-wave blocks the X list, whose rows follow the V rows in Yv)
+template <int P, int NW, bool GROUP = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW >= 8 ? 2 : 1))) void k_m2l_gemm_h32(
     const int *__restrict__ blk_key, const int *__restrict__ blk_j0, const int *__restrict__ key_off,
     const int *__restrict__ pA, const int *__restrict__ keyslot, const _Float16 *__restrict__ ops16u,
     const int *__restrict__ esig, int MRP, const _Float16 *__restrict__ Mh, const int *__restrict__ ebox,
-    unsigned *__restrict__ Y, int yspare, int nblocks, int per_xcd) {
+    unsigned *__restrict__ Y, int yspare, int nblocks, int per_xcd, GroupLaunch g = {}) {
     constexpr int R = P * P * P;
     constexpr int R4 = 4 * R;
     constexpr int KP = (R4 + 63) / 64 * 64;
@@ -667,13 +697,32 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW >= 8
     __shared__ __attribute__((aligned(1024))) _Float16 sA[NST][CH];
     __shared__ int sEA[NPB];
     __shared__ __attribute__((aligned(16))) int sE[MR];  // the operator's row exponents
-    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= nblocks) return;
+    if constexpr (GROUP) {
+        static_assert(kH32WavesV != kH32WavesX, "the block size tells the lists apart");
+        constexpr bool XL = NW == kH32WavesX;
+        constexpr int YS1 = (4 * P * P * P + 31) / 32 * 32;
+        const MemberView &v = g.mv[group_member(g, tile, tile)];
+        blk_key = uni(XL ? v.hxblk : v.hvblk);
+        blk_j0 = uni(XL ? v.hxblk_j0 : v.hvblk_j0);
+        key_off = uni(XL ? v.hxkey_off : v.vkey_off);
+        pA = uni(XL ? v.xiota : v.vA);
+        keyslot = uni(XL ? v.xkeyslot : v.keyslot);
+        ops16u = uni(v.ops16);
+        esig = uni(v.esig);
+        MRP = uni(v.MRP);
+        Mh = uni(XL ? v.Bx : v.Mh);
+        ebox = uni(XL ? v.ebx : v.ebox);
+        const int npv = uni(v.vnpairs), rows = uni(v.y_rows);
+        Y = uni(v.Yv) + (XL ? fp24_offset(npv, YS1, 0) : 0);
+        yspare = XL ? rows - npv : rows;
+    }
     const int tid = threadIdx.x;
-    const int key = blk_key[tile];
-    const int j0 = blk_j0[tile];
-    const int jend = key_off[key + 1];
-    const int sl = keyslot[key];
+    const int key = gu<GROUP>(blk_key[tile]);
+    const int j0 = gu<GROUP>(blk_j0[tile]);
+    const int jend = gu<GROUP>(key_off[key + 1]);
+    const int sl = gu<GROUP>(keyslot[key]);
     const int w = tid >> 6, l = tid & 63;
     const _Float16 *Aop = ops16u + (size_t)sl * MRP * KP;
     const int nrt = MRP / 64;
@@ -980,12 +1029,24 @@ __global__ __launch_bounds__(512) void k_m2l_gemm_ht(
 // X pair j: its source multipole reflected by the pair's mask (the gather
 // table's permutation and input signs), so that the class operator applies
 // unreflected; and the source box's exponent
+template <bool GROUP = false>
 __global__ __launch_bounds__(64) void k_x_reflect_half(const int *__restrict__ A, const unsigned char *__restrict__ mask,
                                                        const int *__restrict__ gtab, int R4, int KP,
                                                        const _Float16 *__restrict__ Mh, const int *__restrict__ ebox,
-                                                       _Float16 *__restrict__ Bx, int *__restrict__ ebx) {
-    const int j = blockIdx.x;
-    const int a = A[j], m = mask[j];
+                                                       _Float16 *__restrict__ Bx, int *__restrict__ ebx,
+                                                       GroupLaunch g = {}) {
+    int j = blockIdx.x;
+    if constexpr (GROUP) {  // (every X pair of every member)
+        const MemberView &v = g.mv[group_member(g, blockIdx.x, j)];
+        A = uni(v.xA);
+        mask = uni(v.xmask);
+        gtab = uni(v.gtab);
+        Mh = uni(v.Mh);
+        ebox = uni(v.ebox);
+        Bx = uni(v.Bx);
+        ebx = uni(v.ebx);
+    }
+    const int a = gu<GROUP>(A[j]), m = gu<GROUP>((int)mask[j]);
     const _Float16 *src = Mh + (size_t)a * KP;
     _Float16 *dst = Bx + (size_t)j * KP;
     constexpr int U = 8;  // elements per lane per pass, their loads in flight together
@@ -1015,19 +1076,32 @@ __global__ __launch_bounds__(64) void k_x_reflect_half(const int *__restrict__ A
 // result: element e of the target takes row element stab[m][e] of the
 // pair's mask m, with its sign; such a row is decoded into LDS first.
 // Fixed order: the sum is deterministic.
-template <int R4>
+template <int R4, bool GROUP = false>
 __global__ __launch_bounds__(256) void k_v_reduce(const int *__restrict__ work, const int *__restrict__ tptr,
                                                   const int *__restrict__ tlist, const unsigned *__restrict__ Y,
                                                   int npv, const unsigned char *__restrict__ xmask,
-                                                  const int *__restrict__ stab, double *__restrict__ L) {
+                                                  const int *__restrict__ stab, double *__restrict__ L,
+                                                  GroupLaunch g = {}) {
     constexpr int YS = (R4 + 31) / 32 * 32;
     static_assert(R4 % 4 == 0 && R4 / 4 <= 256, "one pass");
     __shared__ __attribute__((aligned(16))) float sx[R4];
-    const int b = work[blockIdx.x];
-    const int q0 = tptr[b], q1 = tptr[b + 1];
+    int bid = blockIdx.x;
+    if constexpr (GROUP) {  // (the nodes needing a local expansion, every member)
+        const MemberView &v = g.mv[group_member(g, blockIdx.x, bid)];
+        work = uni(v.m2l_work);
+        tptr = uni(v.ytptr);
+        tlist = uni(v.ytlist);
+        Y = uni(v.Yv);
+        npv = uni(v.vnpairs);
+        xmask = uni(v.xmask);
+        stab = uni(v.stab);
+        L = uni(v.L);
+    }
+    const int b = gu<GROUP>(work[bid]);
+    const int q0 = gu<GROUP>(tptr[b]), q1 = gu<GROUP>(tptr[b + 1]);
     if (q0 == q1) return;
     int qx = q1;  // the X rows close the list
-    while (qx > q0 && tlist[qx - 1] >= npv) --qx;
+    while (qx > q0 && gu<GROUP>(tlist[qx - 1]) >= npv) --qx;
     const int e = 4 * threadIdx.x;
     const bool act = e < R4;
     f4v ax = {0.f, 0.f, 0.f, 0.f};
@@ -1040,8 +1114,8 @@ __global__ __launch_bounds__(256) void k_v_reduce(const int *__restrict__ work, 
 #pragma unroll
         for (int i = 0; i < XB; ++i) {
             const int q = min(qb + i, q1 - 1);
-            const int row = tlist[q];
-            xm[i] = xmask[row - npv];
+            const int row = gu<GROUP>(tlist[q]);
+            xm[i] = gu<GROUP>((int)xmask[row - npv]);
             xv[i] = act ? load4_fp24(Y + fp24_offset(row, YS, e)) : f4v{};
         }
 #pragma unroll
@@ -1065,12 +1139,12 @@ __global__ __launch_bounds__(256) void k_v_reduce(const int *__restrict__ work, 
         f4v a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0, a2 = a0, a3 = a0;
         int q = q0;
         for (; q + 3 < qx; q += 4) {
-            a0 += load4_fp24(Y + fp24_offset(tlist[q], YS, e));
-            a1 += load4_fp24(Y + fp24_offset(tlist[q + 1], YS, e));
-            a2 += load4_fp24(Y + fp24_offset(tlist[q + 2], YS, e));
-            a3 += load4_fp24(Y + fp24_offset(tlist[q + 3], YS, e));
+            a0 += load4_fp24(Y + fp24_offset(gu<GROUP>(tlist[q]), YS, e));
+            a1 += load4_fp24(Y + fp24_offset(gu<GROUP>(tlist[q + 1]), YS, e));
+            a2 += load4_fp24(Y + fp24_offset(gu<GROUP>(tlist[q + 2]), YS, e));
+            a3 += load4_fp24(Y + fp24_offset(gu<GROUP>(tlist[q + 3]), YS, e));
         }
-        for (; q < qx; ++q) a0 += load4_fp24(Y + fp24_offset(tlist[q], YS, e));
+        for (; q < qx; ++q) a0 += load4_fp24(Y + fp24_offset(gu<GROUP>(tlist[q]), YS, e));
         const f4v t = ((a0 + a1) + (a2 + a3)) + ax;
         double *dst = L + (size_t)b * R4 + e;
         dst[0] += (double)t[0];
@@ -1105,7 +1179,9 @@ int launch_gemm(const VPairs &vp, const ibim_plan *pl, const T *ops, hipStream_t
 
 template <int P>
 int gemm_p(ibim_plan *pl, const VPairs &vp, bool mixed) {
-    if (vp.nblocks == 0) return IBIM_OK;
+    // (the FP16 stage also evaluates the X list: it runs when either list
+    // has pairs)
+    if (vp.nblocks == 0 && (&vp != &pl->vp || pl->xp.npairs == 0)) return IBIM_OK;
     constexpr int R4 = 4 * P * P * P;
     hipStream_t st = pl->stream;
     if constexpr (P <= 6) {
@@ -1832,6 +1908,105 @@ int build_m2l_operators_f32(ibim_plan *pl) {
     k_to_float<<<nblk(n), 256, 0, pl->stream>>>(pl->ops, n, pl->ops32);
     IBIM_LAUNCH_CHECK();
     return IBIM_OK;
+}
+
+// ------------------------------------------------------- fused groups ---
+// The M2L stage of a fused group matvec, in the order of the single-plan
+// stage (FP64/FP32: X list, then V list, both into L by atomics; FP16:
+// multipoles to FP16, V pairs, reflected X multipoles, X pairs, with the
+// per-target reduction in group_m2l_reduce)
+namespace {
+template <int P>
+int group_m2l_p(MatvecGroup *g) {
+    constexpr int R4 = 4 * P * P * P;
+    constexpr int KP = (R4 + 63) / 64 * 64;
+    hipStream_t st = g->st;
+    auto gl = [&](const MatvecGroup::Launch &L) {
+        return GroupLaunch{g->d_mv, g->d_tab + L.off, L.aux >= 0 ? g->d_tab + L.aux : nullptr, g->k};
+    };
+    if (g->form == IBIM_PREC_MIXED_F16) {
+        if constexpr (P <= 5) {
+            if (g->m_to_half.blocks)
+                k_m_to_half<true><<<g->m_to_half.blocks, 64, 0, st>>>(nullptr, R4, KP, nullptr, nullptr,
+                                                                       gl(g->m_to_half));
+            if (const int t = g->v_gemm.blocks) {
+                const int pxcd = (t + 7) / 8;
+                k_m2l_gemm_h32<P, kH32WavesV, true><<<8 * pxcd, 64 * kH32WavesV, 0, st>>>(
+                    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr, 0, t,
+                    pxcd, gl(g->v_gemm));
+            }
+            if (g->x_reflect.blocks)
+                k_x_reflect_half<true><<<g->x_reflect.blocks, 64, 0, st>>>(nullptr, nullptr, nullptr, R4, KP, nullptr,
+                                                                            nullptr, nullptr, nullptr, gl(g->x_reflect));
+            if (const int t = g->x_gemm.blocks) {
+                const int pxcd = (t + 7) / 8;
+                k_m2l_gemm_h32<P, kH32WavesX, true><<<8 * pxcd, 64 * kH32WavesX, 0, st>>>(
+                    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr, 0, t,
+                    pxcd, gl(g->x_gemm));
+            }
+            IBIM_LAUNCH_CHECK();
+            return IBIM_OK;
+        }
+        set_error("matvec group: the FP16 form runs p <= 5");
+        return IBIM_ERR_ARG;
+    }
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        if (const int t = g->x_gemm.blocks) {
+            const int pxcd = (t + 7) / 8;
+            k_m2l_gemm<P, T, true, true><<<8 * pxcd, 256, 0, st>>>(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                                   nullptr, nullptr, 0, 0, nullptr, nullptr, 0, t, pxcd,
+                                                                   gl(g->x_gemm));
+        }
+        if (const int t = g->v_gemm.blocks) {
+            const int pxcd = (t + 7) / 8;
+            k_m2l_gemm<P, T, true, false><<<8 * pxcd, 256, 0, st>>>(nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                                    nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, 0,
+                                                                    t, pxcd, gl(g->v_gemm));
+        }
+    };
+    if (g->form == IBIM_PREC_MIXED) run(float{});
+    else run(double{});
+    IBIM_LAUNCH_CHECK();
+    return IBIM_OK;
+}
+
+template <int P>
+int group_m2l_reduce_p(MatvecGroup *g) {
+    constexpr int R4 = 4 * P * P * P;
+    if constexpr (P <= 6) {
+        if (g->v_reduce.blocks)
+            k_v_reduce<R4, true><<<g->v_reduce.blocks, R4 / 4 <= 128 ? 128 : 256, 0, g->st>>>(
+                nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
+                GroupLaunch{g->d_mv, g->d_tab + g->v_reduce.off, nullptr, g->k});
+        IBIM_LAUNCH_CHECK();
+    }
+    return IBIM_OK;
+}
+}  // namespace
+
+int group_m2l(MatvecGroup *g) {
+    switch (g->p) {
+    case 2: return group_m2l_p<2>(g);
+    case 3: return group_m2l_p<3>(g);
+    case 4: return group_m2l_p<4>(g);
+    case 5: return group_m2l_p<5>(g);
+    case 6: return group_m2l_p<6>(g);
+    }
+    set_error("matvec group: unsupported Chebyshev order");
+    return IBIM_ERR_ARG;
+}
+
+int group_m2l_reduce(MatvecGroup *g) {
+    switch (g->p) {
+    case 2: return group_m2l_reduce_p<2>(g);
+    case 3: return group_m2l_reduce_p<3>(g);
+    case 4: return group_m2l_reduce_p<4>(g);
+    case 5: return group_m2l_reduce_p<5>(g);
+    case 6: return group_m2l_reduce_p<6>(g);
+    }
+    set_error("matvec group: unsupported Chebyshev order");
+    return IBIM_ERR_ARG;
 }
 
 }  // namespace ibim

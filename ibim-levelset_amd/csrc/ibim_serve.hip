@@ -9,8 +9,16 @@
 // alone.  Each request builds its own level set, tube nodes and plan on the
 // worker's stream, so requests of different sizes overlap on the device;
 // everything a solve needs stays in HBM until it finishes.
+//
+// Fusion: the GMRES matvecs of small requests (at most fuse_max tube
+// points) go through a batcher.  A solve that needs a matvec waits until
+// every registered solve waits too (they fall into lockstep: each GMRES
+// iteration is one matvec plus short orthogonalisation steps), or until
+// fuse_wait has passed, and then all waiting matvecs of one form run as one
+// fused group matvec: one launch per FMM stage over all of them.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cmath>
@@ -21,6 +29,143 @@
 #include <vector>
 
 #include "ibim_internal.h"
+
+namespace ibim {
+
+struct Batcher {
+    using clock = std::chrono::steady_clock;
+    struct Req {
+        ibim_plan *pl;
+        const double *x;
+        double *y;
+        bool mixed;
+        int rc = IBIM_OK;
+        bool done = false;
+    };
+    struct Cached {
+        std::vector<ibim_plan *> plans;  // in the group's member order
+        bool mixed;
+        MatvecGroup *g;
+        uint64_t used;
+    };
+    std::mutex mu;  // every field; the batch that completes the set is launched under it
+    std::condition_variable cv;
+    int active = 0;                      // registered solves
+    std::vector<Req *> pending;
+    std::vector<Cached> cache;           // groups by member set
+    uint64_t tick = 0;
+    std::chrono::microseconds wait{2000};
+    int64_t group_matvecs = 0, member_matvecs = 0;
+    ibim_stage_report acc{};             // stage reports of released groups
+    static constexpr size_t kMaxCached = 8;
+
+    void add_report(MatvecGroup *g) {
+        ibim_stage_report r;
+        if (timer_report(g->timer, &r, 1) != IBIM_OK) return;
+        for (int i = 0; i < IBIM_STAGE_SLOTS; ++i) {
+            acc.ms[i] += r.ms[i];
+            acc.launches[i] += r.launches[i];
+            acc.valu_flops[i] += r.valu_flops[i];
+            acc.mfma_flops[i] += r.mfma_flops[i];
+            acc.bytes[i] += r.bytes[i];
+        }
+    }
+    void release(size_t i) {
+        add_report(cache[i].g);
+        group_destroy(cache[i].g);
+        cache.erase(cache.begin() + (long)i);
+    }
+    // the group of these plans (sorted), built on first use; nullptr if they
+    // cannot run as one group (the caller runs them one by one)
+    MatvecGroup *group_for(const std::vector<ibim_plan *> &plans, bool mixed) {
+        for (Cached &c : cache)
+            if (c.mixed == mixed && c.plans == plans) {
+                c.used = ++tick;
+                return c.g;
+            }
+        MatvecGroup *g = nullptr;
+        if (group_create(plans.data(), (int)plans.size(), mixed, &g) != IBIM_OK) return nullptr;
+        if (cache.size() >= kMaxCached) {
+            size_t lru = 0;
+            for (size_t i = 1; i < cache.size(); ++i)
+                if (cache[i].used < cache[lru].used) lru = i;
+            release(lru);
+        }
+        cache.push_back({plans, mixed, g, ++tick});
+        return g;
+    }
+    // run every pending matvec (under mu): per form, one fused group of all
+    // its requests, or single matvecs where a group cannot be formed
+    void launch_pending() {
+        std::vector<Req *> batch;
+        batch.swap(pending);
+        for (int form = 0; form < 2; ++form) {
+            std::vector<Req *> part;
+            for (Req *r : batch)
+                if (r->mixed == (form == 1)) part.push_back(r);
+            for (size_t i0 = 0; i0 < part.size(); i0 += kMaxGroup) {
+                const size_t i1 = std::min(part.size(), i0 + (size_t)kMaxGroup);
+                std::vector<Req *> chunk(part.begin() + (long)i0, part.begin() + (long)i1);
+                std::sort(chunk.begin(), chunk.end(), [](Req *a, Req *b) { return a->pl < b->pl; });
+                MatvecGroup *g = nullptr;
+                if (chunk.size() > 1) {
+                    std::vector<ibim_plan *> plans;
+                    for (Req *r : chunk) plans.push_back(r->pl);
+                    g = group_for(plans, form == 1);
+                }
+                if (g) {
+                    std::vector<const double *> xs;
+                    std::vector<double *> ys;
+                    for (Req *r : chunk) {
+                        xs.push_back(r->x);
+                        ys.push_back(r->y);
+                    }
+                    const int rc = group_matvec_ordered(g, xs.data(), ys.data());
+                    for (Req *r : chunk) r->rc = rc;
+                    group_matvecs++;
+                    member_matvecs += (int64_t)chunk.size();
+                } else {
+                    for (Req *r : chunk) r->rc = matvec_single(r->pl, r->x, r->y, r->mixed);
+                }
+            }
+        }
+        for (Req *r : batch) r->done = true;
+    }
+    void enter(ibim_plan *pl) {
+        std::lock_guard<std::mutex> lk(mu);
+        pl->batcher = this;
+        active++;
+    }
+    void leave(ibim_plan *pl) {
+        std::lock_guard<std::mutex> lk(mu);
+        pl->batcher = nullptr;
+        active--;
+        for (size_t i = cache.size(); i-- > 0;)
+            if (std::find(cache[i].plans.begin(), cache[i].plans.end(), pl) != cache[i].plans.end()) release(i);
+        cv.notify_all();  // the waiting solves may now be all there is
+    }
+    ~Batcher() {
+        while (!cache.empty()) release(cache.size() - 1);
+    }
+};
+
+int batch_matvec(Batcher *b, ibim_plan *pl, const double *d_x, double *d_y, bool mixed) {
+    Batcher::Req r{pl, d_x, d_y, mixed};
+    std::unique_lock<std::mutex> lk(b->mu);
+    b->pending.push_back(&r);
+    const auto deadline = Batcher::clock::now() + b->wait;
+    while (!r.done) {
+        if ((int)b->pending.size() >= b->active || Batcher::clock::now() >= deadline) {
+            b->launch_pending();
+            b->cv.notify_all();
+        } else {
+            b->cv.wait_until(lk, deadline);
+        }
+    }
+    return r.rc;
+}
+
+}  // namespace ibim
 
 struct ibim_server {
     struct Job {
@@ -44,6 +189,8 @@ struct ibim_server {
     int device = 0;
     unsigned stage_mask = 0;    // stages recorded in each request's plan
     ibim_stage_report acc{};    // summed over finished requests
+    int64_t fuse_max = 0;       // requests of at most this many tube points fuse their matvecs
+    ibim::Batcher batcher;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -131,7 +278,15 @@ void run_job(ibim_server *s, ibim_server::Job &j, hipStream_t st) {
             rc = IBIM_ERR_HIP;
     }
     if (rc == IBIM_OK) rc = ibim_rhs_device(plan, m, d_at, d_at + 3 * m, d_b);
+    int64_t fuse_max = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        fuse_max = s->fuse_max;
+    }
+    const bool fuse = rc == IBIM_OK && fuse_max > 0 && inf.n_nodes <= fuse_max;
+    if (fuse) s->batcher.enter(plan);
     if (rc == IBIM_OK) rc = ibim_gmres_device(plan, d_b, d_x, j.req.restart, j.req.maxit, j.req.tol, &res.info);
+    if (fuse) s->batcher.leave(plan);
     const int solve_rc = rc;
     if (rc >= 0) rc = ibim_polarize_device(plan, m, d_at, d_x, d_phi);
     if (rc >= 0) {
@@ -310,9 +465,50 @@ int ibim_server_stage_report(ibim_server *s, ibim_stage_report *out, int reset) 
         set_error("bad argument to ibim_server_stage_report");
         return IBIM_ERR_ARG;
     }
+    // the fused launches' stage intervals: released groups, then the live ones
+    ibim_stage_report fused;
+    {
+        ibim::Batcher &b = s->batcher;
+        std::lock_guard<std::mutex> lk(b.mu);
+        for (auto &c : b.cache) b.add_report(c.g);
+        fused = b.acc;
+        if (reset) b.acc = ibim_stage_report{};
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     *out = s->acc;
+    for (int i = 0; i < IBIM_STAGE_SLOTS; ++i) {
+        out->ms[i] += fused.ms[i];
+        out->launches[i] += fused.launches[i];
+        out->valu_flops[i] += fused.valu_flops[i];
+        out->mfma_flops[i] += fused.mfma_flops[i];
+        out->bytes[i] += fused.bytes[i];
+    }
     if (reset) s->acc = ibim_stage_report{};
+    return IBIM_OK;
+}
+
+int ibim_server_set_fusion(ibim_server *s, int64_t max_points, int wait_us) {
+    if (!s || max_points < 0 || wait_us < 0 || wait_us > 1000000) {
+        set_error("bad argument to ibim_server_set_fusion (wait 0..1e6 us)");
+        return IBIM_ERR_ARG;
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->fuse_max = max_points;
+    }
+    std::lock_guard<std::mutex> lk(s->batcher.mu);
+    s->batcher.wait = std::chrono::microseconds(wait_us);
+    return IBIM_OK;
+}
+
+int ibim_server_fusion_stats(ibim_server *s, int64_t *group_matvecs, int64_t *member_matvecs) {
+    if (!s) {
+        set_error("server is NULL");
+        return IBIM_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(s->batcher.mu);
+    if (group_matvecs) *group_matvecs = s->batcher.group_matvecs;
+    if (member_matvecs) *member_matvecs = s->batcher.member_matvecs;
     return IBIM_OK;
 }
 

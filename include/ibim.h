@@ -168,6 +168,30 @@ int ibim_plan_stage_report(ibim_plan *plan, ibim_stage_report *out, int reset);
 int ibim_matvec(ibim_plan *plan, const double *x, double *y);
 int ibim_matvec_device(ibim_plan *plan, const double *d_x, double *d_y);
 
+/* ---- fused matvecs over several plans --------------------------------------
+ * No counterpart in the reference, which applies one molecule's operator at a
+ * time (electric.cpp:294-388 per GMRES step, one molecule per process).  A
+ * group holds k (1..16) plans of one device, Chebyshev order (p <= 6) and
+ * kernel family; ibim_group_matvec_device then computes y_m = A_m x_m for
+ * every member with ONE launch per FMM stage (charges, P2M, each M2M and L2L
+ * level, M2L, near field, L2P, combination), the members' blocks one after
+ * another, on the group's own streams: small solves that would each leave
+ * most of the GPU idle share its launches.  mixed = 0: the FP64 matvec of
+ * every member; mixed != 0: the mixed-precision matvec of the members'
+ * precision (all the same; the FP16 form for p <= 5).  Each member's result
+ * is the one ibim_matvec_device gives (the same kernels on the same data;
+ * bit for bit where those are deterministic).  A call is ordered after the
+ * work already queued on every member's stream, and work queued there later
+ * runs after it.  The group must be destroyed before any member, and a
+ * member's plan must not be used elsewhere while a group call runs. */
+typedef struct ibim_group ibim_group;
+int ibim_group_create(ibim_plan *const *plans, int k, int mixed, ibim_group **out);
+int ibim_group_matvec_device(ibim_group *g, const double *const *d_x, double *const *d_y);
+/* stage timers of the fused launches follow the first member's selection
+   (ibim_plan_profile_stages) at creation */
+int ibim_group_stage_report(ibim_group *g, ibim_stage_report *out, int reset);
+void ibim_group_destroy(ibim_group *g);
+
 /* one black-box FMM: potential[t] = sum_s K(src_s, tgt_t) charge[s] for one
  * of the 16 kernels (kernel_id = IBIM_K*), far field by Chebyshev FMM */
 int ibim_kernel_fmm(ibim_plan *plan, int kernel_id, const double *charge, double *potential);
@@ -327,6 +351,18 @@ void ibim_server_destroy(ibim_server *s);
  * have finished.  No counterpart in the reference. */
 int ibim_server_profile_stages(ibim_server *s, unsigned stage_mask);
 int ibim_server_stage_report(ibim_server *s, ibim_stage_report *out, int reset);
+/* Fused stage launches across concurrent solves (no counterpart in the
+ * reference).  Requests of at most max_points tube points run their GMRES
+ * matvecs through a shared batcher: a solve that needs a matvec waits up to
+ * wait_us microseconds for the other such solves to need one too, and then
+ * all waiting matvecs of one form run as one fused group matvec (see
+ * ibim_group_create); larger requests, and solves that find no partner,
+ * run their own.  Each result is the same as alone (same kernels, same
+ * data).  max_points = 0 switches fusion off (the default); applies to
+ * requests started after the call. */
+int ibim_server_set_fusion(ibim_server *s, int64_t max_points, int wait_us);
+/* fused launches so far: group matvecs and the member matvecs they ran */
+int ibim_server_fusion_stats(ibim_server *s, int64_t *group_matvecs, int64_t *member_matvecs);
 
 const char *ibim_last_error(void);
 const char *ibim_version(void);
