@@ -35,6 +35,15 @@ by device footprint):
 3. The FP64 stream: the FP64 subsample as a Poisson process at `--load` times
    its FP64 capacity: served FP64 solves/s and p50/p99.
 
+4. Fusion (small molecules): 32 globules of 10..40 atoms (11k..32k tube
+   points, where one solve's matvec leaves most of the GPU idle) served as a
+   burst on 16 slots with fused stage launches across the concurrent solves
+   (ibim_server_set_fusion) and without; and one matvec of each of 8 of them,
+   issued on their own streams against one fused group matvec.  The
+   headline server fuses solves of at most --fuse-points tube points (the
+   stream's smallest requests, about 100k points, fill the GPU alone: fusion
+   measured neutral there).
+
 --steps does not change the stream: its subsample fixes the timed requests.
 --warmup W serves W small molecules in each precision before timing.
 
@@ -129,14 +138,14 @@ def stream_molecules(ibim, cfg):
     return sizes, mols
 
 
-def serve(ibim, mols, cfg, precision, slots, rate=None, profile=None, arrival_seed=5, label=None):
+def serve(ibim, mols, cfg, precision, slots, rate=None, profile=None, arrival_seed=5, label=None, fuse=0):
     """Push molecules through a server: all at t = 0 (rate None) or as a
     Poisson process of the given rate.  Returns the results (in molecule
     order), the wall time from the first arrival to the last completion and
     the server's stage report."""
     h = cfg["h"]
     kw = fmm_kw(cfg, h, precision)
-    with ibim.Server(slots=slots) as srv:
+    with ibim.Server(slots=slots, fuse_points=fuse) as srv:
         if profile:
             srv.profile(profile)
         ids = [None] * len(mols)
@@ -168,8 +177,10 @@ def serve(ibim, mols, cfg, precision, slots, rate=None, profile=None, arrival_se
         wall = time.perf_counter() - t0
         win += [time.clock_gettime_ns(time.CLOCK_MONOTONIC), time.clock_gettime_ns(time.CLOCK_BOOTTIME)]
         rep = srv.stage_report() if profile else None
+        fused = srv.fusion_stats()
     if profile is not None:
-        rep = dict(rep or {}, _window_ns={"monotonic": [win[0], win[2]], "boottime": [win[1], win[3]]})
+        rep = dict(rep or {}, _window_ns={"monotonic": [win[0], win[2]], "boottime": [win[1], win[3]]},
+                   _fused=fused)
     return res, wall, rep
 
 
@@ -254,7 +265,7 @@ def run_stream(ibim, args, cfg):
     lam_m = args.load * mu_m_sub * scale
     stages = ["P2P_MIXED", "M2L_MIXED", "MATVEC_MIXED", "P2P", "M2L", "MATVEC"]
     res_m, wall_m, rep = serve(ibim, head_mols, cfg, ibim.PREC_MIXED_F16, slots, rate=lam_m, profile=stages,
-                               label="mixed stream")
+                               label="mixed stream", fuse=args.fuse_points)
     mixed = summarize(res_m, wall_m)
     mixed["arrival_rate_per_s"] = lam_m
     mixed["subsample"] = f"every {args.every}th request of the {len(mols)} ({len(head)} requests)"
@@ -268,7 +279,7 @@ def run_stream(ibim, args, cfg):
     # 3. the FP64 stream on the FP64 subsample
     lam_d = args.load * mu_d_sub
     res_d, wall_d, _ = serve(ibim, sub_mols, cfg, ibim.PREC_FP64, slots, rate=lam_d, arrival_seed=6,
-                             label="fp64 stream")
+                             label="fp64 stream", fuse=args.fuse_points)
     fp64 = summarize(res_d, wall_d)
     fp64["arrival_rate_per_s"] = lam_d
     fp64["subsample"] = f"every {args.fp64_every}th request of the {len(mols)} ({len(sub)} requests)"
@@ -277,6 +288,7 @@ def run_stream(ibim, args, cfg):
     max_err = max(errs + errs_in_stream)
     ok = max_err <= 1e-6 and mixed["all_converged"] and fp64["all_converged"]
     window = rep.pop("_window_ns")
+    mixed["fused_group_matvecs"], mixed["fused_member_matvecs"] = rep.pop("_fused")
     roof = roofline_entries(rep, cfg["p"])
     stage_tot = {k: {"ms": v["ms"], "launches": v["launches"]} for k, v in rep.items()}
     return {
@@ -291,6 +303,76 @@ def run_stream(ibim, args, cfg):
                                f"FP64 burst"},
         "ok": ok, "roof": roof, "stages": stage_tot, "window": window,
     }
+
+
+# ----------------------------------------------------------------- fusion --
+def run_fusion(ibim, cfg):
+    """Small molecules: served bursts with and without fused stage launches
+    across the concurrent solves, and a fused group matvec against the same
+    matvecs issued one per plan on their own streams."""
+    import ctypes as C
+    import torch
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(9)
+    atoms = np.exp(rng.uniform(np.log(10), np.log(40), 32)).astype(int)
+    mols = [ibim.synthetic_globule(int(m), 50000 + i, cfg["confine"] * m ** (1 / 3)) for i, m in enumerate(atoms)]
+    kw = fmm_kw(cfg, cfg["h"], ibim.PREC_MIXED_F16)
+    out = {"workload": f"32 synthetic globules of {atoms.min()}..{atoms.max()} atoms (log-uniform, seed 9), h="
+                       f"{cfg['h']} A, p={cfg['p']}, leaf {cfg['leaf']}, FP16 mixed form, served as one burst on "
+                       f"16 slots"}
+
+    def burst(fuse):
+        with ibim.Server(slots=16, fuse_points=fuse, fuse_wait_us=2000) as srv:
+            t0 = time.perf_counter()
+            ids = [srv.submit(a, r, q, h=cfg["h"], restart=cfg["restart"], tol=cfg["tol"], **kw) for a, r, q in mols]
+            res = [srv.wait(i) for i in ids]
+            wall = time.perf_counter() - t0
+            return res, wall, srv.fusion_stats()
+
+    burst(0)  # warm-up
+    res0, w0, _ = burst(0)
+    res1, w1, st = burst(10 ** 7)
+    out["tube_points"] = [int(r["n_points"]) for r in res0]
+    out["unfused_solves_per_s"] = len(res0) / w0
+    out["fused_solves_per_s"] = len(res1) / w1
+    out["fused_speedup"] = w0 / w1
+    out["fused_group_matvecs"], out["fused_member_matvecs"] = st
+    out["max_rel_energy_diff_fused_vs_unfused"] = max(abs(a["energy"] - b["energy"]) / abs(a["energy"])
+                                                      for a, b in zip(res0, res1))
+    out["all_converged"] = all(r["status"] == 0 for r in res0 + res1)
+    # one matvec per plan on its own stream vs one fused group matvec
+    surfs = [ibim.Surface.from_atoms(a, r, h=cfg["h"], probe=1.4, thickness=2) for a, r, _ in mols[:8]]
+    plans = [ibim.Plan(device_ptrs=s.device_ptrs(), **kw) for s in surfs]
+    xs = [torch.rand(2 * pl.n, dtype=torch.float64, device=dev) for pl in plans]
+    ys = [torch.empty_like(x) for x in xs]
+    P = lambda t: C.c_void_p(t.data_ptr())
+    torch.cuda.synchronize()
+    with ibim.Group(plans, mixed=True) as grp:
+        def singles():
+            for pl, x, y in zip(plans, xs, ys):
+                pl.matvec_device(P(x), P(y))
+            for pl in plans:
+                pl.sync()
+
+        def fused():
+            grp.matvec_device([P(x) for x in xs], [P(y) for y in ys])
+            for pl in plans:
+                pl.sync()
+
+        t = {}
+        for name, f in (("single", singles), ("group", fused)):
+            f()
+            t0 = time.perf_counter()
+            for _ in range(40):
+                f()
+            t[name] = 1e3 * (time.perf_counter() - t0) / 40
+    for pl in plans:
+        pl.close()
+    for s_ in surfs:
+        s_.close()
+    out["matvec_round_8_plans_ms"] = {"one_per_plan_on_own_streams": t["single"], "fused_group": t["group"],
+                                      "points": [int(pl_n) for pl_n in [len(x) // 2 for x in xs]]}
+    return out
 
 
 # ---------------------------------------------------------- CPU baseline --
@@ -464,6 +546,10 @@ def main():
                     help="the capacity bursts and the FP64 stream serve every k-th request")
     ap.add_argument("--requests", type=int, default=None, help="stream length (default: the workload's 256)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fusion", action="store_true", help="skip the small-molecule fusion leg")
+    ap.add_argument("--fuse-points", type=int, default=150000,
+                    help="the headline server fuses the matvecs of concurrent solves of at most this many tube "
+                         "points (0: off)")
     ap.add_argument("--leaf", type=int, default=None, help="override the leaf capacity")
     ap.add_argument("--molecule", type=int, default=None, metavar="I",
                     help="stream256: time single solves of the stream's molecule I instead of serving")
@@ -522,6 +608,12 @@ def main():
     if args.requests:
         cfg["requests"] = args.requests
     st = run_stream(ibim, args, cfg)
+    fusion = None
+    if not args.no_fusion:
+        fusion = run_fusion(ibim, cfg)
+        progress(f"fusion (small molecules): {fusion['fused_solves_per_s']:.1f} vs {fusion['unfused_solves_per_s']:.1f} "
+                 f"solves/s, matvec round {fusion['matvec_round_8_plans_ms']['fused_group']:.2f} vs "
+                 f"{fusion['matvec_round_8_plans_ms']['one_per_plan_on_own_streams']:.2f} ms")
     cpu = None
     if not args.no_cpu:
         try:
@@ -558,7 +650,8 @@ def main():
                                f"h={cfg['h']} A, p={cfg['p']}, leaf {cfg['leaf']}, kappa={cfg['kappa']}, eps "
                                f"{cfg['eps_in']:g}/{cfg['eps_out']:g}, GMRES({cfg['restart']}) tol {cfg['tol']}; "
                                f"Poisson arrivals at {args.load:g} of the saturated rate; geometry, plan, solve and "
-                               f"energy per request; {args.slots} server slots; timed on every {args.every}th "
+                               f"energy per request; {args.slots} server slots, fused matvecs of concurrent solves "
+                               f"of at most {args.fuse_points} tube points; timed on every {args.every}th "
                                f"request (mixed) and every {args.fp64_every}th (FP64)",
                    "steps": "one step = one served request of the timed mixed stream",
                    "requests": cfg["requests"], "timed_requests": mixed["requests"],
@@ -574,6 +667,7 @@ def main():
                                           "spans on the two streams of a matvec, and of two concurrent requests, "
                                           "overlap in time, so they do not add up to the wall time"),
         "timed_window_ns": st["window"],
+        "fusion_small_molecules": fusion,
         "cpu_baseline": cpu,
         "mixed_vs_cpu": (mixed["solves_per_s"] / cpu["value"]) if cpu and "value" in cpu else None,
     }
