@@ -13,8 +13,11 @@ kappa = 0.1257 / A, eps 2 / 80.
   p = 8 the FP16 stage adds its results into L by atomics, so two mixed
   matvecs agree to rounding, not bit for bit.)
 
-About 10 minutes of GPU time, so it runs only when IBIM_GLOBULE50000=1; GMRES
-prints its iteration table (run pytest with -s to see it).
+The two solves take about 10 minutes of GPU time, so they run only when
+IBIM_GLOBULE50000=1 (GMRES prints its iteration table; run pytest with -s to
+see it).  The quick check always runs (about a minute): one kernel's FMM at
+full size against the exact polynomial sums, and the FP16-mixed matvec
+against the FP64 one on the same plan.
 """
 import ctypes as C
 import os
@@ -25,9 +28,9 @@ import pytest
 
 from poly_moments import poly_kernel_sum
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(os.environ.get("IBIM_GLOBULE50000") != "1",
-                                 reason="about 10 minutes of GPU time: set IBIM_GLOBULE50000=1 to run")]
+pytestmark = pytest.mark.gpu
+long_run = pytest.mark.skipif(os.environ.get("IBIM_GLOBULE50000") != "1",
+                              reason="about 10 minutes of GPU time: set IBIM_GLOBULE50000=1 to run")
 
 H, P, LEAF = 0.3, 8, 512
 PB = dict(p=P, max_points=LEAF, max_level=10, kappa=0.1257, eps_in=2.0, eps_out=80.0, vacant_radius=H)
@@ -47,6 +50,46 @@ def globule(ibim, gpu):
     s.close()
 
 
+def test_quick_full_size_fmm_and_mixed_matvec(ibim, globule, gpu):
+    """K2 with the polynomial surrogate kernel exact at full size (every box,
+    list entry, M2L tile and near-field unit of the p = 8 tree), and the
+    FP16-mixed matvec within 2e-5 of the FP64 one on the PB plan (the bar of
+    the smaller mixed-precision tests), repeating to rounding."""
+    import torch
+    _, _, s = globule
+    pts, _, _ = s.arrays()
+    q = np.random.default_rng(50001).normal(size=s.n)
+    with ibim.Plan(device_ptrs=s.device_ptrs(), p=P, max_points=LEAF, max_level=10,
+                   kernel_mode=ibim.KERNEL_POLY) as plan:
+        t = time.time()
+        g = plan.kernel_fmm("K2", q)
+        ref = poly_kernel_sum("K2", pts, q)
+        err = np.abs(g - ref).max() / np.abs(ref).max()
+        say(f"K2 (polynomial) at full size: {time.time() - t:.1f} s, max error {err:.2e} of max |sum|")
+        assert err < 1e-9
+    x = torch.from_numpy(np.random.default_rng(50002).uniform(-1, 1, 2 * s.n)).to(gpu)
+    y64, y16, y16b = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
+    torch.cuda.synchronize()
+    Pt = lambda v: C.c_void_p(v.data_ptr())
+    with ibim.Plan(device_ptrs=s.device_ptrs(), precision=ibim.PREC_FP64, **PB) as plan:
+        t = time.time()
+        plan.matvec_device(Pt(x), Pt(y64))
+        plan.sync()
+        say(f"FP64 matvec {time.time() - t:.1f} s")
+        plan.set_precision(ibim.PREC_MIXED_F16)
+        t = time.time()
+        plan.matvec_device(Pt(x), Pt(y16))
+        plan.matvec_device(Pt(x), Pt(y16b))
+        plan.sync()
+        say(f"FP16 operators and two mixed matvecs {time.time() - t:.1f} s")
+    err = float(torch.linalg.norm(y16 - y64) / torch.linalg.norm(y64))
+    rep = float(torch.linalg.norm(y16b - y16) / torch.linalg.norm(y16))
+    say(f"mixed vs FP64 matvec: {err:.2e}; two mixed matvecs differ by {rep:.1e}")
+    assert 0 < err < 2e-5
+    assert rep < 1e-12
+
+
+@long_run
 def test_fmm_exact_for_polynomial_kernels(ibim, globule):
     _, _, s = globule
     pts, _, _ = s.arrays()
@@ -64,6 +107,7 @@ def test_fmm_exact_for_polynomial_kernels(ibim, globule):
             assert err < 1e-9, k
 
 
+@long_run
 def test_identity_when_media_match(ibim, globule):
     _, _, s = globule
     x = np.random.default_rng(1).normal(size=2 * s.n)
@@ -72,6 +116,7 @@ def test_identity_when_media_match(ibim, globule):
     assert np.array_equal(y, x)
 
 
+@long_run
 def test_fp64_and_mixed_solves_agree(ibim, globule, gpu):
     import torch
     xyz, qa, s = globule
