@@ -1847,9 +1847,16 @@ int group_matvec_ordered(MatvecGroup *g, const double *const *x, double *const *
         IBIM_HIP_CHECK(hipEventRecord(g->ev_in[m], g->pl[m]->stream));
         IBIM_HIP_CHECK(hipStreamWaitEvent(g->st, g->ev_in[m], 0));
     }
-    if (int rc = group_matvec(g, x, y)) return rc;
+    const int rc = group_matvec(g, x, y);
+    // (also after a failed launch: whatever was queued on the group's streams
+    // comes before the members' later work on their buffers)
+    if (rc) {
+        IBIM_HIP_CHECK(hipEventRecord(g->ev_join, g->st2));
+        IBIM_HIP_CHECK(hipStreamWaitEvent(g->st, g->ev_join, 0));
+        IBIM_HIP_CHECK(hipEventRecord(g->ev_done, g->st));
+    }
     for (int m = 0; m < g->k; ++m) IBIM_HIP_CHECK(hipStreamWaitEvent(g->pl[m]->stream, g->ev_done, 0));
-    return IBIM_OK;
+    return rc;
 }
 
 int group_matvec(MatvecGroup *g, const double *const *x, double *const *y) {
