@@ -547,6 +547,8 @@ def main():
     ap.add_argument("--requests", type=int, default=None, help="stream length (default: the workload's 256)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fusion", action="store_true", help="skip the small-molecule fusion leg")
+    ap.add_argument("--fusion-only", action="store_true",
+                    help="run only the small-molecule fusion leg and print its JSON (kernel-trace mode)")
     ap.add_argument("--fuse-points", type=int, default=150000,
                     help="the headline server fuses the matvecs of concurrent solves of at most this many tube "
                          "points (0: off)")
@@ -607,6 +609,9 @@ def main():
 
     if args.requests:
         cfg["requests"] = args.requests
+    if args.fusion_only:
+        print(json.dumps({"mode": "fusion-only", "fusion_small_molecules": run_fusion(ibim, cfg)}))
+        return
     st = run_stream(ibim, args, cfg)
     fusion = None
     if not args.no_fusion:
