@@ -46,20 +46,14 @@ __device__ __forceinline__ double gget(const double *__restrict__ g, const GridD
     return g[((size_t)i * d.ny + j) * d.nz + k];
 }
 
-// WENO5 forward (up) and backward (un) derivatives along dir (levelset.cpp:330-396)
-__device__ __forceinline__ void weno(const double *__restrict__ g, const GridDims &d, int i, int j, int k, int dir,
-                                     double &up, double &un) {
-    double w[7];
-#pragma unroll
-    for (int l = 0; l < 7; ++l) {
-        const int o = l - 3;
-        w[l] = gget(g, d, i + (dir == 0 ? o : 0), j + (dir == 1 ? o : 0), k + (dir == 2 ? o : 0));
-    }
+// WENO5 forward (up) and backward (un) derivatives (levelset.cpp:330-396)
+// from the seven values w[l] at offsets l - 3 along one axis
+__device__ __forceinline__ void weno7(const double (&w)[7], double rh, double &up, double &un) {
     // the divisions of the reference are multiplications by constant
     // reciprocals and Newton-refined reciprocals here (each within an ulp;
     // an FP64 division costs about four times as many instructions)
     const double EPS_LS = 1e-6;
-    const double r12 = 1.0 / 12.0, r3 = 1.0 / 3.0, r6 = 1.0 / 6.0, rh = d.rh;
+    const double r12 = 1.0 / 12.0, r3 = 1.0 / 3.0, r6 = 1.0 / 6.0;
     double a = w[1] - w[0], b = w[2] - w[1], c = w[3] - w[2], dd = w[4] - w[3], e = w[5] - w[4], f = w[6] - w[5];
     const double df = (-b + 7.0 * (c + dd) - e) * r12;
     a = b - a;
@@ -90,6 +84,18 @@ __device__ __forceinline__ void weno(const double *__restrict__ g, const GridDim
     w0 = p0 * rs;
     w2 = p2 * rs;
     un = (df - w0 * (a - 2 * b + c) * r3 - (w2 - 0.5) * (b - 2 * c + dd) * r6) * rh;
+}
+
+// the same along dir at grid point (i, j, k), clamped reads
+__device__ __forceinline__ void weno(const double *__restrict__ g, const GridDims &d, int i, int j, int k, int dir,
+                                     double &up, double &un) {
+    double w[7];
+#pragma unroll
+    for (int l = 0; l < 7; ++l) {
+        const int o = l - 3;
+        w[l] = gget(g, d, i + (dir == 0 ? o : 0), j + (dir == 1 ? o : 0), k + (dir == 2 ? o : 0));
+    }
+    weno7(w, d.rh, up, un);
 }
 
 // Godunov norm getNorm(A, B) = sqrt(sum max(A+, B-)^2) (levelset.cpp:294-301)
@@ -160,37 +166,95 @@ __global__ void k_expand(unsigned long long *__restrict__ g, GridDims d, const d
     (void)wmax;
 }
 
-// one TVD-RK3 stage of evolve (mode 0) or reinitialize (mode 1)
+// one TVD-RK3 stage of evolve (MODE 0) or reinitialize (MODE 1)
 //   stage 1: out = base - T(base)
 //   stage 2: out = (3 base + src - T(src)) / 4
 //   stage 3: out = (base + 2 (src - T(src))) / 3      (out may alias base)
-// T(u) = dtv |grad u|_G (evolve) or dtv sign(phi0) (|grad u|_G - 1) (reinit)
+// T(u) = dtv |grad u|_G (evolve) or dtv sgn (|grad u|_G - 1) (reinit, sgn =
+// phi0 / sqrt(phi0^2 + h^2) per point, k_sign).  A block is a tile of 64 (k)
+// x 4 (j) points of one i-plane: the source's rows j0 - 3 .. j0 + 6 and
+// columns k0 - 3 .. k0 + 66 of the plane are staged in LDS, clamped at the
+// grid's faces as gget clamps, and the j and k stencils read them there; the
+// i stencil reads the seven planes directly through plane pointers that are
+// uniform over the block, at a 32-bit in-plane byte offset.  Needs
+// ny * nz * 8 < 2^32 (the host checks).
+constexpr int kLsTk = 64, kLsTj = 4;
+constexpr int kLsRows = kLsTj + 6, kLsCols = kLsTk + 6;
+
+__device__ __forceinline__ double at_byte(const double *p, unsigned off) {
+    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(p) + off);
+}
+
+template <int MODE, int STAGE>
 __global__ __launch_bounds__(256) void k_ls_stage(const double *base, const double *__restrict__ src,
-                                                  const double *__restrict__ phi0, double *out, GridDims d, int mode,
-                                                  int stage, double dtv, double eps) {
-    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
-    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= n) return;
-    int i, j, k;
-    index3(I, d, i, j, k);
+                                                  const double *__restrict__ sgn, double *out, GridDims d,
+                                                  double dtv) {
+    __shared__ double t[kLsRows][kLsCols];
+    const int k0 = blockIdx.x * kLsTk, j0 = blockIdx.y * kLsTj, i = blockIdx.z;
+    const size_t plane = (size_t)d.ny * d.nz;
+    const double *sp = src + (size_t)i * plane;
+    for (int e = threadIdx.y * kLsTk + threadIdx.x; e < kLsRows * kLsCols; e += kLsTk * kLsTj) {
+        const int r = e / kLsCols, c = e - r * kLsCols;
+        const int jj = min(max(j0 - 3 + r, 0), d.ny - 1);
+        const int kk = min(max(k0 - 3 + c, 0), d.nz - 1);
+        t[r][c] = at_byte(sp, 8u * (unsigned)(jj * d.nz + kk));
+    }
+    __syncthreads();
+    const int tk = threadIdx.x, tj = threadIdx.y;
+    const int k = k0 + tk, j = j0 + tj;
+    if (k >= d.nz || j >= d.ny) return;
+    const unsigned off = 8u * (unsigned)(j * d.nz + k);
     double up[3], un[3];
+    {
+        double w[7];
 #pragma unroll
-    for (int dir = 0; dir < 3; ++dir) weno(src, d, i, j, k, dir, up[dir], un[dir]);
+        for (int l = 0; l < 7; ++l) {
+            const int ii = min(max(i + l - 3, 0), d.nx - 1);
+            w[l] = l == 3 ? t[tj + 3][tk + 3] : at_byte(src + (size_t)ii * plane, off);
+        }
+        weno7(w, d.rh, up[0], un[0]);
+    }
+    {
+        double w[7];
+#pragma unroll
+        for (int l = 0; l < 7; ++l) w[l] = t[tj + l][tk + 3];
+        weno7(w, d.rh, up[1], un[1]);
+    }
+    {
+        double w[7];
+#pragma unroll
+        for (int l = 0; l < 7; ++l) w[l] = t[tj + 3][tk + l];
+        weno7(w, d.rh, up[2], un[2]);
+    }
     double T;
-    if (mode == 0) {
+    if constexpr (MODE == 0) {
         T = dtv * godunov(un, up);
     } else {
-        const double p0 = phi0[I];
-        const double sgn = p0 / sqrt(p0 * p0 + eps);
-        const double nd = sgn > 0. ? godunov(un, up) : godunov(up, un);
-        T = dtv * sgn * (nd - 1.0);
+        const double s = at_byte(sgn + (size_t)i * plane, off);
+        const double nd = s > 0. ? godunov(un, up) : godunov(up, un);
+        T = dtv * s * (nd - 1.0);
     }
-    const double b = base[I];
+    const double c = t[tj + 3][tk + 3];  // src at the point
+    double *op = out + (size_t)i * plane;
     double v;
-    if (stage == 1) v = b - T;
-    else if (stage == 2) v = (3 * b + src[I] - T) * 0.25;
-    else v = (b + 2 * (src[I] - T)) * (1.0 / 3.0);
-    out[I] = v;
+    if constexpr (STAGE == 1) {
+        v = c - T;  // base is src
+    } else {
+        const double b = at_byte(base + (size_t)i * plane, off);
+        if constexpr (STAGE == 2) v = (3 * b + c - T) * 0.25;
+        else v = (b + 2 * (c - T)) * (1.0 / 3.0);
+    }
+    *reinterpret_cast<double *>(reinterpret_cast<char *>(op) + off) = v;
+}
+
+// phi0 -> sign(phi0) = phi0 / sqrt(phi0^2 + eps) in place (the reinitialisation's
+// smoothed sign, levelset.cpp:191-287; constant over its steps)
+__global__ void k_sign(double *__restrict__ g, int64_t n, double eps) {
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I < n) {
+        const double p0 = g[I];
+        g[I] = p0 / sqrt(p0 * p0 + eps);
+    }
 }
 
 __global__ void k_scale(double *__restrict__ g, int64_t n, double s) {
@@ -395,6 +459,8 @@ static int ls_rk3(LsRun &L, int mode, double final_t, double vel, double cfl, do
     const double dtv = dt * vel;
     const double eps = L.d.h * L.d.h;
     const int nb = nblk(L.n);
+    // the tiles of k_ls_stage: 64 (k) x 4 (j) points of one i-plane
+    const dim3 tgrid((L.d.nz + kLsTk - 1) / kLsTk, (L.d.ny + kLsTj - 1) / kLsTj, L.d.nx), tblk(kLsTk, kLsTj);
     unsigned long long *cnt = nullptr;
     double *err = nullptr, *pe = nullptr;
     unsigned int *pc = nullptr;
@@ -419,11 +485,18 @@ static int ls_rk3(LsRun &L, int mode, double final_t, double vel, double cfl, do
     int step = 0, rc = IBIM_OK;
     unsigned long long hc[2] = {0, 1};
     if (mode == 1 && (rc = count(hc))) return rc;
+    if (mode == 1) k_sign<<<nb, 256, 0, L.st>>>(L.phi0, L.n, eps);
     while (step < num_steps && (mode == 0 || hc[1] != 0)) {
         ++step;
-        k_ls_stage<<<nb, 256, 0, L.st>>>(L.g, L.g, L.phi0, L.u1, L.d, mode, 1, dtv, eps);
-        k_ls_stage<<<nb, 256, 0, L.st>>>(L.g, L.u1, L.phi0, L.u2, L.d, mode, 2, dtv, eps);
-        k_ls_stage<<<nb, 256, 0, L.st>>>(L.g, L.u2, L.phi0, L.g, L.d, mode, 3, dtv, eps);
+        if (mode == 0) {
+            k_ls_stage<0, 1><<<tgrid, tblk, 0, L.st>>>(L.g, L.g, nullptr, L.u1, L.d, dtv);
+            k_ls_stage<0, 2><<<tgrid, tblk, 0, L.st>>>(L.g, L.u1, nullptr, L.u2, L.d, dtv);
+            k_ls_stage<0, 3><<<tgrid, tblk, 0, L.st>>>(L.g, L.u2, nullptr, L.g, L.d, dtv);
+        } else {
+            k_ls_stage<1, 1><<<tgrid, tblk, 0, L.st>>>(L.g, L.g, L.phi0, L.u1, L.d, dtv);
+            k_ls_stage<1, 2><<<tgrid, tblk, 0, L.st>>>(L.g, L.u1, L.phi0, L.u2, L.d, dtv);
+            k_ls_stage<1, 3><<<tgrid, tblk, 0, L.st>>>(L.g, L.u2, L.phi0, L.g, L.d, dtv);
+        }
         IBIM_LAUNCH_CHECK();
         if (mode == 1 && (rc = count(hc))) return rc;
     }
@@ -533,8 +606,9 @@ static int check_geom(const ibim_geom_params *gp) {
 
 static int alloc_grid(LsRun &L, bool full) {
     L.n = (int64_t)L.d.nx * L.d.ny * L.d.nz;
-    if (L.d.nx < 7 || L.d.ny < 7 || L.d.nz < 7 || L.n > ((int64_t)1 << 31) - 2) {
-        set_error("grid dimensions must be >= 7 and hold < 2^31 points");
+    if (L.d.nx < 7 || L.d.ny < 7 || L.d.nz < 7 || L.n > ((int64_t)1 << 31) - 2 ||
+        (int64_t)L.d.ny * L.d.nz > ((int64_t)1 << 29) - 1 || L.d.nx > 65535) {
+        set_error("grid dimensions must be >= 7 and hold < 2^31 points, < 2^29 in a plane, <= 65535 planes");
         return IBIM_ERR_ARG;
     }
     IBIM_HIP_CHECK(dev_malloc(&L.g, (size_t)L.n * sizeof(double)));
