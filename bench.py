@@ -11,7 +11,9 @@ probe 1.4 A) and solved with eps_in = 2, eps_out = 80, kappa = 0.1257 /A,
 tau = 1, Chebyshev order p = 5, leaf capacity 192, max level 10, GMRES(40)
 to 1e-8, then its solvation energy is formed from the reaction potential at
 the atoms.  A request is the whole chain: level set, tube nodes, plan (tree,
-lists, operators), right-hand side, GMRES, reaction potential, energy.
+lists, operators), right-hand side, GMRES, reaction potential, energy.  The
+mixed form refines in FP64 at order p; its inner solves apply the FP16-mixed
+operator of order p - 1 (`--inner-order`, 0 for order p).
 
 One step = one served request.  The whole stream holds about 9.2e8 tube
 points (the walk-in-a-sphere globules are porous: ~800 tube points per atom
@@ -75,12 +77,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 # MI355X published peaks: vector FP32 / FP64 and dense MFMA, TFLOP/s; HBM TB/s
 PEAK = {"valu_f32": 157.3, "valu_f64": 78.6, "mfma_f16": 2500.0, "mfma_f32": 157.3, "mfma_f64": 78.6, "hbm": 8.0}
 METRIC = "PB solves/s + p50/p99 latency, mixed vs FP64 GPU at <=1e-6 rel energy error"
+PREC_FP64 = 0  # (IBIM_PREC_FP64 of include/ibim.h)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "p2p_mixed_stream256_traffic.json")
 
 FMM = dict(p=5, leaf=192, max_level=10, kappa=0.1257, eps_in=2.0, eps_out=80.0, tau=1.0, restart=40, tol=1e-8)
 CONFIGS = {
+    # (the mixed inner solves at order p - 1; the FP64 residual stays at order p)
     "stream256": dict(FMM, requests=256, h=0.35, atoms_lo=100, atoms_hi=20000, size_seed=4, mol_seed=10000,
-                      confine=2.6),
+                      confine=2.6, inner_order=4),
     # single molecules (not the default workload)
     "globule500": dict(FMM, atoms=500, seed=500, confine=20.0, h=0.25),
     "globule5000": dict(FMM, atoms=5000, seed=5000, confine=45.0, h=0.25, p=6, leaf=256),
@@ -125,8 +129,11 @@ def pct(xs, q):
 
 
 def fmm_kw(cfg, h, precision):
+    """Plan parameters; the mixed forms' inner solves run at cfg's inner_order
+    (0: the plan's order p)."""
     return dict(p=cfg["p"], max_points=cfg["leaf"], max_level=cfg["max_level"], kappa=cfg["kappa"],
-                eps_in=cfg["eps_in"], eps_out=cfg["eps_out"], vacant_radius=cfg["tau"] * h, precision=precision)
+                eps_in=cfg["eps_in"], eps_out=cfg["eps_out"], vacant_radius=cfg["tau"] * h, precision=precision,
+                inner_order=cfg.get("inner_order", 0) if precision != PREC_FP64 else 0)
 
 
 # ------------------------------------------------------------ the stream --
@@ -289,7 +296,7 @@ def run_stream(ibim, args, cfg):
     ok = max_err <= 1e-6 and mixed["all_converged"] and fp64["all_converged"]
     window = rep.pop("_window_ns")
     mixed["fused_group_matvecs"], mixed["fused_member_matvecs"] = rep.pop("_fused")
-    roof = roofline_entries(rep, cfg["p"])
+    roof = roofline_entries(rep, cfg.get("inner_order") or cfg["p"])  # (the order of the mixed matvecs)
     stage_tot = {k: {"ms": v["ms"], "launches": v["launches"]} for k, v in rep.items()}
     return {
         "sizes": sizes, "mols": head_mols, "res_m": res_m, "res_d_sub": res_d_sub,
@@ -316,7 +323,7 @@ def run_fusion(ibim, cfg):
     rng = np.random.default_rng(9)
     atoms = np.exp(rng.uniform(np.log(10), np.log(40), 32)).astype(int)
     mols = [ibim.synthetic_globule(int(m), 50000 + i, cfg["confine"] * m ** (1 / 3)) for i, m in enumerate(atoms)]
-    kw = fmm_kw(cfg, cfg["h"], ibim.PREC_MIXED_F16)
+    kw = dict(fmm_kw(cfg, cfg["h"], ibim.PREC_MIXED_F16), inner_order=0)  # (fused solves use order p)
     out = {"workload": f"32 synthetic globules of {atoms.min()}..{atoms.max()} atoms (log-uniform, seed 9), h="
                        f"{cfg['h']} A, p={cfg['p']}, leaf {cfg['leaf']}, FP16 mixed form, served as one burst on "
                        f"16 slots"}
@@ -530,7 +537,8 @@ def run_single(ibim, args, cfg):
     with plan_for(ibim.PREC_FP64) as pl:
         stats = pl.stats()
     surf.close()
-    return {"legs": legs, "tube_points": n, "atoms": m, "tree": stats, "roof": roofline_entries(acc, cfg["p"])}
+    return {"legs": legs, "tube_points": n, "atoms": m, "tree": stats,
+            "roof": roofline_entries(acc, cfg.get("inner_order") or cfg["p"])}
 
 
 def main():
@@ -553,6 +561,9 @@ def main():
                     help="the headline server fuses the matvecs of concurrent solves of at most this many tube "
                          "points (0: off)")
     ap.add_argument("--leaf", type=int, default=None, help="override the leaf capacity")
+    ap.add_argument("--inner-order", type=int, default=None,
+                    help="Chebyshev order of the mixed forms' inner solves (0: the plan's p; default: the "
+                         "workload's, p - 1 for the serving stream)")
     ap.add_argument("--molecule", type=int, default=None, metavar="I",
                     help="stream256: time single solves of the stream's molecule I instead of serving")
     ap.add_argument("--forms", default=None, help="single-solve forms to time (comma list of fp64, mixed_f32, "
@@ -576,6 +587,8 @@ def main():
 
     if args.leaf:
         cfg["leaf"] = args.leaf
+    if args.inner_order is not None:
+        cfg["inner_order"] = args.inner_order
     if args.config == "stream256" and (args.matvecs_only or args.molecule is not None):
         # one molecule of the stream (counter mode: the median-size one)
         sizes, _ = stream_molecules(ibim, dict(cfg, requests=cfg["requests"]))
@@ -648,11 +661,14 @@ def main():
         "steps": mixed["requests"], "warmup": args.warmup, "ms_per_step": 1e3 * mixed["wall_s"] / mixed["requests"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f16/f32/f64 (FP16 M2L on MFMA with FP32 accumulation, FP32 near field, FP64 translations, "
-                 "Krylov and refinement)",
+                 "Krylov and refinement" + (f"; inner solves at Chebyshev order {cfg['inner_order']}, the FP64 "
+                                            f"residual at order {cfg['p']}" if cfg.get("inner_order") else "") + ")",
         "data": "synthetic",
         "config": {"workload": f"stream256: serving stream of {cfg['requests']} synthetic globules, "
                                f"{cfg['atoms_lo']}..{cfg['atoms_hi']} atoms log-uniform (seed {cfg['size_seed']}), "
-                               f"h={cfg['h']} A, p={cfg['p']}, leaf {cfg['leaf']}, kappa={cfg['kappa']}, eps "
+                               f"h={cfg['h']} A, p={cfg['p']}"
+                               + (f" (mixed inner solves at order {cfg['inner_order']})" if cfg.get("inner_order")
+                                  else "") + f", leaf {cfg['leaf']}, kappa={cfg['kappa']}, eps "
                                f"{cfg['eps_in']:g}/{cfg['eps_out']:g}, GMRES({cfg['restart']}) tol {cfg['tol']}; "
                                f"Poisson arrivals at {args.load:g} of the saturated rate; geometry, plan, solve and "
                                f"energy per request; {args.slots} server slots, fused matvecs of concurrent solves "

@@ -42,7 +42,8 @@ class IbimError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [("p", C.c_int), ("max_points", C.c_int), ("max_level", C.c_int),
                 ("kappa", C.c_double), ("eps_in", C.c_double), ("eps_out", C.c_double),
-                ("vacant_radius", C.c_double), ("precision", C.c_int), ("kernel_mode", C.c_int)]
+                ("vacant_radius", C.c_double), ("precision", C.c_int), ("kernel_mode", C.c_int),
+                ("inner_order", C.c_int)]
 
 
 class TreeStats(C.Structure):
@@ -228,8 +229,11 @@ def _f64(a, shape=None):
 
 
 def params(p=4, max_points=64, max_level=10, kappa=0.0, eps_in=1.0, eps_out=80.0, vacant_radius=0.0,
-           precision=PREC_FP64, kernel_mode=KERNEL_PB):
-    return Params(p, max_points, max_level, kappa, eps_in, eps_out, vacant_radius, precision, kernel_mode)
+           precision=PREC_FP64, kernel_mode=KERNEL_PB, inner_order=0):
+    """inner_order (mixed forms): the Chebyshev order of the inner solves'
+    operator, 2..p-1, or 0 for p; the FP64 residual keeps order p."""
+    return Params(p, max_points, max_level, kappa, eps_in, eps_out, vacant_radius, precision, kernel_mode,
+                  inner_order)
 
 
 class Plan:

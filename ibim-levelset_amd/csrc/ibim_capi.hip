@@ -62,6 +62,13 @@ int matvec_device(ibim_plan *pl, const double *d_x, double *d_y, bool mixed) {
 }
 
 int matvec_single(ibim_plan *pl, const double *d_x, double *d_y, bool mixed) {
+    if (mixed && pl->prm.inner_order > 0) {
+        // a plan whose inner solves use a lower order builds its own mixed
+        // operators on its first mixed matvec (they both return at once when built)
+        int rc = pl->prm.precision == IBIM_PREC_MIXED_F16 ? build_m2l_operators_f16(pl)
+                                                          : build_m2l_operators_f32(pl);
+        if (rc) return rc;
+    }
     StageScope sc(pl, mixed ? kStMatvecMixed : kStMatvec);
     const StageWork w = matvec_work(pl, mixed);
     sc.add(w.valu, w.mfma, w.bytes);
@@ -108,6 +115,10 @@ static int check_params(const ibim_params *q, int64_t n) {
     }
     if (q->precision != IBIM_PREC_FP64 && q->precision != IBIM_PREC_MIXED && q->precision != IBIM_PREC_MIXED_F16) {
         set_error("unknown precision");
+        return IBIM_ERR_ARG;
+    }
+    if (q->inner_order != 0 && (q->inner_order < 2 || q->inner_order >= q->p)) {
+        set_error("inner_order must be 0 or in 2..p-1");
         return IBIM_ERR_ARG;
     }
     return IBIM_OK;
@@ -245,6 +256,7 @@ const char *ibim_version(void) { return "ibim-levelset-amd 0.1 (gfx950)"; }
 
 void ibim_plan_destroy(ibim_plan *pl) {
     if (!pl) return;
+    ibim_plan_destroy(pl->inner);  // (it runs on this plan's stream; it synchronises first)
     // both streams idle before any block returns to the shared cache (a
     // matvec that failed between its fork and join may still have near-field
     // work in flight on the second stream)
@@ -305,7 +317,8 @@ int ibim_plan_create_device(const ibim_params *prm, int64_t n, const double *d_x
         ibim_plan_destroy(pl);
         return rc;
     }
-    if (pl->prm.precision != IBIM_PREC_FP64 &&
+    // (with a lower inner order the mixed operators wait for this plan's first mixed matvec)
+    if (pl->prm.precision != IBIM_PREC_FP64 && pl->prm.inner_order == 0 &&
         (rc = pl->prm.precision == IBIM_PREC_MIXED ? build_m2l_operators_f32(pl) : build_m2l_operators_f16(pl))) {
         ibim_plan_destroy(pl);
         return rc;
@@ -401,7 +414,9 @@ int ibim_plan_set_precision(ibim_plan *pl, int precision) {
     return rc;
 }
 
-int64_t ibim_plan_footprint(const ibim_plan *pl) { return pl ? pl->footprint : 0; }
+int64_t ibim_plan_footprint(const ibim_plan *pl) {
+    return pl ? pl->footprint + (pl->inner ? pl->inner->footprint : 0) : 0;
+}
 
 int ibim_plan_profile(ibim_plan *pl, int enable) {
     if (!pl) {
@@ -410,6 +425,7 @@ int ibim_plan_profile(ibim_plan *pl, int enable) {
     }
     pl->timer.on = enable != 0;
     pl->timer.mask = enable ? ~0u : 0u;
+    if (pl->inner) ibim_plan_profile(pl->inner, enable);
     return IBIM_OK;
 }
 
@@ -419,6 +435,7 @@ int ibim_plan_set_trace(ibim_plan *pl, int on) {
         return IBIM_ERR_ARG;
     }
     pl->trace = on != 0;
+    if (pl->inner) pl->inner->trace = pl->trace;
     return IBIM_OK;
 }
 
@@ -429,6 +446,7 @@ int ibim_plan_profile_stages(ibim_plan *pl, unsigned stage_mask) {
     }
     pl->timer.mask = stage_mask & ((1u << IBIM_STAGE_COUNT) - 1u);
     pl->timer.on = pl->timer.mask != 0;
+    if (pl->inner) ibim_plan_profile_stages(pl->inner, stage_mask);
     return IBIM_OK;
 }
 
@@ -437,7 +455,19 @@ int ibim_plan_stage_report(ibim_plan *pl, ibim_stage_report *out, int reset) {
         set_error("plan is NULL");
         return IBIM_ERR_ARG;
     }
-    return timer_report(pl->timer, out, reset);
+    int rc = timer_report(pl->timer, out, reset);
+    if (rc || !pl->inner) return rc;
+    // the inner plan's stages (the mixed matvecs of the inner solves) count as this plan's
+    ibim_stage_report r;
+    if ((rc = timer_report(pl->inner->timer, out ? &r : nullptr, reset)) || !out) return rc;
+    for (int i = 0; i < IBIM_STAGE_SLOTS; ++i) {
+        out->ms[i] += r.ms[i];
+        out->launches[i] += r.launches[i];
+        out->valu_flops[i] += r.valu_flops[i];
+        out->mfma_flops[i] += r.mfma_flops[i];
+        out->bytes[i] += r.bytes[i];
+    }
+    return IBIM_OK;
 }
 
 int ibim_plan_stage_times(ibim_plan *pl, double *ms, int64_t *launches, int reset) {

@@ -550,6 +550,7 @@ struct ibim_plan {
     bool defer_reduce = false;           // the FP16 M2L stage leaves the reduction to the caller
     bool trace = false;                  // GMRES prints its iteration table to stdout
     ibim::Batcher *batcher = nullptr;    // a server's matvec batcher, while the plan's solve uses it
+    ibim_plan *inner = nullptr;          // the order-prm.inner_order plan of the mixed inner solves (same points)
 
     // sorted geometry
     double *sx = nullptr, *sy = nullptr, *sz = nullptr;

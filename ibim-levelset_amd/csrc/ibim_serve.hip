@@ -224,18 +224,27 @@ int64_t estimate_footprint(const ibim_request &r, const std::vector<double> &xyz
     }
     const double box_area = 2 * (ext[0] * ext[1] + ext[1] * ext[2] + ext[0] * ext[2]);
     const double npts = std::min(area, 3.0 * box_area) * (2.0 * r.geom.thickness) / (h * h);
-    const int p = r.fmm.p, rr3 = p * p * p;
-    const double per_point = 2.0 * 4 * rr3 * 8 / 12.0 + 8.0 * (2 * (r.restart + 1) + 40);
-    const double ops = (r.fmm.precision != IBIM_PREC_FP64 ? 1.5 : 1.0) * 12 * 56 * pow(4.0 * rr3, 2) * 8;
-    // the FP16 V list keeps one FP24 row of 4 p^3 (rounded up to 32) per V
-    // interaction; about 340 / leaf capacity V interactions per tube point
-    // (plus the X list's, a few per cent), and one unfolded FP16 operator
-    // per (class, mask) key with pairs, V and X (a few thousand keys)
-    const double yv = (r.fmm.precision == IBIM_PREC_MIXED_F16 && p <= 6)
-                          ? npts * 350.0 / std::max(1, r.fmm.max_points) * ((4 * rr3 + 31) / 32 * 32) * 3 +
-                                2500.0 * pow(4.0 * rr3, 2) * 2
-                          : 0.0;
-    return (int64_t)(std::max(grid * 8 * 4.2, npts * per_point + ops + yv) + npts * 56);
+    // one plan of order p: expansions, vectors, the GMRES basis if it holds
+    // it, its operators and (FP16 form) its pair rows
+    auto plan_bytes = [&](int p, bool basis) {
+        const int rr3 = p * p * p;
+        const double per_point = 2.0 * 4 * rr3 * 8 / 12.0 + 8.0 * ((basis ? 2 * (r.restart + 1) : 6) + 40);
+        const double ops = (r.fmm.precision != IBIM_PREC_FP64 ? 1.5 : 1.0) * 12 * 56 * pow(4.0 * rr3, 2) * 8;
+        // the FP16 V list keeps one FP24 row of 4 p^3 (rounded up to 32) per V
+        // interaction; about 340 / leaf capacity V interactions per tube point
+        // (plus the X list's, a few per cent), and one unfolded FP16 operator
+        // per (class, mask) key with pairs, V and X (a few thousand keys)
+        const double yv = (r.fmm.precision == IBIM_PREC_MIXED_F16 && p <= 6)
+                              ? npts * 350.0 / std::max(1, r.fmm.max_points) * ((4 * rr3 + 31) / 32 * 32) * 3 +
+                                    2500.0 * pow(4.0 * rr3, 2) * 2
+                              : 0.0;
+        return npts * per_point + ops + yv;
+    };
+    // a mixed solve with a lower inner order also holds the inner plan, which
+    // then keeps the Krylov basis
+    const bool two = r.fmm.precision != IBIM_PREC_FP64 && r.fmm.inner_order > 0;
+    const double plans = plan_bytes(r.fmm.p, !two) + (two ? plan_bytes(r.fmm.inner_order, true) : 0.0);
+    return (int64_t)(std::max(grid * 8 * 4.2, plans) + npts * 56);
 }
 
 // a request passed over this many times is started before any later one
@@ -252,7 +261,17 @@ void run_job(ibim_server *s, ibim_server::Job &j, hipStream_t st) {
     ibim_surface_info inf{};
     if (rc == IBIM_OK) rc = ibim_surface_get_info(surf, &inf);
     if (rc == IBIM_OK) rc = ibim_surface_device(surf, &sx, &sn, &sw);
-    if (rc == IBIM_OK) rc = ibim_plan_create_device(&j.req.fmm, inf.n_nodes, sx, sn, sw, &plan);
+    int64_t fuse_max = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        fuse_max = s->fuse_max;
+    }
+    // a solve whose matvecs go through the batcher applies this plan's own
+    // operator in its inner solves (a second plan's would bypass the batch)
+    const bool fuse = rc == IBIM_OK && fuse_max > 0 && inf.n_nodes <= fuse_max;
+    ibim_params fmm = j.req.fmm;
+    if (fuse) fmm.inner_order = 0;
+    if (rc == IBIM_OK) rc = ibim_plan_create_device(&fmm, inf.n_nodes, sx, sn, sw, &plan);
     const size_t n2 = 2 * (size_t)inf.n_nodes;
     unsigned mask = 0;
     {
@@ -278,15 +297,10 @@ void run_job(ibim_server *s, ibim_server::Job &j, hipStream_t st) {
             rc = IBIM_ERR_HIP;
     }
     if (rc == IBIM_OK) rc = ibim_rhs_device(plan, m, d_at, d_at + 3 * m, d_b);
-    int64_t fuse_max = 0;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        fuse_max = s->fuse_max;
-    }
-    const bool fuse = rc == IBIM_OK && fuse_max > 0 && inf.n_nodes <= fuse_max;
-    if (fuse) s->batcher.enter(plan);
+    const bool entered = fuse && rc == IBIM_OK;
+    if (entered) s->batcher.enter(plan);
     if (rc == IBIM_OK) rc = ibim_gmres_device(plan, d_b, d_x, j.req.restart, j.req.maxit, j.req.tol, &res.info);
-    if (fuse) s->batcher.leave(plan);
+    if (entered) s->batcher.leave(plan);
     const int solve_rc = rc;
     if (rc >= 0) rc = ibim_polarize_device(plan, m, d_at, d_x, d_phi);
     if (rc >= 0) {

@@ -547,7 +547,7 @@ constexpr int kHostScalars = 8;
 // the workspace for GMRES(restart): the FP64 basis, and (cgs) the
 // projection partials of classical Gram-Schmidt for the inner solves,
 // allocated when first needed
-static int gmres_alloc(ibim_plan *pl, int restart, bool cgs) {
+static int gmres_alloc(ibim_plan *pl, int restart, bool cgs, bool basis = true) {
     const size_t n2 = 2 * (size_t)pl->n;
     int rc;
     // earlier buffers stay on the plan's allocation list and are freed with it
@@ -561,8 +561,8 @@ static int gmres_alloc(ibim_plan *pl, int restart, bool cgs) {
         pl->gm_restart = restart;
     }
     const int rs = pl->gm_restart;
-    if (!pl->gm_V && (rc = dalloc(pl, &pl->gm_V, (size_t)(rs + 1) * n2))) return rc;
-    if (cgs && !pl->gm_hpart &&
+    if (basis && !pl->gm_V && (rc = dalloc(pl, &pl->gm_V, (size_t)(rs + 1) * n2))) return rc;
+    if (basis && cgs && !pl->gm_hpart &&
         ((rc = dalloc(pl, &pl->gm_hpart, (size_t)(rs + 1) * kRG)) || (rc = dalloc(pl, &pl->gm_coef, (size_t)rs + 1))))
         return rc;
     if (!pl->gm_host) IBIM_HIP_CHECK(hipHostMalloc((void **)&pl->gm_host, kHostScalars * sizeof(double)));
@@ -733,7 +733,70 @@ __global__ void k_axpy1(double *__restrict__ x, const double *__restrict__ d, in
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] += d[i];
 }
+
+// the plan's sorted points back in the caller's order (perm[sorted] = original)
+__global__ void k_unsort_points(const int *__restrict__ perm, int n, const double *__restrict__ sx,
+                                const double *__restrict__ sy, const double *__restrict__ sz,
+                                const double *__restrict__ snx, const double *__restrict__ sny,
+                                const double *__restrict__ snz, const double *__restrict__ sw, double *__restrict__ xyz,
+                                double *__restrict__ nrm, double *__restrict__ w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t o = (size_t)perm[i];
+    xyz[3 * o] = sx[i];
+    xyz[3 * o + 1] = sy[i];
+    xyz[3 * o + 2] = sz[i];
+    nrm[3 * o] = snx[i];
+    nrm[3 * o + 1] = sny[i];
+    nrm[3 * o + 2] = snz[i];
+    w[o] = sw[i];
+}
 }  // namespace
+
+// The plan whose matvec the inner solves use when prm.inner_order is set:
+// the same points, leaf capacity and kernels at Chebyshev order inner_order,
+// in this plan's mixed form, built on first use (its points are this plan's,
+// put back in the caller's order) and kept with this plan.  Its work is
+// ordered on this plan's stream, and it records the stages this plan does.
+static int inner_plan(ibim_plan *pl, ibim_plan **out) {
+    if (!pl->inner) {
+        const int n = pl->n;
+        double *xyz = nullptr, *nrm = nullptr, *w = nullptr;
+        int rc = IBIM_OK;
+        if (dev_malloc(&xyz, 3 * (size_t)n * sizeof(double)) != hipSuccess ||
+            dev_malloc(&nrm, 3 * (size_t)n * sizeof(double)) != hipSuccess ||
+            dev_malloc(&w, (size_t)n * sizeof(double)) != hipSuccess) {
+            set_error("hipMalloc (inner plan points)");
+            rc = IBIM_ERR_NOMEM;
+        }
+        if (rc == IBIM_OK) {
+            k_unsort_points<<<(n + 255) / 256, 256, 0, pl->stream>>>(pl->perm, n, pl->sx, pl->sy, pl->sz, pl->snx,
+                                                                      pl->sny, pl->snz, pl->sw, xyz, nrm, w);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
+            if (e != hipSuccess) {
+                set_error(std::string("inner plan points: ") + hipGetErrorString(e));
+                rc = IBIM_ERR_HIP;
+            }
+        }
+        if (rc == IBIM_OK) {
+            ibim_params q = pl->prm;
+            q.p = pl->prm.inner_order;
+            q.inner_order = 0;
+            rc = ibim_plan_create_device(&q, n, xyz, nrm, w, &pl->inner);  // (synchronises its build)
+        }
+        for (double *p : {xyz, nrm, w})
+            if (p) (void)hipFree(p);
+        if (rc) return rc;
+        pl->inner->trace = pl->trace;
+        pl->inner->timer.on = pl->timer.on;
+        pl->inner->timer.mask = pl->timer.mask;
+    }
+    int rc = ibim_plan_set_stream(pl->inner, pl->stream);
+    if (rc) return rc;
+    *out = pl->inner;
+    return IBIM_OK;
+}
 
 // Mixed precision: iterative refinement.  The residual r = b - A x is formed
 // with the FP64 matvec; each correction solves A d = r with GMRES on the
@@ -748,7 +811,11 @@ static int gmres_refine(ibim_plan *pl, const double *d_b, double *d_x, int resta
     int rc;
     if (!pl->gm_rr && ((rc = dalloc(pl, &pl->gm_rr, (size_t)n2)) || (rc = dalloc(pl, &pl->gm_d, (size_t)n2))))
         return rc;
-    if ((rc = gmres_alloc(pl, restart, true))) return rc;
+    // the inner solves' operator: this plan's mixed matvec, or that of its
+    // lower-order companion (the FP64 residual below stays this plan's)
+    ibim_plan *ip = pl;
+    if (pl->prm.inner_order > 0 && (rc = inner_plan(pl, &ip))) return rc;
+    if ((rc = gmres_alloc(pl, restart, true, ip == pl))) return rc;
     double *scal = pl->gm_small;  // borrowed scratch for the two norms below (before any inner solve)
     double *part = pl->gm_part + 3 * kRG;
     auto norm = [&](const double *v, double *out) -> int {
@@ -786,13 +853,18 @@ static int gmres_refine(ibim_plan *pl, const double *d_b, double *d_x, int resta
     };
     if ((rc = residual())) return rc;
     // relative residual the inner solve aims at: the FP32 matvec supports 1e-6;
-    // the FP16 V-list (per-box scales make it linear only to ~1e-5) 1e-5
-    const double floor_mixed = pl->prm.precision == IBIM_PREC_MIXED_F16 ? 1e-5 : 1e-6;
+    // the FP16 V-list (per-box scales make it linear only to ~1e-5) 1e-5; an
+    // operator of lower order differs from the order-p one by ~1e-4
+    auto floor_of = [&](const ibim_plan *q) {
+        if (q != pl) return 1e-4;
+        return pl->prm.precision == IBIM_PREC_MIXED_F16 ? 1e-5 : 1e-6;
+    };
+    double floor_mixed = floor_of(ip);
     while (rn / normb >= tol && inner < maxit && outer < 50) {
         const double inner_tol = std::max(floor_mixed, 0.5 * tol * normb / rn);
         IBIM_HIP_CHECK(hipMemsetAsync(pl->gm_d, 0, (size_t)n2 * sizeof(double), st));
         ibim_solve_info ii{};
-        rc = gmres_core(pl, pl->gm_rr, pl->gm_d, restart, maxit - inner, inner_tol, &ii, true);
+        rc = gmres_core(ip, pl->gm_rr, pl->gm_d, restart, maxit - inner, inner_tol, &ii, true);
         if (rc < 0) return rc;
         inner += ii.iterations;
         nmv += ii.matvecs;
@@ -802,9 +874,17 @@ static int gmres_refine(ibim_plan *pl, const double *d_b, double *d_x, int resta
         const double before = rn;
         if ((rc = residual())) return rc;
         if (pl->trace) {
-            std::printf("refinement step %d: %d inner iterations, FP64 relative residual %.6e\n", outer,
-                        ii.iterations, rn / normb);
+            std::printf("refinement step %d: %d inner iterations (order %d), FP64 relative residual %.6e\n", outer,
+                        ii.iterations, ip->p, rn / normb);
             std::fflush(stdout);
+        }
+        if (ip != pl && !(rn < std::max(0.1, 10.0 * inner_tol) * before)) {
+            // the FP64 residual fell by less than a tenth of what the inner
+            // solve reached with the lower-order operator (or by less than
+            // tenfold): the remaining steps use this plan's own operator
+            ip = pl;
+            floor_mixed = floor_of(ip);
+            continue;
         }
         if (!(rn < before)) break;  // no progress: stop instead of looping
     }

@@ -78,6 +78,10 @@ typedef struct ibim_params {
     double vacant_radius; /* tau * h, Angstrom: kernels regularised below it */
     int precision;        /* IBIM_PREC_* */
     int kernel_mode;      /* IBIM_KERNEL_* */
+    int inner_order;      /* mixed forms: Chebyshev order of the operator the inner solves use
+                             (2..p-1; 0: p).  The FP64 residual of the refinement keeps order
+                             p, so the converged answer is still that of the order-p FP64
+                             system.  No counterpart in the reference (FP64 GMRES only). */
 } ibim_params;
 
 typedef struct ibim_tree_stats {

@@ -13,7 +13,7 @@ int main(void) {
     S(ibim_params);
     F(ibim_params, p); F(ibim_params, max_points); F(ibim_params, max_level); F(ibim_params, kappa);
     F(ibim_params, eps_in); F(ibim_params, eps_out); F(ibim_params, vacant_radius); F(ibim_params, precision);
-    F(ibim_params, kernel_mode);
+    F(ibim_params, kernel_mode); F(ibim_params, inner_order);
     S(ibim_tree_stats);
     F(ibim_tree_stats, n_points); F(ibim_tree_stats, n_nodes); F(ibim_tree_stats, n_leaves);
     F(ibim_tree_stats, n_empty_leaves); F(ibim_tree_stats, depth); F(ibim_tree_stats, n_u);

@@ -66,6 +66,8 @@ def test_plan_create_rejects_bad_arguments_before_any_device_call(ibim):
         (params(vacant_radius=0.0), 1, b"vacant_radius"),
         (params(precision=7), 1, b"precision"),
         (params(kernel_mode=5), 1, b"kernel_mode"),
+        (params(inner_order=1), 1, b"inner_order"),
+        (params(inner_order=5), 1, b"inner_order"),
     ]
     for prm, n, msg in cases:
         plan = ctypes.c_void_p(123)
