@@ -25,6 +25,7 @@ void electric(Grid &, levelset &ls, Surface &surf, Molecule &mol, double rescale
     const std::string prec = cfg.gets("precision", "fp64");
     prm.precision = prec == "mixed" ? IBIM_PREC_MIXED : prec == "mixed16" ? IBIM_PREC_MIXED_F16 : IBIM_PREC_FP64;
     prm.kernel_mode = IBIM_KERNEL_PB;
+    if (prm.precision != IBIM_PREC_FP64) prm.inner_order = cfg.geti("inner_order", 0);
 
     const double *dxyz, *dnrm, *dw;
     ibim::check(ibim_surface_device(surf.handle, &dxyz, &dnrm, &dw), "ibim_surface_device");
