@@ -57,6 +57,30 @@ def test_lower_inner_order_solve_matches_fp64(ibim, sphere_case, mode, order):
     assert rep["MATVEC"]["launches"] >= info["outer_steps"]
 
 
+def test_plan_with_inner_order_builds_its_own_mixed_matvec(ibim, sphere_case):
+    """A plan whose inner solves use a lower order skips its own mixed
+    operators until a mixed matvec asks for them; that matvec then equals an
+    order-p FP16 plan's bit for bit, before and after a solve."""
+    pts, nrm, w = sphere_case
+    kw = dict(p=5, max_points=64, vacant_radius=0.125, precision=ibim.PREC_MIXED_F16, **PB)
+    x = np.random.default_rng(5).uniform(-1, 1, 2 * len(pts))
+    with ibim.Plan(pts, nrm, w, **kw) as ref:
+        y_ref = ref.matvec(x)
+    with ibim.Plan(pts, nrm, w, inner_order=4, **kw) as plan:
+        fp0 = plan.footprint()
+        y0 = plan.matvec(x)
+        fp1 = plan.footprint()
+        b = plan.rhs(np.array([[0.0, 0.0, 0.0]]), np.array([1.0]))
+        _, rc, info = plan.gmres(b, restart=20, maxit=300, tol=1e-8)
+        fp2 = plan.footprint()
+        y1 = plan.matvec(x)
+    print(f"footprint {fp0 / 2**20:.1f} -> {fp1 / 2**20:.1f} (own mixed operators) -> {fp2 / 2**20:.1f} MiB "
+          f"(inner plan); {info}")
+    assert rc == 0
+    assert fp0 < fp1 < fp2
+    assert np.array_equal(y0, y_ref) and np.array_equal(y1, y_ref)
+
+
 def test_inner_order_rejected_out_of_range(ibim, sphere_case):
     pts, nrm, w = sphere_case
     for order in (1, 5, 6):
