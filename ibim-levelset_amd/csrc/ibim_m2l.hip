@@ -666,7 +666,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW >= 8
     constexpr int R4 = 4 * R;
     constexpr int KP = (R4 + 63) / 64 * 64;
     constexpr int KS = KP / 16;                  // MFMA k-steps
-    constexpr int KC = KP % 128 == 0 ? 128 : 64; // k per operator chunk
+    // k per operator chunk: 128 where a row tile has at least DIST chunks of
+    // it (p = 5), else 64 (p = 4: four chunks per row tile, so its stores
+    // take the counted, LDS-staged path below)
+    constexpr int KC = KP % 128 == 0 && KP / 128 >= 3 ? 128 : 64;
     constexpr int NKC = KP / KC;
     constexpr int SEGS = KC / 8;                 // 16-byte segments per chunk row
     constexpr int RPI = 64 / SEGS;               // chunk rows per LDS-load instruction (64 lanes x 16 B)
