@@ -814,7 +814,10 @@ static int gmres_refine(ibim_plan *pl, const double *d_b, double *d_x, int resta
     // the inner solves' operator: this plan's mixed matvec, or that of its
     // lower-order companion (the FP64 residual below stays this plan's)
     ibim_plan *ip = pl;
-    if (pl->prm.inner_order > 0 && (rc = inner_plan(pl, &ip))) return rc;
+    if (pl->prm.inner_order > 0 && (rc = inner_plan(pl, &ip))) {
+        if (rc != IBIM_ERR_NOMEM) return rc;
+        ip = pl;  // no room for the companion plan: the inner solves use this plan's own operator
+    }
     if ((rc = gmres_alloc(pl, restart, true, ip == pl))) return rc;
     double *scal = pl->gm_small;  // borrowed scratch for the two norms below (before any inner solve)
     double *part = pl->gm_part + 3 * kRG;
