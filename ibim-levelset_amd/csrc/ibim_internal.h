@@ -217,7 +217,7 @@ __device__ __forceinline__ void pair_pb32(float dx, float dy, float dz, float qx
     const float a3 = ir3 - m;
     const float g1 = fmaf(k.eps, m, -ir3);
     const float g4 = fmaf(k.inv_eps, m, -ir3);
-    const float k2 = reg ? k.k2reg4pi : (1.0f - e) * ir;
+    const float k2 = reg ? k.k2reg4pi : fmaf(-e, ir, ir);  // (1 - e) / r in one FMA
     const float b3 = (fmaf(et1, t + 2.0f, e) - 3.0f) * ir3 * ir2;
     const float dq = dx * qx + dy * qy + dz * qz;
     f0a = fmaf(g1, dq, f0a);
