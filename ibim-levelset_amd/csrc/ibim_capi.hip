@@ -406,9 +406,14 @@ int ibim_plan_set_precision(ibim_plan *pl, int precision) {
         return IBIM_ERR_ARG;
     }
     // the operators a precision needs are built before the plan switches to
-    // it: a failed build leaves the plan in its previous precision
+    // it: a failed build leaves the plan in its previous precision.  With a
+    // lower inner order the plan's own mixed operators wait for its first
+    // mixed matvec (as at creation), and a companion plan already built for
+    // the inner solves switches with it
     int rc = IBIM_OK;
-    if (precision == IBIM_PREC_MIXED) rc = build_m2l_operators_f32(pl);
+    if (pl->prm.inner_order > 0) {
+        if (pl->inner && precision != IBIM_PREC_FP64) rc = ibim_plan_set_precision(pl->inner, precision);
+    } else if (precision == IBIM_PREC_MIXED) rc = build_m2l_operators_f32(pl);
     else if (precision == IBIM_PREC_MIXED_F16) rc = build_m2l_operators_f16(pl);
     if (rc == IBIM_OK) pl->prm.precision = precision;
     return rc;

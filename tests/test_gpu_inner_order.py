@@ -5,8 +5,9 @@ residual (measured again here with the order-p FP64 operator) below the
 1e-8 tolerance, the solution within 1e-6 and the solvation energy within
 1e-7 relative of the FP64 solve's (both solves stop at a residual below
 1e-8; the observed differences are 1e-8 and below); the inner plan's stages
-are in the plan's stage report; served solves with a lower inner order give
-the FP64 energies within 1e-7, fused small solves too."""
+are in the plan's stage report; a precision switch reaches the companion
+plan; served solves with a lower inner order give the FP64 energies within
+1e-7, fused small solves too."""
 import numpy as np
 import pytest
 
@@ -79,6 +80,31 @@ def test_plan_with_inner_order_builds_its_own_mixed_matvec(ibim, sphere_case):
     assert rc == 0
     assert fp0 < fp1 < fp2
     assert np.array_equal(y0, y_ref) and np.array_equal(y1, y_ref)
+
+
+def test_precision_switch_carries_to_the_inner_plan(ibim, sphere_case):
+    """Switching a plan with a lower inner order from the FP16 to the FP32
+    mixed form switches the companion plan built by its first solve: that
+    plan builds its FP32 operators at once (the footprint grows), and the
+    next solve still ends at the FP64 energy within 1e-7."""
+    pts, nrm, w = sphere_case
+    atoms = np.array([[0.0, 0.0, 0.0], [0.4, -0.3, 0.2]])
+    qa = np.array([1.0, -0.5])
+    _, e64, rc64, _, _, _, _ = solve(ibim, sphere_case, ibim.PREC_FP64)
+    with ibim.Plan(pts, nrm, w, p=5, max_points=64, vacant_radius=0.125, precision=ibim.PREC_MIXED_F16,
+                   inner_order=4, **PB) as plan:
+        b = plan.rhs(atoms, qa)
+        _, rc1, _ = plan.gmres(b, restart=20, maxit=300, tol=1e-8)
+        fp1 = plan.footprint()
+        plan.set_precision(ibim.PREC_MIXED)
+        fp2 = plan.footprint()
+        x, rc2, info = plan.gmres(b, restart=20, maxit=300, tol=1e-8)
+        e = 0.5 * float(plan.polarize(atoms, x) @ qa)
+    print(f"footprint {fp1 / 2**20:.1f} -> {fp2 / 2**20:.1f} MiB; {info}; E64={e64:.12e} E={e:.12e}")
+    assert rc64 == 0 and rc1 == 0 and rc2 == 0, info
+    assert fp2 > fp1
+    assert info["outer_steps"] >= 2
+    assert abs(e - e64) <= 1e-7 * abs(e64)
 
 
 def test_inner_order_rejected_out_of_range(ibim, sphere_case):
