@@ -185,24 +185,14 @@ __device__ __forceinline__ double at_byte(const double *p, unsigned off) {
     return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(p) + off);
 }
 
-template <int MODE, int STAGE>
-__global__ __launch_bounds__(256) void k_ls_stage(const double *base, const double *__restrict__ src,
-                                                  const double *__restrict__ sgn, double *out, GridDims d,
-                                                  double dtv) {
-    __shared__ double t[kLsRows][kLsCols];
-    const int k0 = blockIdx.x * kLsTk, j0 = blockIdx.y * kLsTj, i = blockIdx.z;
+// one point of k_ls_stage (its tile's rows and columns in t), and with
+// COUNT the point's contribution to the gradient count
+template <int MODE, int STAGE, bool COUNT>
+__device__ __forceinline__ void ls_point(const double *base, const double *__restrict__ src,
+                                         const double *__restrict__ sgn, double *out, const GridDims &d, double dtv,
+                                         const double (&t)[kLsRows][kLsCols], int i, int j, int k, int tj, int tk,
+                                         double tube, double thres, unsigned int &c0, unsigned int &c1, double &e) {
     const size_t plane = (size_t)d.ny * d.nz;
-    const double *sp = src + (size_t)i * plane;
-    for (int e = threadIdx.y * kLsTk + threadIdx.x; e < kLsRows * kLsCols; e += kLsTk * kLsTj) {
-        const int r = e / kLsCols, c = e - r * kLsCols;
-        const int jj = min(max(j0 - 3 + r, 0), d.ny - 1);
-        const int kk = min(max(k0 - 3 + c, 0), d.nz - 1);
-        t[r][c] = at_byte(sp, 8u * (unsigned)(jj * d.nz + kk));
-    }
-    __syncthreads();
-    const int tk = threadIdx.x, tj = threadIdx.y;
-    const int k = k0 + tk, j = j0 + tj;
-    if (k >= d.nz || j >= d.ny) return;
     const unsigned off = 8u * (unsigned)(j * d.nz + k);
     double up[3], un[3];
     {
@@ -226,6 +216,7 @@ __global__ __launch_bounds__(256) void k_ls_stage(const double *base, const doub
         for (int l = 0; l < 7; ++l) w[l] = t[tj + 3][tk + l];
         weno7(w, d.rh, up[2], un[2]);
     }
+    const double c = t[tj + 3][tk + 3];  // src at the point
     double T;
     if constexpr (MODE == 0) {
         T = dtv * godunov(un, up);
@@ -233,8 +224,15 @@ __global__ __launch_bounds__(256) void k_ls_stage(const double *base, const doub
         const double s = at_byte(sgn + (size_t)i * plane, off);
         const double nd = s > 0. ? godunov(un, up) : godunov(up, un);
         T = dtv * s * (nd - 1.0);
+        if constexpr (COUNT) {
+            if (fabs(c) < tube) {  // a tube point: |grad| by getNorm, bad beyond thres
+                const double nr = s > 0. ? nd : godunov(un, up);
+                c0 = 1;
+                c1 = fabs(nr - 1.0) > thres ? 1 : 0;
+                e = fabs(nr - 1.0);
+            }
+        }
     }
-    const double c = t[tj + 3][tk + 3];  // src at the point
     double *op = out + (size_t)i * plane;
     double v;
     if constexpr (STAGE == 1) {
@@ -246,6 +244,65 @@ __global__ __launch_bounds__(256) void k_ls_stage(const double *base, const doub
     }
     *reinterpret_cast<double *>(reinterpret_cast<char *>(op) + off) = v;
 }
+
+// COUNT (the first stage of a reinitialisation step): also countGradient
+// (levelset.cpp:398-435) of src, the state the step starts from: tube points,
+// bad points and the sum of ||grad| - 1| over the tube, reduced by wave
+// shuffles and LDS to one slot per tile in tile order
+template <int MODE, int STAGE, bool COUNT = false>
+__global__ __launch_bounds__(256) void k_ls_stage(const double *base, const double *__restrict__ src,
+                                                  const double *__restrict__ sgn, double *out, GridDims d,
+                                                  double dtv, double tube = 0.0, double thres = 0.0,
+                                                  unsigned int *__restrict__ pc = nullptr,
+                                                  double *__restrict__ pe = nullptr) {
+    __shared__ double t[kLsRows][kLsCols];
+    const int k0 = blockIdx.x * kLsTk, j0 = blockIdx.y * kLsTj, i = blockIdx.z;
+    const size_t plane = (size_t)d.ny * d.nz;
+    const double *sp = src + (size_t)i * plane;
+    for (int e = threadIdx.y * kLsTk + threadIdx.x; e < kLsRows * kLsCols; e += kLsTk * kLsTj) {
+        const int r = e / kLsCols, c = e - r * kLsCols;
+        const int jj = min(max(j0 - 3 + r, 0), d.ny - 1);
+        const int kk = min(max(k0 - 3 + c, 0), d.nz - 1);
+        t[r][c] = at_byte(sp, 8u * (unsigned)(jj * d.nz + kk));
+    }
+    __syncthreads();
+    const int tk = threadIdx.x, tj = threadIdx.y;
+    const int k = k0 + tk, j = j0 + tj;
+    if constexpr (COUNT) {
+        // every thread reaches the block's count below
+        unsigned int c0 = 0, c1 = 0;
+        double e = 0.0;
+        if (k < d.nz && j < d.ny) ls_point<MODE, STAGE, true>(base, src, sgn, out, d, dtv, t, i, j, k, tj, tk, tube,
+                                                                thres, c0, c1, e);
+        for (int o = 32; o; o >>= 1) {
+            c0 += __shfl_xor(c0, o);
+            c1 += __shfl_xor(c1, o);
+            e += __shfl_xor(e, o);
+        }
+        __shared__ unsigned int s0[4], s1[4];
+        __shared__ double se[4];
+        const int w = (tj * kLsTk + tk) >> 6;
+        if (((tj * kLsTk + tk) & 63) == 0) {
+            s0[w] = c0;
+            s1[w] = c1;
+            se[w] = e;
+        }
+        __syncthreads();
+        if (tj == 0 && tk == 0) {
+            const size_t b = blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+            pc[2 * b] = s0[0] + s0[1] + s0[2] + s0[3];
+            pc[2 * b + 1] = s1[0] + s1[1] + s1[2] + s1[3];
+            pe[b] = (se[0] + se[1]) + (se[2] + se[3]);
+        }
+        return;
+    } else {
+        if (k >= d.nz || j >= d.ny) return;
+        unsigned int c0, c1;
+        double e;
+        ls_point<MODE, STAGE, false>(base, src, sgn, out, d, dtv, t, i, j, k, tj, tk, 0.0, 0.0, c0, c1, e);
+    }
+}
+
 
 // phi0 -> sign(phi0) = phi0 / sqrt(phi0^2 + eps) in place (the reinitialisation's
 // smoothed sign, levelset.cpp:191-287; constant over its steps)
@@ -262,49 +319,8 @@ __global__ void k_scale(double *__restrict__ g, int64_t n, double s) {
     if (I < n) g[I] *= s;
 }
 
-// countGradient (levelset.cpp:398-435): tube points, bad points, sum of
-// errors.  Each block writes its sums (wave shuffles, then LDS) to its own
-// slot; k_count_sum adds the slots in block order (exact counts, a
-// deterministic error sum).
-__global__ __launch_bounds__(256) void k_count(const double *__restrict__ g, GridDims d, double tube, double thres,
-                                               unsigned int *__restrict__ pc, double *__restrict__ pe) {
-    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
-    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned int c0 = 0, c1 = 0;
-    double e = 0.0;
-    if (I < n && fabs(g[I]) < tube) {
-        int i, j, k;
-        index3(I, d, i, j, k);
-        double up[3], un[3];
-#pragma unroll
-        for (int dir = 0; dir < 3; ++dir) weno(g, d, i, j, k, dir, up[dir], un[dir]);
-        const double nr = godunov(un, up);
-        c0 = 1;
-        c1 = fabs(nr - 1.0) > thres ? 1 : 0;
-        e = fabs(nr - 1.0);
-    }
-    for (int o = 32; o; o >>= 1) {
-        c0 += __shfl_xor(c0, o);
-        c1 += __shfl_xor(c1, o);
-        e += __shfl_xor(e, o);
-    }
-    __shared__ unsigned int s0[4], s1[4];
-    __shared__ double se[4];
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        s0[w] = c0;
-        s1[w] = c1;
-        se[w] = e;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        pc[2 * (size_t)blockIdx.x] = s0[0] + s0[1] + s0[2] + s0[3];
-        pc[2 * (size_t)blockIdx.x + 1] = s1[0] + s1[1] + s1[2] + s1[3];
-        pe[blockIdx.x] = (se[0] + se[1]) + (se[2] + se[3]);
-    }
-}
-
-// one block of 1024: the per-block slots of k_count summed in a fixed order
+// one block of 1024: the per-tile slots of the counting stage summed in tile
+// order (exact counts, a deterministic error sum)
 __global__ __launch_bounds__(1024) void k_count_sum(int nb, const unsigned int *__restrict__ pc,
                                                     const double *__restrict__ pe,
                                                     unsigned long long *__restrict__ counts, double *__restrict__ err) {
@@ -461,16 +477,18 @@ static int ls_rk3(LsRun &L, int mode, double final_t, double vel, double cfl, do
     const int nb = nblk(L.n);
     // the tiles of k_ls_stage: 64 (k) x 4 (j) points of one i-plane
     const dim3 tgrid((L.d.nz + kLsTk - 1) / kLsTk, (L.d.ny + kLsTj - 1) / kLsTj, L.d.nx), tblk(kLsTk, kLsTj);
+    const size_t nbt = (size_t)tgrid.x * tgrid.y * tgrid.z;  // count slots: one per tile
     unsigned long long *cnt = nullptr;
     double *err = nullptr, *pe = nullptr;
     unsigned int *pc = nullptr;
     IBIM_HIP_CHECK(dev_malloc(&cnt, 2 * sizeof(unsigned long long)));
     IBIM_HIP_CHECK(dev_malloc(&err, sizeof(double)));
-    IBIM_HIP_CHECK(dev_malloc(&pc, 2 * (size_t)nb * sizeof(unsigned int)));
-    IBIM_HIP_CHECK(dev_malloc(&pe, (size_t)nb * sizeof(double)));
+    IBIM_HIP_CHECK(dev_malloc(&pc, 2 * nbt * sizeof(unsigned int)));
+    IBIM_HIP_CHECK(dev_malloc(&pe, nbt * sizeof(double)));
+    // the slots the first stage of a step wrote for the state it started
+    // from, summed in tile order
     auto count = [&](unsigned long long *h) -> int {
-        k_count<<<nb, 256, 0, L.st>>>(L.g, L.d, tube, thres, pc, pe);
-        k_count_sum<<<1, 1024, 0, L.st>>>(nb, pc, pe, cnt, err);
+        k_count_sum<<<1, 1024, 0, L.st>>>((int)nbt, pc, pe, cnt, err);
         IBIM_LAUNCH_CHECK();
         double e = 0;
         IBIM_HIP_CHECK(hipMemcpyAsync(h, cnt, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, L.st));
@@ -484,21 +502,31 @@ static int ls_rk3(LsRun &L, int mode, double final_t, double vel, double cfl, do
     };
     int step = 0, rc = IBIM_OK;
     unsigned long long hc[2] = {0, 1};
-    if (mode == 1 && (rc = count(hc))) return rc;
-    if (mode == 1) k_sign<<<nb, 256, 0, L.st>>>(L.phi0, L.n, eps);
-    while (step < num_steps && (mode == 0 || hc[1] != 0)) {
-        ++step;
-        if (mode == 0) {
+    if (mode == 0) {
+        while (step < num_steps) {
+            ++step;
             k_ls_stage<0, 1><<<tgrid, tblk, 0, L.st>>>(L.g, L.g, nullptr, L.u1, L.d, dtv);
             k_ls_stage<0, 2><<<tgrid, tblk, 0, L.st>>>(L.g, L.u1, nullptr, L.u2, L.d, dtv);
             k_ls_stage<0, 3><<<tgrid, tblk, 0, L.st>>>(L.g, L.u2, nullptr, L.g, L.d, dtv);
-        } else {
-            k_ls_stage<1, 1><<<tgrid, tblk, 0, L.st>>>(L.g, L.g, L.phi0, L.u1, L.d, dtv);
+            IBIM_LAUNCH_CHECK();
+        }
+    } else {
+        k_sign<<<nb, 256, 0, L.st>>>(L.phi0, L.n, eps);
+        // Steps run while the state has a bad tube point and steps are left
+        // (the gradient count before every step and after the last one).
+        // Each step's first stage counts the state it starts from, so the
+        // count is no pass of its own; the first stage after the last step
+        // only counts (its result is not used).
+        for (;;) {
+            k_ls_stage<1, 1, true><<<tgrid, tblk, 0, L.st>>>(L.g, L.g, L.phi0, L.u1, L.d, dtv, tube, thres, pc, pe);
+            IBIM_LAUNCH_CHECK();
+            if ((rc = count(hc))) return rc;
+            if (step >= num_steps || hc[1] == 0) break;
+            ++step;
             k_ls_stage<1, 2><<<tgrid, tblk, 0, L.st>>>(L.g, L.u1, L.phi0, L.u2, L.d, dtv);
             k_ls_stage<1, 3><<<tgrid, tblk, 0, L.st>>>(L.g, L.u2, L.phi0, L.g, L.d, dtv);
+            IBIM_LAUNCH_CHECK();
         }
-        IBIM_LAUNCH_CHECK();
-        if (mode == 1 && (rc = count(hc))) return rc;
     }
     if (mode == 0) IBIM_HIP_CHECK(hipStreamSynchronize(L.st));
     (void)hipFree(cnt);
