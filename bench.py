@@ -87,7 +87,7 @@ CONFIGS = {
                       confine=2.6, inner_order=4),
     # single molecules (not the default workload)
     "globule500": dict(FMM, atoms=500, seed=500, confine=20.0, h=0.25),
-    "globule5000": dict(FMM, atoms=5000, seed=5000, confine=45.0, h=0.25, p=6, leaf=256),
+    "globule5000": dict(FMM, atoms=5000, seed=5000, confine=45.0, h=0.25, p=6, leaf=256, inner_order=5),
     "globule50000": dict(FMM, atoms=50000, seed=50000, confine=100.0, h=0.3, p=8, leaf=512, inner_order=7),
 }
 
