@@ -18,6 +18,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 
 #include "ibim_internal.h"
@@ -478,6 +479,10 @@ static int ls_rk3(LsRun &L, int mode, double final_t, double vel, double cfl, do
     // the tiles of k_ls_stage: 64 (k) x 4 (j) points of one i-plane
     const dim3 tgrid((L.d.nz + kLsTk - 1) / kLsTk, (L.d.ny + kLsTj - 1) / kLsTj, L.d.nx), tblk(kLsTk, kLsTj);
     const size_t nbt = (size_t)tgrid.x * tgrid.y * tgrid.z;  // count slots: one per tile
+    if (nbt > (size_t)INT_MAX) {
+        set_error("level set: more tiles than the gradient count's slots can index");
+        return IBIM_ERR_ARG;
+    }
     unsigned long long *cnt = nullptr;
     double *err = nullptr, *pe = nullptr;
     unsigned int *pc = nullptr;
