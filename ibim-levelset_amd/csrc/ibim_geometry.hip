@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstring>
 
 #include "ibim_internal.h"
 
@@ -483,22 +484,34 @@ static int ls_rk3(LsRun &L, int mode, double final_t, double vel, double cfl, do
         set_error("level set: more tiles than the gradient count's slots can index");
         return IBIM_ERR_ARG;
     }
-    unsigned long long *cnt = nullptr;
-    double *err = nullptr, *pe = nullptr;
-    unsigned int *pc = nullptr;
-    IBIM_HIP_CHECK(dev_malloc(&cnt, 2 * sizeof(unsigned long long)));
-    IBIM_HIP_CHECK(dev_malloc(&err, sizeof(double)));
-    IBIM_HIP_CHECK(dev_malloc(&pc, 2 * nbt * sizeof(unsigned int)));
-    IBIM_HIP_CHECK(dev_malloc(&pe, nbt * sizeof(double)));
+    // the count's slots, its sums (two counts and the error sum, one
+    // device block) and their pinned host copy, released on every return
+    struct CountBufs {
+        unsigned int *pc = nullptr;
+        double *pe = nullptr;
+        unsigned long long *res = nullptr;
+        unsigned long long *host = nullptr;
+        ~CountBufs() {
+            for (void *p : {(void *)pc, (void *)pe, (void *)res})
+                if (p) (void)hipFree(p);
+            if (host) (void)hipHostFree(host);
+        }
+    } cb;
+    IBIM_HIP_CHECK(dev_malloc(&cb.pc, 2 * nbt * sizeof(unsigned int)));
+    IBIM_HIP_CHECK(dev_malloc(&cb.pe, nbt * sizeof(double)));
+    IBIM_HIP_CHECK(dev_malloc(&cb.res, 3 * sizeof(unsigned long long)));
+    IBIM_HIP_CHECK(hipHostMalloc((void **)&cb.host, 3 * sizeof(unsigned long long)));
     // the slots the first stage of a step wrote for the state it started
-    // from, summed in tile order
+    // from, summed in tile order and read back in one copy
     auto count = [&](unsigned long long *h) -> int {
-        k_count_sum<<<1, 1024, 0, L.st>>>((int)nbt, pc, pe, cnt, err);
+        k_count_sum<<<1, 1024, 0, L.st>>>((int)nbt, cb.pc, cb.pe, cb.res, (double *)(cb.res + 2));
         IBIM_LAUNCH_CHECK();
-        double e = 0;
-        IBIM_HIP_CHECK(hipMemcpyAsync(h, cnt, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, L.st));
-        IBIM_HIP_CHECK(hipMemcpyAsync(&e, err, sizeof(double), hipMemcpyDeviceToHost, L.st));
+        IBIM_HIP_CHECK(hipMemcpyAsync(cb.host, cb.res, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, L.st));
         IBIM_HIP_CHECK(hipStreamSynchronize(L.st));
+        h[0] = cb.host[0];
+        h[1] = cb.host[1];
+        double e;
+        std::memcpy(&e, cb.host + 2, sizeof(double));
         if (quality) {
             quality[0] = h[0] ? e / (double)h[0] : 0.0;
             quality[1] = h[0] ? (double)h[1] / (double)h[0] : 0.0;
@@ -523,7 +536,8 @@ static int ls_rk3(LsRun &L, int mode, double final_t, double vel, double cfl, do
         // count is no pass of its own; the first stage after the last step
         // only counts (its result is not used).
         for (;;) {
-            k_ls_stage<1, 1, true><<<tgrid, tblk, 0, L.st>>>(L.g, L.g, L.phi0, L.u1, L.d, dtv, tube, thres, pc, pe);
+            k_ls_stage<1, 1, true><<<tgrid, tblk, 0, L.st>>>(L.g, L.g, L.phi0, L.u1, L.d, dtv, tube, thres, cb.pc,
+                                                             cb.pe);
             IBIM_LAUNCH_CHECK();
             if ((rc = count(hc))) return rc;
             if (step >= num_steps || hc[1] == 0) break;
@@ -534,10 +548,6 @@ static int ls_rk3(LsRun &L, int mode, double final_t, double vel, double cfl, do
         }
     }
     if (mode == 0) IBIM_HIP_CHECK(hipStreamSynchronize(L.st));
-    (void)hipFree(cnt);
-    (void)hipFree(err);
-    (void)hipFree(pc);
-    (void)hipFree(pe);
     *steps_out = step;
     return IBIM_OK;
 }
